@@ -977,6 +977,8 @@ def _run_blocks(p, Sf, Bf, Vt, alpha, tuples, perts, tuple_ok, pert_ok):
         r = solve_block(Sf[np.ix_(tsel, tsel)], Bf[np.ix_(tsel, tsel)],
                         Vt[np.ix_(tsel, psel)] if len(psel) else np.zeros((len(tsel), 0), complex),
                         np.asarray(alpha)[psel] if len(psel) else np.zeros(0))
+        if r.dimm == 0:          # 'If (dimm == 0) cycle' (solve_A_subspace.F90:162-168): no e2X(isym) line
+            continue
         blocks.append((isym, r))
         e2 += r.e2
         sumc2 += r.sumc2

@@ -348,9 +348,32 @@ def dense_from_records(nmo: int, records, nmom: int) -> Tuple[np.ndarray, int]:
 @dataclass
 class Fixture:
     inp: ActiveInput
-    mr: Mrconee
+    mr: Optional[Mrconee]
     problem: Problem
     conflicts: int
+    gold: Optional[dict] = None
+
+
+def load_fixture_npz(path: str) -> Fixture:
+    """Problem + goldens from a tests/golden/fixtures/*.npz file (see make_fixtures.py)."""
+    z = np.load(path, allow_pickle=False)
+    ninact, nact, nsec, nelec, totsym, selectroot = (int(x) for x in z["dims"])
+    nmom = ninact + nact + nsec
+    idx = z["idx"].astype(np.int64)
+    val = z["val"].astype(np.complex128)
+    recs = ((int(a), int(b), int(c), int(d), float(v.real), float(v.imag))
+            for (a, b, c, d), v in zip(idx, val))
+    eri, conflicts = dense_from_records(nmom, recs, nmom)
+    rv = [int(x) for x in z["ras"]]
+    ras = RasSpec(*rv) if any(rv[:3]) else None
+    sym = Symmetry(int(z["nsymrpa"]), z["multb"], z["irpamo"])
+    p = Problem(ninact=ninact, nact=nact, nsec=nsec, nelec=nelec, h=z["h"], eri=DenseIntegrals(eri),
+                ecore=float(z["ecore"]), selectroot=selectroot, totsym=totsym, eshift=float(z["eshift"]),
+                sym=sym, ras=ras)
+    inp = ActiveInput(ninact=ninact, nact=nact, nsec=nsec, nelec=nelec, totsym=totsym,
+                      selectroot=selectroot, eshift=float(z["eshift"]))
+    gold = {k[5:]: z[k] for k in z.files if k.startswith("gold_")}
+    return Fixture(inp, None, p, conflicts, gold)
 
 
 def load_fixture(dirpath: str) -> Fixture:
