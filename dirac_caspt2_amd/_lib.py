@@ -1,0 +1,81 @@
+"""ctypes binding of libdcaspt2_amd.so (the C ABI declared in include/dcaspt2_amd.h).
+
+The product path has no fallback: when the shared library is missing the
+import of the binding raises, and every call through it fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdcaspt2_amd.so")
+
+c_int = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+P = ctypes.c_void_p
+PD = ctypes.POINTER(ctypes.c_double)
+PI64 = ctypes.POINTER(ctypes.c_int64)
+PI32 = ctypes.POINTER(ctypes.c_int32)
+
+_SIGS = {
+    "dc2_context_create": (c_int, [c_int, ctypes.POINTER(P)]),
+    "dc2_context_destroy": (c_int, [P]),
+    "dc2_last_error": (ctypes.c_char_p, []),
+    "dc2_version": (ctypes.c_char_p, []),
+    "dc2_set_spaces": (c_int, [P, c_int, c_int, c_int, c_int]),
+    "dc2_set_symmetry": (c_int, [P, c_int, P, P, c_int]),
+    "dc2_set_ras": (c_int, [P, c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
+    "dc2_set_one_electron": (c_int, [P, P, c_dbl]),
+    "dc2_set_integral_block": (c_int, [P, ctypes.c_char_p, P, c_i64, c_int]),
+    "dc2_casci": (c_int, [P, c_int, P, P]),
+    "dc2_get_dets": (c_int, [P, P]),
+    "dc2_get_ci_vector": (c_int, [P, c_int, P]),
+    "dc2_caspt2": (c_int, [P, c_int, c_dbl, P, P, P]),
+    "dc2_get_subspace_info": (c_int, [P, P, P]),
+    "dc2_get_fock_rotation": (c_int, [P, P, P]),
+    "dc2_get_rotated_ci": (c_int, [P, P]),
+    "dc2_get_density": (c_int, [P, c_int, P, c_i64]),
+    "dc2_get_transformed_block": (c_int, [P, ctypes.c_char_p, P, c_i64]),
+    "dc2_get_timings": (c_int, [P, P, c_int]),
+    "dc2_casmat": (c_int, [P, P]),
+    "dc2_cdiag": (c_int, [P, P, c_int, P, P, c_dbl]),
+    "dc2_zgemm": (c_int, [P, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, P, P, c_i64, c_i64, P, c_i64, c_i64,
+                          P, P, c_i64, c_i64, c_i64]),
+    "dc2_synchronize": (c_int, [P]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+class Dc2Error(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m dirac_caspt2_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+def check(fn: str, status: int) -> None:
+    if status != 0:
+        raise Dc2Error(fn, status, lib().dc2_last_error().decode(errors="replace"))

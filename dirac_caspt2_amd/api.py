@@ -1,0 +1,239 @@
+"""Host-side mirror of the reference's CASCI/CASPT2 driver over the C ABI.
+
+The reference keeps the problem in module globals (module_global_variables.F90)
+filled by its readers and then calls casci (casci.F90) and r4dcaspt2_tra
+(r4dcaspt2_tra.F90).  ``Caspt2Context`` holds the same state inside a GPU
+context: ``set_spaces`` / ``set_symmetry`` / ``set_ras`` / ``set_one_electron``
+/ ``set_integral_block`` correspond to what read_mrconee, readint2_casci and
+divide_2_elec_integral_into_subspaces produce; ``casci()`` and ``caspt2()``
+run the two subprograms.  Arrays are column-major (Fortran order) complex128,
+given as numpy arrays (host) or torch tensors (device).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+SUBSPACES = "ABCDEFGH"
+BLOCK_SHAPES = {
+    "OOOO": "oooo", "F1": "ssoo", "F2": "soos",
+    "A1": "aiaa", "A2": "aiii", "B": "aiai",
+    "C1": "saaa", "C2": "saii", "C3": "siia", "D1": "siaa", "D2": "saai", "D3": "siii",
+    "E": "siai", "F": "sasa", "G": "sisa", "H": "sisi",
+}
+# blocks the C ABI needs (A1/A2/B are cut out of OOOO on the device when absent)
+REQUIRED_BLOCKS = ("OOOO", "F1", "F2", "C1", "C2", "C3", "D1", "D2", "D3", "E", "F", "G", "H")
+
+
+@dataclass
+class Caspt2Energies:
+    e2: Dict[str, float]
+    sumc2: Dict[str, float]
+    e0: float
+    e2all: float
+    sumc2_all: float
+    weight0: float
+    total_second_order: float
+    total: float
+    info: Dict[str, tuple] = field(default_factory=dict)
+    margin: Dict[str, float] = field(default_factory=dict)
+    timings: List[float] = field(default_factory=list)
+
+
+def _ptr(a) -> int:
+    """Data pointer of a numpy array or torch tensor."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _fortran_complex(a: np.ndarray) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.complex128))
+
+
+class Caspt2Context:
+    """One GPU context of libdcaspt2_amd (one problem resident in HBM at a time)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.lib()
+        h = ctypes.c_void_p()
+        check("dc2_context_create", self.lib.dc2_context_create(device, ctypes.byref(h)))
+        self.h = h
+        self.dims = None
+        self._keep: Dict[str, object] = {}
+
+    def close(self):
+        if self.h:
+            check("dc2_context_destroy", self.lib.dc2_context_destroy(self.h))
+            self.h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- problem definition --------------------------------------------------
+    def set_spaces(self, ninact: int, nact: int, nsec: int, nelec: int):
+        check("dc2_set_spaces", self.lib.dc2_set_spaces(self.h, ninact, nact, nsec, nelec))
+        self.dims = (ninact, nact, nsec, nelec)
+        self._keep.clear()
+
+    def set_symmetry(self, nsymrpa: int, multb: np.ndarray, irpamo: Sequence[int], totsym: int):
+        mb = np.asfortranarray(np.asarray(multb, dtype=np.int64))
+        ir = np.ascontiguousarray(np.asarray(irpamo, dtype=np.int64))
+        check("dc2_set_symmetry", self.lib.dc2_set_symmetry(self.h, nsymrpa, _ptr(mb), _ptr(ir), totsym))
+
+    def set_ras(self, ras1: int, ras2: int, ras3: int, ras1_max_hole: int, ras3_max_elec: int,
+                min_hole_ras1: int = 0, min_elec_ras3: int = 0):
+        check("dc2_set_ras", self.lib.dc2_set_ras(self.h, ras1, ras2, ras3, ras1_max_hole, ras3_max_elec,
+                                                   min_hole_ras1, min_elec_ras3))
+
+    def set_one_electron(self, h: np.ndarray, ecore: float):
+        hf = _fortran_complex(h)
+        check("dc2_set_one_electron", self.lib.dc2_set_one_electron(self.h, _ptr(hf), float(ecore)))
+
+    def set_integral_block(self, name: str, data, borrow: bool = False):
+        """data: numpy array [i,j,k,l] (any order, copied from host) or a torch CUDA
+        tensor whose memory is already column-major (i fastest)."""
+        if isinstance(data, np.ndarray):
+            arr = _fortran_complex(data)
+            where = 0
+        else:
+            arr = data
+            where = 2 if borrow else 1
+            if borrow:
+                self._keep[name] = data
+        n = int(np.prod(arr.shape)) if isinstance(arr, np.ndarray) else int(arr.numel())
+        check("dc2_set_integral_block", self.lib.dc2_set_integral_block(self.h, name.encode(), _ptr(arr), n, where))
+
+    # --- CASCI ----------------------------------------------------------------
+    def casci(self, nroot: int = 1):
+        eig = np.zeros(max(nroot, 1))
+        nd = ctypes.c_int64()
+        check("dc2_casci", self.lib.dc2_casci(self.h, nroot, _ptr(eig), ctypes.byref(nd)))
+        self.ndet = nd.value
+        return eig[:min(nroot, nd.value)]
+
+    def dets(self) -> np.ndarray:
+        out = np.zeros(self.ndet, dtype=np.int64)
+        check("dc2_get_dets", self.lib.dc2_get_dets(self.h, _ptr(out)))
+        return out
+
+    def ci_vector(self, root: int = 1) -> np.ndarray:
+        out = np.zeros(self.ndet, dtype=np.complex128)
+        check("dc2_get_ci_vector", self.lib.dc2_get_ci_vector(self.h, root, _ptr(out)))
+        return out
+
+    def casmat(self) -> np.ndarray:
+        out = np.zeros((self.ndet, self.ndet), dtype=np.complex128, order="F")
+        check("dc2_casmat", self.lib.dc2_casmat(self.h, _ptr(out)))
+        return out
+
+    # --- CASPT2 ---------------------------------------------------------------
+    def caspt2(self, selectroot: int = 1, eshift: float = 0.0) -> Caspt2Energies:
+        e2 = np.zeros(8)
+        s2 = np.zeros(8)
+        sc = np.zeros(6)
+        check("dc2_caspt2", self.lib.dc2_caspt2(self.h, selectroot, float(eshift), _ptr(e2), _ptr(s2), _ptr(sc)))
+        info = np.zeros(32, dtype=np.int64)
+        margin = np.zeros(8)
+        check("dc2_get_subspace_info", self.lib.dc2_get_subspace_info(self.h, _ptr(info), _ptr(margin)))
+        tm = np.zeros(12)
+        check("dc2_get_timings", self.lib.dc2_get_timings(self.h, _ptr(tm), 12))
+        return Caspt2Energies(
+            e2={X: float(e2[k]) for k, X in enumerate(SUBSPACES)},
+            sumc2={X: float(s2[k]) for k, X in enumerate(SUBSPACES)},
+            e0=float(sc[0]), e2all=float(sc[1]), sumc2_all=float(sc[2]), weight0=float(sc[3]),
+            total_second_order=float(sc[4]), total=float(sc[5]),
+            info={X: tuple(int(v) for v in info[4 * k:4 * k + 4]) for k, X in enumerate(SUBSPACES)},
+            margin={X: float(margin[k]) for k, X in enumerate(SUBSPACES)},
+            timings=[float(t) for t in tm])
+
+    def fock_rotation(self):
+        n = sum(self.dims[:3])
+        eps = np.zeros(n)
+        U = np.zeros((n, n), dtype=np.complex128, order="F")
+        check("dc2_get_fock_rotation", self.lib.dc2_get_fock_rotation(self.h, _ptr(eps), _ptr(U)))
+        return eps, U
+
+    def rotated_ci(self) -> np.ndarray:
+        out = np.zeros(self.ndet, dtype=np.complex128)
+        check("dc2_get_rotated_ci", self.lib.dc2_get_rotated_ci(self.h, _ptr(out)))
+        return out
+
+    def density(self, which: int) -> np.ndarray:
+        na = self.dims[1]
+        n = na * na if which in (0, 1) else na ** 4
+        out = np.zeros(n, dtype=np.complex128)
+        check("dc2_get_density", self.lib.dc2_get_density(self.h, which, _ptr(out), n))
+        return out
+
+    def transformed_block(self, name: str) -> np.ndarray:
+        shape = block_shape(self.dims, name)
+        out = np.zeros(shape, dtype=np.complex128, order="F")
+        check("dc2_get_transformed_block",
+              self.lib.dc2_get_transformed_block(self.h, name.encode(), _ptr(out), int(np.prod(shape))))
+        return out
+
+    def synchronize(self):
+        check("dc2_synchronize", self.lib.dc2_synchronize(self.h))
+
+
+def cdiag(ctx: Caspt2Context, c: np.ndarray, threshold: float = 0.0):
+    """diag.F90:66-128 on the GPU: returns (w, eigenvectors, dimm)."""
+    a = _fortran_complex(c).copy(order="F")
+    n = a.shape[0]
+    w = np.zeros(n)
+    dimm = ctypes.c_int32()
+    check("dc2_cdiag", ctx.lib.dc2_cdiag(ctx.h, _ptr(a), n, ctypes.byref(dimm), _ptr(w), float(threshold)))
+    return w, a, dimm.value
+
+
+def zgemm(ctx: Caspt2Context, opA: str, opB: str, M: int, N: int, K: int, alpha: complex, A, lda: int, sA: int,
+          B, ldb: int, sB: int, beta: complex, C, ldc: int, sC: int, batch: int = 1):
+    """C = alpha op(A) op(B) + beta C on device memory (torch tensors)."""
+    al = np.array([alpha.real, alpha.imag])
+    be = np.array([beta.real, beta.imag])
+    check("dc2_zgemm", ctx.lib.dc2_zgemm(ctx.h, opA.encode(), opB.encode(), M, N, K, _ptr(al), _ptr(A), lda, sA,
+                                         _ptr(B), ldb, sB, _ptr(be), _ptr(C), ldc, sC, batch))
+
+
+def space_sizes(dims):
+    ni, na, ns = dims[:3]
+    return {"i": ni, "a": na, "s": ns, "o": ni + na}
+
+
+def space_slices(dims):
+    ni, na, ns = dims[:3]
+    return {"i": slice(0, ni), "a": slice(ni, ni + na), "s": slice(ni + na, ni + na + ns), "o": slice(0, ni + na)}
+
+
+def block_shape(dims, name: str):
+    sz = space_sizes(dims)
+    return tuple(sz[ch] for ch in BLOCK_SHAPES[name])
+
+
+def needed_blocks(dims) -> List[str]:
+    ni, na, ns = dims[:3]
+    names = ["OOOO"]
+    if ns:
+        names += ["F1", "F2", "C1"]
+        if ni:
+            names += ["C2", "C3", "D1", "D2", "D3", "E", "F", "G", "H"]
+        else:
+            names += ["F"]
+    return [n for n in names if all(v > 0 for v in block_shape(dims, n))]

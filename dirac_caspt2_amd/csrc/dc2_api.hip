@@ -1,0 +1,822 @@
+// C ABI and host orchestration of the CASCI / CASPT2 path on one GPU.
+//
+// All heavy arrays live in HBM for the lifetime of the context: integrals,
+// CI vectors, determinant tables, densities, transformed blocks.  The host only
+// moves scalars and a few eigenvalue vectors (to size the S^-1/2 projection).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+
+#include "../../include/dcaspt2_amd.h"
+#include "dc2_kernels.hpp"
+
+using namespace dc2;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define DC2_ROCSOLVER(expr)                                                                \
+  do {                                                                                     \
+    rocblas_status st_ = (expr);                                                           \
+    if (st_ != rocblas_status_success)                                                     \
+      throw Error(3, std::string("rocSOLVER/rocBLAS status ") + std::to_string((int)st_) + \
+                         " at " + __FILE__ + ":" + std::to_string(__LINE__));              \
+  } while (0)
+
+struct Block {
+  DevBuf<cplx> own;
+  const cplx* ptr = nullptr;
+  int64_t count = 0;
+  bool derived = false;   // copied out of OOOO by the library
+};
+
+const char* kSub = "ABCDEFGH";
+
+}  // namespace
+
+struct dc2_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  int ni = -1, na = 0, ns = 0, ne = 0;
+  int nsym = 1, totsym = 1;
+  std::vector<int> irpamo;                 // per spinor, 1-based
+  std::vector<int> multb_d, multb_ds;      // nsym x nsym
+  int ras[7] = {0, 0, 0, 0, 0, 0, 0};
+  double ecore = 0.0;
+  DevBuf<cplx> h;
+  std::map<std::string, Block> blocks;
+  // CASCI
+  int64_t nfull = 0, ndet = 0;
+  DevBuf<int64_t> full_dets, cas_dets;
+  DevBuf<int> rank_full, rank_cas, cas2full, exc;
+  DevBuf<cplx> civec;
+  std::vector<double> ecas;
+  bool have_casci = false;
+  // CASPT2 state
+  DevBuf<cplx> fock, U, hp, cnew, c_full, cF, Y1, Y1F, G1, G1F, G2, G2F, G3, G3F;
+  DevBuf<double> eps, e0, out2, partial;
+  std::map<std::string, DevBuf<cplx>> tblk;
+  DevBuf<int> tpairs, ipairs, spairs;
+  int64_t sub_info[4 * DC2_NSUBSPACE] = {0};
+  double sub_margin[DC2_NSUBSPACE] = {0};
+  double timings[12] = {0};
+  bool have_pt2 = false;
+  // scratch
+  DevBuf<cplx> w1, w2, w3, w4;
+  DevBuf<double> dw1, dw2;
+  DevBuf<int> info;
+
+  int nmo() const { return ni + na + ns; }
+  int nmoc() const { return ni + na; }
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+struct BlockShape {
+  const char* name;
+  const char* sp;
+};
+const BlockShape kShapes[] = {{"OOOO", "oooo"}, {"F1", "ssoo"}, {"F2", "soos"}, {"A1", "aiaa"}, {"A2", "aiii"},
+                              {"B", "aiai"},    {"C1", "saaa"}, {"C2", "saii"}, {"C3", "siia"}, {"D1", "siaa"},
+                              {"D2", "saai"},   {"D3", "siii"}, {"E", "siai"},  {"F", "sasa"},  {"G", "sisa"},
+                              {"H", "sisi"}};
+
+const char* shape_of(const std::string& name) {
+  for (const auto& s : kShapes)
+    if (name == s.name) return s.sp;
+  throw Error(1, "unknown integral block name '" + name + "'");
+}
+
+int space_dim(const dc2_context* c, char s) {
+  switch (s) {
+    case 'i': return c->ni;
+    case 'a': return c->na;
+    case 's': return c->ns;
+    default: return c->ni + c->na;
+  }
+}
+int space_off(const dc2_context* c, char s) {
+  switch (s) {
+    case 'i': return 0;
+    case 'a': return c->ni;
+    case 's': return c->ni + c->na;
+    default: return 0;
+  }
+}
+int64_t block_count(const dc2_context* c, const char* sp) {
+  int64_t n = 1;
+  for (int k = 0; k < 4; ++k) n *= space_dim(c, sp[k]);
+  return n;
+}
+
+void need_spaces(const dc2_context* c) { DC2_CHECK(c->ni >= 0, 1, "dc2_set_spaces has not been called"); }
+
+// pointer to the original-basis block; A1/A2/B fall back to a copy out of OOOO
+const cplx* get_block(dc2_context* c, const std::string& name) {
+  auto it = c->blocks.find(name);
+  if (it != c->blocks.end()) return it->second.ptr;
+  if (name == "A1" || name == "A2" || name == "B") {
+    auto o = c->blocks.find("OOOO");
+    DC2_CHECK(o != c->blocks.end(), 1, "integral block OOOO is required");
+    const char* sp = shape_of(name);
+    Block& b = c->blocks[name];
+    b.derived = true;
+    b.count = block_count(c, sp);
+    b.own.alloc(b.count);
+    const int64_t m = c->nmoc();
+    int64_t ld[3] = {m, m, m};
+    int off[4], n[4];
+    for (int k = 0; k < 4; ++k) { off[k] = space_off(c, sp[k]); n[k] = space_dim(c, sp[k]); }
+    extract4(c->stream, o->second.ptr, ld, off, n, b.own.get());
+    b.ptr = b.own.get();
+    return b.ptr;
+  }
+  throw Error(1, "integral block " + name + " has not been set");
+}
+
+void zheevd(dc2_context* c, int n, cplx* A, double* w) {
+  c->dw2.ensure(std::max(n, 1));
+  c->info.ensure(1);
+  DC2_ROCSOLVER(rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n,
+                                 reinterpret_cast<rocblas_double_complex*>(A), n, w, c->dw2.get(), c->info.get()));
+  int info = 0;
+  DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  DC2_HIP(hipStreamSynchronize(c->stream));
+  if (info != 0) throw Error(100 + info, "zheevd failed, info = " + std::to_string(info));
+}
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    DC2_HIP(hipEventCreate(&a));
+    DC2_HIP(hipEventCreate(&b));
+    DC2_HIP(hipEventRecord(a, s));
+  }
+  double stop() {
+    DC2_HIP(hipEventRecord(b, s));
+    DC2_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    DC2_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms * 1e-3;
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+
+std::vector<int> pair_list(int n) {
+  std::vector<int> v;
+  for (int t = 0; t < n; ++t)
+    for (int u = 0; u < t; ++u) { v.push_back(t); v.push_back(u); }
+  return v;
+}
+
+void upload_ints(hipStream_t s, DevBuf<int>& dst, const std::vector<int>& v) {
+  dst.alloc(std::max<size_t>(v.size(), 1));
+  if (!v.empty()) DC2_HIP(hipMemcpyAsync(dst.get(), v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, s));
+}
+
+// ---------------------------------------------------------------------------
+void run_casci(dc2_context* c) {
+  need_spaces(c);
+  DC2_CHECK(c->h.get() != nullptr, 1, "dc2_set_one_electron has not been called");
+  const cplx* O = get_block(c, "OOOO");
+  hipStream_t s = c->stream;
+  Timer tm(s);
+  const int na = c->na;
+  const int64_t N = 1ll << na;
+  DevBuf<int> ff(N), fc(N), counts(2), irp(std::max(na, 1)), md(std::max<size_t>(c->multb_d.size(), 1)),
+      mds(std::max<size_t>(c->multb_ds.size(), 1));
+  std::vector<int> irp_act(std::max(na, 1), 1);
+  for (int w = 0; w < na; ++w) irp_act[w] = c->irpamo.empty() ? 1 : c->irpamo[c->ni + w];
+  DC2_HIP(hipMemcpyAsync(irp.get(), irp_act.data(), irp_act.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  if (!c->multb_d.empty()) {
+    DC2_HIP(hipMemcpyAsync(md.get(), c->multb_d.data(), c->multb_d.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    DC2_HIP(hipMemcpyAsync(mds.get(), c->multb_ds.data(), c->multb_ds.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  }
+  DetParams p{na, c->ne, c->ni, c->ras[0], c->ras[1], c->ras[2], c->ras[3], c->ras[4], c->ras[5], c->ras[6],
+              c->nsym, c->totsym, irp.get(), md.get(), mds.get()};
+  c->rank_full.alloc(N);
+  c->rank_cas.alloc(N);
+  int64_t nf = 0, nd = 0;
+  enumerate_dets(s, p, ff.get(), fc.get(), c->rank_full.get(), c->rank_cas.get(), nullptr, nullptr, nullptr, &nf, &nd,
+                 counts.get());
+  DC2_CHECK(nd > 0, 1, "the number of CASCI configurations is 0 (check totsym)");
+  c->nfull = nf;
+  c->ndet = nd;
+  c->full_dets.alloc(nf);
+  c->cas_dets.alloc(nd);
+  c->cas2full.alloc(nd);
+  enumerate_dets(s, p, ff.get(), fc.get(), c->rank_full.get(), c->rank_cas.get(), c->full_dets.get(),
+                 c->cas_dets.get(), c->cas2full.get(), &nf, &nd, counts.get());
+  c->civec.alloc(nd * nd);
+  casmat(s, c->ni, na, c->ne, nd, c->cas_dets.get(), c->rank_cas.get(), O, c->nmoc(), c->h.get(), c->nmo(),
+         c->civec.get());
+  DevBuf<double> w(nd);
+  zheevd(c, (int)nd, c->civec.get(), w.get());
+  c->ecas.resize(nd);
+  DC2_HIP(hipMemcpyAsync(c->ecas.data(), w.get(), nd * sizeof(double), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
+  c->have_casci = true;
+  c->have_pt2 = false;
+  c->timings[0] = tm.stop();
+}
+
+// four one-index passes, each contracting the leading index and rotating it to the end
+void transform(dc2_context* c, const std::string& name) {
+  const char* sp = shape_of(name);
+  const cplx* src = get_block(c, name);
+  const int64_t cnt = block_count(c, sp);
+  DevBuf<cplx>& out = c->tblk[name];
+  out.ensure(cnt);
+  c->w1.ensure(cnt);
+  int64_t dims[4];
+  for (int k = 0; k < 4; ++k) dims[k] = space_dim(c, sp[k]);
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const int nmo = c->nmo();
+  const cplx* in = src;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int64_t n = dims[0], rest = dims[1] * dims[2] * dims[3];
+    cplx* dst = (pass % 2 == 0) ? c->w1.get() : out.get();
+    const int off = space_off(c, sp[pass]);
+    const cplx* Ub = c->U.get() + off + (int64_t)nmo * off;
+    // (i'j'|k'l') = sum conj(U_ii') U_jj' conj(U_kk') U_ll' (ij|kl): conj on the 1st and 3rd index
+    zgemm(c->stream, 'T', (pass % 2 == 0) ? 'R' : 'N', rest, n, n, one, in, n, 0, Ub, nmo, 0, zero, dst, rest, 0, 1);
+    in = dst;
+    const int64_t d0 = dims[0];
+    dims[0] = dims[1]; dims[1] = dims[2]; dims[2] = dims[3]; dims[3] = d0;
+  }
+}
+
+struct SolveOut {
+  double e2 = 0, sumc2 = 0, margin = INFINITY;
+  int64_t dimm = 0;
+};
+
+// S -> cdiag(1e-8) -> ccutoff -> ulambda_s_half -> B~ -> cdiag -> E2 (solve_A_subspace.F90:150-266)
+SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B, const cplx* V, const double* alpha) {
+  SolveOut r;
+  if (n == 0) return r;
+  hipStream_t s = c->stream;
+  DevBuf<double> ws(n);
+  zheevd(c, (int)n, S, ws.get());
+  std::vector<double> w(n);
+  DC2_HIP(hipMemcpyAsync(w.data(), ws.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
+  const double thr = 1.0e-8;
+  int64_t dimm = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    if (w[k] >= thr) ++dimm;
+    r.margin = std::min(r.margin, std::fabs(w[k] - thr));
+  }
+  r.dimm = dimm;
+  if (dimm == 0 || npert == 0) return r;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  DevBuf<cplx> uc(n * dimm), T1(dimm * n), bc1(dimm * dimm), Y(dimm * npert), X(dimm * npert);
+  DevBuf<double> wb(dimm);
+  scale_cols_tail(s, n, dimm, S, ws.get(), uc.get());
+  zgemm(s, 'C', 'N', dimm, n, n, one, uc.get(), n, 0, B, n, 0, zero, T1.get(), dimm, 0, 1);
+  zgemm(s, 'N', 'N', dimm, dimm, n, one, T1.get(), dimm, 0, uc.get(), n, 0, zero, bc1.get(), dimm, 0, 1);
+  zheevd(c, (int)dimm, bc1.get(), wb.get());
+  zgemm(s, 'C', 'N', dimm, npert, n, one, uc.get(), n, 0, V, n, 0, zero, Y.get(), dimm, 0, 1);
+  zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1.get(), dimm, 0, Y.get(), dimm, 0, zero, X.get(), dimm, 0, 1);
+  const int64_t np = e2_partials_needed(dimm * npert);
+  c->partial.ensure(2 * np);
+  c->out2.ensure(2);
+  e2_reduce(s, dimm, npert, X.get(), wb.get(), alpha, c->partial.get(), np, c->out2.get());
+  double o[2];
+  DC2_HIP(hipMemcpyAsync(o, c->out2.get(), sizeof(o), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
+  r.e2 = o[0];
+  r.sumc2 = o[1];
+  return r;
+}
+
+void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, double* sumc2, double* scal) {
+  DC2_CHECK(c->have_casci, 1, "dc2_casci must run before dc2_caspt2");
+  DC2_CHECK(selectroot >= 1 && selectroot <= c->ndet, 1, "selectroot out of range");
+  DC2_CHECK(!(c->ni == 0 && c->ns == 0), 1, "The CASPT2 energy cannot be defined when ninact = 0 and nsec = 0.");
+  hipStream_t s = c->stream;
+  const int ni = c->ni, na = c->na, ns = c->ns, nmo = c->nmo();
+  const int64_t nf = c->nfull, nd = c->ndet, na2 = (int64_t)na * na;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  for (double& t : c->timings) t = (&t == &c->timings[0]) ? t : 0.0;
+  std::memset(c->sub_info, 0, sizeof(c->sub_info));
+  for (int k = 0; k < DC2_NSUBSPACE; ++k) c->sub_margin[k] = INFINITY;
+
+  Timer t1(s);
+  // ---- E_pq tables and the CASCI 1-RDM in the original basis
+  c->exc.alloc(nf * na2);
+  build_exc_table(s, na, nf, c->full_dets.get(), c->rank_full.get(), c->exc.get());
+  c->eps.alloc(nmo);
+  c->e0.alloc(1);
+  c->c_full.ensure(nf);
+  c->cF.ensure(nf);
+  c->Y1.ensure(nf * na2);
+  c->G1.ensure(na2);
+  DevBuf<double> zeros_act(std::max(na, 1));
+  DC2_HIP(hipMemsetAsync(zeros_act.get(), 0, std::max(na, 1) * sizeof(double), s));
+  const cplx* c0 = c->civec.get() + (int64_t)(selectroot - 1) * nd;
+  embed_ci(s, na, nf, nd, c->cas2full.get(), c0, c->full_dets.get(), zeros_act.get(), c->c_full.get(), c->cF.get());
+  apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
+  zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
+  // ---- Fock matrix (fockcasci.F90) and its diagonalisation per space and irrep (fockdiag.F90)
+  c->fock.ensure((int64_t)nmo * nmo);
+  const cplx* F1 = ns ? get_block(c, "F1") : nullptr;
+  const cplx* F2 = ns ? get_block(c, "F2") : nullptr;
+  fock_build(s, ni, na, ns, c->h.get(), nmo, get_block(c, "OOOO"), F1, F2, c->G1.get(), c->fock.get());
+  c->U.ensure((int64_t)nmo * nmo);
+  fill_zero(s, c->U.get(), (int64_t)nmo * nmo);
+  for (int sp = 0; sp < 3; ++sp) {
+    const int lo = sp == 0 ? 0 : (sp == 1 ? ni : ni + na), hi = sp == 0 ? ni : (sp == 1 ? ni + na : nmo);
+    for (int isym = 1; isym <= c->nsym; ++isym) {
+      std::vector<int> ind;
+      for (int i = lo; i < hi; ++i)
+        if ((c->irpamo.empty() ? 1 : c->irpamo[i]) == isym) ind.push_back(i);
+      if (ind.empty()) continue;
+      const int n = (int)ind.size();
+      DevBuf<int> dind;
+      upload_ints(s, dind, ind);
+      DevBuf<cplx> blk((int64_t)n * n), scr((int64_t)n * n);
+      DevBuf<double> w(n);
+      gather_block(s, n, dind.get(), c->fock.get(), nmo, blk.get());
+      zheevd(c, n, blk.get(), w.get());
+      canonical_gauge(s, n, w.get(), blk.get(), scr.get());
+      scatter_block(s, n, dind.get(), blk.get(), w.get(), c->U.get(), nmo, c->eps.get());
+      DC2_HIP(hipStreamSynchronize(s));
+    }
+  }
+  // ---- tracic: ds c~ = c (trac.F90:76-156)
+  {
+    DevBuf<cplx> Ua(na2), ds(nd * nd);
+    int64_t ld[3] = {nmo, 1, 1};
+    int off[4] = {ni, ni, 0, 0}, nn[4] = {na, na, 1, 1};
+    extract4(s, c->U.get(), ld, off, nn, Ua.get());
+    tracic_ds(s, na, c->ne, nd, c->cas_dets.get(), Ua.get(), ds.get());
+    c->cnew.ensure(nd);
+    DC2_HIP(hipMemcpyAsync(c->cnew.get(), c0, nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+    DevBuf<int> ipiv(nd);
+    c->info.ensure(1);
+    DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds.get()),
+                                   (int)nd, ipiv.get(), c->info.get()));
+    int info = 0;
+    DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    DC2_HIP(hipStreamSynchronize(s));
+    if (info != 0) throw Error(100 + info, "zgetrf (tracic) failed, info = " + std::to_string(info));
+    DC2_ROCSOLVER(rocsolver_zgetrs(c->blas, rocblas_operation_none, (int)nd, 1,
+                                   reinterpret_cast<rocblas_double_complex*>(ds.get()), (int)nd, ipiv.get(),
+                                   reinterpret_cast<rocblas_double_complex*>(c->cnew.get()), (int)nd));
+    DC2_HIP(hipStreamSynchronize(s));   // ds / ipiv are released at the end of this scope
+  }
+  c->timings[1] = t1.stop();
+
+  // ---- densities of the rotated state (density_matrix.F90)
+  Timer t2(s);
+  const double* eps_act = c->eps.get() + ni;
+  embed_ci(s, na, nf, nd, c->cas2full.get(), c->cnew.get(), c->full_dets.get(), eps_act, c->c_full.get(),
+           c->cF.get());
+  c->Y1F.ensure(nf * na2);
+  apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
+  apply_E(s, na, nf, 1, c->exc.get(), c->cF.get(), c->Y1F.get());
+  c->G1F.ensure(na2);
+  c->G2.ensure(na2 * na2);
+  c->G2F.ensure(na2 * na2);
+  zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
+  zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1F.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1F.get(), na2, 0, 1);
+  calce0(s, na, eps_act, c->G1.get(), c->e0.get());
+  zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1.get(), nf, 0, zero, c->G2.get(), na2, 0, 1);
+  zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1F.get(), nf, 0, zero, c->G2F.get(), na2, 0, 1);
+  const bool doA = ni > 0, doB = ni > 0, doC = ns > 0, doD = ni > 0 && ns > 0, doF = ns > 0;
+  const bool doE = doD, doG = doD, doH = doD;
+  const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
+  if (doA || doC) {
+    const int64_t na4 = na2 * na2;
+    c->G3.ensure(na2 * na4);
+    c->G3F.ensure(na2 * na4);
+    DevBuf<cplx> Y2(nf * na4);
+    apply_E(s, na, nf, na2, c->exc.get(), c->Y1.get(), Y2.get());
+    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2.get(), nf, 0, zero, c->G3.get(), na2, 0, 1);
+    apply_E(s, na, nf, na2, c->exc.get(), c->Y1F.get(), Y2.get());
+    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2.get(), nf, 0, zero, c->G3F.get(), na2, 0, 1);
+    DC2_HIP(hipStreamSynchronize(s));
+  }
+  // h' = U^H h U (tramo1)
+  c->hp.ensure((int64_t)nmo * nmo);
+  {
+    DevBuf<cplx> T((int64_t)nmo * nmo);
+    zgemm(s, 'C', 'N', nmo, nmo, nmo, one, c->U.get(), nmo, 0, c->h.get(), nmo, 0, zero, T.get(), nmo, 0, 1);
+    zgemm(s, 'N', 'N', nmo, nmo, nmo, one, T.get(), nmo, 0, c->U.get(), nmo, 0, zero, c->hp.get(), nmo, 0, 1);
+    DC2_HIP(hipStreamSynchronize(s));
+  }
+  c->timings[2] = t2.stop();
+
+  // ---- transforms of every needed subspace block (module_intra.F90)
+  Timer t3(s);
+  std::vector<std::string> need;
+  if (doA) { need.push_back("A1"); need.push_back("A2"); }
+  if (doB) need.push_back("B");
+  if (doC) { need.push_back("C1"); if (ni) { need.push_back("C2"); need.push_back("C3"); } }
+  if (doD) { need.push_back("D1"); need.push_back("D2"); need.push_back("D3"); }
+  if (doE) need.push_back("E");
+  if (doF) need.push_back("F");
+  if (doG) need.push_back("G");
+  if (doH) need.push_back("H");
+  for (const auto& nm : need) transform(c, nm);
+  c->timings[3] = t3.stop();
+
+  upload_ints(s, c->tpairs, pair_list(na));
+  upload_ints(s, c->ipairs, pair_list(ni));
+  upload_ints(s, c->spairs, pair_list(ns));
+  Dens d{na, c->G1.get(), c->G1F.get(), c->G2.get(), c->G2F.get(), c->G3.get(), c->G3F.get()};
+  const int64_t ntp = (int64_t)na * (na - 1) / 2, nip = (int64_t)ni * (ni - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
+  const int64_t na3 = na2 * na;
+  for (int k = 0; k < 8; ++k) { e2[k] = 0.0; sumc2[k] = 0.0; }
+
+  for (int k = 0; k < 8; ++k) {
+    if (!do_sub[k]) continue;
+    const char X = kSub[k];
+    Timer tx(s);
+    SolveOut r;
+    int64_t n = 0, npert = 0;
+    if (X == 'H') {
+      // eps laid out [secondary..., inactive...] for the kernel
+      DevBuf<double> ep(ns + ni);
+      DC2_HIP(hipMemcpyAsync(ep.get(), c->eps.get() + ni + na, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+      DC2_HIP(hipMemcpyAsync(ep.get() + ns, c->eps.get(), ni * sizeof(double), hipMemcpyDeviceToDevice, s));
+      const int64_t np = e2_partials_needed(nsp * nip);
+      c->partial.ensure(2 * np);
+      c->out2.ensure(2);
+      e2_H(s, ni, ns, c->tblk["H"].get(), ep.get(), eshift, c->ipairs.get(), c->spairs.get(), c->partial.get(), np,
+           c->out2.get());
+      double o[2];
+      DC2_HIP(hipMemcpyAsync(o, c->out2.get(), sizeof(o), hipMemcpyDeviceToHost, s));
+      DC2_HIP(hipStreamSynchronize(s));
+      r.e2 = o[0];
+      r.sumc2 = o[1];
+      npert = nsp * nip;
+    } else {
+      switch (X) {
+        case 'A': n = na3; npert = ni; break;
+        case 'B': n = ntp; npert = nip; break;
+        case 'C': n = na3; npert = ns; break;
+        case 'D': n = na2; npert = (int64_t)ns * ni; break;
+        case 'E': n = na; npert = nip * ns; break;
+        case 'F': n = ntp; npert = nsp; break;
+        case 'G': n = na; npert = nsp * ni; break;
+      }
+      DevBuf<cplx> S(n * n), B(n * n), V(std::max<int64_t>(n * npert, 1));
+      DevBuf<double> alpha(std::max<int64_t>(npert, 1));
+      const int* pairs = (X == 'B' || X == 'F') ? c->tpairs.get() : nullptr;
+      if (n > 0) build_SB(s, X, d, eps_act, c->e0.get(), n, pairs, S.get(), B.get());
+      if (n > 0 && npert > 0) {
+        switch (X) {
+          case 'A': {
+            DevBuf<cplx> Dm(na3 * na3), Am(na3 * ni), eff(na * ni);
+            const cplx* A1 = c->tblk["A1"].get();
+            effh_A(s, ni, na, c->hp.get(), nmo, c->tblk["A2"].get(), eff.get());
+            d3_perm_A(s, na, c->G3.get(), Dm.get());
+            a1_perm(s, na, ni, A1, Am.get());
+            zgemm(s, 'N', 'N', na3, ni, na3, mk(-1.0, 0.0), Dm.get(), na3, 0, Am.get(), na3, 0, zero, V.get(), na3, 0, 1);
+            V_A_rest(s, na, ni, d, A1, eff.get(), V.get());
+          } break;
+          case 'B': V_B(s, na, ni, d, c->tblk["B"].get(), c->tpairs.get(), c->ipairs.get(), V.get()); break;
+          case 'C': {
+            DevBuf<cplx> Dm(na3 * na3), eff((int64_t)ns * na);
+            const cplx* C1 = c->tblk["C1"].get();
+            if (ni) effh_C(s, ni, na, ns, c->hp.get(), nmo, C1, c->tblk["C2"].get(), c->tblk["C3"].get(), eff.get());
+            else effh_C(s, 0, na, ns, c->hp.get(), nmo, C1, C1, C1, eff.get());
+            d3_perm_C(s, na, c->G3.get(), Dm.get());
+            zgemm(s, 'N', 'T', na3, ns, na3, one, Dm.get(), na3, 0, C1, ns, 0, zero, V.get(), na3, 0, 1);
+            V_C_rest(s, na, ns, d, eff.get(), V.get());
+          } break;
+          case 'D': {
+            DevBuf<cplx> eff((int64_t)ns * ni);
+            effh_D(s, ni, na, ns, c->hp.get(), nmo, c->tblk["D3"].get(), eff.get());
+            V_D(s, na, ni, ns, d, c->tblk["D1"].get(), c->tblk["D2"].get(), eff.get(), V.get());
+          } break;
+          case 'E': V_E(s, na, ni, ns, d, c->tblk["E"].get(), c->ipairs.get(), V.get()); break;
+          case 'F': V_F(s, na, ns, d, c->tblk["F"].get(), c->tpairs.get(), c->spairs.get(), V.get()); break;
+          case 'G': V_G(s, na, ni, ns, d, c->tblk["G"].get(), c->spairs.get(), V.get()); break;
+        }
+        alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert,
+               alpha.get());
+      }
+      r = solve_block(c, n, npert, S.get(), B.get(), V.get(), alpha.get());
+    }
+    e2[k] = r.e2;
+    sumc2[k] = r.sumc2;
+    c->sub_info[4 * k + 0] = n;
+    c->sub_info[4 * k + 1] = r.dimm;
+    c->sub_info[4 * k + 2] = npert;
+    c->sub_info[4 * k + 3] = 1;
+    c->sub_margin[k] = r.margin;
+    c->timings[4 + k] = tx.stop();
+  }
+  double e0h = 0.0;
+  DC2_HIP(hipMemcpyAsync(&e0h, c->e0.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
+  double e2all = 0.0, s2 = 0.0;
+  for (int k = 0; k < 8; ++k) { e2all += e2[k]; s2 += sumc2[k]; }
+  scal[0] = e0h;
+  scal[1] = e2all;
+  scal[2] = s2;
+  scal[3] = 1.0 / (1.0 + s2);
+  scal[4] = e2all - eshift * s2;
+  scal[5] = e2all + (c->ecas[selectroot - 1] + c->ecore) - eshift * s2;
+  c->have_pt2 = true;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* dc2_last_error(void) { return g_last_error.c_str(); }
+const char* dc2_version(void) { return "dcaspt2_amd 0.1 (gfx950)"; }
+
+int dc2_context_create(int32_t device, dc2_context** out) {
+  return guarded([&] {
+    DC2_CHECK(out != nullptr, 1, "null output pointer");
+    auto c = std::make_unique<dc2_context>();
+    c->device = device;
+    DC2_HIP(hipSetDevice(device));
+    DC2_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    DC2_ROCSOLVER(rocblas_create_handle(&c->blas));
+    DC2_ROCSOLVER(rocblas_set_stream(c->blas, c->stream));
+    *out = c.release();
+  });
+}
+
+int dc2_context_destroy(dc2_context* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+    hipStream_t s = ctx->stream;
+    delete ctx;
+    if (s) (void)hipStreamDestroy(s);
+  });
+}
+
+int dc2_set_spaces(dc2_context* c, int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    DC2_CHECK(ninact >= 0 && nact >= 1 && nsec >= 0 && nelec >= 1 && nelec <= nact, 1, "invalid space sizes");
+    DC2_CHECK(nact <= 24, 1, "nact > 24 is not supported");
+    c->ni = ninact; c->na = nact; c->ns = nsec; c->ne = nelec;
+    c->blocks.clear();
+    c->tblk.clear();
+    c->have_casci = c->have_pt2 = false;
+    c->nsym = 1; c->totsym = 1; c->irpamo.clear(); c->multb_d.clear(); c->multb_ds.clear();
+    for (int& r : c->ras) r = 0;
+  });
+}
+
+int dc2_set_symmetry(dc2_context* c, int32_t nsymrpa, const int64_t* multb, const int64_t* irpamo, int32_t totsym) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    need_spaces(c);
+    const int n = nsymrpa;
+    DC2_CHECK(n >= 1 && n <= 64, 1, "nsymrpa out of range");
+    c->nsym = n; c->totsym = totsym;
+    c->irpamo.assign(irpamo, irpamo + c->nmo());
+    // MULTB2(i0, MULTB(i0,j0)) = j0; MULTB_D = MULTB2(1:n,1:n) - n; MULTB_DS(i,j) = MULTB(n+j, i)
+    std::vector<int> m2(4 * n * n, 0);
+    for (int i0 = 0; i0 < 2 * n; ++i0)
+      for (int j0 = 0; j0 < 2 * n; ++j0) {
+        const int k0 = (int)multb[i0 + 2 * n * j0];
+        DC2_CHECK(k0 >= 1 && k0 <= 2 * n, 1, "MULTB entry out of range");
+        m2[i0 + 2 * n * (k0 - 1)] = j0 + 1;
+      }
+    c->multb_d.assign(n * n, 0);
+    c->multb_ds.assign(n * n, 0);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        c->multb_d[i + n * j] = m2[i + 2 * n * j] - n;
+        c->multb_ds[i + n * j] = (int)multb[(n + j) + 2 * n * i];
+      }
+    c->have_casci = c->have_pt2 = false;
+  });
+}
+
+int dc2_set_ras(dc2_context* c, int32_t r1, int32_t r2, int32_t r3, int32_t h1, int32_t e3, int32_t mh1,
+                int32_t me3) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    need_spaces(c);
+    DC2_CHECK(r1 + r2 + r3 == 0 || r1 + r2 + r3 == c->na, 1, "RAS spaces must add up to nact");
+    int v[7] = {r1, r2, r3, h1, e3, mh1, me3};
+    std::memcpy(c->ras, v, sizeof(v));
+    c->have_casci = c->have_pt2 = false;
+  });
+}
+
+int dc2_set_one_electron(dc2_context* c, const double* h, double ecore) {
+  return guarded([&] {
+    DC2_CHECK(c && h, 1, "null argument");
+    need_spaces(c);
+    const int64_t n = (int64_t)c->nmo() * c->nmo();
+    c->h.alloc(n);
+    DC2_HIP(hipMemcpyAsync(c->h.get(), h, n * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+    DC2_HIP(hipStreamSynchronize(c->stream));
+    c->ecore = ecore;
+  });
+}
+
+int dc2_set_integral_block(dc2_context* c, const char* name, const double* data, int64_t count, int32_t where) {
+  return guarded([&] {
+    DC2_CHECK(c && name && data, 1, "null argument");
+    need_spaces(c);
+    const std::string nm(name);
+    const int64_t need = block_count(c, shape_of(nm));
+    DC2_CHECK(count == need, 1, "integral block " + nm + ": expected " + std::to_string(need) + " elements");
+    Block& b = c->blocks[nm];
+    b.derived = false;
+    b.count = count;
+    if (where == 2) {
+      b.own.release();
+      b.ptr = reinterpret_cast<const cplx*>(data);
+    } else {
+      b.own.alloc(count);
+      DC2_HIP(hipMemcpyAsync(b.own.get(), data, count * sizeof(cplx),
+                             where == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, c->stream));
+      DC2_HIP(hipStreamSynchronize(c->stream));
+      b.ptr = b.own.get();
+    }
+    c->have_pt2 = false;
+    if (nm == "OOOO") {
+      // copies previously derived from OOOO are stale now
+      for (const char* k : {"A1", "A2", "B"}) {
+        auto it = c->blocks.find(k);
+        if (it != c->blocks.end() && it->second.derived) c->blocks.erase(it);
+      }
+      c->have_casci = false;
+    }
+  });
+}
+
+int dc2_casci(dc2_context* c, int32_t nroot, double* eigen, int64_t* ndet_out) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    DC2_HIP(hipSetDevice(c->device));
+    run_casci(c);
+    if (ndet_out) *ndet_out = c->ndet;
+    for (int k = 0; k < nroot && k < c->ndet; ++k) eigen[k] = c->ecas[k] + c->ecore;
+  });
+}
+
+int dc2_get_dets(dc2_context* c, int64_t* dets) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_casci, 1, "no CASCI result");
+    DC2_HIP(hipMemcpy(dets, c->cas_dets.get(), c->ndet * sizeof(int64_t), hipMemcpyDeviceToHost));
+  });
+}
+
+int dc2_get_ci_vector(dc2_context* c, int32_t root, double* v) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_casci, 1, "no CASCI result");
+    DC2_CHECK(root >= 1 && root <= c->ndet, 1, "root out of range");
+    DC2_HIP(hipMemcpy(v, c->civec.get() + (int64_t)(root - 1) * c->ndet, c->ndet * sizeof(cplx),
+                      hipMemcpyDeviceToHost));
+  });
+}
+
+int dc2_caspt2(dc2_context* c, int32_t selectroot, double eshift, double* e2, double* sumc2, double* scalars) {
+  return guarded([&] {
+    DC2_CHECK(c && e2 && sumc2 && scalars, 1, "null argument");
+    DC2_HIP(hipSetDevice(c->device));
+    run_caspt2(c, selectroot, eshift, e2, sumc2, scalars);
+  });
+}
+
+int dc2_get_subspace_info(dc2_context* c, int64_t* info, double* margin) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_pt2, 1, "no CASPT2 result");
+    std::memcpy(info, c->sub_info, sizeof(c->sub_info));
+    std::memcpy(margin, c->sub_margin, sizeof(c->sub_margin));
+  });
+}
+
+int dc2_get_fock_rotation(dc2_context* c, double* eps, double* U) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_pt2, 1, "no CASPT2 result");
+    const int64_t n = c->nmo();
+    if (eps) DC2_HIP(hipMemcpy(eps, c->eps.get(), n * sizeof(double), hipMemcpyDeviceToHost));
+    if (U) DC2_HIP(hipMemcpy(U, c->U.get(), n * n * sizeof(cplx), hipMemcpyDeviceToHost));
+  });
+}
+
+int dc2_get_rotated_ci(dc2_context* c, double* v) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_pt2, 1, "no CASPT2 result");
+    DC2_HIP(hipMemcpy(v, c->cnew.get(), c->ndet * sizeof(cplx), hipMemcpyDeviceToHost));
+  });
+}
+
+int dc2_get_density(dc2_context* c, int32_t which, double* out, int64_t count) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_pt2, 1, "no CASPT2 result");
+    const int64_t na2 = (int64_t)c->na * c->na;
+    const cplx* src = nullptr;
+    int64_t n = 0;
+    switch (which) {
+      case 0: src = c->G1.get(); n = na2; break;
+      case 1: src = c->G1F.get(); n = na2; break;
+      case 2: src = c->G2.get(); n = na2 * na2; break;
+      case 3: src = c->G2F.get(); n = na2 * na2; break;
+      default: throw Error(1, "density index out of range");
+    }
+    DC2_CHECK(count == n, 1, "density: wrong element count");
+    DC2_HIP(hipMemcpy(out, src, n * sizeof(cplx), hipMemcpyDeviceToHost));
+  });
+}
+
+int dc2_get_transformed_block(dc2_context* c, const char* name, double* out, int64_t count) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_pt2 && name, 1, "no CASPT2 result");
+    auto it = c->tblk.find(name);
+    DC2_CHECK(it != c->tblk.end(), 1, std::string("block ") + name + " was not transformed");
+    const int64_t n = block_count(c, shape_of(name));
+    DC2_CHECK(count == n, 1, "transformed block: wrong element count");
+    DC2_HIP(hipMemcpy(out, it->second.get(), n * sizeof(cplx), hipMemcpyDeviceToHost));
+  });
+}
+
+int dc2_get_timings(dc2_context* c, double* t, int32_t n) {
+  return guarded([&] {
+    DC2_CHECK(c && t, 1, "null argument");
+    for (int k = 0; k < n && k < 12; ++k) t[k] = c->timings[k];
+  });
+}
+
+int dc2_casmat(dc2_context* c, double* mat) {
+  return guarded([&] {
+    DC2_CHECK(c && c->have_casci && mat, 1, "no CASCI result");
+    DevBuf<cplx> m(c->ndet * c->ndet);
+    casmat(c->stream, c->ni, c->na, c->ne, c->ndet, c->cas_dets.get(), c->rank_cas.get(), get_block(c, "OOOO"),
+           c->nmoc(), c->h.get(), c->nmo(), m.get());
+    DC2_HIP(hipMemcpyAsync(mat, m.get(), c->ndet * c->ndet * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
+    DC2_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int dc2_cdiag(dc2_context* c, double* a, int32_t n, int32_t* dimm, double* w, double thr) {
+  return guarded([&] {
+    DC2_CHECK(c && a && w && dimm && n > 0, 1, "invalid argument");
+    DevBuf<cplx> m((int64_t)n * n);
+    DevBuf<double> wd(n);
+    DC2_HIP(hipMemcpyAsync(m.get(), a, (int64_t)n * n * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+    zheevd(c, n, m.get(), wd.get());
+    DC2_HIP(hipMemcpyAsync(a, m.get(), (int64_t)n * n * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
+    DC2_HIP(hipMemcpyAsync(w, wd.get(), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DC2_HIP(hipStreamSynchronize(c->stream));
+    if (thr == 0.0) *dimm = n;
+    else { int k = 0; for (int i = 0; i < n; ++i) k += (w[i] >= thr); *dimm = k; }
+  });
+}
+
+int dc2_zgemm(dc2_context* c, char opA, char opB, int64_t M, int64_t N, int64_t K, const double* alpha,
+              const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, const double* beta,
+              double* C, int64_t ldc, int64_t sC, int64_t batch) {
+  return guarded([&] {
+    DC2_CHECK(c && alpha && beta, 1, "null argument");
+    zgemm(c->stream, opA, opB, M, N, K, mk(alpha[0], alpha[1]), reinterpret_cast<const cplx*>(A), lda, sA,
+          reinterpret_cast<const cplx*>(B), ldb, sB, mk(beta[0], beta[1]), reinterpret_cast<cplx*>(C), ldc, sC,
+          batch);
+  });
+}
+
+int dc2_synchronize(dc2_context* c) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    DC2_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,564 @@
+// CASCI-side kernels: determinant strings, E_pq tables and gathers, CI matrix,
+// Fock matrix, eigenvector gauge, CI-vector rotation determinants.
+//
+// Strings are int64 bit masks over the active spinors (bit w-1 <-> active
+// spinor w), exactly the integers search_cas_configuration.F90 stores in
+// dict_cas_idx; all string work is integer arithmetic and bit-exact.
+#include "dc2_kernels.hpp"
+
+namespace dc2 {
+namespace {
+
+__device__ inline int64_t one_e_exct(int64_t det, int cre, int ann, int nact, int& phase) {
+  // one_e_exct.F90:34-47, 1-based orbital labels
+  const uint64_t ba = 1ull << (ann - 1), bc = 1ull << (cre - 1);
+  const uint64_t d = (uint64_t)det;
+  if (ann == cre && (d & ba)) { phase = 0; return det; }
+  if ((d & ba) && !(d & bc)) {
+    const uint64_t full = (nact >= 64) ? ~0ull : ((1ull << nact) - 1ull);
+    const uint64_t ia = ann < nact ? (full & ~((1ull << ann) - 1ull)) : 0ull;
+    const uint64_t ib = cre < nact ? (full & ~((1ull << cre) - 1ull)) : 0ull;
+    phase = __popcll(d & ia) + __popcll((d - ba) & ib);
+    return (int64_t)(d - ba + bc);
+  }
+  phase = 0;
+  return 0;
+}
+
+__device__ inline bool ras_ok(uint64_t det, const DetParams& p) {
+  // ras_det_check.F90:8-39
+  if (p.ras1 != 0) {
+    const int n = __popcll(det & ((1ull << p.ras1) - 1ull));
+    if (!((p.ras1 - p.ras1_max_hole) <= n && n <= (p.ras1 - p.min_hole_ras1))) return false;
+  }
+  if (p.ras3 != 0) {
+    const int sh = p.ras1 + p.ras2;
+    const uint64_t r3 = (det >> sh) << sh;
+    const int n = __popcll(det & r3);
+    if (!(n <= p.ras3_max_elec && p.min_elec_ras3 <= n)) return false;
+  }
+  return true;
+}
+
+__device__ inline int det_totsym(uint64_t det, const DetParams& p) {
+  // cas_configuration_totsym, search_cas_configuration.F90:112-149
+  int ret = 1, ielec = 0;
+  for (int i = 1; i <= p.nact; ++i) {
+    if ((det >> (i - 1)) & 1ull) {
+      ++ielec;
+      const int jsym = p.irp_act[i - 1];
+      if (ielec & 1) {
+        ret = p.multb_ds[(jsym - 1) + p.nsym * (ret - 1)];
+      } else {
+        const int q = (jsym & 1) ? jsym + 1 : jsym - 1;
+        ret = p.multb_d[(q - 1) + p.nsym * (ret - 1)];
+      }
+    }
+  }
+  if ((ielec & 1) == 0) ret += p.nsym;
+  return ret;
+}
+
+__global__ void k_det_flags(DetParams p, int64_t N, int* ff, int* fc) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const bool full = __popcll((uint64_t)x) == p.nelec;
+  bool cas = full && ras_ok((uint64_t)x, p);
+  if (cas && p.nsym != 1) cas = det_totsym((uint64_t)x, p) == p.totsym;
+  ff[x] = full ? 1 : 0;
+  fc[x] = cas ? 1 : 0;
+}
+
+// exclusive scans of both flag arrays in one workgroup (N <= 2^24)
+__global__ __launch_bounds__(1024) void k_scan2(int64_t N, const int* ff, const int* fc, int* rf, int* rc,
+                                                int* counts) {
+  __shared__ int sf[1024], sc[1024];
+  const int t = threadIdx.x;
+  const int64_t chunk = (N + 1023) / 1024;
+  const int64_t lo = t * chunk, hi = (lo + chunk < N) ? lo + chunk : N;
+  int a = 0, b = 0;
+  for (int64_t x = lo; x < hi; ++x) { a += ff[x]; b += fc[x]; }
+  sf[t] = a;
+  sc[t] = b;
+  __syncthreads();
+  if (t == 0) {
+    int ra = 0, rb = 0;
+    for (int k = 0; k < 1024; ++k) {
+      const int va = sf[k], vb = sc[k];
+      sf[k] = ra; sc[k] = rb;
+      ra += va; rb += vb;
+    }
+    counts[0] = ra;
+    counts[1] = rb;
+  }
+  __syncthreads();
+  int ra = sf[t], rb = sc[t];
+  for (int64_t x = lo; x < hi; ++x) {
+    rf[x] = ff[x] ? ra++ : -1;
+    rc[x] = fc[x] ? rb++ : -1;
+  }
+}
+
+__global__ void k_det_scatter(int64_t N, const int* rf, const int* rc, int64_t* full_dets, int64_t* cas_dets,
+                              int* cas2full) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  if (rf[x] >= 0) full_dets[rf[x]] = x;
+  if (rc[x] >= 0) { cas_dets[rc[x]] = x; cas2full[rc[x]] = rf[x]; }
+}
+
+__global__ void k_exc_table(int na, int64_t nf, const int64_t* dets, const int* rank_full, int* tab) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nf) return;
+  const int pq = blockIdx.y, p = pq % na, q = pq / na;
+  int ph;
+  const int64_t nw = one_e_exct(dets[j], p + 1, q + 1, na, ph);
+  tab[(int64_t)pq * nf + j] = nw ? ((rank_full[nw] << 1) | (ph & 1)) : -1;
+}
+
+__global__ void k_apply_E(int na, int64_t nf, const int* tab, const cplx* __restrict__ X, cplx* __restrict__ Y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nf) return;
+  const int pq = blockIdx.y, p = pq % na, q = pq / na;
+  const int64_t x = blockIdx.z;
+  const int na2 = na * na;
+  // <j|E_pq|i> = <i|E_qp|j>: gather from the E_qp image of j
+  const int e = tab[(int64_t)(q + na * p) * nf + j];
+  cplx v = mk(0.0, 0.0);
+  if (e >= 0) {
+    v = X[(int64_t)(e >> 1) + nf * x];
+    if (e & 1) v = mk(-v.x, -v.y);
+  }
+  Y[j + nf * ((int64_t)pq + na2 * x)] = v;
+}
+
+__global__ void k_embed(int na, int64_t ncas, const int* cas2full, const cplx* c_cas, cplx* c_full) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ncas) c_full[cas2full[k]] = c_cas[k];
+}
+
+__global__ void k_fvec(int na, int64_t nf, const int64_t* dets, const double* eps_act, const cplx* c_full,
+                       cplx* cF) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nf) return;
+  double f = 0.0;
+  const uint64_t d = (uint64_t)dets[j];
+  for (int w = 0; w < na; ++w)
+    if ((d >> w) & 1ull) f += eps_act[w];
+  cF[j] = cscale(c_full[j], f);
+}
+
+__device__ inline cplx wave_sum(cplx v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_down(v.x, o, 64);
+    v.y += __shfl_down(v.y, o, 64);
+  }
+  return v;
+}
+
+// One wave per CI row (casmat.F90:37-216)
+__global__ __launch_bounds__(64) void k_casmat(int ni, int na, int ne, int64_t ndet, const int64_t* dets,
+                                               const int* rank_cas, const cplx* __restrict__ O, int nmoc,
+                                               const cplx* __restrict__ h, int ldh, cplx* mat) {
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  __shared__ int oc[32], vi[32], L[96];
+  __shared__ short op0[512], op1[512], vp0[512], vp1[512];
+  __shared__ int nop_s, nvp_s;
+  const int64_t det = dets[i];
+  const int nv = na - ne;
+  if (lane == 0) {
+    int no = 0, nvv = 0;
+    for (int imo = 1; imo <= na; ++imo) {
+      if ((det >> (imo - 1)) & 1) oc[no++] = imo;
+      else vi[nvv++] = imo;
+    }
+    for (int k = 0; k < ni; ++k) L[k] = k;
+    for (int k = 0; k < ne; ++k) L[ni + k] = ni + oc[k] - 1;
+    int c = 0;
+    for (int a = 0; a < ne; ++a)
+      for (int b = a + 1; b < ne; ++b) { op0[c] = (short)a; op1[c] = (short)b; ++c; }
+    nop_s = c;
+    c = 0;
+    for (int a = 0; a < nv; ++a)
+      for (int b = a + 1; b < nv; ++b) { vp0[c] = (short)a; vp1[c] = (short)b; ++c; }
+    nvp_s = c;
+  }
+  __syncthreads();
+  const int64_t m = nmoc;
+  auto Oi = [&](int a, int b, int c, int d) -> cplx { return O[a + m * (b + m * (c + m * (int64_t)d))]; };
+  // diagonal (casmat.F90:57-104)
+  const int nl = ni + ne;
+  cplx part = mk(0.0, 0.0);
+  for (int a0 = 0; a0 < nl; ++a0) {
+    const int r = L[a0];
+    for (int b0 = a0 + 1 + lane; b0 < nl; b0 += 64) {
+      const int s = L[b0];
+      const cplx x = Oi(r, r, s, s), y = Oi(r, s, s, r);
+      part.x += 0.5 * x.x - 0.5 * y.x;
+      part.y += 0.5 * x.y - 0.5 * y.y;
+    }
+  }
+  part = wave_sum(part);
+  if (lane == 0) {
+    cplx d = mk(0.0, 0.0);
+    for (int k = 0; k < ni; ++k) d = cadd(d, h[k + (int64_t)ldh * k]);
+    for (int k = 0; k < ne; ++k) {
+      const int g = ni + oc[k] - 1;
+      d = cadd(d, h[g + (int64_t)ldh * g]);
+    }
+    mat[i + ndet * i] = cadd(d, cadd(part, cconj(part)));
+  }
+  // single excitations (casmat.F90:108-169)
+  for (int idx = lane; idx < ne * nv; idx += 64) {
+    const int r = oc[idx / nv], a = vi[idx % nv];
+    int ph;
+    const int64_t nw = one_e_exct(det, a, r, na, ph);
+    const int64_t j = rank_cas[nw];
+    if (j <= i) continue;
+    const int ir = ni + r - 1, ia = ni + a - 1;
+    cplx v = h[ir + (int64_t)ldh * ia];
+    for (int k = 0; k < nl; ++k) {
+      const int s = L[k];
+      v = cadd(v, csub(Oi(ir, ia, s, s), Oi(ir, s, s, ia)));
+    }
+    if (ph & 1) v = mk(-v.x, -v.y);
+    mat[i + ndet * j] = v;
+    mat[j + ndet * i] = cconj(v);
+  }
+  // double excitations (casmat.F90:171-213)
+  const int nop = nop_s, nvp = nvp_s;
+  for (int64_t idx = lane; idx < (int64_t)nop * nvp; idx += 64) {
+    const int po = (int)(idx / nvp), pv = (int)(idx % nvp);
+    const int r = oc[op0[po]], s = oc[op1[po]], a = vi[vp0[pv]], b = vi[vp1[pv]];
+    int ph1, ph2;
+    const int64_t n1 = one_e_exct(det, a, r, na, ph1);
+    const int64_t n2 = one_e_exct(n1, b, s, na, ph2);
+    const int64_t j = rank_cas[n2];
+    if (j <= i) continue;
+    const int ir = ni + r - 1, is = ni + s - 1, ia = ni + a - 1, ib = ni + b - 1;
+    cplx v = csub(Oi(ir, ia, is, ib), Oi(ir, ib, is, ia));
+    if ((ph1 + ph2) & 1) v = mk(-v.x, -v.y);
+    mat[i + ndet * j] = v;
+    mat[j + ndet * i] = cconj(v);
+  }
+}
+
+// fockcasci.F90:34-83 ; thread per element of the nmo x nmo matrix
+__global__ void k_fock(int ni, int na, int ns, const cplx* __restrict__ h, int ldh, const cplx* __restrict__ O,
+                       const cplx* __restrict__ F1, const cplx* __restrict__ F2, const cplx* __restrict__ D1,
+                       cplx* fock) {
+  const int nmoc = ni + na, nmo = nmoc + ns;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nmo || i > j) return;
+  const int si = i < ni ? 0 : (i < nmoc ? 1 : 2), sj = j < ni ? 0 : (j < nmoc ? 1 : 2);
+  if (si != sj) return;
+  cplx f = h[i + (int64_t)ldh * j];
+  if (si < 2) {
+    const int64_t m = nmoc;
+    auto Oi = [&](int a, int b, int c, int d) -> cplx { return O[a + m * (b + m * (c + m * (int64_t)d))]; };
+    for (int k = 0; k < ni; ++k) f = cadd(f, csub(Oi(i, j, k, k), Oi(i, k, k, j)));
+    for (int l = 0; l < na; ++l)
+      for (int k = 0; k < na; ++k) {
+        const cplx d = D1[k + na * l];
+        cfma(f, d, csub(Oi(i, j, ni + k, ni + l), Oi(i, ni + l, ni + k, j)));
+      }
+  } else {
+    const int64_t a = i - nmoc, b = j - nmoc, S = ns, m = nmoc;
+    auto f1 = [&](int64_t x, int64_t y, int64_t k, int64_t l) { return F1[x + S * (y + S * (k + m * l))]; };
+    auto f2 = [&](int64_t x, int64_t k, int64_t l, int64_t y) { return F2[x + S * (k + m * (l + m * y))]; };
+    for (int k = 0; k < ni; ++k) f = cadd(f, csub(f1(a, b, k, k), f2(a, k, k, b)));
+    for (int l = 0; l < na; ++l)
+      for (int k = 0; k < na; ++k) {
+        const cplx d = D1[k + na * l];
+        cfma(f, d, csub(f1(a, b, ni + k, ni + l), f2(a, ni + l, ni + k, b)));
+      }
+  }
+  fock[i + (int64_t)nmo * j] = f;
+  if (i != j) fock[j + (int64_t)nmo * i] = cconj(f);
+}
+
+__global__ void k_gather_block(int n, const int* ind, const cplx* fock, int ldf, cplx* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  out[i + (int64_t)n * j] = fock[ind[i] + (int64_t)ldf * ind[j]];
+}
+
+__global__ void k_scatter_block(int n, const int* ind, const cplx* V, const double* w, cplx* U, int ldu,
+                                double* eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  U[ind[i] + (int64_t)ldu * ind[j]] = V[i + (int64_t)n * j];
+  if (i == 0) eps[ind[j]] = w[j];
+}
+
+// Canonical gauge of Hermitian eigenvectors; one workgroup (see the oracle's canonical_gauge)
+__global__ __launch_bounds__(256) void k_gauge(int n, const double* w, cplx* V, cplx* Ub) {
+  __shared__ double dsh[4096];
+  __shared__ double rmax[256];
+  __shared__ int rmin[256];
+  const int t = threadIdx.x;
+  int i0 = 0;
+  while (i0 < n) {
+    int i1 = i0 + 1;
+    while (i1 < n && fabs(w[i1] - w[i0]) <= 1.0e-8 * fmax(1.0, fabs(w[i0]))) ++i1;
+    const int m = i1 - i0;
+    for (int k = 0; k < m; ++k) {
+      double lmax = -1.0;
+      for (int r = t; r < n; r += 256) {
+        double d = 0.0;
+        for (int c = i0; c < i1; ++c) d += cabs2(V[r + (int64_t)n * c]);
+        for (int l = 0; l < k; ++l) d -= cabs2(Ub[r + (int64_t)n * l]);
+        dsh[r] = d;
+        lmax = fmax(lmax, d);
+      }
+      rmax[t] = lmax;
+      __syncthreads();
+      for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) rmax[t] = fmax(rmax[t], rmax[t + o]);
+        __syncthreads();
+      }
+      const double dmax = rmax[0];
+      int lmin = 1 << 30;
+      for (int r = t; r < n; r += 256)
+        if (dsh[r] >= dmax - 1.0e-10 && r < lmin) lmin = r;
+      rmin[t] = lmin;
+      __syncthreads();
+      for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) rmin[t] = min(rmin[t], rmin[t + o]);
+        __syncthreads();
+      }
+      const int rk = rmin[0] < n ? rmin[0] : 0;   // NaN input: no row qualifies
+      const double inv = 1.0 / sqrt(dsh[rk]);
+      for (int r = t; r < n; r += 256) {
+        cplx u = mk(0.0, 0.0);
+        for (int c = i0; c < i1; ++c) cfma(u, V[r + (int64_t)n * c], cconj(V[rk + (int64_t)n * c]));
+        for (int l = 0; l < k; ++l) {
+          const cplx a = Ub[r + (int64_t)n * l], b = cconj(Ub[rk + (int64_t)n * l]);
+          u = csub(u, cmul(a, b));
+        }
+        Ub[r + (int64_t)n * k] = cscale(u, inv);
+      }
+      __syncthreads();
+    }
+    for (int k = 0; k < m; ++k)
+      for (int r = t; r < n; r += 256) V[r + (int64_t)n * (i0 + k)] = Ub[r + (int64_t)n * k];
+    __syncthreads();
+    i0 = i1;
+  }
+}
+
+template <int NE>
+__global__ void k_tracic_ds(int na, int ne, int64_t ndet, const int64_t* dets, const cplx* __restrict__ Ua,
+                            cplx* ds) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t kt = blockIdx.y;
+  if (k >= ndet) return;
+  int ok[NE], on[NE];
+  {
+    const uint64_t a = (uint64_t)dets[k], b = (uint64_t)dets[kt];
+    int x = 0, y = 0;
+    for (int w = 0; w < na; ++w) {
+      if ((a >> w) & 1ull) ok[x++] = w;
+      if ((b >> w) & 1ull) on[y++] = w;
+    }
+  }
+  cplx A[NE][NE];
+  for (int r = 0; r < ne; ++r)
+    for (int c = 0; c < ne; ++c) A[r][c] = Ua[ok[r] + (int64_t)na * on[c]];
+  // LU with partial pivoting, pivot = first max of |re|+|im| (izamax)
+  int swaps = 0;
+  for (int c = 0; c < ne; ++c) {
+    int piv = c;
+    double best = fabs(A[c][c].x) + fabs(A[c][c].y);
+    for (int r = c + 1; r < ne; ++r) {
+      const double v = fabs(A[r][c].x) + fabs(A[r][c].y);
+      if (v > best) { best = v; piv = r; }
+    }
+    if (piv != c) {
+      ++swaps;
+      for (int q = 0; q < ne; ++q) { const cplx tmp = A[c][q]; A[c][q] = A[piv][q]; A[piv][q] = tmp; }
+    }
+    const cplx d = A[c][c];
+    const double den = cabs2(d);
+    if (den == 0.0) continue;
+    const cplx dinv = mk(d.x / den, -d.y / den);
+    for (int r = c + 1; r < ne; ++r) {
+      const cplx l = cmul(A[r][c], dinv);
+      for (int q = c + 1; q < ne; ++q) A[r][q] = csub(A[r][q], cmul(l, A[c][q]));
+    }
+  }
+  cplx det = mk(1.0, 0.0);
+  for (int r = 0; r < ne; ++r) det = cmul(A[r][r], det);
+  if (swaps & 1) det = mk(-det.x, -det.y);
+  ds[k + ndet * kt] = det;
+}
+
+__global__ void k_calce0(int na, const double* eps_act, const cplx* D1, double* e0) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double e = 0.0;
+    for (int w = 0; w < na; ++w) e += D1[w + na * w].x * eps_act[w];
+    *e0 = e;
+  }
+}
+
+struct Perm6 {
+  int dims[6];
+  int64_t src[6];
+};
+__global__ void k_permute6(Perm6 p, const cplx* __restrict__ in, cplx* __restrict__ out, int64_t total) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= total) return;
+  int64_t r = o, s = 0;
+  for (int a = 0; a < 6; ++a) {
+    const int64_t q = r % p.dims[a];
+    r /= p.dims[a];
+    s += q * p.src[a];
+  }
+  out[o] = in[s];
+}
+
+struct Ext4 {
+  int64_t ld[3];
+  int off[4], n[4];
+};
+__global__ void k_extract4(Ext4 e, const cplx* __restrict__ in, cplx* __restrict__ out, int64_t total) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= total) return;
+  int64_t r = o;
+  const int i = (int)(r % e.n[0]); r /= e.n[0];
+  const int j = (int)(r % e.n[1]); r /= e.n[1];
+  const int k = (int)(r % e.n[2]); r /= e.n[2];
+  const int l = (int)r;
+  out[o] = in[(e.off[0] + i) + e.ld[0] * ((e.off[1] + j) + e.ld[1] * ((e.off[2] + k) + e.ld[2] * (int64_t)(e.off[3] + l)))];
+}
+
+__global__ void k_fill_zero(cplx* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = mk(0.0, 0.0);
+}
+
+__global__ void k_scale_cols_tail(int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (r >= n) return;
+  const int64_t src = n - dimm + c;
+  uc[r + n * c] = cscale(vs[r + n * src], pow(ws[src], -0.5));
+}
+
+}  // namespace
+
+void enumerate_dets(hipStream_t s, const DetParams& p, int* ff, int* fc, int* rf, int* rc, int64_t* full_dets,
+                    int64_t* cas_dets, int* cas2full, int64_t* n_full, int64_t* n_cas, int* counts_dev) {
+  DC2_CHECK(p.nact >= 1 && p.nact <= 24, 1, "nact must be in 1..24");
+  DC2_CHECK(p.nelec >= 1 && p.nelec <= p.nact, 1, "nelec must be in 1..nact");
+  const int64_t N = 1ll << p.nact;
+  hipLaunchKernelGGL(k_det_flags, dim3(cdiv(N, 256)), dim3(256), 0, s, p, N, ff, fc);
+  hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, N, ff, fc, rf, rc, counts_dev);
+  int counts[2];
+  DC2_HIP(hipMemcpyAsync(counts, counts_dev, sizeof(counts), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
+  *n_full = counts[0];
+  *n_cas = counts[1];
+  if (full_dets)
+    hipLaunchKernelGGL(k_det_scatter, dim3(cdiv(N, 256)), dim3(256), 0, s, N, rf, rc, full_dets, cas_dets, cas2full);
+  DC2_HIP(hipGetLastError());
+}
+
+void build_exc_table(hipStream_t s, int na, int64_t nf, const int64_t* full_dets, const int* rank_full, int* tab) {
+  hipLaunchKernelGGL(k_exc_table, dim3(cdiv(nf, 256), na * na), dim3(256), 0, s, na, nf, full_dets, rank_full, tab);
+  DC2_HIP(hipGetLastError());
+}
+
+void apply_E(hipStream_t s, int na, int64_t nf, int64_t nx, const int* tab, const cplx* X, cplx* Y) {
+  DC2_CHECK(nx < 65536, 1, "apply_E: too many right-hand sides");
+  hipLaunchKernelGGL(k_apply_E, dim3(cdiv(nf, 256), na * na, (unsigned)nx), dim3(256), 0, s, na, nf, tab, X, Y);
+  DC2_HIP(hipGetLastError());
+}
+
+void embed_ci(hipStream_t s, int na, int64_t nf, int64_t ncas, const int* cas2full, const cplx* c_cas,
+              const int64_t* full_dets, const double* eps_act, cplx* c_full, cplx* cF) {
+  hipLaunchKernelGGL(k_fill_zero, dim3(cdiv(nf, 256)), dim3(256), 0, s, c_full, nf);
+  hipLaunchKernelGGL(k_embed, dim3(cdiv(ncas, 256)), dim3(256), 0, s, na, ncas, cas2full, c_cas, c_full);
+  hipLaunchKernelGGL(k_fvec, dim3(cdiv(nf, 256)), dim3(256), 0, s, na, nf, full_dets, eps_act, c_full, cF);
+  DC2_HIP(hipGetLastError());
+}
+
+void casmat(hipStream_t s, int ni, int na, int ne, int64_t ndet, const int64_t* cas_dets, const int* rank_cas,
+            const cplx* O, int nmoc, const cplx* h, int ldh, cplx* mat) {
+  DC2_CHECK(na <= 32 && ni + ne <= 96, 1, "casmat: nact <= 32 and ninact + nelec <= 96 are supported");
+  hipLaunchKernelGGL(k_fill_zero, dim3(cdiv(ndet * ndet, 256)), dim3(256), 0, s, mat, ndet * ndet);
+  hipLaunchKernelGGL(k_casmat, dim3((unsigned)ndet), dim3(64), 0, s, ni, na, ne, ndet, cas_dets, rank_cas, O, nmoc,
+                     h, ldh, mat);
+  DC2_HIP(hipGetLastError());
+}
+
+void fock_build(hipStream_t s, int ni, int na, int ns, const cplx* h, int ldh, const cplx* O, const cplx* F1,
+                const cplx* F2, const cplx* D1, cplx* fock) {
+  const int nmo = ni + na + ns;
+  hipLaunchKernelGGL(k_fill_zero, dim3(cdiv((int64_t)nmo * nmo, 256)), dim3(256), 0, s, fock, (int64_t)nmo * nmo);
+  hipLaunchKernelGGL(k_fock, dim3(cdiv(nmo, 128), nmo), dim3(128), 0, s, ni, na, ns, h, ldh, O, F1, F2, D1, fock);
+  DC2_HIP(hipGetLastError());
+}
+
+void gather_block(hipStream_t s, int n, const int* ind, const cplx* fock, int ldf, cplx* out) {
+  hipLaunchKernelGGL(k_gather_block, dim3(cdiv(n, 128), n), dim3(128), 0, s, n, ind, fock, ldf, out);
+  DC2_HIP(hipGetLastError());
+}
+
+void scatter_block(hipStream_t s, int n, const int* ind, const cplx* V, const double* w, cplx* U, int ldu,
+                   double* eps) {
+  hipLaunchKernelGGL(k_scatter_block, dim3(cdiv(n, 128), n), dim3(128), 0, s, n, ind, V, w, U, ldu, eps);
+  DC2_HIP(hipGetLastError());
+}
+
+void canonical_gauge(hipStream_t s, int n, const double* w, cplx* V, cplx* scratch) {
+  DC2_CHECK(n <= 4096, 1, "canonical_gauge: block larger than 4096");
+  hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, s, n, w, V, scratch);
+  DC2_HIP(hipGetLastError());
+}
+
+void tracic_ds(hipStream_t s, int na, int ne, int64_t ndet, const int64_t* cas_dets, const cplx* Uact, cplx* ds) {
+  DC2_CHECK(ndet < 65536, 1, "tracic: more than 65535 determinants");
+  dim3 g(cdiv(ndet, 128), (unsigned)ndet);
+  if (ne <= 4) hipLaunchKernelGGL(k_tracic_ds<4>, g, dim3(128), 0, s, na, ne, ndet, cas_dets, Uact, ds);
+  else if (ne <= 8) hipLaunchKernelGGL(k_tracic_ds<8>, g, dim3(128), 0, s, na, ne, ndet, cas_dets, Uact, ds);
+  else if (ne <= 16) hipLaunchKernelGGL(k_tracic_ds<16>, g, dim3(128), 0, s, na, ne, ndet, cas_dets, Uact, ds);
+  else throw Error(1, "tracic: nelec > 16 not supported");
+  DC2_HIP(hipGetLastError());
+}
+
+void calce0(hipStream_t s, int na, const double* eps_act, const cplx* D1, double* e0) {
+  hipLaunchKernelGGL(k_calce0, dim3(1), dim3(64), 0, s, na, eps_act, D1, e0);
+  DC2_HIP(hipGetLastError());
+}
+
+void permute6(hipStream_t s, const int* dims_out, const int* src_stride, const cplx* in, cplx* out, int64_t total) {
+  Perm6 p;
+  for (int a = 0; a < 6; ++a) { p.dims[a] = dims_out[a]; p.src[a] = src_stride[a]; }
+  hipLaunchKernelGGL(k_permute6, dim3(cdiv(total, 256)), dim3(256), 0, s, p, in, out, total);
+  DC2_HIP(hipGetLastError());
+}
+
+void extract4(hipStream_t s, const cplx* in, const int64_t* ld_in, const int* off, const int* n, cplx* out) {
+  Ext4 e;
+  for (int a = 0; a < 3; ++a) e.ld[a] = ld_in[a];
+  for (int a = 0; a < 4; ++a) { e.off[a] = off[a]; e.n[a] = n[a]; }
+  const int64_t total = (int64_t)n[0] * n[1] * n[2] * n[3];
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_extract4, dim3(cdiv(total, 256)), dim3(256), 0, s, e, in, out, total);
+  DC2_HIP(hipGetLastError());
+}
+
+void fill_zero(hipStream_t s, cplx* p, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_zero, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n);
+  DC2_HIP(hipGetLastError());
+}
+
+void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc) {
+  if (dimm <= 0) return;
+  hipLaunchKernelGGL(k_scale_cols_tail, dim3(cdiv(n, 256), (unsigned)dimm), dim3(256), 0, s, n, dimm, vs, ws, uc);
+  DC2_HIP(hipGetLastError());
+}
+
+}  // namespace dc2

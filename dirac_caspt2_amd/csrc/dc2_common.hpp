@@ -1,0 +1,93 @@
+// Shared definitions for the CDNA4 (gfx950) CASPT2 kernels.
+//
+// Complex numbers are double2 {x = re, y = im}, the memory layout of Fortran
+// complex*16 / C double _Complex.  Matrices are column-major (Fortran order)
+// with explicit leading dimensions, as the reference's arrays are.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace dc2 {
+
+typedef double2 cplx;
+
+__host__ __device__ inline cplx mk(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline cplx cadd(cplx a, cplx b) { return mk(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline cplx csub(cplx a, cplx b) { return mk(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline cplx cmul(cplx a, cplx b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline cplx cconj(cplx a) { return mk(a.x, -a.y); }
+__host__ __device__ inline cplx cscale(cplx a, double s) { return mk(a.x * s, a.y * s); }
+__host__ __device__ inline double cabs2(cplx a) { return a.x * a.x + a.y * a.y; }
+// a += b*c
+__host__ __device__ inline void cfma(cplx& a, cplx b, cplx c) {
+  a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
+  a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+}
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define DC2_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::dc2::Error(2, std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                                    __FILE__ + ":" + std::to_string(__LINE__));        \
+  } while (0)
+
+#define DC2_CHECK(cond, code, msg)                       \
+  do {                                                   \
+    if (!(cond)) throw ::dc2::Error((code), (msg));      \
+  } while (0)
+
+// Device allocation owned by the library context.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) DC2_HIP(hipMalloc(&p, count * sizeof(T)));
+  }
+  // keep the allocation if it is large enough
+  void ensure(size_t count) {
+    if (count > n || p == nullptr) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T* get() const { return p; }
+};
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// --- GEMM (dc2_gemm.hip) -----------------------------------------------------
+// C[m,n] = alpha * sum_k op(A)[m,k] op(B)[k,n] + beta * C[m,n], batched by strides.
+// opA: 'N' A[m + lda*k], 'T' A[k + lda*m], 'C' conj(A[k + lda*m])
+// opB: 'N' B[k + ldb*n], 'R' conj(B[k + ldb*n]), 'T' B[n + ldb*k], 'C' conj(B[n + ldb*k])
+void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, cplx alpha,
+           const cplx* A, int64_t lda, int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta,
+           cplx* C, int64_t ldc, int64_t sC, int64_t batch);
+
+}  // namespace dc2

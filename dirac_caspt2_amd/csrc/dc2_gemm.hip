@@ -1,0 +1,161 @@
+// Complex FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// One workgroup = 4 wave64s computing a 64x64 complex tile of C; every wave
+// owns a 32x32 sub-tile = 2x2 MFMA tiles of 16x16.  A complex multiply-add is
+// four real MFMAs on separate real/imaginary planes staged in LDS.
+// The LDS row stride (80 doubles) puts the four k-rows read by one wave in
+// disjoint bank halves.
+//
+// Used for: the four one-index integral transforms (module_intra.F90), the
+// RDM contractions (density_matrix.F90), the V contractions and the
+// (U L^-1/2)^H B (U L^-1/2) products of every solve_X_subspace.
+#include "dc2_common.hpp"
+
+namespace dc2 {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+namespace {
+constexpr int BM = 64, BN = 64, BK = 16, LDS_STRIDE = 80;
+
+template <char OPA>
+__device__ inline cplx loadA(const cplx* __restrict__ A, int64_t lda, int64_t m, int64_t k) {
+  if (OPA == 'N') return A[m + lda * k];
+  cplx v = A[k + lda * m];
+  return OPA == 'C' ? cconj(v) : v;
+}
+
+template <char OPB>
+__device__ inline cplx loadB(const cplx* __restrict__ B, int64_t ldb, int64_t k, int64_t n) {
+  if (OPB == 'N') return B[k + ldb * n];
+  if (OPB == 'R') return cconj(B[k + ldb * n]);
+  cplx v = B[n + ldb * k];
+  return OPB == 'C' ? cconj(v) : v;
+}
+
+template <char OPA, char OPB>
+__global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t K, cplx alpha,
+                                               const cplx* __restrict__ A, int64_t lda, int64_t sA,
+                                               const cplx* __restrict__ B, int64_t ldb, int64_t sB,
+                                               cplx beta, cplx* __restrict__ C, int64_t ldc, int64_t sC) {
+  __shared__ double Ar[BK][LDS_STRIDE], Ai[BK][LDS_STRIDE], Br[BK][LDS_STRIDE], Bi[BK][LDS_STRIDE];
+  const int64_t bz = blockIdx.z;
+  A += bz * sA;
+  B += bz * sB;
+  C += bz * sC;
+  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  d4 accr[2][2], acci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accr[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      acci[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    // stage A (BM x BK) and B (BK x BN) tiles; the index order follows the
+    // contiguous direction of the operand in memory
+#pragma unroll
+    for (int e = tid; e < BM * BK; e += 256) {
+      int mm, kk;
+      if (OPA == 'N') { mm = e % BM; kk = e / BM; }
+      else { kk = e % BK; mm = e / BK; }
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      cplx v = (gm < M && gk < K) ? loadA<OPA>(A, lda, gm, gk) : mk(0.0, 0.0);
+      Ar[kk][mm] = v.x;
+      Ai[kk][mm] = v.y;
+    }
+#pragma unroll
+    for (int e = tid; e < BN * BK; e += 256) {
+      int nn, kk;
+      if (OPB == 'N' || OPB == 'R') { kk = e % BK; nn = e / BK; }
+      else { nn = e % BN; kk = e / BN; }
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      cplx v = (gn < N && gk < K) ? loadB<OPB>(B, ldb, gk, gn) : mk(0.0, 0.0);
+      Br[kk][nn] = v.x;
+      Bi[kk][nn] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kl = kk + (lane >> 4);
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Ar[kl][wm + 16 * i + (lane & 15)];
+        ai[i] = Ai[kl][wm + 16 * i + (lane & 15)];
+        br[i] = Br[kl][wn + 16 * i + (lane & 15)];
+        bi[i] = Bi[kl][wn + 16 * i + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accr[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // D[row][col] of a 16x16 MFMA tile sits in element (row % 4) of lane col + 16 * (row / 4)
+  const bool beta_zero = (beta.x == 0.0 && beta.y == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int64_t col = n0 + wn + 16 * j + (lane & 15);
+        if (row < M && col < N) {
+          cplx acc = mk(accr[i][j][r], acci[i][j][r]);
+          cplx out = cmul(alpha, acc);
+          if (!beta_zero) cfma(out, beta, C[row + ldc * col]);
+          C[row + ldc * col] = out;
+        }
+      }
+}
+
+template <char OPA, char OPB>
+void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda, int64_t sA,
+            const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC, int64_t batch) {
+  dim3 grid(cdiv(M, BM), cdiv(N, BN), (unsigned)batch);
+  hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                     C, ldc, sC);
+  DC2_HIP(hipGetLastError());
+}
+
+template <char OPA>
+void dispatchB(char opB, hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda,
+               int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC,
+               int64_t batch) {
+  switch (opB) {
+    case 'N': launch<OPA, 'N'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    case 'R': launch<OPA, 'R'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    case 'T': launch<OPA, 'T'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    case 'C': launch<OPA, 'C'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    default: throw Error(1, "zgemm: bad opB");
+  }
+}
+}  // namespace
+
+void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A,
+           int64_t lda, int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc,
+           int64_t sC, int64_t batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  DC2_CHECK(batch < 65536, 1, "zgemm: batch too large for one grid dimension");
+  DC2_CHECK(cdiv(M, BM) < (1u << 31) && cdiv(N, BN) < 65536, 1, "zgemm: matrix too large for the grid");
+  switch (opA) {
+    case 'N': dispatchB<'N'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    case 'T': dispatchB<'T'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    case 'C': dispatchB<'C'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    default: throw Error(1, "zgemm: bad opA");
+  }
+}
+
+}  // namespace dc2

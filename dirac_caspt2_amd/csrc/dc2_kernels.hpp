@@ -1,0 +1,92 @@
+// Launchers of the CASCI / CASPT2 kernels (defined in dc2_ci.hip and dc2_pt2.hip).
+#pragma once
+
+#include "dc2_common.hpp"
+
+namespace dc2 {
+
+// ---------------------------------------------------------------- dc2_ci.hip
+struct DetParams {
+  int nact, nelec, ninact;
+  int ras1, ras2, ras3, ras1_max_hole, ras3_max_elec, min_hole_ras1, min_elec_ras3;
+  int nsym, totsym;
+  const int* irp_act;   // device, nact entries (1-based fermion irreps of the active spinors)
+  const int* multb_d;   // device, nsym x nsym (MULTB_D, column-major)
+  const int* multb_ds;  // device, nsym x nsym (MULTB_DS, column-major)
+};
+
+// Enumerate all nelec-in-nact strings (full) and the CAS/RAS/symmetry-selected
+// subset, both in ascending bit-string order (the Gosper order of
+// search_cas_configuration.F90).  Returns counts through the host pointers.
+void enumerate_dets(hipStream_t s, const DetParams& p, int* flag_full, int* flag_cas, int* rank_full,
+                    int* rank_cas, int64_t* full_dets, int64_t* cas_dets, int* cas2full, int64_t* n_full,
+                    int64_t* n_cas, int* counts_dev);
+// tab[(p + na*q)*nf + j] = (rank(E_pq |j>) << 1) | odd_phase, or -1 when E_pq|j> = 0
+void build_exc_table(hipStream_t s, int na, int64_t nf, const int64_t* full_dets, const int* rank_full, int* tab);
+// Y[j + nf*((p + na*q) + na*na*x)] = (E_pq X_x)_j for x < nx
+void apply_E(hipStream_t s, int na, int64_t nf, int64_t nx, const int* tab, const cplx* X, cplx* Y);
+// c_full = embed(c_cas) (zeros elsewhere), cF = f * c_full with f_j = sum_w eps_w n_w(j)
+void embed_ci(hipStream_t s, int na, int64_t nf, int64_t ncas, const int* cas2full, const cplx* c_cas,
+              const int64_t* full_dets, const double* eps_act, cplx* c_full, cplx* cF);
+// CASCI matrix (casmat.F90:3-222)
+void casmat(hipStream_t s, int ni, int na, int ne, int64_t ndet, const int64_t* cas_dets, const int* rank_cas,
+            const cplx* O, int nmoc, const cplx* h, int ldh, cplx* mat);
+// Fock matrix, occupied and secondary diagonal blocks (fockcasci.F90:4-90); upper
+// triangles written as computed, lower triangles as conjugates
+void fock_build(hipStream_t s, int ni, int na, int ns, const cplx* h, int ldh, const cplx* O, const cplx* F1,
+                const cplx* F2, const cplx* D1, cplx* fock);
+// gather a (space, irrep) block: out(i,j) = fock(ind_i, ind_j) (upper values)
+void gather_block(hipStream_t s, int n, const int* ind, const cplx* fock, int ldf, cplx* out);
+void scatter_block(hipStream_t s, int n, const int* ind, const cplx* V, const double* w, cplx* U, int ldu,
+                   double* eps);
+// canonical eigenvector gauge (see canonical_gauge in the oracle); scratch >= n*n
+void canonical_gauge(hipStream_t s, int n, const double* w, cplx* V, cplx* scratch);
+// ds(k,kt) = det(Uact[occ_k, occ_kt]) with partial pivoting (trac.F90:105-131, sdet.F90:87-108)
+void tracic_ds(hipStream_t s, int na, int ne, int64_t ndet, const int64_t* cas_dets, const cplx* Uact, cplx* ds);
+// e0 = sum_w eps_w Re D1(w,w) (calce0.F90)
+void calce0(hipStream_t s, int na, const double* eps_act, const cplx* D1, double* e0);
+// out[perm] generic 6-index permutation of a column-major tensor
+void permute6(hipStream_t s, const int* dims_out, const int* src_stride_of_out_axis, const cplx* in, cplx* out,
+              int64_t total);
+// copy a 4-index sub-block of a column-major tensor
+void extract4(hipStream_t s, const cplx* in, const int64_t* ld_in, const int* off, const int* n, cplx* out);
+void fill_zero(hipStream_t s, cplx* p, int64_t n);
+void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc);
+
+// ---------------------------------------------------------------- dc2_pt2.hip
+struct Dens {
+  int na;
+  const cplx *G1, *G1F, *G2, *G2F, *G3, *G3F;  // see dc2_pt2.hip for the index conventions
+};
+void build_SB(hipStream_t s, char X, const Dens& d, const double* eps_act, const double* e0_dev, int64_t n,
+              const int* pairs, cplx* S, cplx* B);
+// effective one-electron terms of A, C, D (tramo1 + integral sums)
+void effh_A(hipStream_t s, int ni, int na, const cplx* hp, int ldh, const cplx* A2, cplx* effh);
+void effh_C(hipStream_t s, int ni, int na, int ns, const cplx* hp, int ldh, const cplx* C1, const cplx* C2,
+            const cplx* C3, cplx* effh);
+void effh_D(hipStream_t s, int ni, int na, int ns, const cplx* hp, int ldh, const cplx* D3b, cplx* effh);
+// D3 reshapes used by the A and C V-matrix GEMMs
+void d3_perm_A(hipStream_t s, int na, const cplx* G3, cplx* Dm);
+void d3_perm_C(hipStream_t s, int na, const cplx* G3, cplx* Dm);
+void a1_perm(hipStream_t s, int na, int ni, const cplx* A1, cplx* Am);
+void V_A_rest(hipStream_t s, int na, int ni, const Dens& d, const cplx* A1, const cplx* effh, cplx* V);
+void V_C_rest(hipStream_t s, int na, int ns, const Dens& d, const cplx* effh, cplx* V);
+void V_B(hipStream_t s, int na, int ni, const Dens& d, const cplx* Bb, const int* tpairs, const int* ipairs,
+         cplx* V);
+void V_D(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* D1b, const cplx* D2b, const cplx* effh,
+         cplx* V);
+void V_E(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Eb, const int* ipairs, cplx* V);
+void V_F(hipStream_t s, int na, int ns, const Dens& d, const cplx* Fb, const int* tpairs, const int* spairs, cplx* V);
+void V_G(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Gb, const int* spairs, cplx* V);
+// alpha per perturber of subspace X (solve_X_subspace.F90 'alpha =' lines)
+void alphas(hipStream_t s, char X, int ni, int na, int ns, const double* eps, const double* e0_dev, double eshift,
+            const int* ipairs, const int* spairs, int64_t npert, double* alpha);
+// e2 = -sum |X_jk|^2/(alpha_k + wb_j), sumc2 = sum |X_jk|^2/(alpha_k + wb_j)^2 ; partial sums per block,
+// then a single-block final pass (deterministic order)
+void e2_reduce(hipStream_t s, int64_t dimm, int64_t npert, const cplx* X, const double* wb, const double* alpha,
+               double* partial, int64_t npartial_max, double* out2);
+void e2_H(hipStream_t s, int ni, int ns, const cplx* Hb, const double* eps, double eshift, const int* ipairs,
+          const int* spairs, double* partial, int64_t npartial_max, double* out2);
+int64_t e2_partials_needed(int64_t nwork);
+
+}  // namespace dc2

@@ -1,0 +1,500 @@
+// PT2-side kernels: S and B (= H0 - E0 S) assembly, V = <X|H|0> builders and
+// the E2 / sumc2 reductions of the eight internally contracted subspaces
+// (solve_[A-H]_subspace.F90).
+//
+// Density conventions (all column-major, na active spinors, na2 = na*na):
+//   G1 [a + na*b]                          = <E_ab>
+//   G1F[a + na*b]                          = <E_ab F>,      F = sum_w eps_w E_ww
+//   G2 [(b + na*a) + na2*(c + na*d)]       = <E_ab E_cd>    (bra pair stored swapped:
+//                                              G2 = Y1^H Y1 with Y1 = E|0>)
+//   G2F                                     = <E_ab E_cd F>
+//   G3 [(b + na*a) + na2*((c + na*d) + na2*(e + na*f))] = <E_ab E_cd E_ef>,  G3F likewise.
+// Integral blocks are the transformed subspace blocks, column-major over their
+// four indices, e.g. the (s,i|s,i) block H[a + ns*(i + ni*(b + ns*j))] = (ai|bj).
+#include "dc2_kernels.hpp"
+
+namespace dc2 {
+namespace {
+
+struct DV {
+  int na;
+  const cplx *G1, *G1F, *G2, *G2F, *G3, *G3F;
+  __device__ cplx D1(int a, int b) const { return G1[a + na * b]; }
+  __device__ cplx D1F(int a, int b) const { return G1F[a + na * b]; }
+  __device__ int64_t i2(int a, int b, int c, int d) const {
+    return (int64_t)(b + na * a) + (int64_t)na * na * (c + na * d);
+  }
+  __device__ int64_t i3(int a, int b, int c, int d, int e, int f) const {
+    const int64_t na2 = (int64_t)na * na;
+    return (int64_t)(b + na * a) + na2 * ((c + na * d) + na2 * (e + na * f));
+  }
+  __device__ cplx D2(int a, int b, int c, int d) const { return G2[i2(a, b, c, d)]; }
+  __device__ cplx D2F(int a, int b, int c, int d) const { return G2F[i2(a, b, c, d)]; }
+  __device__ cplx D3(int a, int b, int c, int d, int e, int f) const { return G3[i3(a, b, c, d, e, f)]; }
+  __device__ cplx D3F(int a, int b, int c, int d, int e, int f) const { return G3F[i3(a, b, c, d, e, f)]; }
+};
+
+inline DV dv(const Dens& d) { return DV{d.na, d.G1, d.G1F, d.G2, d.G2F, d.G3, d.G3F}; }
+
+__device__ inline cplx neg(cplx a) { return mk(-a.x, -a.y); }
+
+// S and B formula values for j >= i (the computed upper triangle of every sXmat/bXmat)
+template <char X>
+__device__ void sb_formula(const DV& d, const double* e, double e0, const int* pairs, int64_t i, int64_t j,
+                           cplx& s, cplx& b) {
+  const int na = d.na;
+  if (X == 'A') {  // solve_A_subspace.F90:312-330, 374-409
+    const int x = (int)(i / (na * na)), y = (int)((i / na) % na), z = (int)(i % na);
+    const int t = (int)(j / (na * na)), u = (int)((j / na) % na), v = (int)(j % na);
+    s = neg(d.D3(z, y, t, x, u, v));
+    b = neg(d.D3F(z, y, t, x, u, v));
+    if (t == x) { s = cadd(s, d.D2(z, y, u, v)); b = cadd(b, d.D2F(z, y, u, v)); }
+    b = cadd(b, cscale(s, e[u] + e[t] - e[v]));
+  } else if (X == 'B') {  // solve_B_subspace.F90:317-345, 404-447
+    const int x = pairs[2 * i], y = pairs[2 * i + 1], t = pairs[2 * j], u = pairs[2 * j + 1];
+    s = d.D2(t, x, u, y);
+    b = d.D2F(t, x, u, y);
+    if (t == x) { s = csub(s, d.D1(u, y)); b = csub(b, d.D1F(u, y)); }
+    if (u == y) { s = csub(s, d.D1(t, x)); b = csub(b, d.D1F(t, x)); }
+    if (t == y) { s = csub(s, d.D1(u, x)); b = csub(b, d.D1F(u, x)); }
+    if (t == x && u == y) { s.x += 1.0; b.x += e0; }
+    b = cadd(b, cscale(s, e[t] + e[u]));
+  } else if (X == 'C') {  // solve_C_subspace.F90:308-330, 384-414
+    const int x = (int)(i / (na * na)), y = (int)((i / na) % na), z = (int)(i % na);
+    const int t = (int)(j / (na * na)), u = (int)((j / na) % na), v = (int)(j % na);
+    s = d.D3(z, y, x, t, u, v);
+    b = cadd(d.D3F(z, y, x, t, u, v), cscale(s, e[u] - e[v] - e[t]));
+  } else if (X == 'D') {  // solve_D_subspace.F90:326-337, 387-406
+    const int x = (int)(i / na), y = (int)(i % na), t = (int)(j / na), u = (int)(j % na);
+    s = d.D2(y, x, t, u);
+    b = cadd(d.D2F(y, x, t, u), cscale(s, e[t] - e[u]));
+  } else if (X == 'E') {  // solve_E_subspace.F90:303-318, 362-392
+    const int u = (int)i, t = (int)j;
+    s = neg(d.D1(t, u));
+    if (u == t) s.x += 1.0;
+    b = neg(d.D1F(t, u));
+    if (u == t) b.x += e0;
+    b = cadd(b, cscale(s, e[t]));
+  } else if (X == 'F') {  // solve_F_subspace.F90:325-340, 395-420
+    const int v = pairs[2 * i], x = pairs[2 * i + 1], t = pairs[2 * j], u = pairs[2 * j + 1];
+    s = d.D2(v, t, x, u);
+    b = d.D2F(v, t, x, u);
+    if (x == t) { s = csub(s, d.D1(v, u)); b = csub(b, d.D1F(v, u)); }
+    b = cadd(b, cscale(s, -e[u] - e[t]));
+  } else {  // 'G', solve_G_subspace.F90:313-325, 380-395
+    const int ti = (int)i, tj = (int)j;
+    s = d.D1(ti, tj);
+    b = csub(d.D1F(ti, tj), cscale(s, e[ti]));
+  }
+}
+
+template <char X>
+__global__ void k_build_SB(DV d, const double* eps_act, const double* e0p, int64_t n, const int* pairs, cplx* S,
+                           cplx* B) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  const double e0 = *e0p;
+  cplx s, b;
+  if (i <= j) {
+    sb_formula<X>(d, eps_act, e0, pairs, i, j, s, b);
+    if (i == j) { s = cconj(s); b = cconj(b); }   // sc(j,i) = DCONJG(sc(i,j)) with i == j
+  } else {
+    sb_formula<X>(d, eps_act, e0, pairs, j, i, s, b);
+    s = cconj(s);
+    b = cconj(b);
+  }
+  S[i + n * j] = s;
+  B[i + n * j] = b;
+}
+
+// effh(p,i) = h'(p,i) + sum_k [(pi|kk) - (pk|ki)]      (solve_A_subspace.F90:531-612), A2 = (a,i|i,i)
+__global__ void k_effh_A(int ni, int na, const cplx* hp, int ldh, const cplx* A2, cplx* effh) {
+  const int p = threadIdx.x, i = blockIdx.x;
+  if (p >= na) return;
+  const int64_t Na = na, Ni = ni;
+  cplx v = hp[(ni + p) + (int64_t)ldh * i];
+  for (int k = 0; k < ni; ++k) {
+    v = cadd(v, A2[p + Na * (i + Ni * (k + Ni * k))]);
+    v = csub(v, A2[p + Na * (k + Ni * (k + Ni * i))]);
+  }
+  effh[p + na * i] = v;
+}
+
+// effh(a,p) = h'(a,p) + sum_k[(ap|kk) - (ak|kp)] - sum_w (aw|wp)   (solve_C_subspace.F90:530-608)
+__global__ void k_effh_C(int ni, int na, int ns, const cplx* hp, int ldh, const cplx* C1, const cplx* C2,
+                         const cplx* C3, cplx* effh) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, p = blockIdx.y;
+  if (a >= ns) return;
+  const int64_t S = ns, Na = na, Ni = ni;
+  cplx v = hp[(ni + na + a) + (int64_t)ldh * (ni + p)];
+  for (int k = 0; k < ni; ++k) {
+    v = cadd(v, C2[a + S * (p + Na * (k + Ni * k))]);
+    v = csub(v, C3[a + S * (k + Ni * (k + Ni * p))]);
+  }
+  for (int w = 0; w < na; ++w) v = csub(v, C1[a + S * (w + Na * (w + Na * p))]);
+  effh[a + ns * p] = v;
+}
+
+// effh(a,i) = h'(a,i) + sum_k [(ai|kk) - (ak|ki)]       (solve_D_subspace.F90:455-597), D3b = (s,i|i,i)
+__global__ void k_effh_D(int ni, int na, int ns, const cplx* hp, int ldh, const cplx* D3b, cplx* effh) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (a >= ns) return;
+  const int64_t S = ns, Ni = ni;
+  cplx v = hp[(ni + na + a) + (int64_t)ldh * i];
+  for (int k = 0; k < ni; ++k) {
+    v = cadd(v, D3b[a + S * (i + Ni * (k + Ni * k))]);
+    v = csub(v, D3b[a + S * (k + Ni * (k + Ni * i))]);
+  }
+  effh[a + ns * i] = v;
+}
+
+// Dm[(v + na u + na2 t) + na3 (p + na q + na2 r)] = D3(v,u,p,t,q,r)
+__global__ void k_d3_perm_A(DV d, cplx* Dm) {
+  const int na = d.na;
+  const int64_t na3 = (int64_t)na * na * na;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
+  if (row >= na3) return;
+  const int v = (int)(row % na), u = (int)((row / na) % na), t = (int)(row / (na * na));
+  const int p = (int)(col % na), q = (int)((col / na) % na), r = (int)(col / (na * na));
+  Dm[row + na3 * col] = d.D3(v, u, p, t, q, r);
+}
+// Dm[(v + na u + na2 t) + na3 (j + na k + na2 l)] = D3(v,u,t,j,k,l)
+__global__ void k_d3_perm_C(DV d, cplx* Dm) {
+  const int na = d.na;
+  const int64_t na3 = (int64_t)na * na * na;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
+  if (row >= na3) return;
+  const int v = (int)(row % na), u = (int)((row / na) % na), t = (int)(row / (na * na));
+  const int j = (int)(col % na), k = (int)((col / na) % na), l = (int)(col / (na * na));
+  Dm[row + na3 * col] = d.D3(v, u, t, j, k, l);
+}
+// Am[(p + na q + na2 r) + na3 i] = A1[p + na (i + ni (q + na r))],  A1 = (a,i|a,a)
+__global__ void k_a1_perm(int na, int ni, const cplx* A1, cplx* Am) {
+  const int64_t na3 = (int64_t)na * na * na;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (row >= na3) return;
+  const int p = (int)(row % na), q = (int)((row / na) % na), r = (int)(row / (na * na));
+  Am[row + na3 * i] = A1[p + (int64_t)na * (i + (int64_t)ni * (q + (int64_t)na * r))];
+}
+
+// V_A(tuv, i) += sum_pq <E_vu E_pq>(ti|pq) - sum_p <E_vu E_pt> effh(p,i) + <E_vu> effh(t,i)
+__global__ void k_V_A_rest(int na, int ni, DV d, const cplx* A1, const cplx* effh, cplx* V) {
+  const int64_t na3 = (int64_t)na * na * na;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (row >= na3) return;
+  const int v = (int)(row % na), u = (int)((row / na) % na), t = (int)(row / (na * na));
+  const int64_t Na = na, Ni = ni;
+  cplx acc = V[row + na3 * i];
+  for (int q = 0; q < na; ++q)
+    for (int p = 0; p < na; ++p) cfma(acc, d.D2(v, u, p, q), A1[t + Na * (i + Ni * (p + Na * q))]);
+  for (int p = 0; p < na; ++p) acc = csub(acc, cmul(d.D2(v, u, p, t), effh[p + na * i]));
+  cfma(acc, d.D1(v, u), effh[t + na * i]);
+  V[row + na3 * i] = acc;
+}
+
+// V_C(tuv, a) += sum_p effh(a,p) <E_vu E_tp>, skipping |effh| < 1e-10 (solve_C_subspace.F90:629)
+__global__ void k_V_C_rest(int na, int ns, DV d, const cplx* effh, cplx* V) {
+  const int64_t na3 = (int64_t)na * na * na;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y;
+  if (row >= na3) return;
+  const int v = (int)(row % na), u = (int)((row / na) % na), t = (int)(row / (na * na));
+  cplx acc = V[row + na3 * a];
+  for (int p = 0; p < na; ++p) {
+    const cplx ef = effh[a + (int64_t)ns * p];
+    if (sqrt(cabs2(ef)) < 1.0e-10) continue;
+    cfma(acc, ef, d.D2(v, u, t, p));
+  }
+  V[row + na3 * a] = acc;
+}
+
+// V_B(tu; i>j)  (solve_B_subspace.F90:530-585), Bb = (a,i|a,i)
+__global__ void k_V_B(int na, int ni, DV d, const cplx* Bb, const int* tp, const int* ip, int64_t ntp, cplx* V) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
+  if (k >= ntp) return;
+  const int t = tp[2 * k], u = tp[2 * k + 1], i = ip[2 * m], j = ip[2 * m + 1];
+  const int64_t Na = na, Ni = ni;
+  auto Bv = [&](int p, int ii, int q, int jj) { return Bb[p + Na * (ii + Ni * (q + Na * jj))]; };
+  cplx acc = csub(Bv(t, i, u, j), Bv(u, i, t, j));
+  for (int q = 0; q < na; ++q)
+    for (int p = 0; p < na; ++p) cfma(acc, Bv(p, i, q, j), d.D2(p, t, q, u));
+  for (int p = 0; p < na; ++p) {
+    cfma(acc, Bv(u, i, p, j), d.D1(p, t));
+    acc = csub(acc, cmul(Bv(t, i, p, j), d.D1(p, u)));
+    acc = csub(acc, cmul(Bv(p, i, u, j), d.D1(p, t)));
+  }
+  V[k + ntp * m] = acc;
+}
+
+// V_D(tu; a,i) (solve_D_subspace.F90:455-597), D1b = (s,i|a,a), D2b = (s,a|a,i)
+__global__ void k_V_D(int na, int ni, int ns, DV d, const cplx* D1b, const cplx* D2b, const cplx* effh, cplx* V) {
+  const int64_t tu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, ai = blockIdx.y;
+  const int64_t na2 = (int64_t)na * na;
+  if (tu >= na2) return;
+  const int t = (int)(tu / na), u = (int)(tu % na), a = (int)(ai / ni), i = (int)(ai % ni);
+  const int64_t S = ns, Na = na, Ni = ni;
+  cplx acc = cmul(effh[a + S * i], d.D1(u, t));
+  for (int q = 0; q < na; ++q)
+    for (int p = 0; p < na; ++p) {
+      const cplx g = csub(D1b[a + S * (i + Ni * (p + Na * q))], D2b[a + S * (q + Na * (p + Na * i))]);
+      cfma(acc, d.D2(u, t, p, q), g);
+    }
+  V[tu + na2 * ai] = acc;
+}
+
+// V_E(t; a,I>J) (solve_E_subspace.F90:436-466), Eb = (s,i|a,i)
+__global__ void k_V_E(int na, int ni, int ns, DV d, const cplx* Eb, const int* ip, cplx* V) {
+  const int t = threadIdx.x;
+  const int64_t pa = blockIdx.x;   // pair * ns + a
+  if (t >= na) return;
+  const int64_t S = ns, Ni = ni, Na = na;
+  const int a = (int)(pa % ns), m = (int)(pa / ns);
+  const int I = ip[2 * m], J = ip[2 * m + 1];
+  auto Ev = [&](int aa, int ii, int p, int jj) { return Eb[aa + S * (ii + Ni * (p + Na * jj))]; };
+  cplx acc = csub(Ev(a, J, t, I), Ev(a, I, t, J));
+  for (int p = 0; p < na; ++p) cfma(acc, d.D1(t, p), csub(Ev(a, I, p, J), Ev(a, J, p, I)));
+  V[t + Na * pa] = acc;
+}
+
+// V_F(tu; a>b) (solve_F_subspace.F90:505-535), Fb = (s,a|s,a)
+__global__ void k_V_F(int na, int ns, DV d, const cplx* Fb, const int* tp, const int* sp, int64_t ntp, cplx* V) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
+  if (k >= ntp) return;
+  const int t = tp[2 * k], u = tp[2 * k + 1], a = sp[2 * m], b = sp[2 * m + 1];
+  const int64_t S = ns, Na = na;
+  auto Fv = [&](int aa, int p, int bb, int q) { return Fb[aa + S * (p + Na * (bb + S * q))]; };
+  cplx acc = mk(0.0, 0.0);
+  for (int q = 0; q < na; ++q)
+    for (int p = 0; p < na; ++p) cfma(acc, d.D2(t, p, u, q), Fv(a, p, b, q));
+  for (int p = 0; p < na; ++p) acc = csub(acc, cmul(d.D1(t, p), Fv(a, u, b, p)));
+  V[k + ntp * m] = acc;
+}
+
+// V_G(t; a>b, i) (solve_G_subspace.F90:445-462), Gb = (s,i|s,a)
+__global__ void k_V_G(int na, int ni, int ns, DV d, const cplx* Gb, const int* sp, int64_t nsp, cplx* V) {
+  const int t = threadIdx.x % 32 < na ? threadIdx.x % 32 : -1;
+  const int64_t pi = (int64_t)blockIdx.x * (blockDim.x / 32) + threadIdx.x / 32;  // pair*ni + i
+  if (t < 0 || pi >= nsp * ni) return;
+  const int64_t S = ns, Ni = ni;
+  const int m = (int)(pi / ni), i = (int)(pi % ni), a = sp[2 * m], b = sp[2 * m + 1];
+  auto Gv = [&](int aa, int ii, int bb, int p) { return Gb[aa + S * (ii + Ni * (bb + S * p))]; };
+  cplx acc = mk(0.0, 0.0);
+  for (int p = 0; p < na; ++p) cfma(acc, d.D1(t, p), csub(Gv(a, i, b, p), Gv(b, i, a, p)));
+  V[t + (int64_t)na * pi] = acc;
+}
+
+__global__ void k_alphas(char X, int ni, int na, int ns, const double* eps, const double* e0p, double eshift,
+                         const int* ip, const int* sp, int64_t npert, double* alpha) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npert) return;
+  const double e0 = *e0p;
+  const int nmoc = ni + na;
+  double a = 0.0;
+  switch (X) {
+    case 'A': a = -eps[k]; break;                                              // :249
+    case 'B': a = -eps[ip[2 * k]] - eps[ip[2 * k + 1]]; break;                 // :251
+    case 'C': a = eps[nmoc + k]; break;                                        // :247
+    case 'D': a = eps[nmoc + k / ni] - eps[k % ni]; break;                     // :262
+    case 'E': { const int64_t m = k / ns, s = k % ns;                          // :238
+      a = eps[nmoc + s] - eps[ip[2 * m]] - eps[ip[2 * m + 1]]; } break;
+    case 'F': a = eps[nmoc + sp[2 * k]] + eps[nmoc + sp[2 * k + 1]]; break;    // :262
+    case 'G': { const int64_t m = k / ni, i = k % ni;                          // :252
+      a = -eps[i] + eps[nmoc + sp[2 * m]] + eps[nmoc + sp[2 * m + 1]]; } break;
+  }
+  alpha[k] = a - e0 + eshift;
+}
+
+constexpr int RB = 256;
+__device__ inline void block_sum2(double& a, double& b, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = a;
+  sh[RB + t] = b;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if (t < o) { sh[t] += sh[t + o]; sh[RB + t] += sh[RB + t + o]; }
+    __syncthreads();
+  }
+  a = sh[0];
+  b = sh[RB];
+}
+
+__global__ __launch_bounds__(RB) void k_e2_partial(int64_t dimm, int64_t npert, const cplx* X, const double* wb,
+                                                   const double* alpha, double* partial) {
+  __shared__ double sh[2 * RB];
+  double e = 0.0, s = 0.0;
+  const int64_t total = dimm * npert;
+  for (int64_t q = (int64_t)blockIdx.x * RB + threadIdx.x; q < total; q += (int64_t)gridDim.x * RB) {
+    const int64_t j = q % dimm, k = q / dimm;
+    const double den = alpha[k] + wb[j];
+    const double a2 = cabs2(X[q]);
+    e -= a2 / den;
+    s += a2 / (den * den);
+  }
+  block_sum2(e, s, sh);
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = e; partial[2 * blockIdx.x + 1] = s; }
+}
+
+__global__ __launch_bounds__(RB) void k_e2_H_partial(int ni, int ns, const cplx* Hb, const double* eps, double eshift,
+                                                     const int* ip, const int* sp, int64_t nsp, int64_t nip,
+                                                     double* partial) {
+  __shared__ double sh[2 * RB];
+  double e = 0.0, s = 0.0;
+  const int nmoc0 = 0;
+  (void)nmoc0;
+  const int64_t S = ns, Ni = ni, total = nsp * nip;
+  const int off = blockIdx.y;  // unused second dimension guard
+  (void)off;
+  for (int64_t q = (int64_t)blockIdx.x * RB + threadIdx.x; q < total; q += (int64_t)gridDim.x * RB) {
+    const int64_t m = q % nsp, n = q / nsp;
+    const int a = sp[2 * m], b = sp[2 * m + 1], i = ip[2 * n], j = ip[2 * n + 1];
+    const cplx v = csub(Hb[a + S * (i + Ni * (b + S * j))], Hb[a + S * (j + Ni * (b + S * i))]);
+    const double den = eps[a] + eps[b] - eps[ns + i] - eps[ns + j] + eshift;
+    const double a2 = cabs2(v);
+    e -= a2 / den;
+    s += a2 / (den * den);
+  }
+  block_sum2(e, s, sh);
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = e; partial[2 * blockIdx.x + 1] = s; }
+}
+
+__global__ __launch_bounds__(RB) void k_final2(int64_t np, const double* partial, double* out2) {
+  __shared__ double sh[2 * RB];
+  double e = 0.0, s = 0.0;
+  for (int64_t q = threadIdx.x; q < np; q += RB) { e += partial[2 * q]; s += partial[2 * q + 1]; }
+  block_sum2(e, s, sh);
+  if (threadIdx.x == 0) { out2[0] = e; out2[1] = s; }
+}
+
+}  // namespace
+
+int64_t e2_partials_needed(int64_t nwork) {
+  int64_t nb = (nwork + RB - 1) / RB;
+  if (nb > 2048) nb = 2048;
+  if (nb < 1) nb = 1;
+  return nb;
+}
+
+void build_SB(hipStream_t s, char X, const Dens& d, const double* eps_act, const double* e0_dev, int64_t n,
+              const int* pairs, cplx* S, cplx* B) {
+  DC2_CHECK(n < 65536, 1, "build_SB: dimension too large");
+  dim3 g(cdiv(n, 128), (unsigned)n);
+  DV v = dv(d);
+  switch (X) {
+    case 'A': hipLaunchKernelGGL(k_build_SB<'A'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    case 'B': hipLaunchKernelGGL(k_build_SB<'B'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    case 'C': hipLaunchKernelGGL(k_build_SB<'C'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    case 'D': hipLaunchKernelGGL(k_build_SB<'D'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    case 'E': hipLaunchKernelGGL(k_build_SB<'E'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    case 'F': hipLaunchKernelGGL(k_build_SB<'F'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    case 'G': hipLaunchKernelGGL(k_build_SB<'G'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    default: throw Error(1, "build_SB: bad subspace");
+  }
+  DC2_HIP(hipGetLastError());
+}
+
+void effh_A(hipStream_t s, int ni, int na, const cplx* hp, int ldh, const cplx* A2, cplx* effh) {
+  hipLaunchKernelGGL(k_effh_A, dim3(ni), dim3(64), 0, s, ni, na, hp, ldh, A2, effh);
+  DC2_HIP(hipGetLastError());
+}
+void effh_C(hipStream_t s, int ni, int na, int ns, const cplx* hp, int ldh, const cplx* C1, const cplx* C2,
+            const cplx* C3, cplx* effh) {
+  hipLaunchKernelGGL(k_effh_C, dim3(cdiv(ns, 64), na), dim3(64), 0, s, ni, na, ns, hp, ldh, C1, C2, C3, effh);
+  DC2_HIP(hipGetLastError());
+}
+void effh_D(hipStream_t s, int ni, int na, int ns, const cplx* hp, int ldh, const cplx* D3b, cplx* effh) {
+  hipLaunchKernelGGL(k_effh_D, dim3(cdiv(ns, 64), ni), dim3(64), 0, s, ni, na, ns, hp, ldh, D3b, effh);
+  DC2_HIP(hipGetLastError());
+}
+void d3_perm_A(hipStream_t s, int na, const cplx* G3, cplx* Dm) {
+  Dens d{na, nullptr, nullptr, nullptr, nullptr, G3, nullptr};
+  const int64_t na3 = (int64_t)na * na * na;
+  hipLaunchKernelGGL(k_d3_perm_A, dim3(cdiv(na3, 256), (unsigned)na3), dim3(256), 0, s, dv(d), Dm);
+  DC2_HIP(hipGetLastError());
+}
+void d3_perm_C(hipStream_t s, int na, const cplx* G3, cplx* Dm) {
+  Dens d{na, nullptr, nullptr, nullptr, nullptr, G3, nullptr};
+  const int64_t na3 = (int64_t)na * na * na;
+  hipLaunchKernelGGL(k_d3_perm_C, dim3(cdiv(na3, 256), (unsigned)na3), dim3(256), 0, s, dv(d), Dm);
+  DC2_HIP(hipGetLastError());
+}
+void a1_perm(hipStream_t s, int na, int ni, const cplx* A1, cplx* Am) {
+  const int64_t na3 = (int64_t)na * na * na;
+  hipLaunchKernelGGL(k_a1_perm, dim3(cdiv(na3, 256), ni), dim3(256), 0, s, na, ni, A1, Am);
+  DC2_HIP(hipGetLastError());
+}
+void V_A_rest(hipStream_t s, int na, int ni, const Dens& d, const cplx* A1, const cplx* effh, cplx* V) {
+  const int64_t na3 = (int64_t)na * na * na;
+  hipLaunchKernelGGL(k_V_A_rest, dim3(cdiv(na3, 128), ni), dim3(128), 0, s, na, ni, dv(d), A1, effh, V);
+  DC2_HIP(hipGetLastError());
+}
+void V_C_rest(hipStream_t s, int na, int ns, const Dens& d, const cplx* effh, cplx* V) {
+  const int64_t na3 = (int64_t)na * na * na;
+  hipLaunchKernelGGL(k_V_C_rest, dim3(cdiv(na3, 128), ns), dim3(128), 0, s, na, ns, dv(d), effh, V);
+  DC2_HIP(hipGetLastError());
+}
+void V_B(hipStream_t s, int na, int ni, const Dens& d, const cplx* Bb, const int* tpairs, const int* ipairs,
+         cplx* V) {
+  const int64_t ntp = (int64_t)na * (na - 1) / 2, nip = (int64_t)ni * (ni - 1) / 2;
+  if (ntp == 0 || nip == 0) return;
+  hipLaunchKernelGGL(k_V_B, dim3(cdiv(ntp, 64), (unsigned)nip), dim3(64), 0, s, na, ni, dv(d), Bb, tpairs, ipairs,
+                     ntp, V);
+  DC2_HIP(hipGetLastError());
+}
+void V_D(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* D1b, const cplx* D2b, const cplx* effh,
+         cplx* V) {
+  const int64_t na2 = (int64_t)na * na;
+  hipLaunchKernelGGL(k_V_D, dim3(cdiv(na2, 64), (unsigned)((int64_t)ns * ni)), dim3(64), 0, s, na, ni, ns, dv(d),
+                     D1b, D2b, effh, V);
+  DC2_HIP(hipGetLastError());
+}
+void V_E(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Eb, const int* ipairs, cplx* V) {
+  const int64_t nip = (int64_t)ni * (ni - 1) / 2;
+  if (nip == 0) return;
+  DC2_CHECK(na <= 64, 1, "V_E: nact <= 64");
+  hipLaunchKernelGGL(k_V_E, dim3((unsigned)(nip * ns)), dim3(64), 0, s, na, ni, ns, dv(d), Eb, ipairs, V);
+  DC2_HIP(hipGetLastError());
+}
+void V_F(hipStream_t s, int na, int ns, const Dens& d, const cplx* Fb, const int* tpairs, const int* spairs,
+         cplx* V) {
+  const int64_t ntp = (int64_t)na * (na - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
+  if (ntp == 0 || nsp == 0) return;
+  DC2_CHECK(nsp < 65536 * 64, 1, "V_F: too many secondary pairs");
+  hipLaunchKernelGGL(k_V_F, dim3(cdiv(ntp, 64), (unsigned)nsp), dim3(64), 0, s, na, ns, dv(d), Fb, tpairs, spairs,
+                     ntp, V);
+  DC2_HIP(hipGetLastError());
+}
+void V_G(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Gb, const int* spairs, cplx* V) {
+  const int64_t nsp = (int64_t)ns * (ns - 1) / 2;
+  if (nsp == 0 || ni == 0) return;
+  DC2_CHECK(na <= 32, 1, "V_G: nact <= 32");
+  const int64_t items = nsp * ni;   // 8 items of 32 lanes per 256-thread block
+  hipLaunchKernelGGL(k_V_G, dim3(cdiv(items, 8)), dim3(256), 0, s, na, ni, ns, dv(d), Gb, spairs, nsp, V);
+  DC2_HIP(hipGetLastError());
+}
+void alphas(hipStream_t s, char X, int ni, int na, int ns, const double* eps, const double* e0_dev, double eshift,
+            const int* ipairs, const int* spairs, int64_t npert, double* alpha) {
+  if (npert <= 0) return;
+  hipLaunchKernelGGL(k_alphas, dim3(cdiv(npert, 256)), dim3(256), 0, s, X, ni, na, ns, eps, e0_dev, eshift, ipairs,
+                     spairs, npert, alpha);
+  DC2_HIP(hipGetLastError());
+}
+void e2_reduce(hipStream_t s, int64_t dimm, int64_t npert, const cplx* X, const double* wb, const double* alpha,
+               double* partial, int64_t npartial_max, double* out2) {
+  int64_t nb = e2_partials_needed(dimm * npert);
+  if (nb > npartial_max) nb = npartial_max;
+  hipLaunchKernelGGL(k_e2_partial, dim3((unsigned)nb), dim3(RB), 0, s, dimm, npert, X, wb, alpha, partial);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(RB), 0, s, nb, partial, out2);
+  DC2_HIP(hipGetLastError());
+}
+void e2_H(hipStream_t s, int ni, int ns, const cplx* Hb, const double* eps, double eshift, const int* ipairs,
+          const int* spairs, double* partial, int64_t npartial_max, double* out2) {
+  const int64_t nsp = (int64_t)ns * (ns - 1) / 2, nip = (int64_t)ni * (ni - 1) / 2;
+  int64_t nb = e2_partials_needed(nsp * nip);
+  if (nb > npartial_max) nb = npartial_max;
+  // eps here is laid out [secondary..., inactive...] by the caller
+  hipLaunchKernelGGL(k_e2_H_partial, dim3((unsigned)nb), dim3(RB), 0, s, ni, ns, Hb, eps, eshift, ipairs, spairs, nsp,
+                     nip, partial);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(RB), 0, s, nb, partial, out2);
+  DC2_HIP(hipGetLastError());
+}
+
+}  // namespace dc2

@@ -1,0 +1,134 @@
+/*
+ * dcaspt2_amd.h -- C ABI of the MI355X (gfx950) CASCI/CASPT2 hot path.
+ *
+ * Plain C: fixed-width integers, double pointers.  Complex arrays are
+ * interleaved (re, im) doubles, i.e. the memory of Fortran complex*16 or C
+ * double _Complex, column-major (Fortran order), 0-based in C / 1-based
+ * labels where the reference uses them.  Every function returns 0 on success
+ * or a non-zero status (1 bad argument/state, 2 HIP error, 3 rocSOLVER error,
+ * >= 100: 100 + LAPACK-style info of a failed factorisation); the message is
+ * available from dc2_last_error().  A Fortran host turns a non-zero status into
+ * stop_with_errorcode(status) (reference: module_error.F90:15-39).
+ *
+ * The entry points map onto the reference's Fortran subroutines (paths relative
+ * to the reference repository):
+ *
+ *   dc2_set_spaces / dc2_set_symmetry / dc2_set_ras / dc2_set_one_electron /
+ *   dc2_set_integral_block   -- the module_global_variables state the reference's
+ *       readers fill (module_global_variables.F90:55-183: ninact/nact/nsec/nelec,
+ *       irpamo, MULTB_*, ras*_list, one_elec_int_r/i, inttwr/inttwi,
+ *       int2r/i_f1, int2r/i_f2) and the subspace integral files A1int..Hint
+ *       (divide_2_elec_integral_into_subspaces.F90:133-384)
+ *   dc2_casci        -- casci (casci.F90:3-72): search_cas_configuration
+ *                       (search_cas_configuration.F90:3-181) + casmat_complex
+ *                       (casmat.F90:3-222) + cdiag (diag.F90:66-128)
+ *   dc2_caspt2       -- r4dcaspt2_tra (r4dcaspt2_tra.F90:4-482): fockcasci_complex
+ *                       (fockcasci.F90:4-90), fockdiag (fockdiag.F90:4-78) with
+ *                       tracic (trac.F90:76-156), calce0 (calce0.F90:4-50),
+ *                       intra_1/2/3 (module_intra.F90:19-51), tramo1
+ *                       (tramo.F90:4-68) and solve_A..H_subspace
+ *                       (solve_[A-H]_subspace.F90, with cutoff.F90:37-70 and
+ *                       ulambda_s_half.F90:47-75)
+ *   dc2_casmat       -- casmat_complex alone (CI matrix to host memory)
+ *   dc2_cdiag        -- cdiag (diag.F90:66-128) on a host matrix, on the GPU
+ *   dc2_zgemm        -- gemm (module_blas.F90:32-57) on device memory
+ */
+#ifndef DCASPT2_AMD_H
+#define DCASPT2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dc2_context dc2_context;
+
+/* subspace indices used by the result arrays: A=0 ... H=7 */
+#define DC2_NSUBSPACE 8
+
+/* ---- context -------------------------------------------------------------- */
+int dc2_context_create(int32_t device, dc2_context** out);
+int dc2_context_destroy(dc2_context* ctx);
+const char* dc2_last_error(void);
+/* version string of the library build */
+const char* dc2_version(void);
+
+/* ---- problem definition (module_global_variables.F90) ---------------------- */
+/* spinor spaces: ninact, nact, nsec, nelec (module_global_variables.F90:21-24) */
+int dc2_set_spaces(dc2_context* ctx, int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec);
+/* double-group symmetry: nsymrpa, MULTB (2n x 2n, column-major, 1-based values),
+ * irpamo (nmo entries, CAS order) and the target total symmetry totsym
+ * (read_mrconee.F90:383-453).  Without this call the problem is C1. */
+int dc2_set_symmetry(dc2_context* ctx, int32_t nsymrpa, const int64_t* multb, const int64_t* irpamo, int32_t totsym);
+/* RAS restriction (read_input_module.F90:150-172; ras_det_check.F90) */
+int dc2_set_ras(dc2_context* ctx, int32_t ras1_size, int32_t ras2_size, int32_t ras3_size, int32_t ras1_max_hole,
+                int32_t ras3_max_elec, int32_t min_hole_ras1, int32_t min_elec_ras3);
+/* one-electron integrals h (nmo x nmo complex, CAS order) and the core energy */
+int dc2_set_one_electron(dc2_context* ctx, const double* h, double ecore);
+/* Two-electron integral blocks in the original spinor basis, column-major over
+ * (i,j,k,l) = (ij|kl).  name / index spaces (i inactive, a active, s secondary,
+ * o = inactive+active):
+ *   "OOOO" (o,o|o,o)  = inttwr/inttwi      (module_2integrals.F90:395-434)
+ *   "F1"   (s,s|o,o)  = int2r/i_f1, "F2" (s,o|o,s) = int2r/i_f2
+ *   "A1" aiaa "A2" aiii "B" aiai (optional: taken from OOOO when absent)
+ *   "C1" saaa "C2" saii "C3" siia "D1" siaa "D2" saai "D3" siii
+ *   "E" siai "F" sasa "G" sisa "H" sisi      (the A1int..Hint subspace files)
+ * where: 0 = host memory (copied), 1 = device memory (copied),
+ *        2 = device memory borrowed (must stay valid until the context is
+ *            destroyed or the block replaced). */
+int dc2_set_integral_block(dc2_context* ctx, const char* name, const double* data, int64_t count, int32_t where);
+
+/* ---- CASCI (casci.F90:31-72) --------------------------------------------------
+ * nroot energies eigen(k) = ecas(k) + ecore are written to eigen[0..nroot-1]
+ * (fewer if ndet < nroot; *ndet_out receives ndet). */
+int dc2_casci(dc2_context* ctx, int32_t nroot, double* eigen, int64_t* ndet_out);
+/* determinant bit strings (ndet int64) and the CI vector of a root (ndet complex) */
+int dc2_get_dets(dc2_context* ctx, int64_t* dets);
+int dc2_get_ci_vector(dc2_context* ctx, int32_t root, double* c);
+
+/* ---- CASPT2 (r4dcaspt2_tra.F90:113-442) ------------------------------------------
+ * selectroot is 1-based.  Outputs: e2[8], sumc2[8] per subspace A..H (0 when
+ * skipped), scalars[6] = { e0, e2all, sumc2_all, weight0,
+ * total second-order energy (e2all - eshift*sumc2), total energy }. */
+int dc2_caspt2(dc2_context* ctx, int32_t selectroot, double eshift, double* e2, double* sumc2, double* scalars);
+/* Per subspace diagnostics of the last dc2_caspt2: info[4*k..4*k+3] =
+ * {dimn, dimm, npert, ran}, margin[k] = min |w_S - 1e-8| (distance of the
+ * closest S eigenvalue to the linear-dependency threshold). */
+int dc2_get_subspace_info(dc2_context* ctx, int64_t* info, double* margin);
+/* orbital energies after fockdiag (nmo doubles) and the rotation U (nmo x nmo complex) */
+int dc2_get_fock_rotation(dc2_context* ctx, double* eps, double* U);
+/* rotated CI vector of the last dc2_caspt2 (ndet complex) */
+int dc2_get_rotated_ci(dc2_context* ctx, double* c);
+/* density tensors of the rotated state, 0:<E_ab> (na^2) 1:<E_ab F> 2:<E_ab E_cd> (na^4)
+ * 3:<E_ab E_cd F>; layouts documented in dirac_caspt2_amd/csrc/dc2_pt2.hip */
+int dc2_get_density(dc2_context* ctx, int32_t which, double* out, int64_t count);
+/* a transformed subspace block of the last dc2_caspt2 (same names as above) */
+int dc2_get_transformed_block(dc2_context* ctx, const char* name, double* out, int64_t count);
+/* wall-clock seconds of the stages of the last dc2_casci/dc2_caspt2 call,
+ * measured with HIP events on the library stream:
+ * t[0] casci, t[1] fock+fockdiag+tracic, t[2] densities, t[3] transforms,
+ * t[4..11] subspaces A..H (assembly + solves, transforms excluded) */
+int dc2_get_timings(dc2_context* ctx, double* t, int32_t n);
+
+/* ---- stand-alone pieces --------------------------------------------------------- */
+/* CI matrix <i|H|j> (ndet x ndet complex, host) after dc2_casci (casmat.F90:3-222) */
+int dc2_casmat(dc2_context* ctx, double* mat);
+/* cdiag (diag.F90:66-128): Hermitian eigendecomposition of the host matrix c
+ * (n x n complex, upper triangle read) on the GPU; c is overwritten by the
+ * eigenvectors, w by the ascending eigenvalues, *dimm = count(w >= threshold)
+ * (or n when threshold == 0) */
+int dc2_cdiag(dc2_context* ctx, double* c, int32_t n, int32_t* dimm, double* w, double threshold);
+/* C = alpha op(A) op(B) + beta C on device memory, batched by strides;
+ * opA in 'N','T','C'; opB in 'N','R' (conjugate, no transpose),'T','C' */
+int dc2_zgemm(dc2_context* ctx, char opA, char opB, int64_t M, int64_t N, int64_t K, const double* alpha,
+              const double* A, int64_t lda, int64_t strideA, const double* B, int64_t ldb, int64_t strideB,
+              const double* beta, double* C, int64_t ldc, int64_t strideC, int64_t batch);
+/* wait for all work queued on the library stream */
+int dc2_synchronize(dc2_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCASPT2_AMD_H */

@@ -1,0 +1,95 @@
+"""Parity of the HIP CASCI/CASPT2 path with the oracle and the reference goldens.
+
+Tolerance: 1e-9 Eh on every per-subspace e2 and sumc2, on e0 and on the total
+energy (the plain-FP64 bar of the north star); CASCI energies 1e-10 Eh;
+determinant strings bit-exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import FIXTURES
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small():
+    from dirac_caspt2_amd import synth
+    from gpu_helpers import oracle_problem
+    from oracle import caspt2_ref as R
+    sp = synth.generate(synth.CONFIG_SMALL)
+    p = oracle_problem(sp)
+    cas = R.casci(p)
+    return sp, p, cas, R.run_caspt2(p, cas)
+
+
+def test_small_casci_and_dets(small):
+    from gpu_helpers import context_from_synth, say
+    sp, p, cas, ref = small
+    say("small synthetic: CASCI")
+    with context_from_synth(sp) as ctx:
+        eig = ctx.casci(5)
+        assert list(ctx.dets()) == cas[0]
+        np.testing.assert_allclose(eig, cas[3][:5], atol=1e-10, rtol=0)
+        from oracle import caspt2_ref as R
+        M = ctx.casmat()
+        np.testing.assert_allclose(M, R.casmat(p, cas[0]), atol=1e-12, rtol=0)
+
+
+def test_small_caspt2(small):
+    from gpu_helpers import assert_energies, context_from_synth, say
+    sp, p, cas, ref = small
+    say("small synthetic: CASPT2")
+    with context_from_synth(sp) as ctx:
+        ctx.casci(1)
+        out = ctx.caspt2(1, 0.0)
+        say("gpu", out.e2, out.total)
+        eps, U = ctx.fock_rotation()
+        np.testing.assert_allclose(eps, ref.eps, atol=1e-10)
+        assert_energies(out, ref, 1e-9, "small")
+
+
+def test_small_caspt2_eshift(small):
+    from gpu_helpers import assert_energies, context_from_synth, say
+    from oracle import caspt2_ref as R
+    sp, p, cas, _ = small
+    say("small synthetic: CASPT2 with eshift 0.1, root 2")
+    p2 = R.Problem(p.ninact, p.nact, p.nsec, p.nelec, p.h, p.eri, p.ecore, selectroot=2, eshift=0.1)
+    ref = R.run_caspt2(p2, cas)
+    with context_from_synth(sp) as ctx:
+        ctx.casci(2)
+        out = ctx.caspt2(2, 0.1)
+        assert_energies(out, ref, 1e-9, "eshift")
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_goldens(name, fixture_cache):
+    from gpu_helpers import context_from_problem, say
+    say("fixture", name)
+    fx = fixture_cache(name)
+    g = fx.gold
+    with context_from_problem(fx.problem) as ctx:
+        eig = ctx.casci(len(g["casci"]))
+        np.testing.assert_allclose(eig, g["casci"][:len(eig)], atol=1e-10, rtol=0)
+        out = ctx.caspt2(fx.problem.selectroot, fx.problem.eshift)
+        for k, X in enumerate("ABCDEFGH"):
+            assert abs(out.e2[X] - g["e2"][k]) < 1e-9, (X, out.e2[X], g["e2"][k])
+            assert abs(out.sumc2[X] - g["sumc2"][k]) < 1e-9, X
+        assert abs(out.total - g["total"]) < 1e-9
+
+
+def test_medium_caspt2():
+    """192 spinors (16/12/164, 6 electrons, 924 determinants)."""
+    from dirac_caspt2_amd import synth
+    from gpu_helpers import assert_energies, context_from_synth, oracle_problem, say
+    from oracle import caspt2_ref as R
+    sp = synth.generate(synth.CONFIG_FP64)
+    p = oracle_problem(sp)
+    say("medium: oracle")
+    ref = R.run_caspt2(p)
+    say("medium: gpu")
+    with context_from_synth(sp) as ctx:
+        ctx.casci(1)
+        out = ctx.caspt2(1, 0.0)
+        say("gpu", out.e2, out.total, "timings", out.timings)
+        assert_energies(out, ref, 1e-9, "medium")

@@ -1,0 +1,66 @@
+"""MFMA FP64 complex GEMM of the library against a plain numpy product (all op combinations)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from dirac_caspt2_amd.api import Caspt2Context
+    c = Caspt2Context(0)
+    yield c
+    c.close()
+
+
+def _op(x, op):
+    if op == "N":
+        return x
+    if op == "T":
+        return x.T
+    if op == "C":
+        return np.conj(x).T
+    if op == "R":
+        return np.conj(x)
+
+
+@pytest.mark.parametrize("opA", ["N", "T", "C"])
+@pytest.mark.parametrize("opB", ["N", "R", "T", "C"])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 97), (12, 200, 280)])
+def test_zgemm_ops(ctx, opA, opB, shape):
+    import torch
+    from dirac_caspt2_amd.api import zgemm
+    from gpu_helpers import say
+    M, N, K = shape
+    say(f"zgemm {opA}{opB} {shape}")
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    a_shape = (M, K) if opA == "N" else (K, M)
+    b_shape = (K, N) if opB in "NR" else (N, K)
+    A = rng.normal(size=a_shape) + 1j * rng.normal(size=a_shape)
+    B = rng.normal(size=b_shape) + 1j * rng.normal(size=b_shape)
+    C = rng.normal(size=(M, N)) + 1j * rng.normal(size=(M, N))
+    alpha, beta = 0.7 - 0.3j, -0.2 + 0.5j
+    ref = alpha * _op(A, opA) @ _op(B, opB) + beta * C
+    dev = lambda x: torch.from_numpy(np.asfortranarray(x).ravel(order="F")).cuda()
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    zgemm(ctx, opA, opB, M, N, K, alpha, dA, a_shape[0], 0, dB, b_shape[0], 0, beta, dC, M, 0, 1)
+    ctx.synchronize()
+    out = dC.cpu().numpy().reshape((M, N), order="F")
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-12 * max(1, K))
+
+
+def test_zgemm_batched(ctx):
+    import torch
+    from dirac_caspt2_amd.api import zgemm
+    rng = np.random.default_rng(3)
+    M, N, K, nb = 40, 24, 30, 5
+    A = rng.normal(size=(nb, K, M)) + 1j * rng.normal(size=(nb, K, M))
+    B = rng.normal(size=(nb, N, K)) + 1j * rng.normal(size=(nb, N, K))
+    dA = torch.from_numpy(A.copy()).cuda()       # batch-major; each (M,K) column-major == (K,M) C-order
+    dB = torch.from_numpy(B.copy()).cuda()
+    dC = torch.zeros(nb * M * N, dtype=torch.complex128, device="cuda")
+    zgemm(ctx, "N", "N", M, N, K, 1.0 + 0j, dA, M, M * K, dB, K, K * N, 0j, dC, M, M * N, nb)
+    ctx.synchronize()
+    out = dC.cpu().numpy().reshape(nb, N, M)
+    for b in range(nb):
+        np.testing.assert_allclose(out[b].T, A[b].T @ B[b].T, atol=1e-12)
