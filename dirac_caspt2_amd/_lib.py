@@ -44,6 +44,10 @@ _SIGS = {
     "dc2_zgemm": (c_int, [P, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, P, P, c_i64, c_i64, P, c_i64, c_i64,
                           P, P, c_i64, c_i64, c_i64]),
     "dc2_synchronize": (c_int, [P]),
+    "dc2_malloc": (c_int, [P, c_i64, ctypes.POINTER(P)]),
+    "dc2_free": (c_int, [P, P]),
+    "dc2_memcpy": (c_int, [P, P, P, c_i64, c_int]),
+    "dc2_set_integral_factors": (c_int, [P, P, c_i64, c_int]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -7,7 +7,7 @@ context: ``set_spaces`` / ``set_symmetry`` / ``set_ras`` / ``set_one_electron``
 / ``set_integral_block`` correspond to what read_mrconee, readint2_casci and
 divide_2_elec_integral_into_subspaces produce; ``casci()`` and ``caspt2()``
 run the two subprograms.  Arrays are column-major (Fortran order) complex128,
-given as numpy arrays (host) or torch tensors (device).
+given as numpy arrays (host) or DeviceArray buffers (device).
 """
 from __future__ import annotations
 
@@ -47,10 +47,45 @@ class Caspt2Energies:
 
 
 def _ptr(a) -> int:
-    """Data pointer of a numpy array or torch tensor."""
+    """Data pointer of a numpy array or a DeviceArray."""
     if isinstance(a, np.ndarray):
         return a.ctypes.data
+    if isinstance(a, DeviceArray):
+        return a.ptr
     return a.data_ptr()
+
+
+class DeviceArray:
+    """Device allocation made through the library's own HIP runtime (dc2_malloc)."""
+
+    def __init__(self, ctx: "Caspt2Context", host: Optional[np.ndarray] = None, nbytes: int = 0):
+        self.ctx = ctx
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            nbytes = host.nbytes
+        p = ctypes.c_void_p()
+        check("dc2_malloc", ctx.lib.dc2_malloc(ctx.h, int(nbytes), ctypes.byref(p)))
+        self.ptr = p.value or 0
+        self.nbytes = int(nbytes)
+        if host is not None and nbytes:
+            check("dc2_memcpy", ctx.lib.dc2_memcpy(ctx.h, self.ptr, host.ctypes.data, self.nbytes, 1))
+
+    def to_host(self, dtype=np.complex128) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        if self.nbytes:
+            check("dc2_memcpy", self.ctx.lib.dc2_memcpy(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes, 2))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            check("dc2_free", self.ctx.lib.dc2_free(self.ctx.h, self.ptr))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def _fortran_complex(a: np.ndarray) -> np.ndarray:
@@ -107,7 +142,7 @@ class Caspt2Context:
         check("dc2_set_one_electron", self.lib.dc2_set_one_electron(self.h, _ptr(hf), float(ecore)))
 
     def set_integral_block(self, name: str, data, borrow: bool = False):
-        """data: numpy array [i,j,k,l] (any order, copied from host) or a torch CUDA
+        """data: numpy array [i,j,k,l] (any order, copied from host) or a DeviceArray
         tensor whose memory is already column-major (i fastest)."""
         if isinstance(data, np.ndarray):
             arr = _fortran_complex(data)
@@ -117,8 +152,13 @@ class Caspt2Context:
             where = 2 if borrow else 1
             if borrow:
                 self._keep[name] = data
-        n = int(np.prod(arr.shape)) if isinstance(arr, np.ndarray) else int(arr.numel())
+        n = int(np.prod(arr.shape)) if isinstance(arr, np.ndarray) else arr.nbytes // 16
         check("dc2_set_integral_block", self.lib.dc2_set_integral_block(self.h, name.encode(), _ptr(arr), n, where))
+
+    def set_integral_factors(self, B: np.ndarray):
+        """(pq|rs) = sum_P B[P,p,q] B[P,r,s]: every block is formed on the device."""
+        Bf = np.asfortranarray(np.asarray(B, dtype=np.complex128))
+        check("dc2_set_integral_factors", self.lib.dc2_set_integral_factors(self.h, _ptr(Bf), Bf.shape[0], 0))
 
     # --- CASCI ----------------------------------------------------------------
     def casci(self, nroot: int = 1):
@@ -205,7 +245,7 @@ def cdiag(ctx: Caspt2Context, c: np.ndarray, threshold: float = 0.0):
 
 def zgemm(ctx: Caspt2Context, opA: str, opB: str, M: int, N: int, K: int, alpha: complex, A, lda: int, sA: int,
           B, ldb: int, sB: int, beta: complex, C, ldc: int, sC: int, batch: int = 1):
-    """C = alpha op(A) op(B) + beta C on device memory (torch tensors)."""
+    """C = alpha op(A) op(B) + beta C on device memory (DeviceArray buffers)."""
     al = np.array([alpha.real, alpha.imag])
     be = np.array([beta.real, beta.imag])
     check("dc2_zgemm", ctx.lib.dc2_zgemm(ctx.h, opA.encode(), opB.encode(), M, N, K, _ptr(al), _ptr(A), lda, sA,

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -74,6 +75,7 @@ struct dc2_context {
   DevBuf<cplx> w1, w2, w3, w4;
   DevBuf<double> dw1, dw2;
   DevBuf<int> info;
+  bool debug = false;
 
   int nmo() const { return ni + na + ns; }
   int nmoc() const { return ni + na; }
@@ -157,15 +159,59 @@ const cplx* get_block(dc2_context* c, const std::string& name) {
   throw Error(1, "integral block " + name + " has not been set");
 }
 
-void zheevd(dc2_context* c, int n, cplx* A, double* w) {
+// diagnostics of a matrix handed to an eigensolver that failed
+std::string matrix_report(dc2_context* c, int n, const cplx* A) {
+  std::vector<cplx> h((size_t)n * n);
+  (void)hipMemcpy(h.data(), A, h.size() * sizeof(cplx), hipMemcpyDeviceToHost);
+  int64_t nonfinite = 0;
+  double herm = 0.0, amax = 0.0, dimag = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const cplx a = h[i + (size_t)n * j], b = h[j + (size_t)n * i];
+      if (!std::isfinite(a.x) || !std::isfinite(a.y)) ++nonfinite;
+      amax = std::max(amax, std::hypot(a.x, a.y));
+      herm = std::max(herm, std::hypot(a.x - b.x, a.y + b.y));
+      if (i == j) dimag = std::max(dimag, std::fabs(a.y));
+    }
+  return " (n=" + std::to_string(n) + ", non-finite=" + std::to_string(nonfinite) + ", max|a|=" + std::to_string(amax) +
+         ", max|a-a^H|=" + std::to_string(herm) + ", max|Im diag|=" + std::to_string(dimag) + ")";
+}
+
+// debug mode (DC2_DEBUG=1): fail at the first device array holding a non-finite value
+template <class T>
+void check_finite(dc2_context* c, const char* what, const T* p, int64_t count) {
+  if (!c->debug || count <= 0 || p == nullptr) return;
+  std::vector<T> h(count);
+  DC2_HIP(hipMemcpyAsync(h.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  DC2_HIP(hipStreamSynchronize(c->stream));
+  const double* d = reinterpret_cast<const double*>(h.data());
+  const int64_t nd = count * (int64_t)(sizeof(T) / sizeof(double));
+  for (int64_t k = 0; k < nd; ++k)
+    if (!std::isfinite(d[k]))
+      throw Error(4, std::string("non-finite value in ") + what + " at double index " + std::to_string(k) + " of " +
+                         std::to_string(nd));
+}
+
+void zheevd(dc2_context* c, int n, cplx* A, double* w, const char* what = "") {
   c->dw2.ensure(std::max(n, 1));
   c->info.ensure(1);
-  DC2_ROCSOLVER(rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n,
-                                 reinterpret_cast<rocblas_double_complex*>(A), n, w, c->dw2.get(), c->info.get()));
+  DevBuf<cplx> keep;
+  if (c->debug) {
+    keep.alloc((int64_t)n * n);
+    DC2_HIP(hipMemcpyAsync(keep.get(), A, (int64_t)n * n * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+  }
+  // Tridiagonal QR (the ZHEEV algorithm of diag.F90:109).  The divide-and-conquer
+  // variant returned non-finite eigenvectors with info = 0 for the rank-deficient
+  // S matrices of the A/C subspaces (hundreds of zero eigenvalues).
+  DC2_ROCSOLVER(rocsolver_zheev(c->blas, rocblas_evect_original, rocblas_fill_upper, n,
+                                reinterpret_cast<rocblas_double_complex*>(A), n, w, c->dw2.get(), c->info.get()));
   int info = 0;
   DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, c->stream));
   DC2_HIP(hipStreamSynchronize(c->stream));
-  if (info != 0) throw Error(100 + info, "zheevd failed, info = " + std::to_string(info));
+  check_finite(c, what, A, (int64_t)n * n);
+  if (info != 0)
+    throw Error(100 + info, std::string("zheev failed for ") + what + ", info = " + std::to_string(info) +
+                                (c->debug ? matrix_report(c, n, keep.get()) : std::string(" (n=") + std::to_string(n) + ")"));
 }
 
 struct Timer {
@@ -238,7 +284,7 @@ void run_casci(dc2_context* c) {
   casmat(s, c->ni, na, c->ne, nd, c->cas_dets.get(), c->rank_cas.get(), O, c->nmoc(), c->h.get(), c->nmo(),
          c->civec.get());
   DevBuf<double> w(nd);
-  zheevd(c, (int)nd, c->civec.get(), w.get());
+  zheevd(c, (int)nd, c->civec.get(), w.get(), "the CASCI matrix");
   c->ecas.resize(nd);
   DC2_HIP(hipMemcpyAsync(c->ecas.data(), w.get(), nd * sizeof(double), hipMemcpyDeviceToHost, s));
   DC2_HIP(hipStreamSynchronize(s));
@@ -284,7 +330,7 @@ SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B,
   if (n == 0) return r;
   hipStream_t s = c->stream;
   DevBuf<double> ws(n);
-  zheevd(c, (int)n, S, ws.get());
+  zheevd(c, (int)n, S, ws.get(), "an S matrix");
   std::vector<double> w(n);
   DC2_HIP(hipMemcpyAsync(w.data(), ws.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
   DC2_HIP(hipStreamSynchronize(s));
@@ -299,10 +345,15 @@ SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B,
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   DevBuf<cplx> uc(n * dimm), T1(dimm * n), bc1(dimm * dimm), Y(dimm * npert), X(dimm * npert);
   DevBuf<double> wb(dimm);
+  check_finite(c, "S eigenvectors", S, n * n);
+  check_finite(c, "S eigenvalues", ws.get(), n);
   scale_cols_tail(s, n, dimm, S, ws.get(), uc.get());
+  check_finite(c, "U L^-1/2", uc.get(), n * dimm);
   zgemm(s, 'C', 'N', dimm, n, n, one, uc.get(), n, 0, B, n, 0, zero, T1.get(), dimm, 0, 1);
+  check_finite(c, "(U L^-1/2)^H B", T1.get(), n * dimm);
   zgemm(s, 'N', 'N', dimm, dimm, n, one, T1.get(), dimm, 0, uc.get(), n, 0, zero, bc1.get(), dimm, 0, 1);
-  zheevd(c, (int)dimm, bc1.get(), wb.get());
+  check_finite(c, "B~", bc1.get(), dimm * dimm);
+  zheevd(c, (int)dimm, bc1.get(), wb.get(), "a B~ matrix");
   zgemm(s, 'C', 'N', dimm, npert, n, one, uc.get(), n, 0, V, n, 0, zero, Y.get(), dimm, 0, 1);
   zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1.get(), dimm, 0, Y.get(), dimm, 0, zero, X.get(), dimm, 0, 1);
   const int64_t np = e2_partials_needed(dimm * npert);
@@ -345,11 +396,13 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   embed_ci(s, na, nf, nd, c->cas2full.get(), c0, c->full_dets.get(), zeros_act.get(), c->c_full.get(), c->cF.get());
   apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
   zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
+  check_finite(c, "the CASCI 1-RDM", c->G1.get(), na2);
   // ---- Fock matrix (fockcasci.F90) and its diagonalisation per space and irrep (fockdiag.F90)
   c->fock.ensure((int64_t)nmo * nmo);
   const cplx* F1 = ns ? get_block(c, "F1") : nullptr;
   const cplx* F2 = ns ? get_block(c, "F2") : nullptr;
   fock_build(s, ni, na, ns, c->h.get(), nmo, get_block(c, "OOOO"), F1, F2, c->G1.get(), c->fock.get());
+  check_finite(c, "the Fock matrix", c->fock.get(), (int64_t)nmo * nmo);
   c->U.ensure((int64_t)nmo * nmo);
   fill_zero(s, c->U.get(), (int64_t)nmo * nmo);
   for (int sp = 0; sp < 3; ++sp) {
@@ -365,7 +418,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       DevBuf<cplx> blk((int64_t)n * n), scr((int64_t)n * n);
       DevBuf<double> w(n);
       gather_block(s, n, dind.get(), c->fock.get(), nmo, blk.get());
-      zheevd(c, n, blk.get(), w.get());
+      zheevd(c, n, blk.get(), w.get(), "a Fock block");
       canonical_gauge(s, n, w.get(), blk.get(), scr.get());
       scatter_block(s, n, dind.get(), blk.get(), w.get(), c->U.get(), nmo, c->eps.get());
       DC2_HIP(hipStreamSynchronize(s));
@@ -394,6 +447,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     DC2_HIP(hipStreamSynchronize(s));   // ds / ipiv are released at the end of this scope
   }
   c->timings[1] = t1.stop();
+  check_finite(c, "U", c->U.get(), (int64_t)nmo * nmo);
+  check_finite(c, "eps", c->eps.get(), nmo);
+  check_finite(c, "the rotated CI vector", c->cnew.get(), nd);
 
   // ---- densities of the rotated state (density_matrix.F90)
   Timer t2(s);
@@ -434,6 +490,16 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     DC2_HIP(hipStreamSynchronize(s));
   }
   c->timings[2] = t2.stop();
+  check_finite(c, "e0", c->e0.get(), 1);
+  check_finite(c, "G1", c->G1.get(), na2);
+  check_finite(c, "G1F", c->G1F.get(), na2);
+  check_finite(c, "G2", c->G2.get(), na2 * na2);
+  check_finite(c, "G2F", c->G2F.get(), na2 * na2);
+  if (doA || doC) {
+    check_finite(c, "G3", c->G3.get(), na2 * na2 * na2);
+    check_finite(c, "G3F", c->G3F.get(), na2 * na2 * na2);
+  }
+  check_finite(c, "h'", c->hp.get(), (int64_t)nmo * nmo);
 
   // ---- transforms of every needed subspace block (module_intra.F90)
   Timer t3(s);
@@ -448,6 +514,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   if (doH) need.push_back("H");
   for (const auto& nm : need) transform(c, nm);
   c->timings[3] = t3.stop();
+  for (const auto& nm : need) check_finite(c, ("transformed " + nm).c_str(), c->tblk[nm].get(), block_count(c, shape_of(nm)));
 
   upload_ints(s, c->tpairs, pair_list(na));
   upload_ints(s, c->ipairs, pair_list(ni));
@@ -526,6 +593,11 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
         alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert,
                alpha.get());
       }
+      const std::string tag(1, X);
+      check_finite(c, ("S" + tag).c_str(), S.get(), n * n);
+      check_finite(c, ("B" + tag).c_str(), B.get(), n * n);
+      check_finite(c, ("V" + tag).c_str(), V.get(), n * npert);
+      check_finite(c, ("alpha" + tag).c_str(), alpha.get(), npert);
       r = solve_block(c, n, npert, S.get(), B.get(), V.get(), alpha.get());
     }
     e2[k] = r.e2;
@@ -564,6 +636,8 @@ int dc2_context_create(int32_t device, dc2_context** out) {
     DC2_CHECK(out != nullptr, 1, "null output pointer");
     auto c = std::make_unique<dc2_context>();
     c->device = device;
+    const char* dbg = std::getenv("DC2_DEBUG");
+    c->debug = dbg && dbg[0] && dbg[0] != '0';
     DC2_HIP(hipSetDevice(device));
     DC2_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     DC2_ROCSOLVER(rocblas_create_handle(&c->blas));
@@ -809,6 +883,80 @@ int dc2_zgemm(dc2_context* c, char opA, char opB, int64_t M, int64_t N, int64_t 
     zgemm(c->stream, opA, opB, M, N, K, mk(alpha[0], alpha[1]), reinterpret_cast<const cplx*>(A), lda, sA,
           reinterpret_cast<const cplx*>(B), ldb, sB, mk(beta[0], beta[1]), reinterpret_cast<cplx*>(C), ldc, sC,
           batch);
+  });
+}
+
+int dc2_malloc(dc2_context* c, int64_t bytes, void** ptr) {
+  return guarded([&] {
+    DC2_CHECK(c && ptr && bytes >= 0, 1, "invalid argument");
+    DC2_HIP(hipSetDevice(c->device));
+    *ptr = nullptr;
+    if (bytes) DC2_HIP(hipMalloc(ptr, bytes));
+  });
+}
+
+int dc2_free(dc2_context* c, void* ptr) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    if (ptr) DC2_HIP(hipFree(ptr));
+  });
+}
+
+int dc2_memcpy(dc2_context* c, void* dst, const void* src, int64_t bytes, int32_t kind) {
+  return guarded([&] {
+    DC2_CHECK(c && (bytes == 0 || (dst && src)), 1, "invalid argument");
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    DC2_CHECK(kind >= 1 && kind <= 3, 1, "memcpy kind must be 1, 2 or 3");
+    if (bytes) DC2_HIP(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    DC2_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int dc2_set_integral_factors(dc2_context* c, const double* B, int64_t naux, int32_t where) {
+  return guarded([&] {
+    DC2_CHECK(c && B && naux > 0, 1, "invalid argument");
+    need_spaces(c);
+    hipStream_t s = c->stream;
+    const int64_t nmo = c->nmo();
+    DevBuf<cplx> Bd;
+    const cplx* Bp = reinterpret_cast<const cplx*>(B);
+    if (where == 0) {
+      Bd.alloc(naux * nmo * nmo);
+      DC2_HIP(hipMemcpyAsync(Bd.get(), B, naux * nmo * nmo * sizeof(cplx), hipMemcpyHostToDevice, s));
+      Bp = Bd.get();
+    }
+    const bool has_s = c->ns > 0, has_i = c->ni > 0;
+    std::vector<std::string> names = {"OOOO"};
+    if (has_s) {
+      for (const char* n : {"F1", "F2", "C1", "F"}) names.push_back(n);
+      if (has_i) for (const char* n : {"C2", "C3", "D1", "D2", "D3", "E", "G", "H"}) names.push_back(n);
+    }
+    const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+    for (const auto& nm : names) {
+      const char* sp = shape_of(nm);
+      int64_t n[4];
+      for (int k = 0; k < 4; ++k) n[k] = space_dim(c, sp[k]);
+      const int64_t m1 = n[0] * n[1], m2 = n[2] * n[3];
+      DevBuf<cplx> L(naux * m1), R(naux * m2);
+      int64_t ld[3] = {naux, nmo, nmo};
+      int offL[4] = {0, space_off(c, sp[0]), space_off(c, sp[1]), 0}, nL[4] = {(int)naux, (int)n[0], (int)n[1], 1};
+      int offR[4] = {0, space_off(c, sp[2]), space_off(c, sp[3]), 0}, nR[4] = {(int)naux, (int)n[2], (int)n[3], 1};
+      extract4(s, Bp, ld, offL, nL, L.get());
+      extract4(s, Bp, ld, offR, nR, R.get());
+      Block& b = c->blocks[nm];
+      b.derived = false;
+      b.count = m1 * m2;
+      b.own.alloc(b.count);
+      b.ptr = b.own.get();
+      zgemm(s, 'T', 'N', m1, m2, naux, one, L.get(), naux, 0, R.get(), naux, 0, zero, b.own.get(), m1, 0, 1);
+      DC2_HIP(hipStreamSynchronize(s));
+    }
+    for (const char* k : {"A1", "A2", "B"}) {
+      auto it = c->blocks.find(k);
+      if (it != c->blocks.end() && it->second.derived) c->blocks.erase(it);
+    }
+    c->have_casci = c->have_pt2 = false;
   });
 }
 

@@ -2,26 +2,15 @@
 
 Integral blocks are built in the column-major layout the C ABI expects.  For
 the synthetic workloads the blocks (pq|rs) = sum_P B^P_pq B^P_rs are formed on
-the GPU with torch (input preparation, outside any timed region) and handed
-over as device pointers.
+the GPU by the library itself (input preparation, outside any timed region).
 """
 from __future__ import annotations
 
-from typing import Callable, Optional
+from typing import Callable
 
 import numpy as np
 
 from .api import BLOCK_SHAPES, Caspt2Context, needed_blocks, space_slices
-
-
-def block_from_factors_torch(Bt, sl):
-    """Column-major (i fastest) device tensor of block (s1 s2|s3 s4) from B[P,p,q] (torch, complex128)."""
-    L = Bt[:, sl[0], sl[1]]
-    R = Bt[:, sl[2], sl[3]]
-    naux = Bt.shape[0]
-    Lm = L.permute(0, 2, 1).reshape(naux, -1)                 # (P, n2*n1): column j*n1 + i
-    Rm = R.permute(2, 1, 0).reshape(-1, naux)                 # (n4*n3, P): row l*n3 + k
-    return (Rm @ Lm).contiguous()                              # memory = Fortran order of (i,j,k,l)
 
 
 def block_from_factors_numpy(B: np.ndarray, sl) -> np.ndarray:
@@ -32,21 +21,14 @@ def block_from_factors_numpy(B: np.ndarray, sl) -> np.ndarray:
 
 
 def load_factors(ctx: Caspt2Context, dims, h: np.ndarray, B: np.ndarray, ecore: float,
-                 device: Optional[str] = "cuda", Bt=None):
-    """dims = (ninact, nact, nsec, nelec)."""
+                 on_device: bool = True):
+    """dims = (ninact, nact, nsec, nelec).  on_device: form the blocks with the
+    library's GEMM (dc2_set_integral_factors); otherwise build them with numpy."""
     ctx.set_spaces(*dims)
     ctx.set_one_electron(h, ecore)
     sls = space_slices(dims)
-    if device is not None:
-        import torch
-        if Bt is None:
-            Bt = torch.from_numpy(np.ascontiguousarray(B)).to(device)
-        for name in needed_blocks(dims):
-            sl = [sls[ch] for ch in BLOCK_SHAPES[name]]
-            blk = block_from_factors_torch(Bt, sl)
-            ctx.set_integral_block(name, blk)          # device-to-device copy into the context
-            del blk
-        torch.cuda.synchronize()
+    if on_device:
+        ctx.set_integral_factors(B)
     else:
         for name in needed_blocks(dims):
             sl = [sls[ch] for ch in BLOCK_SHAPES[name]]

@@ -127,6 +127,18 @@ int dc2_zgemm(dc2_context* ctx, char opA, char opB, int64_t M, int64_t N, int64_
 /* wait for all work queued on the library stream */
 int dc2_synchronize(dc2_context* ctx);
 
+/* ---- device memory for hosts without their own GPU runtime (e.g. Fortran) ----------
+ * kind: 1 host->device, 2 device->host, 3 device->device (synchronous) */
+int dc2_malloc(dc2_context* ctx, int64_t bytes, void** ptr);
+int dc2_free(dc2_context* ctx, void* ptr);
+int dc2_memcpy(dc2_context* ctx, void* dst, const void* src, int64_t bytes, int32_t kind);
+/* Input preparation for factorised integrals (pq|rs) = sum_P B[P,p,q] B[P,r,s]:
+ * B is (naux, nmo, nmo) complex, column-major (P fastest); every block the path
+ * needs is formed on the device (no counterpart in the reference, which reads
+ * the four-index MDCINT files; this feeds the same blocks without a host round
+ * trip).  where: 0 host, 1 device. */
+int dc2_set_integral_factors(dc2_context* ctx, const double* B, int64_t naux, int32_t where);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,7 @@ def context_from_problem(p: R.Problem, device: int = 0) -> Caspt2Context:
 def context_from_synth(sp, device: int = 0, on_gpu: bool = True) -> Caspt2Context:
     s = sp.spec
     ctx = Caspt2Context(device)
-    load_factors(ctx, (s.ninact, s.nact, s.nsec, s.nelec), sp.h, sp.B, sp.ecore, device="cuda" if on_gpu else None)
+    load_factors(ctx, (s.ninact, s.nact, s.nsec, s.nelec), sp.h, sp.B, sp.ecore, on_device=on_gpu)
     return ctx
 
 

@@ -28,8 +28,7 @@ def _op(x, op):
 @pytest.mark.parametrize("opB", ["N", "R", "T", "C"])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 97), (12, 200, 280)])
 def test_zgemm_ops(ctx, opA, opB, shape):
-    import torch
-    from dirac_caspt2_amd.api import zgemm
+    from dirac_caspt2_amd.api import DeviceArray, zgemm
     from gpu_helpers import say
     M, N, K = shape
     say(f"zgemm {opA}{opB} {shape}")
@@ -41,26 +40,25 @@ def test_zgemm_ops(ctx, opA, opB, shape):
     C = rng.normal(size=(M, N)) + 1j * rng.normal(size=(M, N))
     alpha, beta = 0.7 - 0.3j, -0.2 + 0.5j
     ref = alpha * _op(A, opA) @ _op(B, opB) + beta * C
-    dev = lambda x: torch.from_numpy(np.asfortranarray(x).ravel(order="F")).cuda()
+    dev = lambda x: DeviceArray(ctx, np.asfortranarray(x).ravel(order="F"))
     dA, dB, dC = dev(A), dev(B), dev(C)
     zgemm(ctx, opA, opB, M, N, K, alpha, dA, a_shape[0], 0, dB, b_shape[0], 0, beta, dC, M, 0, 1)
     ctx.synchronize()
-    out = dC.cpu().numpy().reshape((M, N), order="F")
+    out = dC.to_host().reshape((M, N), order="F")
     np.testing.assert_allclose(out, ref, rtol=0, atol=1e-12 * max(1, K))
 
 
 def test_zgemm_batched(ctx):
-    import torch
-    from dirac_caspt2_amd.api import zgemm
+    from dirac_caspt2_amd.api import DeviceArray, zgemm
     rng = np.random.default_rng(3)
     M, N, K, nb = 40, 24, 30, 5
     A = rng.normal(size=(nb, K, M)) + 1j * rng.normal(size=(nb, K, M))
     B = rng.normal(size=(nb, N, K)) + 1j * rng.normal(size=(nb, N, K))
-    dA = torch.from_numpy(A.copy()).cuda()       # batch-major; each (M,K) column-major == (K,M) C-order
-    dB = torch.from_numpy(B.copy()).cuda()
-    dC = torch.zeros(nb * M * N, dtype=torch.complex128, device="cuda")
+    dA = DeviceArray(ctx, A)        # batch-major; each (M,K) column-major == (K,M) C-order
+    dB = DeviceArray(ctx, B)
+    dC = DeviceArray(ctx, np.zeros(nb * M * N, dtype=np.complex128))
     zgemm(ctx, "N", "N", M, N, K, 1.0 + 0j, dA, M, M * K, dB, K, K * N, 0j, dC, M, M * N, nb)
     ctx.synchronize()
-    out = dC.cpu().numpy().reshape(nb, N, M)
+    out = dC.to_host().reshape(nb, N, M)
     for b in range(nb):
         np.testing.assert_allclose(out[b].T, A[b].T @ B[b].T, atol=1e-12)
