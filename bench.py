@@ -1,0 +1,146 @@
+"""Benchmark: CASPT2 energies/s (+ p50/p99 latency) of the plain-FP64 GPU path.
+
+One step = one complete CASPT2 energy (CASCI: determinant strings, CI matrix,
+eigensolve; CASPT2: Fock build and diagonalisation, CI rotation, densities,
+integral transforms, subspaces A-H) for the 192-spinor workload
+(16 inactive / 12 active with 6 electrons (924 determinants) / 164 secondary,
+complex128, seeded synthetic Kramers-paired integrals), with the integrals
+already resident in HBM.  Only one GPU is used.
+
+    python bench.py [--steps K] [--warmup W] [--workload fp64|small|split]
+                    [--no-cpu-baseline] [--verify]
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X dense FP64 (matrix and vector)
+PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
+
+
+def workload_spec(name):
+    from dirac_caspt2_amd import synth
+    return {"small": synth.CONFIG_SMALL, "fp64": synth.CONFIG_FP64, "split": synth.CONFIG_SPLIT}[name]
+
+
+def cpu_baseline(sp, steps=1):
+    """The numpy oracle on the same workload, timed on this host (bounded sample: one energy)."""
+    from oracle import caspt2_ref as R
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max((i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"), default=1)
+    except Exception:
+        cores = os.cpu_count() or 1
+    s = sp.spec
+    p = R.Problem(s.ninact, s.nact, s.nsec, s.nelec, sp.h, R.FactorIntegrals(sp.B), sp.ecore, s.selectroot,
+                  eshift=s.eshift)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = R.run_caspt2(p)
+    dt = (time.perf_counter() - t0) / steps
+    return res, {"value": 1.0 / dt, "unit": "energies/s", "cores": int(cores), "kind": "port",
+                 "sample": f"{steps} full CASPT2 energy of the same workload with the numpy oracle "
+                           f"({dt:.2f} s per energy, BLAS threads = {cores})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="fp64", choices=["small", "fp64", "split"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verify", action="store_true", help="compare every energy with the oracle's")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the sharding design is documented in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from dirac_caspt2_amd import synth
+    from dirac_caspt2_amd.api import Caspt2Context
+    from dirac_caspt2_amd.problem import load_factors
+
+    spec = workload_spec(args.workload)
+    sp = synth.generate(spec)
+    ctx = Caspt2Context(0)
+    load_factors(ctx, (spec.ninact, spec.nact, spec.nsec, spec.nelec), sp.h, sp.B, sp.ecore, on_device=True)
+
+    def step():
+        ctx.casci(spec.selectroot)
+        return ctx.caspt2(spec.selectroot, spec.eshift)
+
+    for _ in range(args.warmup):
+        out = step()
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        out = step()            # synchronous: returns after the energies are on the host
+        lat.append(time.perf_counter() - t)
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    g_all = ctx.gemm_stats("all")
+    g_tr = ctx.gemm_stats("transform")
+    ctx.set_profiling(False)
+    stage = out.timings
+    lat_ms = np.array(lat) * 1e3
+
+    # roofline of the MFMA GEMM family (all hand-written complex FP64 GEMM launches of the
+    # timed steps): algorithmic flops 8*M*N*K per launch over HIP-event durations
+    achieved = g_all["flops"] / g_all["seconds"] / 1e12 if g_all["seconds"] > 0 else 0.0
+    ach_tr = g_tr["flops"] / g_tr["seconds"] / 1e12 if g_tr["seconds"] > 0 else 0.0
+    result = {
+        "metric": "CASPT2 energies/s (plain FP64 GPU path)",
+        "value": args.steps / total,
+        "unit": "energies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (complex128)",
+        "data": "synthetic (seeded Kramers-paired integrals, in HBM before timing)",
+        "config": {"workload": f"{args.workload}: ninact {spec.ninact}, nact {spec.nact}, nsec {spec.nsec}, "
+                               f"nelec {spec.nelec}, seed {spec.seed}, selectroot {spec.selectroot}",
+                   "global_batch": 1, "parallelism": "single GPU"},
+        "stage_seconds_last_step": {"casci": stage[0], "fock_fockdiag_tracic": stage[1], "densities": stage[2],
+                                    "transforms": stage[3],
+                                    **{f"subspace_{X}": stage[4 + k] for k, X in enumerate("ABCDEFGH")}},
+        "energies_last_step": {"total": out.total, "e2": out.e2},
+        "roofline": {"kernel": "k_zgemm (MFMA FP64 complex GEMM, all launches)", "bound": "mfma",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "launches_per_step": g_all["launches"] / args.steps,
+                     "algorithmic_flops_per_step": g_all["flops"] / args.steps,
+                     "transform_launches_achieved_tflops": ach_tr},
+        "eig_fallbacks": ctx.eig_fallbacks(),
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        ref, cb = cpu_baseline(sp)
+        result["cpu_baseline"] = cb
+        result["parity_vs_oracle_max_abs_e2"] = max(abs(out.e2[X] - ref.e2[X]) for X in "ABCDEFGH")
+        result["parity_vs_oracle_total"] = abs(out.total - ref.total)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

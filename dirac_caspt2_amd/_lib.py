@@ -48,6 +48,9 @@ _SIGS = {
     "dc2_free": (c_int, [P, P]),
     "dc2_memcpy": (c_int, [P, P, P, c_i64, c_int]),
     "dc2_set_integral_factors": (c_int, [P, P, c_i64, c_int]),
+    "dc2_set_profiling": (c_int, [P, c_int]),
+    "dc2_get_gemm_stats": (c_int, [P, c_int, P]),
+    "dc2_get_eig_fallbacks": (c_int, [P, P]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -232,6 +232,21 @@ class Caspt2Context:
     def synchronize(self):
         check("dc2_synchronize", self.lib.dc2_synchronize(self.h))
 
+    def set_profiling(self, on: bool):
+        check("dc2_set_profiling", self.lib.dc2_set_profiling(self.h, 1 if on else 0))
+
+    GEMM_TAGS = {"all": -1, "other": 0, "transform": 1, "density": 2, "vbuild": 3, "projection": 4, "factors": 5}
+
+    def gemm_stats(self, tag: str = "all"):
+        out = np.zeros(4)
+        check("dc2_get_gemm_stats", self.lib.dc2_get_gemm_stats(self.h, self.GEMM_TAGS[tag], _ptr(out)))
+        return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
+
+    def eig_fallbacks(self) -> int:
+        n = ctypes.c_int64()
+        check("dc2_get_eig_fallbacks", self.lib.dc2_get_eig_fallbacks(self.h, ctypes.byref(n)))
+        return n.value
+
 
 def cdiag(ctx: Caspt2Context, c: np.ndarray, threshold: float = 0.0):
     """diag.F90:66-128 on the GPU: returns (w, eigenvectors, dimm)."""

@@ -76,6 +76,9 @@ struct dc2_context {
   DevBuf<double> dw1, dw2;
   DevBuf<int> info;
   bool debug = false;
+  int eig_method = 2;        // 0 QR (zheev), 1 divide and conquer, 2 D&C + Jacobi, 3 Jacobi
+  int64_t eig_fallbacks = 0;
+  GemmStats gstats;        // GEMM launch accounting (dc2_set_profiling)
 
   int nmo() const { return ni + na + ns; }
   int nmoc() const { return ni + na; }
@@ -192,25 +195,60 @@ void check_finite(dc2_context* c, const char* what, const T* p, int64_t count) {
                          std::to_string(nd));
 }
 
-void zheevd(dc2_context* c, int n, cplx* A, double* w, const char* what = "") {
+bool all_finite(dc2_context* c, const cplx* A, int64_t count) {
+  std::vector<double> h(2 * count);
+  DC2_HIP(hipMemcpyAsync(h.data(), A, count * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
+  DC2_HIP(hipStreamSynchronize(c->stream));
+  for (double v : h)
+    if (!std::isfinite(v)) return false;
+  return true;
+}
+
+int run_eig(dc2_context* c, int method, int n, cplx* A, double* w) {
+  auto* a = reinterpret_cast<rocblas_double_complex*>(A);
   c->dw2.ensure(std::max(n, 1));
-  c->info.ensure(1);
-  DevBuf<cplx> keep;
-  if (c->debug) {
-    keep.alloc((int64_t)n * n);
-    DC2_HIP(hipMemcpyAsync(keep.get(), A, (int64_t)n * n * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+  c->info.ensure(2);
+  switch (method) {
+    case 1:
+      DC2_ROCSOLVER(rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, c->dw2.get(),
+                                     c->info.get()));
+      break;
+    case 2:
+      DC2_ROCSOLVER(rocsolver_zheevdj(c->blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, c->info.get()));
+      break;
+    case 3:
+      DC2_ROCSOLVER(rocsolver_zheevj(c->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n, a,
+                                     n, 0.0, c->dw2.get(), 100, c->info.get() + 1, w, c->info.get()));
+      break;
+    default:
+      // Tridiagonal QR, the ZHEEV algorithm of diag.F90:109
+      DC2_ROCSOLVER(rocsolver_zheev(c->blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, c->dw2.get(),
+                                    c->info.get()));
   }
-  // Tridiagonal QR (the ZHEEV algorithm of diag.F90:109).  The divide-and-conquer
-  // variant returned non-finite eigenvectors with info = 0 for the rank-deficient
-  // S matrices of the A/C subspaces (hundreds of zero eigenvalues).
-  DC2_ROCSOLVER(rocsolver_zheev(c->blas, rocblas_evect_original, rocblas_fill_upper, n,
-                                reinterpret_cast<rocblas_double_complex*>(A), n, w, c->dw2.get(), c->info.get()));
   int info = 0;
   DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, c->stream));
   DC2_HIP(hipStreamSynchronize(c->stream));
-  check_finite(c, what, A, (int64_t)n * n);
+  return info;
+}
+
+// Hermitian eigendecomposition (upper triangle), ascending eigenvalues.  The
+// configured rocSOLVER variant runs first; a failure or a non-finite result
+// (the divide-and-conquer variants returned NaN eigenvectors with info = 0 for
+// rank-deficient S matrices with hundreds of zero eigenvalues) falls back to the
+// tridiagonal QR algorithm on a saved copy.
+void zheevd(dc2_context* c, int n, cplx* A, double* w, const char* what = "") {
+  const int64_t nn = (int64_t)n * n;
+  DevBuf<cplx> keep(nn);
+  DC2_HIP(hipMemcpyAsync(keep.get(), A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+  int info = run_eig(c, c->eig_method, n, A, w);
+  if (c->eig_method != 0 && (info != 0 || !all_finite(c, A, nn))) {
+    ++c->eig_fallbacks;
+    DC2_HIP(hipMemcpyAsync(A, keep.get(), nn * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+    info = run_eig(c, 0, n, A, w);
+  }
+  check_finite(c, what, A, nn);
   if (info != 0)
-    throw Error(100 + info, std::string("zheev failed for ") + what + ", info = " + std::to_string(info) +
+    throw Error(100 + info, std::string("Hermitian eigensolver failed for ") + what + ", info = " + std::to_string(info) +
                                 (c->debug ? matrix_report(c, n, keep.get()) : std::string(" (n=") + std::to_string(n) + ")"));
 }
 
@@ -348,6 +386,7 @@ SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B,
   check_finite(c, "S eigenvectors", S, n * n);
   check_finite(c, "S eigenvalues", ws.get(), n);
   scale_cols_tail(s, n, dimm, S, ws.get(), uc.get());
+  c->gstats.tag = 4;
   check_finite(c, "U L^-1/2", uc.get(), n * dimm);
   zgemm(s, 'C', 'N', dimm, n, n, one, uc.get(), n, 0, B, n, 0, zero, T1.get(), dimm, 0, 1);
   check_finite(c, "(U L^-1/2)^H B", T1.get(), n * dimm);
@@ -356,6 +395,7 @@ SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B,
   zheevd(c, (int)dimm, bc1.get(), wb.get(), "a B~ matrix");
   zgemm(s, 'C', 'N', dimm, npert, n, one, uc.get(), n, 0, V, n, 0, zero, Y.get(), dimm, 0, 1);
   zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1.get(), dimm, 0, Y.get(), dimm, 0, zero, X.get(), dimm, 0, 1);
+  c->gstats.tag = 0;
   const int64_t np = e2_partials_needed(dimm * npert);
   c->partial.ensure(2 * np);
   c->out2.ensure(2);
@@ -453,6 +493,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
 
   // ---- densities of the rotated state (density_matrix.F90)
   Timer t2(s);
+  c->gstats.tag = 2;
   const double* eps_act = c->eps.get() + ni;
   embed_ci(s, na, nf, nd, c->cas2full.get(), c->cnew.get(), c->full_dets.get(), eps_act, c->c_full.get(),
            c->cF.get());
@@ -489,6 +530,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     zgemm(s, 'N', 'N', nmo, nmo, nmo, one, T.get(), nmo, 0, c->U.get(), nmo, 0, zero, c->hp.get(), nmo, 0, 1);
     DC2_HIP(hipStreamSynchronize(s));
   }
+  c->gstats.tag = 0;
   c->timings[2] = t2.stop();
   check_finite(c, "e0", c->e0.get(), 1);
   check_finite(c, "G1", c->G1.get(), na2);
@@ -512,7 +554,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   if (doF) need.push_back("F");
   if (doG) need.push_back("G");
   if (doH) need.push_back("H");
+  c->gstats.tag = 1;
   for (const auto& nm : need) transform(c, nm);
+  c->gstats.tag = 0;
   c->timings[3] = t3.stop();
   for (const auto& nm : need) check_finite(c, ("transformed " + nm).c_str(), c->tblk[nm].get(), block_count(c, shape_of(nm)));
 
@@ -568,7 +612,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
             effh_A(s, ni, na, c->hp.get(), nmo, c->tblk["A2"].get(), eff.get());
             d3_perm_A(s, na, c->G3.get(), Dm.get());
             a1_perm(s, na, ni, A1, Am.get());
+            c->gstats.tag = 3;
             zgemm(s, 'N', 'N', na3, ni, na3, mk(-1.0, 0.0), Dm.get(), na3, 0, Am.get(), na3, 0, zero, V.get(), na3, 0, 1);
+            c->gstats.tag = 0;
             V_A_rest(s, na, ni, d, A1, eff.get(), V.get());
           } break;
           case 'B': V_B(s, na, ni, d, c->tblk["B"].get(), c->tpairs.get(), c->ipairs.get(), V.get()); break;
@@ -578,7 +624,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
             if (ni) effh_C(s, ni, na, ns, c->hp.get(), nmo, C1, c->tblk["C2"].get(), c->tblk["C3"].get(), eff.get());
             else effh_C(s, 0, na, ns, c->hp.get(), nmo, C1, C1, C1, eff.get());
             d3_perm_C(s, na, c->G3.get(), Dm.get());
+            c->gstats.tag = 3;
             zgemm(s, 'N', 'T', na3, ns, na3, one, Dm.get(), na3, 0, C1, ns, 0, zero, V.get(), na3, 0, 1);
+            c->gstats.tag = 0;
             V_C_rest(s, na, ns, d, eff.get(), V.get());
           } break;
           case 'D': {
@@ -638,6 +686,10 @@ int dc2_context_create(int32_t device, dc2_context** out) {
     c->device = device;
     const char* dbg = std::getenv("DC2_DEBUG");
     c->debug = dbg && dbg[0] && dbg[0] != '0';
+    if (const char* em = std::getenv("DC2_EIGEN")) {
+      const std::string m(em);
+      c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : 2;
+    }
     DC2_HIP(hipSetDevice(device));
     DC2_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     DC2_ROCSOLVER(rocblas_create_handle(&c->blas));
@@ -651,6 +703,8 @@ int dc2_context_destroy(dc2_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->gstats.on) set_gemm_stats(nullptr);
+    for (auto& r : ctx->gstats.rec) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     hipStream_t s = ctx->stream;
     delete ctx;
@@ -949,7 +1003,9 @@ int dc2_set_integral_factors(dc2_context* c, const double* B, int64_t naux, int3
       b.count = m1 * m2;
       b.own.alloc(b.count);
       b.ptr = b.own.get();
+      c->gstats.tag = 5;
       zgemm(s, 'T', 'N', m1, m2, naux, one, L.get(), naux, 0, R.get(), naux, 0, zero, b.own.get(), m1, 0, 1);
+      c->gstats.tag = 0;
       DC2_HIP(hipStreamSynchronize(s));
     }
     for (const char* k : {"A1", "A2", "B"}) {
@@ -957,6 +1013,43 @@ int dc2_set_integral_factors(dc2_context* c, const double* B, int64_t naux, int3
       if (it != c->blocks.end() && it->second.derived) c->blocks.erase(it);
     }
     c->have_casci = c->have_pt2 = false;
+  });
+}
+
+int dc2_set_profiling(dc2_context* c, int32_t on) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    DC2_HIP(hipStreamSynchronize(c->stream));
+    for (auto& r : c->gstats.rec) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    c->gstats.rec.clear();
+    c->gstats.on = on != 0;
+    c->gstats.tag = 0;
+    set_gemm_stats(on ? &c->gstats : nullptr);
+  });
+}
+
+int dc2_get_gemm_stats(dc2_context* c, int32_t tag, double* out) {
+  return guarded([&] {
+    DC2_CHECK(c && out, 1, "null argument");
+    DC2_HIP(hipStreamSynchronize(c->stream));
+    double n = 0, sec = 0, fl = 0, mx = 0;
+    for (auto& r : c->gstats.rec) {
+      if (tag >= 0 && r.tag != tag) continue;
+      float ms = 0;
+      DC2_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      n += 1;
+      sec += ms * 1e-3;
+      fl += r.flops;
+      mx = std::max(mx, r.flops);
+    }
+    out[0] = n; out[1] = sec; out[2] = fl; out[3] = mx;
+  });
+}
+
+int dc2_get_eig_fallbacks(dc2_context* c, int64_t* n) {
+  return guarded([&] {
+    DC2_CHECK(c && n, 1, "null argument");
+    *n = c->eig_fallbacks;
   });
 }
 

@@ -83,6 +83,20 @@ struct DevBuf {
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // --- GEMM (dc2_gemm.hip) -----------------------------------------------------
+// Optional accounting of GEMM launches: HIP events around every launch on the
+// launch stream, algorithmic flops 8*M*N*K*batch per launch, grouped by a tag.
+struct GemmRecord {
+  hipEvent_t a, b;
+  double flops;
+  int tag;
+};
+struct GemmStats {
+  bool on = false;
+  int tag = 0;
+  std::vector<GemmRecord> rec;
+};
+void set_gemm_stats(GemmStats* st);
+
 // C[m,n] = alpha * sum_k op(A)[m,k] op(B)[k,n] + beta * C[m,n], batched by strides.
 // opA: 'N' A[m + lda*k], 'T' A[k + lda*m], 'C' conj(A[k + lda*m])
 // opB: 'N' B[k + ldb*n], 'R' conj(B[k + ldb*n]), 'T' B[n + ldb*k], 'C' conj(B[n + ldb*k])

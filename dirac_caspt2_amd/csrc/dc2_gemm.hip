@@ -122,13 +122,28 @@ __global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t K, 
       }
 }
 
+GemmStats* g_stats = nullptr;
+
 template <char OPA, char OPB>
 void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda, int64_t sA,
             const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC, int64_t batch) {
   dim3 grid(cdiv(M, BM), cdiv(N, BN), (unsigned)batch);
+  GemmRecord r{};
+  const bool acct = g_stats && g_stats->on;
+  if (acct) {
+    DC2_HIP(hipEventCreate(&r.a));
+    DC2_HIP(hipEventCreate(&r.b));
+    DC2_HIP(hipEventRecord(r.a, s));
+  }
   hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
                      C, ldc, sC);
   DC2_HIP(hipGetLastError());
+  if (acct) {
+    DC2_HIP(hipEventRecord(r.b, s));
+    r.flops = 8.0 * (double)M * (double)N * (double)K * (double)batch;
+    r.tag = g_stats->tag;
+    g_stats->rec.push_back(r);
+  }
 }
 
 template <char OPA>
@@ -144,6 +159,8 @@ void dispatchB(char opB, hipStream_t s, int64_t M, int64_t N, int64_t K, cplx al
   }
 }
 }  // namespace
+
+void set_gemm_stats(GemmStats* st) { g_stats = st; }
 
 void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A,
            int64_t lda, int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc,

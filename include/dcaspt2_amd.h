@@ -126,6 +126,15 @@ int dc2_zgemm(dc2_context* ctx, char opA, char opB, int64_t M, int64_t N, int64_
               const double* beta, double* C, int64_t ldc, int64_t strideC, int64_t batch);
 /* wait for all work queued on the library stream */
 int dc2_synchronize(dc2_context* ctx);
+/* GEMM launch accounting with HIP events on the library stream (resets on each call):
+ * dc2_get_gemm_stats(tag) returns out[4] = {launches, seconds, algorithmic flops
+ * (8*M*N*K per complex GEMM), largest single launch flops} of the launches with that
+ * tag (-1: all): 1 integral transforms, 2 density contractions, 3 V contractions,
+ * 4 S^-1/2 projections, 5 integral blocks from factors, 0 other */
+int dc2_set_profiling(dc2_context* ctx, int32_t on);
+int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
+/* number of eigensolves that fell back to the tridiagonal QR algorithm */
+int dc2_get_eig_fallbacks(dc2_context* ctx, int64_t* n);
 
 /* ---- device memory for hosts without their own GPU runtime (e.g. Fortran) ----------
  * kind: 1 host->device, 2 device->host, 3 device->device (synchronous) */
