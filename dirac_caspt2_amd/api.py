@@ -242,6 +242,14 @@ class Caspt2Context:
         check("dc2_get_gemm_stats", self.lib.dc2_get_gemm_stats(self.h, self.GEMM_TAGS[tag], _ptr(out)))
         return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
 
+    OPTIONS = {"eigensolver": {"qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
+               "ac_metric": {"auto": 0, "formula": 1, "determinant": 2}}
+
+    def set_option(self, key: str, value):
+        """Numerical options: eigensolver in qr|dc|dcj|jacobi, ac_metric in auto|formula|determinant."""
+        v = self.OPTIONS[key][value] if isinstance(value, str) else int(value)
+        check("dc2_set_option", self.lib.dc2_set_option(self.h, key.encode(), v))
+
     def eig_fallbacks(self) -> int:
         n = ctypes.c_int64()
         check("dc2_get_eig_fallbacks", self.lib.dc2_get_eig_fallbacks(self.h, ctypes.byref(n)))

@@ -77,6 +77,7 @@ struct dc2_context {
   DevBuf<int> info;
   bool debug = false;
   int eig_method = 2;        // 0 QR (zheev), 1 divide and conquer, 2 D&C + Jacobi, 3 Jacobi
+  int ac_route = 0;          // A/C metric: 0 automatic, 1 RDM formulas (nact^3), 2 determinant space
   int64_t eig_fallbacks = 0;
   GemmStats gstats;        // GEMM launch accounting (dc2_set_profiling)
 
@@ -362,39 +363,17 @@ struct SolveOut {
   int64_t dimm = 0;
 };
 
-// S -> cdiag(1e-8) -> ccutoff -> ulambda_s_half -> B~ -> cdiag -> E2 (solve_A_subspace.F90:150-266)
-SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B, const cplx* V, const double* alpha) {
-  SolveOut r;
-  if (n == 0) return r;
+// cdiag(B~) -> X = W^H Y -> E2 / sumc2 (solve_A_subspace.F90:197-257); bc1 is overwritten
+void finish_block(dc2_context* c, int64_t dimm, int64_t npert, cplx* bc1, const cplx* Y, const double* alpha,
+                  SolveOut& r) {
   hipStream_t s = c->stream;
-  DevBuf<double> ws(n);
-  zheevd(c, (int)n, S, ws.get(), "an S matrix");
-  std::vector<double> w(n);
-  DC2_HIP(hipMemcpyAsync(w.data(), ws.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
-  DC2_HIP(hipStreamSynchronize(s));
-  const double thr = 1.0e-8;
-  int64_t dimm = 0;
-  for (int64_t k = 0; k < n; ++k) {
-    if (w[k] >= thr) ++dimm;
-    r.margin = std::min(r.margin, std::fabs(w[k] - thr));
-  }
-  r.dimm = dimm;
-  if (dimm == 0 || npert == 0) return r;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  DevBuf<cplx> uc(n * dimm), T1(dimm * n), bc1(dimm * dimm), Y(dimm * npert), X(dimm * npert);
+  DevBuf<cplx> X(dimm * npert);
   DevBuf<double> wb(dimm);
-  check_finite(c, "S eigenvectors", S, n * n);
-  check_finite(c, "S eigenvalues", ws.get(), n);
-  scale_cols_tail(s, n, dimm, S, ws.get(), uc.get());
+  check_finite(c, "B~", bc1, dimm * dimm);
+  zheevd(c, (int)dimm, bc1, wb.get(), "a B~ matrix");
   c->gstats.tag = 4;
-  check_finite(c, "U L^-1/2", uc.get(), n * dimm);
-  zgemm(s, 'C', 'N', dimm, n, n, one, uc.get(), n, 0, B, n, 0, zero, T1.get(), dimm, 0, 1);
-  check_finite(c, "(U L^-1/2)^H B", T1.get(), n * dimm);
-  zgemm(s, 'N', 'N', dimm, dimm, n, one, T1.get(), dimm, 0, uc.get(), n, 0, zero, bc1.get(), dimm, 0, 1);
-  check_finite(c, "B~", bc1.get(), dimm * dimm);
-  zheevd(c, (int)dimm, bc1.get(), wb.get(), "a B~ matrix");
-  zgemm(s, 'C', 'N', dimm, npert, n, one, uc.get(), n, 0, V, n, 0, zero, Y.get(), dimm, 0, 1);
-  zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1.get(), dimm, 0, Y.get(), dimm, 0, zero, X.get(), dimm, 0, 1);
+  zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1, dimm, 0, Y, dimm, 0, zero, X.get(), dimm, 0, 1);
   c->gstats.tag = 0;
   const int64_t np = e2_partials_needed(dimm * npert);
   c->partial.ensure(2 * np);
@@ -405,6 +384,89 @@ SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B,
   DC2_HIP(hipStreamSynchronize(s));
   r.e2 = o[0];
   r.sumc2 = o[1];
+}
+
+// count of retained eigenvalues (w >= 1e-8, cutoff.F90:64-69) and the distance of the
+// closest eigenvalue to the threshold
+int64_t count_retained(dc2_context* c, int64_t n, const double* w_dev, double& margin) {
+  std::vector<double> w(n);
+  DC2_HIP(hipMemcpyAsync(w.data(), w_dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DC2_HIP(hipStreamSynchronize(c->stream));
+  const double thr = 1.0e-8;
+  int64_t dimm = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    if (w[k] >= thr) ++dimm;
+    margin = std::min(margin, std::fabs(w[k] - thr));
+  }
+  return dimm;
+}
+
+// S -> cdiag(1e-8) -> ccutoff -> ulambda_s_half -> B~ -> cdiag -> E2 (solve_A_subspace.F90:150-266)
+SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B, const cplx* V, const double* alpha) {
+  SolveOut r;
+  if (n == 0) return r;
+  hipStream_t s = c->stream;
+  DevBuf<double> ws(n);
+  zheevd(c, (int)n, S, ws.get(), "an S matrix");
+  const int64_t dimm = count_retained(c, n, ws.get(), r.margin);
+  r.dimm = dimm;
+  if (dimm == 0 || npert == 0) return r;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  DevBuf<cplx> uc(n * dimm), T1(dimm * n), bc1(dimm * dimm), Y(dimm * npert);
+  check_finite(c, "S eigenvectors", S, n * n);
+  check_finite(c, "S eigenvalues", ws.get(), n);
+  scale_cols_tail(s, n, dimm, S, ws.get(), uc.get());
+  c->gstats.tag = 4;
+  check_finite(c, "U L^-1/2", uc.get(), n * dimm);
+  zgemm(s, 'C', 'N', dimm, n, n, one, uc.get(), n, 0, B, n, 0, zero, T1.get(), dimm, 0, 1);
+  zgemm(s, 'N', 'N', dimm, dimm, n, one, T1.get(), dimm, 0, uc.get(), n, 0, zero, bc1.get(), dimm, 0, 1);
+  zgemm(s, 'C', 'N', dimm, npert, n, one, uc.get(), n, 0, V, n, 0, zero, Y.get(), dimm, 0, 1);
+  c->gstats.tag = 0;
+  finish_block(c, dimm, npert, bc1.get(), Y.get(), alpha, r);
+  return r;
+}
+
+// A and C through their determinant-space factor.  With Phi the (m x n) matrix of the
+// vectors a+_t E_uv|0> (A) or a_t E_uv|0> (C) over the full (nelec+-1)-electron string
+// space, the formula matrices of solve_A/C_subspace.F90 are S = Phi^H Phi and
+// B = Phi^H F Phi, F = diag(sum_w eps_w n_w) (F commutes past the ladder operators).
+// The nonzero spectrum of S is that of G = Phi Phi^H (m x m): for G g = l g, the S
+// eigenvector is u = Phi^H g / sqrt(l), so the retained set (l >= 1e-8) and dimm are those
+// of the reference, U L^-1/2 = Phi^H G_k L^-1, B~ = G_k^H F G_k and
+// (U L^-1/2)^H V = L^-1 G_k^H (Phi V).  Used when m < n = nact^3.
+SolveOut solve_block_gram(dc2_context* c, bool create, int64_t n, int64_t npert, const cplx* V,
+                          const double* alpha) {
+  SolveOut r;
+  hipStream_t s = c->stream;
+  const int na = c->na;
+  const int64_t m = binom_host(na, create ? c->ne + 1 : c->ne - 1);
+  const double thr = 1.0e-8;
+  if (n > m) r.margin = thr;   // the n - m eigenvalues of S that vanish identically
+  if (m == 0) return r;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  DevBuf<int64_t> strings(m);
+  DevBuf<cplx> Phi(m * n), G(m * m);
+  DevBuf<double> f(m), lam(m);
+  build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), c->Y1.get(), c->eps.get() + c->ni, m,
+            strings.get(), Phi.get(), f.get());
+  c->gstats.tag = 4;
+  zgemm(s, 'N', 'C', m, m, n, one, Phi.get(), m, 0, Phi.get(), m, 0, zero, G.get(), m, 0, 1);
+  c->gstats.tag = 0;
+  check_finite(c, "Phi Phi^H", G.get(), m * m);
+  zheevd(c, (int)m, G.get(), lam.get(), "a determinant-space metric");
+  const int64_t dimm = count_retained(c, m, lam.get(), r.margin);
+  r.dimm = dimm;
+  if (dimm == 0 || npert == 0) return r;
+  const cplx* Gk = G.get() + m * (m - dimm);
+  DevBuf<cplx> FG(m * dimm), GL(m * dimm), bc1(dimm * dimm), P(m * npert), Y(dimm * npert);
+  scale_rows(s, m, dimm, f.get(), Gk, FG.get());
+  scale_cols_tail(s, m, dimm, G.get(), lam.get(), GL.get(), true);
+  c->gstats.tag = 4;
+  zgemm(s, 'C', 'N', dimm, dimm, m, one, Gk, m, 0, FG.get(), m, 0, zero, bc1.get(), dimm, 0, 1);
+  zgemm(s, 'N', 'N', m, npert, n, one, Phi.get(), m, 0, V, n, 0, zero, P.get(), m, 0, 1);
+  zgemm(s, 'C', 'N', dimm, npert, m, one, GL.get(), m, 0, P.get(), m, 0, zero, Y.get(), dimm, 0, 1);
+  c->gstats.tag = 0;
+  finish_block(c, dimm, npert, bc1.get(), Y.get(), alpha, r);
   return r;
 }
 
@@ -600,10 +662,13 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
         case 'F': n = ntp; npert = nsp; break;
         case 'G': n = na; npert = nsp * ni; break;
       }
-      DevBuf<cplx> S(n * n), B(n * n), V(std::max<int64_t>(n * npert, 1));
+      // A and C: determinant-space metric when the (nelec+-1)-electron space is smaller than nact^3
+      const bool gram = (X == 'A' || X == 'C') && c->ac_route != 1 &&
+                        (c->ac_route == 2 || binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) < n);
+      DevBuf<cplx> S(gram ? 0 : n * n), B(gram ? 0 : n * n), V(std::max<int64_t>(n * npert, 1));
       DevBuf<double> alpha(std::max<int64_t>(npert, 1));
       const int* pairs = (X == 'B' || X == 'F') ? c->tpairs.get() : nullptr;
-      if (n > 0) build_SB(s, X, d, eps_act, c->e0.get(), n, pairs, S.get(), B.get());
+      if (n > 0 && !gram) build_SB(s, X, d, eps_act, c->e0.get(), n, pairs, S.get(), B.get());
       if (n > 0 && npert > 0) {
         switch (X) {
           case 'A': {
@@ -642,11 +707,14 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
                alpha.get());
       }
       const std::string tag(1, X);
-      check_finite(c, ("S" + tag).c_str(), S.get(), n * n);
-      check_finite(c, ("B" + tag).c_str(), B.get(), n * n);
+      if (!gram) {
+        check_finite(c, ("S" + tag).c_str(), S.get(), n * n);
+        check_finite(c, ("B" + tag).c_str(), B.get(), n * n);
+      }
       check_finite(c, ("V" + tag).c_str(), V.get(), n * npert);
       check_finite(c, ("alpha" + tag).c_str(), alpha.get(), npert);
-      r = solve_block(c, n, npert, S.get(), B.get(), V.get(), alpha.get());
+      r = gram ? solve_block_gram(c, X == 'A', n, npert, V.get(), alpha.get())
+               : solve_block(c, n, npert, S.get(), B.get(), V.get(), alpha.get());
     }
     e2[k] = r.e2;
     sumc2[k] = r.sumc2;
@@ -689,6 +757,10 @@ int dc2_context_create(int32_t device, dc2_context** out) {
     if (const char* em = std::getenv("DC2_EIGEN")) {
       const std::string m(em);
       c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : 2;
+    }
+    if (const char* r = std::getenv("DC2_AC_METRIC")) {
+      const std::string m(r);
+      c->ac_route = m == "formula" ? 1 : m == "determinant" ? 2 : 0;
     }
     DC2_HIP(hipSetDevice(device));
     DC2_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1043,6 +1115,22 @@ int dc2_get_gemm_stats(dc2_context* c, int32_t tag, double* out) {
       mx = std::max(mx, r.flops);
     }
     out[0] = n; out[1] = sec; out[2] = fl; out[3] = mx;
+  });
+}
+
+int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
+  return guarded([&] {
+    DC2_CHECK(c && key, 1, "null argument");
+    const std::string k(key);
+    if (k == "eigensolver") {
+      DC2_CHECK(value >= 0 && value <= 3, 1, "eigensolver must be 0..3");
+      c->eig_method = (int)value;
+    } else if (k == "ac_metric") {
+      DC2_CHECK(value >= 0 && value <= 2, 1, "ac_metric must be 0..2");
+      c->ac_route = (int)value;
+    } else {
+      throw Error(1, "unknown option '" + k + "'");
+    }
   });
 }
 

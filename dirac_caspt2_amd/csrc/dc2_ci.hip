@@ -438,11 +438,72 @@ __global__ void k_fill_zero(cplx* p, int64_t n) {
   if (i < n) p[i] = mk(0.0, 0.0);
 }
 
-__global__ void k_scale_cols_tail(int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc) {
+__global__ void k_scale_cols_tail(int64_t n, int64_t dimm, const cplx* vs, const double* ws, int inverse,
+                                  cplx* uc) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (r >= n) return;
   const int64_t src = n - dimm + c;
-  uc[r + n * c] = cscale(vs[r + n * src], pow(ws[src], -0.5));
+  uc[r + n * c] = cscale(vs[r + n * src], inverse ? 1.0 / ws[src] : pow(ws[src], -0.5));
+}
+
+// C(n, k), exact in 64-bit for n <= 62
+__device__ inline int64_t binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  int64_t r = 1;
+  for (int i = 0; i < k; ++i) r = r * (n - i) / (i + 1);
+  return r;
+}
+
+// strings[r] = the r-th k-bit string below 2^na in ascending order (colex unranking:
+// rank = sum_j C(p_j, j) over the set bit positions p_1 < ... < p_k)
+__global__ void k_unrank(int na, int k, int64_t m, int64_t* strings) {
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r0 >= m) return;
+  int64_t r = r0, d = 0;
+  int p = na - 1;
+  for (int j = k; j >= 1; --j) {
+    while (binom(p, j) > r) --p;
+    d |= 1ll << p;
+    r -= binom(p, j);
+    --p;
+  }
+  strings[r0] = d;
+}
+
+// Phi[r + m*(t*na*na + u*na + v)] = <r| a+_t E_uv |0> (create) or <r| a_t E_uv |0>, gathered
+// from Y1[:, u + na*v] = E_uv|0>.  The ladder phase counts the occupied spinors above t,
+// the convention of one_e_exct, so that a+_t a_x = E_tx holds with the tables of apply_E.
+__global__ void k_phi(int na, int create, int64_t m, const int64_t* __restrict__ strings, int64_t nf,
+                      const int* __restrict__ rank_full, const cplx* __restrict__ Y1, cplx* __restrict__ Phi) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const int t = blockIdx.y, uv = blockIdx.z;
+  const int u = uv / na, v = uv % na;
+  const uint64_t d = (uint64_t)strings[r], bit = 1ull << t;
+  cplx val = mk(0.0, 0.0);
+  if (((d & bit) != 0) == (create != 0)) {
+    const int k = rank_full[d ^ bit];
+    val = Y1[(int64_t)k + nf * (u + na * v)];
+    if (__popcll(d >> (t + 1)) & 1) val = mk(-val.x, -val.y);
+  }
+  Phi[r + m * ((int64_t)t * na * na + uv)] = val;
+}
+
+// f[r] = sum_w eps_w n_w(strings[r])
+__global__ void k_fdet(int na, int64_t m, const int64_t* strings, const double* eps_act, double* f) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const uint64_t d = (uint64_t)strings[r];
+  double s = 0.0;
+  for (int w = 0; w < na; ++w)
+    if ((d >> w) & 1ull) s += eps_act[w];
+  f[r] = s;
+}
+
+__global__ void k_scale_rows(int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (r >= m) return;
+  out[r + m * c] = cscale(in[r + m * c], f[r]);
 }
 
 }  // namespace
@@ -555,9 +616,38 @@ void fill_zero(hipStream_t s, cplx* p, int64_t n) {
   DC2_HIP(hipGetLastError());
 }
 
-void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc) {
-  if (dimm <= 0) return;
-  hipLaunchKernelGGL(k_scale_cols_tail, dim3(cdiv(n, 256), (unsigned)dimm), dim3(256), 0, s, n, dimm, vs, ws, uc);
+void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc,
+                     bool inverse) {
+  if (dimm <= 0 || n <= 0) return;
+  DC2_CHECK(dimm < 65536, 1, "scale_cols_tail: too many columns for one grid dimension");
+  hipLaunchKernelGGL(k_scale_cols_tail, dim3(cdiv(n, 256), (unsigned)dimm), dim3(256), 0, s, n, dimm, vs, ws,
+                     inverse ? 1 : 0, uc);
+  DC2_HIP(hipGetLastError());
+}
+
+int64_t binom_host(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  int64_t r = 1;
+  for (int i = 0; i < k; ++i) r = r * (n - i) / (i + 1);
+  return r;
+}
+
+void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int* rank_full, const cplx* Y1,
+               const double* eps_act, int64_t m, int64_t* strings, cplx* Phi, double* fdiag) {
+  DC2_CHECK(m == binom_host(na, create ? ne + 1 : ne - 1), 1, "build_phi: wrong string count");
+  if (m == 0) return;
+  DC2_CHECK(na >= 1 && na <= 24 && (int64_t)na * na < 65536, 1, "build_phi: nact out of range");
+  hipLaunchKernelGGL(k_unrank, dim3(cdiv(m, 256)), dim3(256), 0, s, na, create ? ne + 1 : ne - 1, m, strings);
+  hipLaunchKernelGGL(k_phi, dim3(cdiv(m, 256), na, na * na), dim3(256), 0, s, na, create ? 1 : 0, m, strings, nf,
+                     rank_full, Y1, Phi);
+  hipLaunchKernelGGL(k_fdet, dim3(cdiv(m, 256)), dim3(256), 0, s, na, m, strings, eps_act, fdiag);
+  DC2_HIP(hipGetLastError());
+}
+
+void scale_rows(hipStream_t s, int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out) {
+  if (m <= 0 || ncol <= 0) return;
+  DC2_CHECK(ncol < 65536, 1, "scale_rows: too many columns for one grid dimension");
+  hipLaunchKernelGGL(k_scale_rows, dim3(cdiv(m, 256), (unsigned)ncol), dim3(256), 0, s, m, ncol, f, in, out);
   DC2_HIP(hipGetLastError());
 }
 

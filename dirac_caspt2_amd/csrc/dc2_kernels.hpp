@@ -51,7 +51,16 @@ void permute6(hipStream_t s, const int* dims_out, const int* src_stride_of_out_a
 // copy a 4-index sub-block of a column-major tensor
 void extract4(hipStream_t s, const cplx* in, const int64_t* ld_in, const int* off, const int* n, cplx* out);
 void fill_zero(hipStream_t s, cplx* p, int64_t n);
-void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc);
+// uc[:, c] = vs[:, n-dimm+c] * ws[n-dimm+c]^(-1/2)  (or ^-1 with inverse)
+void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc,
+                     bool inverse = false);
+int64_t binom_host(int n, int k);
+// Determinant-space factor of the A (create) / C metric: the (ne+-1)-electron strings
+// (ascending), Phi[r + m*(t*na*na + u*na + v)] = <r|a+_t E_uv|0> or <r|a_t E_uv|0>, so that
+// S = Phi^H Phi and B = Phi^H diag(fdiag) Phi with fdiag[r] = sum_w eps_w n_w(r)
+void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int* rank_full, const cplx* Y1,
+               const double* eps_act, int64_t m, int64_t* strings, cplx* Phi, double* fdiag);
+void scale_rows(hipStream_t s, int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out);
 
 // ---------------------------------------------------------------- dc2_pt2.hip
 struct Dens {

@@ -133,6 +133,12 @@ int dc2_synchronize(dc2_context* ctx);
  * 4 S^-1/2 projections, 5 integral blocks from factors, 0 other */
 int dc2_set_profiling(dc2_context* ctx, int32_t on);
 int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
+/* Numerical options (no counterpart in the reference):
+ *   "eigensolver": 0 tridiagonal QR (the ZHEEV algorithm), 1 divide and conquer,
+ *                  2 divide and conquer with Jacobi leaves (default), 3 Jacobi;
+ *   "ac_metric":   0 automatic (default), 1 the RDM-formula S/B matrices of size nact^3,
+ *                  2 the determinant-space factorisation of the A and C metrics. */
+int dc2_set_option(dc2_context* ctx, const char* key, int64_t value);
 /* number of eigensolves that fell back to the tridiagonal QR algorithm */
 int dc2_get_eig_fallbacks(dc2_context* ctx, int64_t* n);
 

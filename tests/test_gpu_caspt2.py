@@ -36,11 +36,18 @@ def test_small_casci_and_dets(small):
         np.testing.assert_allclose(M, R.casmat(p, cas[0]), atol=1e-12, rtol=0)
 
 
-def test_small_caspt2(small):
+METRICS = ["formula", "determinant"]
+
+
+@pytest.mark.parametrize("metric", METRICS)
+def test_small_caspt2(small, metric):
+    """Both routes of the A/C metric (RDM formulas of size nact^3 and the determinant-space
+    factorisation) against the oracle."""
     from gpu_helpers import assert_energies, context_from_synth, say
     sp, p, cas, ref = small
-    say("small synthetic: CASPT2")
+    say("small synthetic: CASPT2, A/C metric", metric)
     with context_from_synth(sp) as ctx:
+        ctx.set_option("ac_metric", metric)
         ctx.casci(1)
         out = ctx.caspt2(1, 0.0)
         say("gpu", out.e2, out.total)
@@ -62,13 +69,15 @@ def test_small_caspt2_eshift(small):
         assert_energies(out, ref, 1e-9, "eshift")
 
 
+@pytest.mark.parametrize("metric", METRICS)
 @pytest.mark.parametrize("name", FIXTURES)
-def test_fixture_goldens(name, fixture_cache):
+def test_fixture_goldens(name, metric, fixture_cache):
     from gpu_helpers import context_from_problem, say
-    say("fixture", name)
+    say("fixture", name, "A/C metric", metric)
     fx = fixture_cache(name)
     g = fx.gold
     with context_from_problem(fx.problem) as ctx:
+        ctx.set_option("ac_metric", metric)
         eig = ctx.casci(len(g["casci"]))
         np.testing.assert_allclose(eig, g["casci"][:len(eig)], atol=1e-10, rtol=0)
         out = ctx.caspt2(fx.problem.selectroot, fx.problem.eshift)
@@ -78,18 +87,30 @@ def test_fixture_goldens(name, fixture_cache):
         assert abs(out.total - g["total"]) < 1e-9
 
 
-def test_medium_caspt2():
-    """192 spinors (16/12/164, 6 electrons, 924 determinants)."""
+@pytest.fixture(scope="module")
+def medium():
     from dirac_caspt2_amd import synth
-    from gpu_helpers import assert_energies, context_from_synth, oracle_problem, say
+    from gpu_helpers import oracle_problem, say
     from oracle import caspt2_ref as R
     sp = synth.generate(synth.CONFIG_FP64)
-    p = oracle_problem(sp)
     say("medium: oracle")
-    ref = R.run_caspt2(p)
-    say("medium: gpu")
+    return sp, R.run_caspt2(oracle_problem(sp))
+
+
+@pytest.mark.parametrize("metric", METRICS)
+def test_medium_caspt2(medium, metric):
+    """192 spinors (16/12/164, 6 electrons, 924 determinants); the retained dimensions of
+    every subspace agree between the two A/C metric routes."""
+    from gpu_helpers import assert_energies, context_from_synth, say
+    sp, ref = medium
+    say("medium: gpu, A/C metric", metric)
     with context_from_synth(sp) as ctx:
+        ctx.set_option("ac_metric", metric)
         ctx.casci(1)
         out = ctx.caspt2(1, 0.0)
-        say("gpu", out.e2, out.total, "timings", out.timings)
+        say("gpu", out.e2, out.total, "timings", out.timings, "info", out.info)
         assert_energies(out, ref, 1e-9, "medium")
+        for X in "ABCDEFGH":
+            blocks = ref.e2_blocks[X]
+            if blocks:
+                assert out.info[X][1] == sum(b.dimm for _, b in blocks), X   # (dimn, dimm, npert, ran)
