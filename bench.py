@@ -8,7 +8,7 @@ complex128, seeded synthetic Kramers-paired integrals), with the integrals
 already resident in HBM.  Only one GPU is used.
 
     python bench.py [--steps K] [--warmup W] [--workload fp64|small|split]
-                    [--no-cpu-baseline] [--verify]
+                    [--no-cpu-baseline] [--copy-calibration MIB]
 
 Prints ONE JSON line.
 """
@@ -54,6 +54,31 @@ def cpu_baseline(sp, steps=1):
                            f"({dt:.2f} s per energy, BLAS threads = {cores})"}
 
 
+def copy_calibration(ctx, mib):
+    from dirac_caspt2_amd.api import DeviceArray
+    n = mib << 20
+    src, dst = DeviceArray(ctx, nbytes=n), DeviceArray(ctx, nbytes=n)
+    for _ in range(3):
+        ctx.lib.dc2_memcpy(ctx.h, dst.ptr, src.ptr, n, 3)
+    ctx.synchronize()
+    src.free()
+    dst.free()
+
+
+def measured_traffic(workload, op):
+    """HBM bytes per launch of k_zgemm<op> from the committed rocprofv3 counter summary of this
+    workload (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known size),
+    or None when no summary for this kernel exists."""
+    path = os.path.join(HERE, "profiles", f"k_zgemm_{workload}_hbm_counters.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        w = line.split()
+        if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == op:
+            return float(w[2])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,7 +86,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="fp64", choices=["small", "fp64", "split"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--verify", action="store_true", help="compare every energy with the oracle's")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, run 3 device-to-device copies of MIB MiB (a plain copy kernel "
+                         "of known size, the reference for the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -95,14 +122,19 @@ def main():
     total = time.perf_counter() - t0
     g_all = ctx.gemm_stats("all")
     g_tr = ctx.gemm_stats("transform")
+    ops = {f"{oa}{ob}": ctx.gemm_op_stats(oa, ob) for oa in "NTC" for ob in "NRTC"}
     ctx.set_profiling(False)
+    dom_op, dom = max(ops.items(), key=lambda kv: kv[1]["seconds"])
     stage = out.timings
     lat_ms = np.array(lat) * 1e3
 
-    # roofline of the MFMA GEMM family (all hand-written complex FP64 GEMM launches of the
-    # timed steps): algorithmic flops 8*M*N*K per launch over HIP-event durations
-    achieved = g_all["flops"] / g_all["seconds"] / 1e12 if g_all["seconds"] > 0 else 0.0
+    # roofline of the dominant hand-written kernel, the k_zgemm<opA,opB> instantiation with the
+    # largest summed HIP-event time over the timed steps: algorithmic flops (8*M*N*K per
+    # complex GEMM) per launch over the average launch duration
+    achieved = dom["flops"] / dom["seconds"] / 1e12 if dom["seconds"] > 0 else 0.0
+    ach_all = g_all["flops"] / g_all["seconds"] / 1e12 if g_all["seconds"] > 0 else 0.0
     ach_tr = g_tr["flops"] / g_tr["seconds"] / 1e12 if g_tr["seconds"] > 0 else 0.0
+    traffic = measured_traffic(args.workload, dom_op)
     result = {
         "metric": "CASPT2 energies/s (plain FP64 GPU path)",
         "value": args.steps / total,
@@ -125,14 +157,19 @@ def main():
                                     "transforms": stage[3],
                                     **{f"subspace_{X}": stage[4 + k] for k, X in enumerate("ABCDEFGH")}},
         "energies_last_step": {"total": out.total, "e2": out.e2},
-        "roofline": {"kernel": "k_zgemm (MFMA FP64 complex GEMM, all launches)", "bound": "mfma",
+        "roofline": {"kernel": f"k_zgemm<'{dom_op[0]}','{dom_op[1]}'> (MFMA FP64 complex GEMM)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "launches_per_step": g_all["launches"] / args.steps,
-                     "algorithmic_flops_per_step": g_all["flops"] / args.steps,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "launches_per_step": dom["launches"] / args.steps,
+                     "avg_launch_us": 1e6 * dom["seconds"] / max(dom["launches"], 1),
+                     "algorithmic_flops_per_launch": dom["flops"] / max(dom["launches"], 1),
+                     "all_gemm_launches_per_step": g_all["launches"] / args.steps,
+                     "all_gemm_achieved_tflops": ach_all,
                      "transform_launches_achieved_tflops": ach_tr},
         "eig_fallbacks": ctx.eig_fallbacks(),
     }
+    if args.copy_calibration:
+        copy_calibration(ctx, args.copy_calibration)
     ctx.close()
     if not args.no_cpu_baseline:
         ref, cb = cpu_baseline(sp)

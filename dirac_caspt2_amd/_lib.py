@@ -50,6 +50,7 @@ _SIGS = {
     "dc2_set_integral_factors": (c_int, [P, P, c_i64, c_int]),
     "dc2_set_profiling": (c_int, [P, c_int]),
     "dc2_set_option": (c_int, [P, ctypes.c_char_p, c_i64]),
+    "dc2_get_gemm_op_stats": (c_int, [P, ctypes.c_char, ctypes.c_char, P]),
     "dc2_get_gemm_stats": (c_int, [P, c_int, P]),
     "dc2_get_eig_fallbacks": (c_int, [P, P]),
 }

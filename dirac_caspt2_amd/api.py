@@ -242,6 +242,12 @@ class Caspt2Context:
         check("dc2_get_gemm_stats", self.lib.dc2_get_gemm_stats(self.h, self.GEMM_TAGS[tag], _ptr(out)))
         return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
 
+    def gemm_op_stats(self, opA: str, opB: str):
+        """GEMM accounting of the launches of kernel instantiation k_zgemm<opA, opB>."""
+        out = np.zeros(4)
+        check("dc2_get_gemm_op_stats", self.lib.dc2_get_gemm_op_stats(self.h, opA.encode(), opB.encode(), _ptr(out)))
+        return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
+
     OPTIONS = {"eigensolver": {"qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
                "ac_metric": {"auto": 0, "formula": 1, "determinant": 2}}
 

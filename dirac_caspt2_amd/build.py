@@ -56,12 +56,24 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN_DRIVER_SRC = os.path.join(HERE, "..", "tests", "fortran", "drive_caspt2.F90")
+FORTRAN_DRIVER = os.path.join(BUILD, "drive_caspt2")
+
+
 def build_fortran_interface() -> None:
-    """Compile the iso_c_binding interface module with amdflang (checks it against the C ABI)."""
+    """Compile the iso_c_binding interface module with amdflang and link the Fortran
+    test driver against libdcaspt2_amd.so (a Fortran host calling the C ABI)."""
     flang = shutil.which("amdflang") or os.path.join(ROCM, "bin", "amdflang")
     if not os.path.exists(FORTRAN_SRC) or not os.path.exists(flang):
         return
-    _run([flang, "-c", FORTRAN_SRC, "-module-dir", BUILD, "-o", os.path.join(BUILD, "dcaspt2_amd_interface.o")])
+    obj = os.path.join(BUILD, "dcaspt2_amd_interface.o")
+    _run([flang, "-O2", "-c", FORTRAN_SRC, "-module-dir", BUILD, "-o", obj])
+    if os.path.exists(FORTRAN_DRIVER_SRC):
+        llvm_lib = os.path.join(ROCM, "llvm", "lib")
+        _run([flang, "-O2", "-module-dir", BUILD, "-I", BUILD, FORTRAN_DRIVER_SRC, obj, "-o", FORTRAN_DRIVER + ".tmp",
+              "-L", HERE, "-ldcaspt2_amd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + llvm_lib,
+              "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+        os.replace(FORTRAN_DRIVER + ".tmp", FORTRAN_DRIVER)
 
 
 if __name__ == "__main__":

@@ -1134,6 +1134,24 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
   });
 }
 
+int dc2_get_gemm_op_stats(dc2_context* c, char opA, char opB, double* out) {
+  return guarded([&] {
+    DC2_CHECK(c && out, 1, "null argument");
+    DC2_HIP(hipStreamSynchronize(c->stream));
+    double n = 0, sec = 0, fl = 0, mx = 0;
+    for (auto& r : c->gstats.rec) {
+      if (r.opA != opA || r.opB != opB) continue;
+      float ms = 0;
+      DC2_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      n += 1;
+      sec += ms * 1e-3;
+      fl += r.flops;
+      mx = std::max(mx, r.flops);
+    }
+    out[0] = n; out[1] = sec; out[2] = fl; out[3] = mx;
+  });
+}
+
 int dc2_get_eig_fallbacks(dc2_context* c, int64_t* n) {
   return guarded([&] {
     DC2_CHECK(c && n, 1, "null argument");

@@ -89,6 +89,7 @@ struct GemmRecord {
   hipEvent_t a, b;
   double flops;
   int tag;
+  char opA, opB;   // the kernel instantiation k_zgemm<opA, opB>
 };
 struct GemmStats {
   bool on = false;

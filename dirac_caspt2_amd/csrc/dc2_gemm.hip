@@ -142,6 +142,8 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     DC2_HIP(hipEventRecord(r.b, s));
     r.flops = 8.0 * (double)M * (double)N * (double)K * (double)batch;
     r.tag = g_stats->tag;
+    r.opA = OPA;
+    r.opB = OPB;
     g_stats->rec.push_back(r);
   }
 }

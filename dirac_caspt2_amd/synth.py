@@ -12,8 +12,15 @@ h_{p~q~} = conj(h_pq), h_{p~q} = -conj(h_{pq~}).
 
 Two-electron integrals: (pq|rs) = sum_P B^P_pq B^P_rs, naux = 3*nspinor,
 B^P Hermitian with the same quaternion structure and magnitudes
-CN(0,1)*exp(-|e_p-e_q|/4)/sqrt(naux).  This gives (pq|rs)=(rs|pq),
+amplitude*CN(0,1)*exp(-|e_p-e_q|/4)/sqrt(naux).  This gives (pq|rs)=(rs|pq),
 (pq|rs)=conj(qp|sr) and (p~q~|rs)=(qp|rs).
+
+amplitude defaults to 0.25.  With amplitude 1 the 192-spinor workload
+(16/12/164, 6 electrons) is unphysical: the two-electron terms push the lowest
+secondary Fock eigenvalue to -5.05 Eh (below the inactive ones), the D
+subspace returns E2 = +21.2 Eh and the smallest |alpha + w| denominator is
+2.3e-4 (an intruder state).  At 0.25 every orbital space keeps its energy
+window and the smallest denominator is above 0.1 Eh.
 """
 from __future__ import annotations
 

@@ -133,6 +133,8 @@ int dc2_synchronize(dc2_context* ctx);
  * 4 S^-1/2 projections, 5 integral blocks from factors, 0 other */
 int dc2_set_profiling(dc2_context* ctx, int32_t on);
 int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
+/* the same figures for the launches of one kernel instantiation k_zgemm<opA, opB> */
+int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
 /* Numerical options (no counterpart in the reference):
  *   "eigensolver": 0 tridiagonal QR (the ZHEEV algorithm), 1 divide and conquer,
  *                  2 divide and conquer with Jacobi leaves (default), 3 Jacobi;
