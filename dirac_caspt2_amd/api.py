@@ -248,11 +248,13 @@ class Caspt2Context:
         check("dc2_get_gemm_op_stats", self.lib.dc2_get_gemm_op_stats(self.h, opA.encode(), opB.encode(), _ptr(out)))
         return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
 
-    OPTIONS = {"eigensolver": {"qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
-               "ac_metric": {"auto": 0, "formula": 1, "determinant": 2}}
+    OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
+               "ac_metric": {"auto": 0, "formula": 1, "determinant": 2},
+               "lanes": {}, "schur": {"off": 0, "on": 1}}
 
     def set_option(self, key: str, value):
-        """Numerical options: eigensolver in qr|dc|dcj|jacobi, ac_metric in auto|formula|determinant."""
+        """Options: eigensolver auto|qr|dc|dcj|jacobi, ac_metric auto|formula|determinant,
+        lanes 1..3 (concurrent subspace streams), schur on|off."""
         v = self.OPTIONS[key][value] if isinstance(value, str) else int(value)
         check("dc2_set_option", self.lib.dc2_set_option(self.h, key.encode(), v))
 

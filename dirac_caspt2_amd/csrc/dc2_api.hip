@@ -7,6 +7,9 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -40,6 +43,49 @@ struct Block {
 };
 
 const char* kSub = "ABCDEFGH";
+
+// An execution lane: a stream, its rocBLAS handle and grow-only scratch buffers.
+// Subspace solves run on several lanes at once, one host thread per lane; the
+// scratch buffers persist across calls, so repeated energies allocate nothing
+// once the sizes have been seen.
+struct Lane {
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  bool owned = false;
+  std::map<std::string, DevBuf<cplx>> zb;
+  std::map<std::string, DevBuf<double>> db;
+  std::map<std::string, DevBuf<int64_t>> lb;
+  DevBuf<int> info;
+  cplx* z(const char* name, int64_t n) {
+    DevBuf<cplx>& b = zb[name];
+    b.ensure((size_t)std::max<int64_t>(n, 1));
+    return b.get();
+  }
+  double* d(const char* name, int64_t n) {
+    DevBuf<double>& b = db[name];
+    b.ensure((size_t)std::max<int64_t>(n, 1));
+    return b.get();
+  }
+  int64_t* l(const char* name, int64_t n) {
+    DevBuf<int64_t>& b = lb[name];
+    b.ensure((size_t)std::max<int64_t>(n, 1));
+    return b.get();
+  }
+  Lane() = default;
+  Lane(const Lane&) = delete;
+  Lane& operator=(const Lane&) = delete;
+  ~Lane() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    zb.clear();
+    db.clear();
+    lb.clear();
+    info.release();
+    if (owned) {
+      if (blas) rocblas_destroy_handle(blas);
+      if (stream) (void)hipStreamDestroy(stream);
+    }
+  }
+};
 
 }  // namespace
 
@@ -76,9 +122,15 @@ struct dc2_context {
   DevBuf<double> dw1, dw2;
   DevBuf<int> info;
   bool debug = false;
-  int eig_method = 2;        // 0 QR (zheev), 1 divide and conquer, 2 D&C + Jacobi, 3 Jacobi
+  // eigensolver: -1 automatic (divide and conquer; D&C with Jacobi leaves for the
+  // rank-deficient nact^3 S matrices), else 0 QR (zheev), 1 D&C, 2 D&C + Jacobi, 3 Jacobi
+  int eig_method = -1;
   int ac_route = 0;          // A/C metric: 0 automatic, 1 RDM formulas (nact^3), 2 determinant space
-  int64_t eig_fallbacks = 0;
+  int nlanes = 3;            // concurrent subspace lanes (1 = sequential on the library stream)
+  bool schur = true;         // E2 of the determinant-space route through the compressed resolvent
+  std::atomic<int64_t> eig_fallbacks{0};
+  std::unique_ptr<Lane> lane0;                  // the library stream as a lane
+  std::vector<std::unique_ptr<Lane>> lanes;     // worker lanes (created on first use)
   GemmStats gstats;        // GEMM launch accounting (dc2_set_profiling)
 
   int nmo() const { return ni + na + ns; }
@@ -183,11 +235,11 @@ std::string matrix_report(dc2_context* c, int n, const cplx* A) {
 
 // debug mode (DC2_DEBUG=1): fail at the first device array holding a non-finite value
 template <class T>
-void check_finite(dc2_context* c, const char* what, const T* p, int64_t count) {
+void check_finite(dc2_context* c, hipStream_t s, const char* what, const T* p, int64_t count) {
   if (!c->debug || count <= 0 || p == nullptr) return;
   std::vector<T> h(count);
-  DC2_HIP(hipMemcpyAsync(h.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-  DC2_HIP(hipStreamSynchronize(c->stream));
+  DC2_HIP(hipMemcpyAsync(h.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
   const double* d = reinterpret_cast<const double*>(h.data());
   const int64_t nd = count * (int64_t)(sizeof(T) / sizeof(double));
   for (int64_t k = 0; k < nd; ++k)
@@ -195,63 +247,74 @@ void check_finite(dc2_context* c, const char* what, const T* p, int64_t count) {
       throw Error(4, std::string("non-finite value in ") + what + " at double index " + std::to_string(k) + " of " +
                          std::to_string(nd));
 }
+template <class T>
+void check_finite(dc2_context* c, const char* what, const T* p, int64_t count) {
+  check_finite(c, c->stream, what, p, count);
+}
 
-bool all_finite(dc2_context* c, const cplx* A, int64_t count) {
+bool all_finite(hipStream_t s, const cplx* A, int64_t count) {
   std::vector<double> h(2 * count);
-  DC2_HIP(hipMemcpyAsync(h.data(), A, count * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
-  DC2_HIP(hipStreamSynchronize(c->stream));
+  DC2_HIP(hipMemcpyAsync(h.data(), A, count * sizeof(cplx), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
   for (double v : h)
     if (!std::isfinite(v)) return false;
   return true;
 }
 
-int run_eig(dc2_context* c, int method, int n, cplx* A, double* w) {
+int run_eig(Lane& L, int method, int n, cplx* A, double* w) {
   auto* a = reinterpret_cast<rocblas_double_complex*>(A);
-  c->dw2.ensure(std::max(n, 1));
-  c->info.ensure(2);
+  double* e = L.d("eig_e", n);
+  L.info.ensure(2);
   switch (method) {
     case 1:
-      DC2_ROCSOLVER(rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, c->dw2.get(),
-                                     c->info.get()));
+      DC2_ROCSOLVER(rocsolver_zheevd(L.blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, e,
+                                     L.info.get()));
       break;
     case 2:
-      DC2_ROCSOLVER(rocsolver_zheevdj(c->blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, c->info.get()));
+      DC2_ROCSOLVER(rocsolver_zheevdj(L.blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, L.info.get()));
       break;
     case 3:
-      DC2_ROCSOLVER(rocsolver_zheevj(c->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n, a,
-                                     n, 0.0, c->dw2.get(), 100, c->info.get() + 1, w, c->info.get()));
+      DC2_ROCSOLVER(rocsolver_zheevj(L.blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n, a,
+                                     n, 0.0, e, 100, L.info.get() + 1, w, L.info.get()));
       break;
     default:
       // Tridiagonal QR, the ZHEEV algorithm of diag.F90:109
-      DC2_ROCSOLVER(rocsolver_zheev(c->blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, c->dw2.get(),
-                                    c->info.get()));
+      DC2_ROCSOLVER(rocsolver_zheev(L.blas, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, e,
+                                    L.info.get()));
   }
   int info = 0;
-  DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  DC2_HIP(hipStreamSynchronize(c->stream));
+  DC2_HIP(hipMemcpyAsync(&info, L.info.get(), sizeof(int), hipMemcpyDeviceToHost, L.stream));
+  DC2_HIP(hipStreamSynchronize(L.stream));
   return info;
 }
 
-// Hermitian eigendecomposition (upper triangle), ascending eigenvalues.  The
-// configured rocSOLVER variant runs first; a failure or a non-finite result
-// (the divide-and-conquer variants returned NaN eigenvectors with info = 0 for
+enum class EigKind { general, rank_deficient };
+
+// Hermitian eigendecomposition (upper triangle), ascending eigenvalues, on lane L.
+// The configured rocSOLVER variant runs first; a failure or a non-finite result (the
+// divide-and-conquer variants returned NaN eigenvectors with info = 0 for
 // rank-deficient S matrices with hundreds of zero eigenvalues) falls back to the
 // tridiagonal QR algorithm on a saved copy.
-void zheevd(dc2_context* c, int n, cplx* A, double* w, const char* what = "") {
+void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what = "",
+            EigKind kind = EigKind::general) {
+  if (n <= 0) return;
   const int64_t nn = (int64_t)n * n;
-  DevBuf<cplx> keep(nn);
-  DC2_HIP(hipMemcpyAsync(keep.get(), A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
-  int info = run_eig(c, c->eig_method, n, A, w);
-  if (c->eig_method != 0 && (info != 0 || !all_finite(c, A, nn))) {
+  const int method = c->eig_method >= 0 ? c->eig_method : (kind == EigKind::rank_deficient ? 2 : 1);
+  cplx* keep = L.z("eig_keep", nn);
+  DC2_HIP(hipMemcpyAsync(keep, A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
+  int info = run_eig(L, method, n, A, w);
+  if (method != 0 && (info != 0 || !all_finite(L.stream, A, nn))) {
     ++c->eig_fallbacks;
-    DC2_HIP(hipMemcpyAsync(A, keep.get(), nn * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
-    info = run_eig(c, 0, n, A, w);
+    DC2_HIP(hipMemcpyAsync(A, keep, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
+    info = run_eig(L, 0, n, A, w);
   }
-  check_finite(c, what, A, nn);
+  check_finite(c, L.stream, what, A, nn);
   if (info != 0)
     throw Error(100 + info, std::string("Hermitian eigensolver failed for ") + what + ", info = " + std::to_string(info) +
-                                (c->debug ? matrix_report(c, n, keep.get()) : std::string(" (n=") + std::to_string(n) + ")"));
+                                (c->debug ? matrix_report(c, n, keep) : std::string(" (n=") + std::to_string(n) + ")"));
 }
+
+void zheevd(dc2_context* c, int n, cplx* A, double* w, const char* what = "") { zheevd(c, *c->lane0, n, A, w, what); }
 
 struct Timer {
   hipEvent_t a, b;
@@ -363,35 +426,40 @@ struct SolveOut {
   int64_t dimm = 0;
 };
 
-// cdiag(B~) -> X = W^H Y -> E2 / sumc2 (solve_A_subspace.F90:197-257); bc1 is overwritten
-void finish_block(dc2_context* c, int64_t dimm, int64_t npert, cplx* bc1, const cplx* Y, const double* alpha,
-                  SolveOut& r) {
-  hipStream_t s = c->stream;
-  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  DevBuf<cplx> X(dimm * npert);
-  DevBuf<double> wb(dimm);
-  check_finite(c, "B~", bc1, dimm * dimm);
-  zheevd(c, (int)dimm, bc1, wb.get(), "a B~ matrix");
-  c->gstats.tag = 4;
-  zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1, dimm, 0, Y, dimm, 0, zero, X.get(), dimm, 0, 1);
-  c->gstats.tag = 0;
-  const int64_t np = e2_partials_needed(dimm * npert);
-  c->partial.ensure(2 * np);
-  c->out2.ensure(2);
-  e2_reduce(s, dimm, npert, X.get(), wb.get(), alpha, c->partial.get(), np, c->out2.get());
+// e2 and sumc2 from the two-double result of a reduction on lane L
+void read_out2(Lane& L, const double* out2, SolveOut& r) {
   double o[2];
-  DC2_HIP(hipMemcpyAsync(o, c->out2.get(), sizeof(o), hipMemcpyDeviceToHost, s));
-  DC2_HIP(hipStreamSynchronize(s));
+  DC2_HIP(hipMemcpyAsync(o, out2, sizeof(o), hipMemcpyDeviceToHost, L.stream));
+  DC2_HIP(hipStreamSynchronize(L.stream));
   r.e2 = o[0];
   r.sumc2 = o[1];
 }
 
+// cdiag(B~) -> X = W^H Y -> E2 / sumc2 (solve_A_subspace.F90:197-257); bc1 is overwritten
+void finish_block(dc2_context* c, Lane& L, int64_t dimm, int64_t npert, cplx* bc1, const cplx* Y, const double* alpha,
+                  SolveOut& r) {
+  hipStream_t s = L.stream;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  cplx* X = L.z("fin_X", dimm * npert);
+  double* wb = L.d("fin_wb", dimm);
+  check_finite(c, s, "B~", bc1, dimm * dimm);
+  zheevd(c, L, (int)dimm, bc1, wb, "a B~ matrix");
+  set_gemm_tag(4);
+  zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1, dimm, 0, Y, dimm, 0, zero, X, dimm, 0, 1);
+  set_gemm_tag(0);
+  const int64_t np = e2_partials_needed(dimm * npert);
+  double* partial = L.d("partial", 2 * np);
+  double* out2 = L.d("out2", 2);
+  e2_reduce(s, dimm, npert, X, wb, alpha, partial, np, out2);
+  read_out2(L, out2, r);
+}
+
 // count of retained eigenvalues (w >= 1e-8, cutoff.F90:64-69) and the distance of the
 // closest eigenvalue to the threshold
-int64_t count_retained(dc2_context* c, int64_t n, const double* w_dev, double& margin) {
+int64_t count_retained(Lane& L, int64_t n, const double* w_dev, double& margin) {
   std::vector<double> w(n);
-  DC2_HIP(hipMemcpyAsync(w.data(), w_dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  DC2_HIP(hipStreamSynchronize(c->stream));
+  DC2_HIP(hipMemcpyAsync(w.data(), w_dev, n * sizeof(double), hipMemcpyDeviceToHost, L.stream));
+  DC2_HIP(hipStreamSynchronize(L.stream));
   const double thr = 1.0e-8;
   int64_t dimm = 0;
   for (int64_t k = 0; k < n; ++k) {
@@ -402,27 +470,29 @@ int64_t count_retained(dc2_context* c, int64_t n, const double* w_dev, double& m
 }
 
 // S -> cdiag(1e-8) -> ccutoff -> ulambda_s_half -> B~ -> cdiag -> E2 (solve_A_subspace.F90:150-266)
-SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B, const cplx* V, const double* alpha) {
+SolveOut solve_block(dc2_context* c, Lane& L, int64_t n, int64_t npert, cplx* S, cplx* B, const cplx* V,
+                     const double* alpha, EigKind kind) {
   SolveOut r;
   if (n == 0) return r;
-  hipStream_t s = c->stream;
-  DevBuf<double> ws(n);
-  zheevd(c, (int)n, S, ws.get(), "an S matrix");
-  const int64_t dimm = count_retained(c, n, ws.get(), r.margin);
+  hipStream_t s = L.stream;
+  double* ws = L.d("sb_ws", n);
+  zheevd(c, L, (int)n, S, ws, "an S matrix", kind);
+  const int64_t dimm = count_retained(L, n, ws, r.margin);
   r.dimm = dimm;
   if (dimm == 0 || npert == 0) return r;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  DevBuf<cplx> uc(n * dimm), T1(dimm * n), bc1(dimm * dimm), Y(dimm * npert);
-  check_finite(c, "S eigenvectors", S, n * n);
-  check_finite(c, "S eigenvalues", ws.get(), n);
-  scale_cols_tail(s, n, dimm, S, ws.get(), uc.get());
-  c->gstats.tag = 4;
-  check_finite(c, "U L^-1/2", uc.get(), n * dimm);
-  zgemm(s, 'C', 'N', dimm, n, n, one, uc.get(), n, 0, B, n, 0, zero, T1.get(), dimm, 0, 1);
-  zgemm(s, 'N', 'N', dimm, dimm, n, one, T1.get(), dimm, 0, uc.get(), n, 0, zero, bc1.get(), dimm, 0, 1);
-  zgemm(s, 'C', 'N', dimm, npert, n, one, uc.get(), n, 0, V, n, 0, zero, Y.get(), dimm, 0, 1);
-  c->gstats.tag = 0;
-  finish_block(c, dimm, npert, bc1.get(), Y.get(), alpha, r);
+  cplx* uc = L.z("sb_uc", n * dimm);
+  cplx* T1 = L.z("sb_T1", dimm * n);
+  cplx* bc1 = L.z("sb_bc1", dimm * dimm);
+  cplx* Y = L.z("sb_Y", dimm * npert);
+  check_finite(c, s, "S eigenvectors", S, n * n);
+  scale_cols_tail(s, n, dimm, S, ws, uc);
+  set_gemm_tag(4);
+  zgemm(s, 'C', 'N', dimm, n, n, one, uc, n, 0, B, n, 0, zero, T1, dimm, 0, 1);
+  zgemm(s, 'N', 'N', dimm, dimm, n, one, T1, dimm, 0, uc, n, 0, zero, bc1, dimm, 0, 1);
+  zgemm(s, 'C', 'N', dimm, npert, n, one, uc, n, 0, V, n, 0, zero, Y, dimm, 0, 1);
+  set_gemm_tag(0);
+  finish_block(c, L, dimm, npert, bc1, Y, alpha, r);
   return r;
 }
 
@@ -433,41 +503,181 @@ SolveOut solve_block(dc2_context* c, int64_t n, int64_t npert, cplx* S, cplx* B,
 // The nonzero spectrum of S is that of G = Phi Phi^H (m x m): for G g = l g, the S
 // eigenvector is u = Phi^H g / sqrt(l), so the retained set (l >= 1e-8) and dimm are those
 // of the reference, U L^-1/2 = Phi^H G_k L^-1, B~ = G_k^H F G_k and
-// (U L^-1/2)^H V = L^-1 G_k^H (Phi V).  Used when m < n = nact^3.
-SolveOut solve_block_gram(dc2_context* c, bool create, int64_t n, int64_t npert, const cplx* V,
+// (U L^-1/2)^H V = L^-1 G_k^H (Phi V).  Used when m < n = nact^3.  When at most
+// e2_schur_max_dropped() directions are dropped, E2 comes from the compressed resolvent
+// of diag(F + alpha) (k_e2_schur) instead of an eigendecomposition of B~.
+SolveOut solve_block_gram(dc2_context* c, Lane& L, bool create, int64_t n, int64_t npert, const cplx* V,
                           const double* alpha) {
   SolveOut r;
-  hipStream_t s = c->stream;
+  hipStream_t s = L.stream;
   const int na = c->na;
   const int64_t m = binom_host(na, create ? c->ne + 1 : c->ne - 1);
   const double thr = 1.0e-8;
   if (n > m) r.margin = thr;   // the n - m eigenvalues of S that vanish identically
   if (m == 0) return r;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  DevBuf<int64_t> strings(m);
-  DevBuf<cplx> Phi(m * n), G(m * m);
-  DevBuf<double> f(m), lam(m);
-  build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), c->Y1.get(), c->eps.get() + c->ni, m,
-            strings.get(), Phi.get(), f.get());
-  c->gstats.tag = 4;
-  zgemm(s, 'N', 'C', m, m, n, one, Phi.get(), m, 0, Phi.get(), m, 0, zero, G.get(), m, 0, 1);
-  c->gstats.tag = 0;
-  check_finite(c, "Phi Phi^H", G.get(), m * m);
-  zheevd(c, (int)m, G.get(), lam.get(), "a determinant-space metric");
-  const int64_t dimm = count_retained(c, m, lam.get(), r.margin);
+  int64_t* strings = L.l("g_strings", m);
+  cplx* Phi = L.z("g_Phi", m * n);
+  cplx* G = L.z("g_G", m * m);
+  double* f = L.d("g_f", m);
+  double* lam = L.d("g_lam", m);
+  build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), c->Y1.get(), c->eps.get() + c->ni, m, strings, Phi,
+            f);
+  set_gemm_tag(4);
+  zgemm(s, 'N', 'C', m, m, n, one, Phi, m, 0, Phi, m, 0, zero, G, m, 0, 1);
+  set_gemm_tag(0);
+  check_finite(c, s, "Phi Phi^H", G, m * m);
+  zheevd(c, L, (int)m, G, lam, "a determinant-space metric");
+  const int64_t dimm = count_retained(L, m, lam, r.margin);
   r.dimm = dimm;
   if (dimm == 0 || npert == 0) return r;
-  const cplx* Gk = G.get() + m * (m - dimm);
-  DevBuf<cplx> FG(m * dimm), GL(m * dimm), bc1(dimm * dimm), P(m * npert), Y(dimm * npert);
-  scale_rows(s, m, dimm, f.get(), Gk, FG.get());
-  scale_cols_tail(s, m, dimm, G.get(), lam.get(), GL.get(), true);
-  c->gstats.tag = 4;
-  zgemm(s, 'C', 'N', dimm, dimm, m, one, Gk, m, 0, FG.get(), m, 0, zero, bc1.get(), dimm, 0, 1);
-  zgemm(s, 'N', 'N', m, npert, n, one, Phi.get(), m, 0, V, n, 0, zero, P.get(), m, 0, 1);
-  zgemm(s, 'C', 'N', dimm, npert, m, one, GL.get(), m, 0, P.get(), m, 0, zero, Y.get(), dimm, 0, 1);
-  c->gstats.tag = 0;
-  finish_block(c, dimm, npert, bc1.get(), Y.get(), alpha, r);
+  const int64_t nd = m - dimm;
+  const cplx* Gk = G + m * nd;
+  cplx* GL = L.z("g_GL", m * dimm);
+  cplx* P = L.z("g_P", m * npert);
+  cplx* Y = L.z("g_Y", dimm * npert);
+  scale_cols_tail(s, m, dimm, G, lam, GL, true);
+  set_gemm_tag(4);
+  zgemm(s, 'N', 'N', m, npert, n, one, Phi, m, 0, V, n, 0, zero, P, m, 0, 1);
+  zgemm(s, 'C', 'N', dimm, npert, m, one, GL, m, 0, P, m, 0, zero, Y, dimm, 0, 1);
+  set_gemm_tag(0);
+  if (c->schur && nd <= e2_schur_max_dropped()) {
+    cplx* Z = L.z("g_Z", m * npert);
+    set_gemm_tag(4);
+    zgemm(s, 'N', 'N', m, npert, dimm, one, Gk, m, 0, Y, dimm, 0, zero, Z, m, 0, 1);
+    set_gemm_tag(0);
+    double* partial = L.d("partial", 2 * npert);
+    double* out2 = L.d("out2", 2);
+    e2_schur(s, m, (int)nd, npert, G, Z, f, alpha, partial, out2);
+    read_out2(L, out2, r);
+    if (std::isfinite(r.e2) && std::isfinite(r.sumc2)) return r;
+    // a singular compressed system: fall through to the eigendecomposition of B~
+  }
+  cplx* FG = L.z("g_FG", m * dimm);
+  cplx* bc1 = L.z("g_bc1", dimm * dimm);
+  scale_rows(s, m, dimm, f, Gk, FG);
+  set_gemm_tag(4);
+  zgemm(s, 'C', 'N', dimm, dimm, m, one, Gk, m, 0, FG, m, 0, zero, bc1, dimm, 0, 1);
+  set_gemm_tag(0);
+  finish_block(c, L, dimm, npert, bc1, Y, alpha, r);
   return r;
+}
+
+// one subspace k (0..7 = A..H) on lane L: S/B/V/alpha assembly and the solve
+void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, double* sumc2) {
+  hipStream_t s = L.stream;
+  const int ni = c->ni, na = c->na, ns = c->ns, nmo = c->nmo();
+  const int64_t na2 = (int64_t)na * na, na3 = na2 * na;
+  const int64_t ntp = (int64_t)na * (na - 1) / 2, nip = (int64_t)ni * (ni - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const double* eps_act = c->eps.get() + ni;
+  Dens d{na, c->G1.get(), c->G1F.get(), c->G2.get(), c->G2F.get(), c->G3.get(), c->G3F.get()};
+  auto blk = [&](const char* name) { return c->tblk.at(name).get(); };
+  const char X = kSub[k];
+  Timer tx(s);
+  SolveOut r;
+  int64_t n = 0, npert = 0;
+  if (X == 'H') {
+    // eps laid out [secondary..., inactive...] for the kernel
+    double* ep = L.d("H_eps", ns + ni);
+    DC2_HIP(hipMemcpyAsync(ep, c->eps.get() + ni + na, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+    DC2_HIP(hipMemcpyAsync(ep + ns, c->eps.get(), ni * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const int64_t np = e2_partials_needed(nsp * nip);
+    double* partial = L.d("partial", 2 * np);
+    double* out2 = L.d("out2", 2);
+    e2_H(s, ni, ns, blk("H"), ep, eshift, c->ipairs.get(), c->spairs.get(), partial, np, out2);
+    read_out2(L, out2, r);
+    npert = nsp * nip;
+  } else {
+    switch (X) {
+      case 'A': n = na3; npert = ni; break;
+      case 'B': n = ntp; npert = nip; break;
+      case 'C': n = na3; npert = ns; break;
+      case 'D': n = na2; npert = (int64_t)ns * ni; break;
+      case 'E': n = na; npert = nip * ns; break;
+      case 'F': n = ntp; npert = nsp; break;
+      case 'G': n = na; npert = nsp * ni; break;
+    }
+    // A and C: determinant-space metric when the (nelec+-1)-electron space is smaller than nact^3
+    const bool gram = (X == 'A' || X == 'C') && c->ac_route != 1 &&
+                      (c->ac_route == 2 || binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) < n);
+    cplx* S = gram ? nullptr : L.z("S", n * n);
+    cplx* B = gram ? nullptr : L.z("B", n * n);
+    cplx* V = L.z("V", n * npert);
+    double* alpha = L.d("alpha", npert);
+    const int* pairs = (X == 'B' || X == 'F') ? c->tpairs.get() : nullptr;
+    if (n > 0 && !gram) build_SB(s, X, d, eps_act, c->e0.get(), n, pairs, S, B);
+    if (n > 0 && npert > 0) {
+      switch (X) {
+        case 'A': {
+          cplx* Dm = L.z("A_Dm", na3 * na3);
+          cplx* Am = L.z("A_Am", na3 * ni);
+          cplx* eff = L.z("A_eff", na * ni);
+          const cplx* A1 = blk("A1");
+          effh_A(s, ni, na, c->hp.get(), nmo, blk("A2"), eff);
+          d3_perm_A(s, na, c->G3.get(), Dm);
+          a1_perm(s, na, ni, A1, Am);
+          set_gemm_tag(3);
+          zgemm(s, 'N', 'N', na3, ni, na3, mk(-1.0, 0.0), Dm, na3, 0, Am, na3, 0, zero, V, na3, 0, 1);
+          set_gemm_tag(0);
+          V_A_rest(s, na, ni, d, A1, eff, V);
+        } break;
+        case 'B': V_B(s, na, ni, d, blk("B"), c->tpairs.get(), c->ipairs.get(), V); break;
+        case 'C': {
+          cplx* Dm = L.z("C_Dm", na3 * na3);
+          cplx* eff = L.z("C_eff", (int64_t)ns * na);
+          const cplx* C1 = blk("C1");
+          if (ni) effh_C(s, ni, na, ns, c->hp.get(), nmo, C1, blk("C2"), blk("C3"), eff);
+          else effh_C(s, 0, na, ns, c->hp.get(), nmo, C1, C1, C1, eff);
+          d3_perm_C(s, na, c->G3.get(), Dm);
+          set_gemm_tag(3);
+          zgemm(s, 'N', 'T', na3, ns, na3, one, Dm, na3, 0, C1, ns, 0, zero, V, na3, 0, 1);
+          set_gemm_tag(0);
+          V_C_rest(s, na, ns, d, eff, V);
+        } break;
+        case 'D': {
+          cplx* eff = L.z("D_eff", (int64_t)ns * ni);
+          effh_D(s, ni, na, ns, c->hp.get(), nmo, blk("D3"), eff);
+          V_D(s, na, ni, ns, d, blk("D1"), blk("D2"), eff, V);
+        } break;
+        case 'E': V_E(s, na, ni, ns, d, blk("E"), c->ipairs.get(), V); break;
+        case 'F': V_F(s, na, ns, d, blk("F"), c->tpairs.get(), c->spairs.get(), V); break;
+        case 'G': V_G(s, na, ni, ns, d, blk("G"), c->spairs.get(), V); break;
+      }
+      alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert, alpha);
+    }
+    const std::string tag(1, X);
+    if (!gram) {
+      check_finite(c, s, ("S" + tag).c_str(), S, n * n);
+      check_finite(c, s, ("B" + tag).c_str(), B, n * n);
+    }
+    check_finite(c, s, ("V" + tag).c_str(), V, n * npert);
+    check_finite(c, s, ("alpha" + tag).c_str(), alpha, npert);
+    const EigKind kind = (X == 'A' || X == 'C') ? EigKind::rank_deficient : EigKind::general;
+    r = gram ? solve_block_gram(c, L, X == 'A', n, npert, V, alpha)
+             : solve_block(c, L, n, npert, S, B, V, alpha, kind);
+  }
+  e2[k] = r.e2;
+  sumc2[k] = r.sumc2;
+  c->sub_info[4 * k + 0] = n;
+  c->sub_info[4 * k + 1] = r.dimm;
+  c->sub_info[4 * k + 2] = npert;
+  c->sub_info[4 * k + 3] = 1;
+  c->sub_margin[k] = r.margin;
+  c->timings[4 + k] = tx.stop();
+}
+
+// worker lane i (1-based), created on first use
+Lane& worker_lane(dc2_context* c, int i) {
+  while ((int)c->lanes.size() < i) {
+    auto L = std::make_unique<Lane>();
+    L->owned = true;
+    DC2_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+    DC2_ROCSOLVER(rocblas_create_handle(&L->blas));
+    DC2_ROCSOLVER(rocblas_set_stream(L->blas, L->stream));
+    c->lanes.push_back(std::move(L));
+  }
+  return *c->lanes[i - 1];
 }
 
 void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, double* sumc2, double* scal) {
@@ -555,7 +765,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
 
   // ---- densities of the rotated state (density_matrix.F90)
   Timer t2(s);
-  c->gstats.tag = 2;
+  set_gemm_tag(2);
   const double* eps_act = c->eps.get() + ni;
   embed_ci(s, na, nf, nd, c->cas2full.get(), c->cnew.get(), c->full_dets.get(), eps_act, c->c_full.get(),
            c->cF.get());
@@ -592,7 +802,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     zgemm(s, 'N', 'N', nmo, nmo, nmo, one, T.get(), nmo, 0, c->U.get(), nmo, 0, zero, c->hp.get(), nmo, 0, 1);
     DC2_HIP(hipStreamSynchronize(s));
   }
-  c->gstats.tag = 0;
+  set_gemm_tag(0);
   c->timings[2] = t2.stop();
   check_finite(c, "e0", c->e0.get(), 1);
   check_finite(c, "G1", c->G1.get(), na2);
@@ -616,115 +826,58 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   if (doF) need.push_back("F");
   if (doG) need.push_back("G");
   if (doH) need.push_back("H");
-  c->gstats.tag = 1;
+  set_gemm_tag(1);
   for (const auto& nm : need) transform(c, nm);
-  c->gstats.tag = 0;
+  set_gemm_tag(0);
   c->timings[3] = t3.stop();
   for (const auto& nm : need) check_finite(c, ("transformed " + nm).c_str(), c->tblk[nm].get(), block_count(c, shape_of(nm)));
 
   upload_ints(s, c->tpairs, pair_list(na));
   upload_ints(s, c->ipairs, pair_list(ni));
   upload_ints(s, c->spairs, pair_list(ns));
-  Dens d{na, c->G1.get(), c->G1F.get(), c->G2.get(), c->G2F.get(), c->G3.get(), c->G3F.get()};
-  const int64_t ntp = (int64_t)na * (na - 1) / 2, nip = (int64_t)ni * (ni - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
-  const int64_t na3 = na2 * na;
   for (int k = 0; k < 8; ++k) { e2[k] = 0.0; sumc2[k] = 0.0; }
 
+  // subspaces: A and C (the large metrics) on their own lanes, the rest on the library
+  // stream; every lane waits for the transforms and densities above
+  std::vector<std::vector<int>> plan(std::max(1, std::min(c->nlanes, 3)));
   for (int k = 0; k < 8; ++k) {
     if (!do_sub[k]) continue;
-    const char X = kSub[k];
-    Timer tx(s);
-    SolveOut r;
-    int64_t n = 0, npert = 0;
-    if (X == 'H') {
-      // eps laid out [secondary..., inactive...] for the kernel
-      DevBuf<double> ep(ns + ni);
-      DC2_HIP(hipMemcpyAsync(ep.get(), c->eps.get() + ni + na, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
-      DC2_HIP(hipMemcpyAsync(ep.get() + ns, c->eps.get(), ni * sizeof(double), hipMemcpyDeviceToDevice, s));
-      const int64_t np = e2_partials_needed(nsp * nip);
-      c->partial.ensure(2 * np);
-      c->out2.ensure(2);
-      e2_H(s, ni, ns, c->tblk["H"].get(), ep.get(), eshift, c->ipairs.get(), c->spairs.get(), c->partial.get(), np,
-           c->out2.get());
-      double o[2];
-      DC2_HIP(hipMemcpyAsync(o, c->out2.get(), sizeof(o), hipMemcpyDeviceToHost, s));
-      DC2_HIP(hipStreamSynchronize(s));
-      r.e2 = o[0];
-      r.sumc2 = o[1];
-      npert = nsp * nip;
-    } else {
-      switch (X) {
-        case 'A': n = na3; npert = ni; break;
-        case 'B': n = ntp; npert = nip; break;
-        case 'C': n = na3; npert = ns; break;
-        case 'D': n = na2; npert = (int64_t)ns * ni; break;
-        case 'E': n = na; npert = nip * ns; break;
-        case 'F': n = ntp; npert = nsp; break;
-        case 'G': n = na; npert = nsp * ni; break;
-      }
-      // A and C: determinant-space metric when the (nelec+-1)-electron space is smaller than nact^3
-      const bool gram = (X == 'A' || X == 'C') && c->ac_route != 1 &&
-                        (c->ac_route == 2 || binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) < n);
-      DevBuf<cplx> S(gram ? 0 : n * n), B(gram ? 0 : n * n), V(std::max<int64_t>(n * npert, 1));
-      DevBuf<double> alpha(std::max<int64_t>(npert, 1));
-      const int* pairs = (X == 'B' || X == 'F') ? c->tpairs.get() : nullptr;
-      if (n > 0 && !gram) build_SB(s, X, d, eps_act, c->e0.get(), n, pairs, S.get(), B.get());
-      if (n > 0 && npert > 0) {
-        switch (X) {
-          case 'A': {
-            DevBuf<cplx> Dm(na3 * na3), Am(na3 * ni), eff(na * ni);
-            const cplx* A1 = c->tblk["A1"].get();
-            effh_A(s, ni, na, c->hp.get(), nmo, c->tblk["A2"].get(), eff.get());
-            d3_perm_A(s, na, c->G3.get(), Dm.get());
-            a1_perm(s, na, ni, A1, Am.get());
-            c->gstats.tag = 3;
-            zgemm(s, 'N', 'N', na3, ni, na3, mk(-1.0, 0.0), Dm.get(), na3, 0, Am.get(), na3, 0, zero, V.get(), na3, 0, 1);
-            c->gstats.tag = 0;
-            V_A_rest(s, na, ni, d, A1, eff.get(), V.get());
-          } break;
-          case 'B': V_B(s, na, ni, d, c->tblk["B"].get(), c->tpairs.get(), c->ipairs.get(), V.get()); break;
-          case 'C': {
-            DevBuf<cplx> Dm(na3 * na3), eff((int64_t)ns * na);
-            const cplx* C1 = c->tblk["C1"].get();
-            if (ni) effh_C(s, ni, na, ns, c->hp.get(), nmo, C1, c->tblk["C2"].get(), c->tblk["C3"].get(), eff.get());
-            else effh_C(s, 0, na, ns, c->hp.get(), nmo, C1, C1, C1, eff.get());
-            d3_perm_C(s, na, c->G3.get(), Dm.get());
-            c->gstats.tag = 3;
-            zgemm(s, 'N', 'T', na3, ns, na3, one, Dm.get(), na3, 0, C1, ns, 0, zero, V.get(), na3, 0, 1);
-            c->gstats.tag = 0;
-            V_C_rest(s, na, ns, d, eff.get(), V.get());
-          } break;
-          case 'D': {
-            DevBuf<cplx> eff((int64_t)ns * ni);
-            effh_D(s, ni, na, ns, c->hp.get(), nmo, c->tblk["D3"].get(), eff.get());
-            V_D(s, na, ni, ns, d, c->tblk["D1"].get(), c->tblk["D2"].get(), eff.get(), V.get());
-          } break;
-          case 'E': V_E(s, na, ni, ns, d, c->tblk["E"].get(), c->ipairs.get(), V.get()); break;
-          case 'F': V_F(s, na, ns, d, c->tblk["F"].get(), c->tpairs.get(), c->spairs.get(), V.get()); break;
-          case 'G': V_G(s, na, ni, ns, d, c->tblk["G"].get(), c->spairs.get(), V.get()); break;
-        }
-        alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert,
-               alpha.get());
-      }
-      const std::string tag(1, X);
-      if (!gram) {
-        check_finite(c, ("S" + tag).c_str(), S.get(), n * n);
-        check_finite(c, ("B" + tag).c_str(), B.get(), n * n);
-      }
-      check_finite(c, ("V" + tag).c_str(), V.get(), n * npert);
-      check_finite(c, ("alpha" + tag).c_str(), alpha.get(), npert);
-      r = gram ? solve_block_gram(c, X == 'A', n, npert, V.get(), alpha.get())
-               : solve_block(c, n, npert, S.get(), B.get(), V.get(), alpha.get());
-    }
-    e2[k] = r.e2;
-    sumc2[k] = r.sumc2;
-    c->sub_info[4 * k + 0] = n;
-    c->sub_info[4 * k + 1] = r.dimm;
-    c->sub_info[4 * k + 2] = npert;
-    c->sub_info[4 * k + 3] = 1;
-    c->sub_margin[k] = r.margin;
-    c->timings[4 + k] = tx.stop();
+    const int lane = plan.size() == 1 ? 0 : (k == 0 ? 1 : (k == 2 ? (int)plan.size() - 1 : 0));
+    plan[lane].push_back(k);
   }
+  hipEvent_t ready = nullptr;
+  DC2_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  DC2_HIP(hipEventRecord(ready, s));
+  std::vector<std::exception_ptr> errs(plan.size());
+  std::vector<Lane*> lane_of(plan.size(), c->lane0.get());
+  for (size_t w = 1; w < plan.size(); ++w) {   // all lanes exist before any thread starts
+    if (plan[w].empty()) continue;
+    lane_of[w] = &worker_lane(c, (int)w);
+    DC2_HIP(hipStreamWaitEvent(lane_of[w]->stream, ready, 0));
+  }
+  std::vector<std::thread> threads;
+  for (size_t w = 1; w < plan.size(); ++w) {
+    if (plan[w].empty()) continue;
+    threads.emplace_back([&, w]() {
+      try {
+        DC2_HIP(hipSetDevice(c->device));
+        Lane& Lw = *lane_of[w];
+        for (int k : plan[w]) run_subspace(c, Lw, k, eshift, e2, sumc2);
+        DC2_HIP(hipStreamSynchronize(Lw.stream));
+      } catch (...) {
+        errs[w] = std::current_exception();
+      }
+    });
+  }
+  try {
+    for (int k : plan[0]) run_subspace(c, *c->lane0, k, eshift, e2, sumc2);
+  } catch (...) {
+    errs[0] = std::current_exception();
+  }
+  for (auto& t : threads) t.join();
+  (void)hipEventDestroy(ready);
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
   double e0h = 0.0;
   DC2_HIP(hipMemcpyAsync(&e0h, c->e0.get(), sizeof(double), hipMemcpyDeviceToHost, s));
   DC2_HIP(hipStreamSynchronize(s));
@@ -756,7 +909,7 @@ int dc2_context_create(int32_t device, dc2_context** out) {
     c->debug = dbg && dbg[0] && dbg[0] != '0';
     if (const char* em = std::getenv("DC2_EIGEN")) {
       const std::string m(em);
-      c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : 2;
+      c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : -1;
     }
     if (const char* r = std::getenv("DC2_AC_METRIC")) {
       const std::string m(r);
@@ -766,6 +919,10 @@ int dc2_context_create(int32_t device, dc2_context** out) {
     DC2_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     DC2_ROCSOLVER(rocblas_create_handle(&c->blas));
     DC2_ROCSOLVER(rocblas_set_stream(c->blas, c->stream));
+    c->lane0 = std::make_unique<Lane>();
+    c->lane0->stream = c->stream;
+    c->lane0->blas = c->blas;
+    if (const char* nl = std::getenv("DC2_LANES")) c->nlanes = std::max(1, std::min(3, std::atoi(nl)));
     *out = c.release();
   });
 }
@@ -1075,9 +1232,9 @@ int dc2_set_integral_factors(dc2_context* c, const double* B, int64_t naux, int3
       b.count = m1 * m2;
       b.own.alloc(b.count);
       b.ptr = b.own.get();
-      c->gstats.tag = 5;
+      set_gemm_tag(5);
       zgemm(s, 'T', 'N', m1, m2, naux, one, L.get(), naux, 0, R.get(), naux, 0, zero, b.own.get(), m1, 0, 1);
-      c->gstats.tag = 0;
+      set_gemm_tag(0);
       DC2_HIP(hipStreamSynchronize(s));
     }
     for (const char* k : {"A1", "A2", "B"}) {
@@ -1095,7 +1252,7 @@ int dc2_set_profiling(dc2_context* c, int32_t on) {
     for (auto& r : c->gstats.rec) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     c->gstats.rec.clear();
     c->gstats.on = on != 0;
-    c->gstats.tag = 0;
+    set_gemm_tag(0);
     set_gemm_stats(on ? &c->gstats : nullptr);
   });
 }
@@ -1123,11 +1280,17 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     DC2_CHECK(c && key, 1, "null argument");
     const std::string k(key);
     if (k == "eigensolver") {
-      DC2_CHECK(value >= 0 && value <= 3, 1, "eigensolver must be 0..3");
+      DC2_CHECK(value >= -1 && value <= 3, 1, "eigensolver must be -1..3");
       c->eig_method = (int)value;
     } else if (k == "ac_metric") {
       DC2_CHECK(value >= 0 && value <= 2, 1, "ac_metric must be 0..2");
       c->ac_route = (int)value;
+    } else if (k == "lanes") {
+      DC2_CHECK(value >= 1 && value <= 3, 1, "lanes must be 1..3");
+      c->nlanes = (int)value;
+    } else if (k == "schur") {
+      DC2_CHECK(value == 0 || value == 1, 1, "schur must be 0 or 1");
+      c->schur = value != 0;
     } else {
       throw Error(1, "unknown option '" + k + "'");
     }

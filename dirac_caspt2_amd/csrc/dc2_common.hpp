@@ -93,10 +93,11 @@ struct GemmRecord {
 };
 struct GemmStats {
   bool on = false;
-  int tag = 0;
   std::vector<GemmRecord> rec;
 };
 void set_gemm_stats(GemmStats* st);
+// tag of the GEMM launches issued by the calling host thread
+void set_gemm_tag(int tag);
 
 // C[m,n] = alpha * sum_k op(A)[m,k] op(B)[k,n] + beta * C[m,n], batched by strides.
 // opA: 'N' A[m + lda*k], 'T' A[k + lda*m], 'C' conj(A[k + lda*m])

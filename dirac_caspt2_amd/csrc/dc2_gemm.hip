@@ -9,6 +9,8 @@
 // Used for: the four one-index integral transforms (module_intra.F90), the
 // RDM contractions (density_matrix.F90), the V contractions and the
 // (U L^-1/2)^H B (U L^-1/2) products of every solve_X_subspace.
+#include <mutex>
+
 #include "dc2_common.hpp"
 
 namespace dc2 {
@@ -123,6 +125,8 @@ __global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t K, 
 }
 
 GemmStats* g_stats = nullptr;
+std::mutex g_stats_mu;
+thread_local int g_tag = 0;
 
 template <char OPA, char OPB>
 void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda, int64_t sA,
@@ -141,9 +145,10 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
   if (acct) {
     DC2_HIP(hipEventRecord(r.b, s));
     r.flops = 8.0 * (double)M * (double)N * (double)K * (double)batch;
-    r.tag = g_stats->tag;
+    r.tag = g_tag;
     r.opA = OPA;
     r.opB = OPB;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     g_stats->rec.push_back(r);
   }
 }
@@ -162,7 +167,11 @@ void dispatchB(char opB, hipStream_t s, int64_t M, int64_t N, int64_t K, cplx al
 }
 }  // namespace
 
-void set_gemm_stats(GemmStats* st) { g_stats = st; }
+void set_gemm_stats(GemmStats* st) {
+  std::lock_guard<std::mutex> lk(g_stats_mu);
+  g_stats = st;
+}
+void set_gemm_tag(int tag) { g_tag = tag; }
 
 void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A,
            int64_t lda, int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc,

@@ -97,5 +97,11 @@ void e2_reduce(hipStream_t s, int64_t dimm, int64_t npert, const cplx* X, const 
 void e2_H(hipStream_t s, int ni, int ns, const cplx* Hb, const double* eps, double eshift, const int* ipairs,
           const int* spairs, double* partial, int64_t npartial_max, double* out2);
 int64_t e2_partials_needed(int64_t nwork);
+// E2/sumc2 of the determinant-space route without the B~ eigensolve: per perturber k the
+// compressed resolvent of diag(f + alpha_k) on the retained directions Gk (Gd = the r
+// dropped eigenvectors of G, Z = Gk Lambda^-1 Gk^H Phi V); partial needs 2*npert doubles
+int e2_schur_max_dropped();
+void e2_schur(hipStream_t s, int64_t m, int r, int64_t npert, const cplx* Gd, const cplx* Z, const double* f,
+              const double* alpha, double* partial, double* out2);
 
 }  // namespace dc2

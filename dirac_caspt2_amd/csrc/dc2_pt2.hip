@@ -338,11 +338,7 @@ __global__ __launch_bounds__(RB) void k_e2_H_partial(int ni, int ns, const cplx*
                                                      double* partial) {
   __shared__ double sh[2 * RB];
   double e = 0.0, s = 0.0;
-  const int nmoc0 = 0;
-  (void)nmoc0;
   const int64_t S = ns, Ni = ni, total = nsp * nip;
-  const int off = blockIdx.y;  // unused second dimension guard
-  (void)off;
   for (int64_t q = (int64_t)blockIdx.x * RB + threadIdx.x; q < total; q += (int64_t)gridDim.x * RB) {
     const int64_t m = q % nsp, n = q / nsp;
     const int a = sp[2 * m], b = sp[2 * m + 1], i = ip[2 * n], j = ip[2 * n + 1];
@@ -354,6 +350,80 @@ __global__ __launch_bounds__(RB) void k_e2_H_partial(int ni, int ns, const cplx*
   }
   block_sum2(e, s, sh);
   if (threadIdx.x == 0) { partial[2 * blockIdx.x] = e; partial[2 * blockIdx.x + 1] = s; }
+}
+
+// E2 of one perturber k through the compressed resolvent (one workgroup per k).  With
+// G = [Gd | Gk] unitary (Gd: the r dropped directions), D = diag(f + alpha_k) and
+// z = Gk y_k, the retained-space resolvent gives u = (Gk^H D Gk)^-1 y_k = Gk^H x with
+//   x = D^-1 (z - Gd c),  (Gd^H D^-1 Gd) c = Gd^H D^-1 z,
+// so e2_k = -Re(z^H x) = -y^H (B~ + alpha_k)^-1 y and sumc2_k = |x|^2.  The r x r system is
+// solved in LDS by Gaussian elimination with partial pivoting; a zero pivot yields NaN,
+// which the host detects.
+constexpr int SCHUR_RMAX = 32;
+__global__ __launch_bounds__(RB) void k_e2_schur(int64_t m, int r, const cplx* __restrict__ Gd,
+                                                 const cplx* __restrict__ Z, const double* __restrict__ f,
+                                                 const double* __restrict__ alpha, double* partial) {
+  __shared__ cplx W[SCHUR_RMAX][SCHUR_RMAX + 1];   // column r holds the right-hand side
+  __shared__ cplx lf[SCHUR_RMAX], cs[SCHUR_RMAX];
+  __shared__ double sh[2 * RB];
+  __shared__ int piv;
+  const int t = threadIdx.x;
+  const int64_t k = blockIdx.x;
+  const double al = alpha[k];
+  const cplx* z = Z + m * k;
+  // W(a,b) = sum_row conj(Gd(row,a)) Gd(row,b)/D(row), W(a,r) = sum_row conj(Gd(row,a)) z(row)/D(row)
+  for (int o = t; o < r * (r + 1); o += RB) {
+    const int a = o % r, b = o / r;
+    const cplx* gb = b < r ? Gd + m * b : z;
+    const cplx* ga = Gd + m * a;
+    cplx acc = mk(0.0, 0.0);
+    for (int64_t row = 0; row < m; ++row) cfma(acc, cconj(ga[row]), cscale(gb[row], 1.0 / (f[row] + al)));
+    W[a][b] = acc;
+  }
+  __syncthreads();
+  for (int j = 0; j < r; ++j) {
+    if (t == 0) {
+      int p = j;
+      double best = cabs2(W[j][j]);
+      for (int i = j + 1; i < r; ++i)
+        if (cabs2(W[i][j]) > best) { best = cabs2(W[i][j]); p = i; }
+      piv = p;
+    }
+    __syncthreads();
+    if (piv != j)
+      for (int b = t; b <= r; b += RB) { const cplx tmp = W[j][b]; W[j][b] = W[piv][b]; W[piv][b] = tmp; }
+    __syncthreads();
+    const cplx d = W[j][j];
+    const double dd = cabs2(d);
+    for (int i = j + 1 + t; i < r; i += RB)
+      lf[i] = dd > 0.0 ? cscale(cmul(W[i][j], cconj(d)), 1.0 / dd) : mk(NAN, NAN);
+    __syncthreads();
+    for (int o = t; o < (r - j - 1) * (r - j); o += RB) {
+      const int i = j + 1 + o % (r - j - 1), b = j + 1 + o / (r - j - 1);
+      W[i][b] = csub(W[i][b], cmul(lf[i], W[j][b]));
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int i = r - 1; i >= 0; --i) {
+      cplx acc = W[i][r];
+      for (int b = i + 1; b < r; ++b) acc = csub(acc, cmul(W[i][b], cs[b]));
+      const double dd = cabs2(W[i][i]);
+      cs[i] = dd > 0.0 ? cscale(cmul(acc, cconj(W[i][i])), 1.0 / dd) : mk(NAN, NAN);
+    }
+  }
+  __syncthreads();
+  double e = 0.0, s = 0.0;
+  for (int64_t row = t; row < m; row += RB) {
+    cplx acc = z[row];
+    for (int a = 0; a < r; ++a) acc = csub(acc, cmul(Gd[row + m * a], cs[a]));
+    const cplx x = cscale(acc, 1.0 / (f[row] + al));
+    const cplx zx = cmul(cconj(z[row]), x);
+    e -= zx.x;
+    s += cabs2(x);
+  }
+  block_sum2(e, s, sh);
+  if (t == 0) { partial[2 * k] = e; partial[2 * k + 1] = s; }
 }
 
 __global__ __launch_bounds__(RB) void k_final2(int64_t np, const double* partial, double* out2) {
@@ -483,6 +553,15 @@ void e2_reduce(hipStream_t s, int64_t dimm, int64_t npert, const cplx* X, const 
   if (nb > npartial_max) nb = npartial_max;
   hipLaunchKernelGGL(k_e2_partial, dim3((unsigned)nb), dim3(RB), 0, s, dimm, npert, X, wb, alpha, partial);
   hipLaunchKernelGGL(k_final2, dim3(1), dim3(RB), 0, s, nb, partial, out2);
+  DC2_HIP(hipGetLastError());
+}
+int e2_schur_max_dropped() { return SCHUR_RMAX; }
+void e2_schur(hipStream_t s, int64_t m, int r, int64_t npert, const cplx* Gd, const cplx* Z, const double* f,
+              const double* alpha, double* partial, double* out2) {
+  DC2_CHECK(r >= 0 && r <= SCHUR_RMAX, 1, "e2_schur: too many dropped directions");
+  DC2_CHECK(npert >= 1 && npert < (1ll << 31), 1, "e2_schur: perturber count out of range");
+  hipLaunchKernelGGL(k_e2_schur, dim3((unsigned)npert), dim3(RB), 0, s, m, r, Gd, Z, f, alpha, partial);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(RB), 0, s, npert, partial, out2);
   DC2_HIP(hipGetLastError());
 }
 void e2_H(hipStream_t s, int ni, int ns, const cplx* Hb, const double* eps, double eshift, const int* ipairs,

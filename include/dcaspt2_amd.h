@@ -136,10 +136,16 @@ int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
 /* the same figures for the launches of one kernel instantiation k_zgemm<opA, opB> */
 int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
 /* Numerical options (no counterpart in the reference):
- *   "eigensolver": 0 tridiagonal QR (the ZHEEV algorithm), 1 divide and conquer,
- *                  2 divide and conquer with Jacobi leaves (default), 3 Jacobi;
+ *   "eigensolver": -1 automatic (default: divide and conquer, with Jacobi leaves for the
+ *                  rank-deficient nact^3 S matrices), 0 tridiagonal QR (the ZHEEV
+ *                  algorithm), 1 divide and conquer, 2 D&C with Jacobi leaves, 3 Jacobi;
  *   "ac_metric":   0 automatic (default), 1 the RDM-formula S/B matrices of size nact^3,
- *                  2 the determinant-space factorisation of the A and C metrics. */
+ *                  2 the determinant-space factorisation of the A and C metrics;
+ *   "lanes":       1..3 concurrent streams (each with a host thread) for the subspace
+ *                  solves (default 3: A, C and the rest);
+ *   "schur":       1 (default) E2 of the determinant-space route through the compressed
+ *                  resolvent when at most 32 directions are cut, 0 always through the
+ *                  eigendecomposition of B~. */
 int dc2_set_option(dc2_context* ctx, const char* key, int64_t value);
 /* number of eigensolves that fell back to the tridiagonal QR algorithm */
 int dc2_get_eig_fallbacks(dc2_context* ctx, int64_t* n);
