@@ -51,6 +51,14 @@ _SIGS = {
     "dc2_set_profiling": (c_int, [P, c_int]),
     "dc2_set_option": (c_int, [P, ctypes.c_char_p, c_i64]),
     "dc2_get_gemm_op_stats": (c_int, [P, ctypes.c_char, ctypes.c_char, P]),
+    "dc2_get_integral_block_device": (c_int, [P, ctypes.c_char_p, P, P]),
+    "dc2_request_footprint": (c_int, [c_int, c_int, c_int, c_int, P]),
+    "dc2_server_create": (c_int, [c_int, c_int, c_i64, c_int, P]),
+    "dc2_server_destroy": (c_int, [P]),
+    "dc2_server_add_molecule": (c_int, [P, c_int, c_int, c_int, c_int, P, c_i64, P]),
+    "dc2_server_submit": (c_int, [P, c_int, P, ctypes.c_double, c_int, ctypes.c_double, P, c_int, P]),
+    "dc2_server_wait": (c_int, [P, ctypes.c_double, P, P, P]),
+    "dc2_server_stats": (c_int, [P, P]),
     "dc2_get_gemm_stats": (c_int, [P, c_int, P]),
     "dc2_get_eig_fallbacks": (c_int, [P, P]),
 }

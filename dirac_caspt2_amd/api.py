@@ -308,3 +308,94 @@ def needed_blocks(dims) -> List[str]:
         else:
             names += ["F"]
     return [n for n in names if all(v > 0 for v in block_shape(dims, n))]
+
+
+@dataclass
+class ServedResult:
+    id: int
+    status: int
+    e2: Dict[str, float]
+    sumc2: Dict[str, float]
+    e0: float
+    e2all: float
+    sumc2_all: float
+    weight0: float
+    total2: float
+    total: float
+    t_submit: float
+    t_start: float
+    t_end: float
+    slot: int
+    error: str = ""
+
+    @property
+    def latency(self) -> float:
+        return self.t_end - self.t_submit
+
+
+class Server:
+    """Continuous-batching request server of the library (dc2_server_*): nslots concurrent
+    energies on one GPU, molecules resident in HBM, admission by device-memory footprint."""
+
+    def __init__(self, device: int = 0, nslots: int = 4, mem_budget: int = 200 << 30, lanes_per_slot: int = 1):
+        self.lib = _lib.lib()
+        h = ctypes.c_void_p()
+        check("dc2_server_create", self.lib.dc2_server_create(device, nslots, int(mem_budget), lanes_per_slot,
+                                                              ctypes.byref(h)))
+        self.h = h.value
+        self.dims: Dict[int, tuple] = {}
+
+    def add_molecule(self, dims, B: np.ndarray) -> int:
+        """dims = (ninact, nact, nsec, nelec); (pq|rs) = sum_P B[P,p,q] B[P,r,s]."""
+        Bf = np.asfortranarray(np.asarray(B, dtype=np.complex128))
+        mid = ctypes.c_int32()
+        check("dc2_server_add_molecule", self.lib.dc2_server_add_molecule(self.h, *[int(x) for x in dims], _ptr(Bf),
+                                                                          Bf.shape[0], ctypes.byref(mid)))
+        self.dims[mid.value] = tuple(dims)
+        return mid.value
+
+    def submit(self, mol: int, h: np.ndarray, ecore: float, selectroot: int = 1, eshift: float = 0.0,
+               ras=None) -> int:
+        hf = _fortran_complex(h)
+        r = None if ras is None else np.ascontiguousarray(ras, dtype=np.int32)
+        rid = ctypes.c_int64()
+        check("dc2_server_submit", self.lib.dc2_server_submit(self.h, mol, _ptr(hf), float(ecore), selectroot,
+                                                              float(eshift), None if r is None else _ptr(r), 0,
+                                                              ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, timeout: float = 1.0) -> Optional[ServedResult]:
+        rid = ctypes.c_int64()
+        st = ctypes.c_int32()
+        out = np.zeros(26)
+        check("dc2_server_wait", self.lib.dc2_server_wait(self.h, float(timeout), ctypes.byref(rid),
+                                                          ctypes.byref(st), _ptr(out)))
+        if rid.value == 0:
+            return None
+        err = self.lib.dc2_last_error().decode() if st.value else ""
+        return ServedResult(rid.value, st.value, dict(zip(SUBSPACES, out[0:8])), dict(zip(SUBSPACES, out[8:16])),
+                            *[float(x) for x in out[16:22]], float(out[22]), float(out[23]), float(out[24]),
+                            int(out[25]), err)
+
+    def stats(self) -> Dict[str, int]:
+        out = np.zeros(5, dtype=np.int64)
+        check("dc2_server_stats", self.lib.dc2_server_stats(self.h, _ptr(out)))
+        return dict(zip(("pending", "running", "completed", "in_use", "peak"), (int(x) for x in out)))
+
+    def close(self):
+        if self.h:
+            check("dc2_server_destroy", self.lib.dc2_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def request_footprint(ninact: int, nact: int, nsec: int, nelec: int) -> int:
+    """Device bytes one served request holds while it runs (the admission estimate)."""
+    b = ctypes.c_int64()
+    check("dc2_request_footprint", _lib.lib().dc2_request_footprint(ninact, nact, nsec, nelec, ctypes.byref(b)))
+    return b.value

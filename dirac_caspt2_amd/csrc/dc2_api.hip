@@ -55,6 +55,7 @@ struct Lane {
   std::map<std::string, DevBuf<cplx>> zb;
   std::map<std::string, DevBuf<double>> db;
   std::map<std::string, DevBuf<int64_t>> lb;
+  std::map<std::string, DevBuf<int>> ib;
   DevBuf<int> info;
   cplx* z(const char* name, int64_t n) {
     DevBuf<cplx>& b = zb[name];
@@ -71,6 +72,11 @@ struct Lane {
     b.ensure((size_t)std::max<int64_t>(n, 1));
     return b.get();
   }
+  int* i(const char* name, int64_t n) {
+    DevBuf<int>& b = ib[name];
+    b.ensure((size_t)std::max<int64_t>(n, 1));
+    return b.get();
+  }
   Lane() = default;
   Lane(const Lane&) = delete;
   Lane& operator=(const Lane&) = delete;
@@ -79,6 +85,7 @@ struct Lane {
     zb.clear();
     db.clear();
     lb.clear();
+    ib.clear();
     info.release();
     if (owned) {
       if (blas) rocblas_destroy_handle(blas);
@@ -345,7 +352,7 @@ std::vector<int> pair_list(int n) {
 }
 
 void upload_ints(hipStream_t s, DevBuf<int>& dst, const std::vector<int>& v) {
-  dst.alloc(std::max<size_t>(v.size(), 1));
+  dst.ensure(std::max<size_t>(v.size(), 1));
   if (!v.empty()) DC2_HIP(hipMemcpyAsync(dst.get(), v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, s));
 }
 
@@ -358,37 +365,43 @@ void run_casci(dc2_context* c) {
   Timer tm(s);
   const int na = c->na;
   const int64_t N = 1ll << na;
-  DevBuf<int> ff(N), fc(N), counts(2), irp(std::max(na, 1)), md(std::max<size_t>(c->multb_d.size(), 1)),
-      mds(std::max<size_t>(c->multb_ds.size(), 1));
+  // scratch from the library lane's grow-only buffers: a hipFree would synchronise every
+  // stream of the device, including other contexts' work
+  Lane& L0 = *c->lane0;
+  int* ff = L0.i("ci_ff", N);
+  int* fc = L0.i("ci_fc", N);
+  int* counts = L0.i("ci_counts", 2);
+  int* irp = L0.i("ci_irp", std::max(na, 1));
+  int* md = L0.i("ci_md", std::max<int64_t>((int64_t)c->multb_d.size(), 1));
+  int* mds = L0.i("ci_mds", std::max<int64_t>((int64_t)c->multb_ds.size(), 1));
   std::vector<int> irp_act(std::max(na, 1), 1);
   for (int w = 0; w < na; ++w) irp_act[w] = c->irpamo.empty() ? 1 : c->irpamo[c->ni + w];
-  DC2_HIP(hipMemcpyAsync(irp.get(), irp_act.data(), irp_act.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  DC2_HIP(hipMemcpyAsync(irp, irp_act.data(), irp_act.size() * sizeof(int), hipMemcpyHostToDevice, s));
   if (!c->multb_d.empty()) {
-    DC2_HIP(hipMemcpyAsync(md.get(), c->multb_d.data(), c->multb_d.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    DC2_HIP(hipMemcpyAsync(mds.get(), c->multb_ds.data(), c->multb_ds.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    DC2_HIP(hipMemcpyAsync(md, c->multb_d.data(), c->multb_d.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    DC2_HIP(hipMemcpyAsync(mds, c->multb_ds.data(), c->multb_ds.size() * sizeof(int), hipMemcpyHostToDevice, s));
   }
   DetParams p{na, c->ne, c->ni, c->ras[0], c->ras[1], c->ras[2], c->ras[3], c->ras[4], c->ras[5], c->ras[6],
-              c->nsym, c->totsym, irp.get(), md.get(), mds.get()};
-  c->rank_full.alloc(N);
-  c->rank_cas.alloc(N);
+              c->nsym, c->totsym, irp, md, mds};
+  c->rank_full.ensure(N);
+  c->rank_cas.ensure(N);
   int64_t nf = 0, nd = 0;
-  enumerate_dets(s, p, ff.get(), fc.get(), c->rank_full.get(), c->rank_cas.get(), nullptr, nullptr, nullptr, &nf, &nd,
-                 counts.get());
+  enumerate_dets(s, p, ff, fc, c->rank_full.get(), c->rank_cas.get(), nullptr, nullptr, nullptr, &nf, &nd, counts);
   DC2_CHECK(nd > 0, 1, "the number of CASCI configurations is 0 (check totsym)");
   c->nfull = nf;
   c->ndet = nd;
-  c->full_dets.alloc(nf);
-  c->cas_dets.alloc(nd);
-  c->cas2full.alloc(nd);
-  enumerate_dets(s, p, ff.get(), fc.get(), c->rank_full.get(), c->rank_cas.get(), c->full_dets.get(),
-                 c->cas_dets.get(), c->cas2full.get(), &nf, &nd, counts.get());
-  c->civec.alloc(nd * nd);
+  c->full_dets.ensure(nf);
+  c->cas_dets.ensure(nd);
+  c->cas2full.ensure(nd);
+  enumerate_dets(s, p, ff, fc, c->rank_full.get(), c->rank_cas.get(), c->full_dets.get(), c->cas_dets.get(),
+                 c->cas2full.get(), &nf, &nd, counts);
+  c->civec.ensure(nd * nd);
   casmat(s, c->ni, na, c->ne, nd, c->cas_dets.get(), c->rank_cas.get(), O, c->nmoc(), c->h.get(), c->nmo(),
          c->civec.get());
-  DevBuf<double> w(nd);
-  zheevd(c, (int)nd, c->civec.get(), w.get(), "the CASCI matrix");
+  double* w = L0.d("ci_w", nd);
+  zheevd(c, (int)nd, c->civec.get(), w, "the CASCI matrix");
   c->ecas.resize(nd);
-  DC2_HIP(hipMemcpyAsync(c->ecas.data(), w.get(), nd * sizeof(double), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipMemcpyAsync(c->ecas.data(), w, nd * sizeof(double), hipMemcpyDeviceToHost, s));
   DC2_HIP(hipStreamSynchronize(s));
   c->have_casci = true;
   c->have_pt2 = false;
@@ -694,18 +707,19 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
 
   Timer t1(s);
   // ---- E_pq tables and the CASCI 1-RDM in the original basis
-  c->exc.alloc(nf * na2);
+  Lane& L0 = *c->lane0;
+  c->exc.ensure(nf * na2);
   build_exc_table(s, na, nf, c->full_dets.get(), c->rank_full.get(), c->exc.get());
-  c->eps.alloc(nmo);
-  c->e0.alloc(1);
+  c->eps.ensure(nmo);
+  c->e0.ensure(1);
   c->c_full.ensure(nf);
   c->cF.ensure(nf);
   c->Y1.ensure(nf * na2);
   c->G1.ensure(na2);
-  DevBuf<double> zeros_act(std::max(na, 1));
-  DC2_HIP(hipMemsetAsync(zeros_act.get(), 0, std::max(na, 1) * sizeof(double), s));
+  double* zeros_act = L0.d("zeros_act", std::max(na, 1));
+  DC2_HIP(hipMemsetAsync(zeros_act, 0, std::max(na, 1) * sizeof(double), s));
   const cplx* c0 = c->civec.get() + (int64_t)(selectroot - 1) * nd;
-  embed_ci(s, na, nf, nd, c->cas2full.get(), c0, c->full_dets.get(), zeros_act.get(), c->c_full.get(), c->cF.get());
+  embed_ci(s, na, nf, nd, c->cas2full.get(), c0, c->full_dets.get(), zeros_act, c->c_full.get(), c->cF.get());
   apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
   zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
   check_finite(c, "the CASCI 1-RDM", c->G1.get(), na2);
@@ -725,38 +739,39 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
         if ((c->irpamo.empty() ? 1 : c->irpamo[i]) == isym) ind.push_back(i);
       if (ind.empty()) continue;
       const int n = (int)ind.size();
-      DevBuf<int> dind;
-      upload_ints(s, dind, ind);
-      DevBuf<cplx> blk((int64_t)n * n), scr((int64_t)n * n);
-      DevBuf<double> w(n);
-      gather_block(s, n, dind.get(), c->fock.get(), nmo, blk.get());
-      zheevd(c, n, blk.get(), w.get(), "a Fock block");
-      canonical_gauge(s, n, w.get(), blk.get(), scr.get());
-      scatter_block(s, n, dind.get(), blk.get(), w.get(), c->U.get(), nmo, c->eps.get());
+      int* dind = L0.i("fd_ind", n);
+      DC2_HIP(hipMemcpyAsync(dind, ind.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+      cplx* blk = L0.z("fd_blk", (int64_t)n * n);
+      cplx* scr = L0.z("fd_scr", (int64_t)n * n);
+      double* w = L0.d("fd_w", n);
+      gather_block(s, n, dind, c->fock.get(), nmo, blk);
+      zheevd(c, n, blk, w, "a Fock block");
+      canonical_gauge(s, n, w, blk, scr);
+      scatter_block(s, n, dind, blk, w, c->U.get(), nmo, c->eps.get());
       DC2_HIP(hipStreamSynchronize(s));
     }
   }
   // ---- tracic: ds c~ = c (trac.F90:76-156)
   {
-    DevBuf<cplx> Ua(na2), ds(nd * nd);
+    cplx* Ua = L0.z("tr_Ua", na2);
+    cplx* ds = L0.z("tr_ds", nd * nd);
     int64_t ld[3] = {nmo, 1, 1};
     int off[4] = {ni, ni, 0, 0}, nn[4] = {na, na, 1, 1};
-    extract4(s, c->U.get(), ld, off, nn, Ua.get());
-    tracic_ds(s, na, c->ne, nd, c->cas_dets.get(), Ua.get(), ds.get());
+    extract4(s, c->U.get(), ld, off, nn, Ua);
+    tracic_ds(s, na, c->ne, nd, c->cas_dets.get(), Ua, ds);
     c->cnew.ensure(nd);
     DC2_HIP(hipMemcpyAsync(c->cnew.get(), c0, nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-    DevBuf<int> ipiv(nd);
+    int* ipiv = L0.i("tr_ipiv", nd);
     c->info.ensure(1);
-    DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds.get()),
-                                   (int)nd, ipiv.get(), c->info.get()));
+    DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds),
+                                   (int)nd, ipiv, c->info.get()));
     int info = 0;
     DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     DC2_HIP(hipStreamSynchronize(s));
     if (info != 0) throw Error(100 + info, "zgetrf (tracic) failed, info = " + std::to_string(info));
     DC2_ROCSOLVER(rocsolver_zgetrs(c->blas, rocblas_operation_none, (int)nd, 1,
-                                   reinterpret_cast<rocblas_double_complex*>(ds.get()), (int)nd, ipiv.get(),
+                                   reinterpret_cast<rocblas_double_complex*>(ds), (int)nd, ipiv,
                                    reinterpret_cast<rocblas_double_complex*>(c->cnew.get()), (int)nd));
-    DC2_HIP(hipStreamSynchronize(s));   // ds / ipiv are released at the end of this scope
   }
   c->timings[1] = t1.stop();
   check_finite(c, "U", c->U.get(), (int64_t)nmo * nmo);
@@ -787,20 +802,18 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     const int64_t na4 = na2 * na2;
     c->G3.ensure(na2 * na4);
     c->G3F.ensure(na2 * na4);
-    DevBuf<cplx> Y2(nf * na4);
-    apply_E(s, na, nf, na2, c->exc.get(), c->Y1.get(), Y2.get());
-    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2.get(), nf, 0, zero, c->G3.get(), na2, 0, 1);
-    apply_E(s, na, nf, na2, c->exc.get(), c->Y1F.get(), Y2.get());
-    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2.get(), nf, 0, zero, c->G3F.get(), na2, 0, 1);
-    DC2_HIP(hipStreamSynchronize(s));
+    cplx* Y2 = L0.z("Y2", nf * na4);
+    apply_E(s, na, nf, na2, c->exc.get(), c->Y1.get(), Y2);
+    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3.get(), na2, 0, 1);
+    apply_E(s, na, nf, na2, c->exc.get(), c->Y1F.get(), Y2);
+    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3F.get(), na2, 0, 1);
   }
   // h' = U^H h U (tramo1)
   c->hp.ensure((int64_t)nmo * nmo);
   {
-    DevBuf<cplx> T((int64_t)nmo * nmo);
-    zgemm(s, 'C', 'N', nmo, nmo, nmo, one, c->U.get(), nmo, 0, c->h.get(), nmo, 0, zero, T.get(), nmo, 0, 1);
-    zgemm(s, 'N', 'N', nmo, nmo, nmo, one, T.get(), nmo, 0, c->U.get(), nmo, 0, zero, c->hp.get(), nmo, 0, 1);
-    DC2_HIP(hipStreamSynchronize(s));
+    cplx* T = L0.z("hT", (int64_t)nmo * nmo);
+    zgemm(s, 'C', 'N', nmo, nmo, nmo, one, c->U.get(), nmo, 0, c->h.get(), nmo, 0, zero, T, nmo, 0, 1);
+    zgemm(s, 'N', 'N', nmo, nmo, nmo, one, T, nmo, 0, c->U.get(), nmo, 0, zero, c->hp.get(), nmo, 0, 1);
   }
   set_gemm_tag(0);
   c->timings[2] = t2.stop();
@@ -894,6 +907,10 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
 
 }  // namespace
 
+namespace dc2 {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace dc2
+
 // ===================================================================== C ABI
 extern "C" {
 
@@ -947,8 +964,7 @@ int dc2_set_spaces(dc2_context* c, int32_t ninact, int32_t nact, int32_t nsec, i
     DC2_CHECK(ninact >= 0 && nact >= 1 && nsec >= 0 && nelec >= 1 && nelec <= nact, 1, "invalid space sizes");
     DC2_CHECK(nact <= 24, 1, "nact > 24 is not supported");
     c->ni = ninact; c->na = nact; c->ns = nsec; c->ne = nelec;
-    c->blocks.clear();
-    c->tblk.clear();
+    c->blocks.clear();   // transformed-block buffers are kept and regrown on demand
     c->have_casci = c->have_pt2 = false;
     c->nsym = 1; c->totsym = 1; c->irpamo.clear(); c->multb_d.clear(); c->multb_ds.clear();
     for (int& r : c->ras) r = 0;
@@ -999,7 +1015,7 @@ int dc2_set_one_electron(dc2_context* c, const double* h, double ecore) {
     DC2_CHECK(c && h, 1, "null argument");
     need_spaces(c);
     const int64_t n = (int64_t)c->nmo() * c->nmo();
-    c->h.alloc(n);
+    c->h.ensure(n);
     DC2_HIP(hipMemcpyAsync(c->h.get(), h, n * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
     DC2_HIP(hipStreamSynchronize(c->stream));
     c->ecore = ecore;
@@ -1272,6 +1288,17 @@ int dc2_get_gemm_stats(dc2_context* c, int32_t tag, double* out) {
       mx = std::max(mx, r.flops);
     }
     out[0] = n; out[1] = sec; out[2] = fl; out[3] = mx;
+  });
+}
+
+int dc2_get_integral_block_device(dc2_context* c, const char* name, const double** ptr, int64_t* count) {
+  return guarded([&] {
+    DC2_CHECK(c && name && ptr && count, 1, "null argument");
+    auto it = c->blocks.find(name);
+    DC2_CHECK(it != c->blocks.end() && it->second.ptr != nullptr, 1, std::string("integral block ") + name +
+                                                                         " is not resident");
+    *ptr = reinterpret_cast<const double*>(it->second.ptr);
+    *count = it->second.count;
   });
 }
 

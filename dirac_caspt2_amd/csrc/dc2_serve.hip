@@ -1,0 +1,322 @@
+// Request server: continuous batching of CASPT2 energies on one GPU.
+//
+// Molecules (spinor spaces + two-electron integrals) are registered once; their
+// integral blocks are formed on the device and stay resident in HBM, shared
+// read-only by every request on that molecule (points of a potential-energy scan
+// differ in their one-electron matrix).  Each of nslots slots owns a dc2_context
+// (its streams, rocBLAS handles and grow-only scratch) and a host thread.  A free
+// slot takes the oldest pending request as soon as the device-memory footprint of
+// the running requests plus its own fits the budget, so requests that arrive while
+// others run are admitted as slots and memory free up, and their kernels overlap
+// on the device.  No counterpart in the reference (its driver runs one problem per
+// process, dcaspt2_run_subprograms.F90:57-62).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/dcaspt2_amd.h"
+
+namespace dc2 {
+void set_last_error(const std::string& msg);   // dc2_api.hip: the message dc2_last_error() returns
+}
+
+namespace {
+
+struct Molecule {
+  dc2_context* ctx = nullptr;
+  int32_t ni = 0, na = 0, ns = 0, ne = 0;
+  int64_t bytes = 0;
+};
+
+struct Request {
+  int64_t id = 0;
+  Molecule* mol = nullptr;
+  int32_t mol_id = 0;
+  std::vector<double> h;
+  double ecore = 0.0, eshift = 0.0;
+  int32_t root = 1, flags = 0, slot = -1;
+  int32_t ras[7] = {0, 0, 0, 0, 0, 0, 0};
+  int64_t footprint = 0;
+  double t_submit = 0, t_start = 0, t_end = 0;
+  int status = 0;
+  std::string err;
+  double e2[8] = {0}, sumc2[8] = {0}, scal[6] = {0};
+};
+
+const char* kBlockNames[] = {"OOOO", "F1", "F2", "A1", "A2", "B", "C1", "C2", "C3",
+                             "D1",   "D2", "D3", "E", "F",  "G", "H"};
+
+int64_t binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  int64_t r = 1;
+  for (int i = 0; i < k; ++i) r = r * (n - i) / (i + 1);
+  return r;
+}
+
+// device bytes one request holds while it runs (besides the molecule's resident blocks):
+// transformed blocks + transform scratch, CI matrix and LU copy, E_pq tables and images,
+// density tensors, the A/C metric work and the V matrices
+int64_t footprint_bytes(int64_t ni, int64_t na, int64_t ns, int64_t ne) {
+  const int64_t z = 16;
+  const int64_t blocks[] = {na * ni * na * na, na * ni * ni * ni, na * ni * na * ni, ns * na * na * na,
+                            ns * na * ni * ni, ns * ni * ni * na, ns * ni * na * na, ns * na * na * ni,
+                            ns * ni * ni * ni, ns * ni * na * ni, ns * na * ns * na, ns * ni * ns * na,
+                            ns * ni * ns * ni};
+  int64_t tb = 0, mx = 0;
+  for (int64_t b : blocks) { tb += b; mx = b > mx ? b : mx; }
+  const int64_t nf = binom((int)na, (int)ne), na2 = na * na, na3 = na2 * na;
+  const int64_t mA = binom((int)na, (int)ne + 1), mC = binom((int)na, (int)ne - 1);
+  const int64_t ci = 3 * nf * nf;
+  const int64_t dens = 4 * nf * na2 + nf * na2 * na2 + 6 * na3 * na3;
+  const int64_t ac = 2 * (std::min(mA, na3) * na3 + 2 * std::min(mA, na3) * std::min(mA, na3)) +
+                     na3 * (ni + ns) * 3;
+  const int64_t other = 8 * (ns + ni) * (ns + ni) + 4 * ns * ns * ni + 4 * ns * ni * ni * na;
+  return z * (tb + mx + ci + dens + ac + other) + (64ll << 20);
+}
+
+}  // namespace
+
+struct dc2_server {
+  int32_t device = 0, nslots = 1, lanes = 1;
+  int64_t budget = 0, in_use = 0, peak = 0, completed = 0;
+  int running = 0;
+  bool stopping = false;
+  int64_t next_id = 1;
+  std::vector<std::unique_ptr<Molecule>> mols;
+  std::vector<dc2_context*> slot_ctx;
+  std::vector<Molecule*> slot_mol;
+  std::deque<Request*> pending, done;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::vector<std::thread> threads;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+  dc2::set_last_error(msg);
+  return code;
+}
+
+// point the slot's context at the molecule's resident blocks (borrowed, no copy)
+int bind_molecule(dc2_context* c, const Molecule& m) {
+  int st = dc2_set_spaces(c, m.ni, m.na, m.ns, m.ne);
+  for (const char* name : kBlockNames) {
+    if (st) break;
+    const double* p = nullptr;
+    int64_t n = 0;
+    if (dc2_get_integral_block_device(m.ctx, name, &p, &n) != 0) continue;   // not needed for this problem
+    st = dc2_set_integral_block(c, name, p, n, 2);
+  }
+  return st;
+}
+
+void run_request(dc2_server* S, int k, Request* r) {
+  dc2_context* c = S->slot_ctx[k];
+  int st = 0;
+  if (S->slot_mol[k] != r->mol) {
+    S->slot_mol[k] = nullptr;
+    st = bind_molecule(c, *r->mol);
+    if (!st) S->slot_mol[k] = r->mol;
+  }
+  if (!st) st = dc2_set_ras(c, r->ras[0], r->ras[1], r->ras[2], r->ras[3], r->ras[4], r->ras[5], r->ras[6]);
+  if (!st) st = dc2_set_one_electron(c, r->h.data(), r->ecore);
+  std::vector<double> eig(r->root);
+  int64_t ndet = 0;
+  if (!st) st = dc2_casci(c, r->root, eig.data(), &ndet);
+  if (!st) st = dc2_caspt2(c, r->root, r->eshift, r->e2, r->sumc2, r->scal);
+  r->status = st;
+  if (st) r->err = dc2_last_error();
+}
+
+void slot_main(dc2_server* S, int k) {
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_work.wait(lk, [&] {
+        if (!S->pending.empty() && (S->running == 0 || S->in_use + S->pending.front()->footprint <= S->budget))
+          return true;
+        return S->stopping && S->pending.empty();
+      });
+      if (S->pending.empty()) return;   // stopping and drained
+      r = S->pending.front();
+      S->pending.pop_front();
+      S->in_use += r->footprint;
+      S->peak = std::max(S->peak, S->in_use);
+      ++S->running;
+      r->t_start = S->now();
+      r->slot = k;
+    }
+    run_request(S, k, r);
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->in_use -= r->footprint;
+      --S->running;
+      ++S->completed;
+      r->t_end = S->now();
+      S->done.push_back(r);
+    }
+    S->cv_done.notify_all();
+    S->cv_work.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dc2_request_footprint(int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec, int64_t* bytes) {
+  if (!bytes || ninact < 0 || nact < 1 || nsec < 0 || nelec < 1 || nelec > nact) return fail(1, "invalid sizes");
+  *bytes = footprint_bytes(ninact, nact, nsec, nelec);
+  return 0;
+}
+
+int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot, dc2_server** out) {
+  if (!out || nslots < 1 || nslots > 64 || mem_budget <= 0 || lanes_per_slot < 1 || lanes_per_slot > 3)
+    return fail(1, "dc2_server_create: invalid argument");
+  auto S = std::make_unique<dc2_server>();
+  S->device = device;
+  S->nslots = nslots;
+  S->lanes = lanes_per_slot;
+  S->budget = mem_budget;
+  for (int k = 0; k < nslots; ++k) {
+    dc2_context* c = nullptr;
+    int st = dc2_context_create(device, &c);
+    if (!st) st = dc2_set_option(c, "lanes", lanes_per_slot);
+    if (st) {
+      const std::string msg = dc2_last_error();
+      if (c) dc2_context_destroy(c);
+      for (auto* x : S->slot_ctx) dc2_context_destroy(x);
+      return fail(st, "dc2_server_create: " + msg);
+    }
+    S->slot_ctx.push_back(c);
+    S->slot_mol.push_back(nullptr);
+  }
+  dc2_server* raw = S.get();
+  for (int k = 0; k < nslots; ++k) S->threads.emplace_back(slot_main, raw, k);
+  *out = S.release();
+  return 0;
+}
+
+int dc2_server_destroy(dc2_server* S) {
+  if (!S) return 0;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stopping = true;
+  }
+  S->cv_work.notify_all();
+  for (auto& t : S->threads) t.join();
+  for (auto* c : S->slot_ctx) dc2_context_destroy(c);
+  for (auto& m : S->mols) dc2_context_destroy(m->ctx);
+  for (auto* r : S->done) delete r;
+  delete S;
+  return 0;
+}
+
+int dc2_server_add_molecule(dc2_server* S, int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec, const double* B,
+                            int64_t naux, int32_t* mol_id) {
+  if (!S || !B || !mol_id || naux < 1) return fail(1, "dc2_server_add_molecule: invalid argument");
+  auto m = std::make_unique<Molecule>();
+  m->ni = ninact; m->na = nact; m->ns = nsec; m->ne = nelec;
+  int st = dc2_context_create(S->device, &m->ctx);
+  if (!st) st = dc2_set_spaces(m->ctx, ninact, nact, nsec, nelec);
+  if (!st) st = dc2_set_integral_factors(m->ctx, B, naux, 0);
+  if (st) {
+    const std::string msg = dc2_last_error();
+    if (m->ctx) dc2_context_destroy(m->ctx);
+    return fail(st, "dc2_server_add_molecule: " + msg);
+  }
+  for (const char* name : kBlockNames) {
+    const double* p = nullptr;
+    int64_t n = 0;
+    if (dc2_get_integral_block_device(m->ctx, name, &p, &n) == 0) m->bytes += 16 * n;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  *mol_id = (int32_t)S->mols.size();
+  S->mols.push_back(std::move(m));
+  return 0;
+}
+
+int dc2_server_submit(dc2_server* S, int32_t mol_id, const double* h, double ecore, int32_t selectroot, double eshift,
+                      const int32_t* ras, int32_t flags, int64_t* req_id) {
+  if (!S || !h || !req_id || selectroot < 1 || flags != 0) return fail(1, "dc2_server_submit: invalid argument");
+  auto r = std::make_unique<Request>();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (mol_id < 0 || mol_id >= (int32_t)S->mols.size()) return fail(1, "dc2_server_submit: unknown molecule");
+    r->mol = S->mols[mol_id].get();
+  }
+  const Molecule& m = *r->mol;
+  const int64_t nmo = (int64_t)m.ni + m.na + m.ns;
+  r->mol_id = mol_id;
+  r->h.assign(h, h + 2 * nmo * nmo);
+  r->ecore = ecore;
+  r->root = selectroot;
+  r->eshift = eshift;
+  r->flags = flags;
+  if (ras) std::memcpy(r->ras, ras, sizeof(r->ras));
+  r->footprint = footprint_bytes(m.ni, m.na, m.ns, m.ne);
+  if (r->footprint > S->budget) return fail(1, "dc2_server_submit: request exceeds the memory budget");
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    r->id = S->next_id++;
+    r->t_submit = S->now();
+    *req_id = r->id;
+    S->pending.push_back(r.release());
+  }
+  S->cv_work.notify_one();
+  return 0;
+}
+
+int dc2_server_wait(dc2_server* S, double timeout_s, int64_t* req_id, int32_t* req_status, double* out) {
+  if (!S || !req_id || !req_status || !out) return fail(1, "dc2_server_wait: invalid argument");
+  Request* r = nullptr;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(std::max(0.0, timeout_s));
+    S->cv_done.wait_until(lk, until, [&] { return !S->done.empty(); });
+    if (S->done.empty()) {
+      *req_id = 0;
+      return 0;
+    }
+    r = S->done.front();
+    S->done.pop_front();
+  }
+  *req_id = r->id;
+  *req_status = r->status;
+  std::memcpy(out, r->e2, 8 * sizeof(double));
+  std::memcpy(out + 8, r->sumc2, 8 * sizeof(double));
+  std::memcpy(out + 16, r->scal, 6 * sizeof(double));
+  out[22] = r->t_submit;
+  out[23] = r->t_start;
+  out[24] = r->t_end;
+  out[25] = r->slot;
+  if (r->status) dc2::set_last_error(r->err);
+  delete r;
+  return 0;
+}
+
+int dc2_server_stats(dc2_server* S, int64_t* out) {
+  if (!S || !out) return fail(1, "dc2_server_stats: invalid argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  out[0] = (int64_t)S->pending.size();
+  out[1] = S->running;
+  out[2] = S->completed;
+  out[3] = S->in_use;
+  out[4] = S->peak;
+  return 0;
+}
+
+}  // extern "C"

@@ -126,3 +126,20 @@ def generate(spec: SynthSpec, naux: Optional[int] = None, amplitude: float = 0.2
     B = _quaternion_pairs(rng, nkr, sc, naux)
     ecore = float(rng.uniform(5.0, 10.0))
     return SynthProblem(spec, eps, h, B, ecore)
+
+
+def scan_displacements(spec: SynthSpec, seed: int, sigma: float = 0.05) -> np.ndarray:
+    """Per-spinor orbital-energy displacement direction of a potential-energy scan: one
+    N(0, sigma) Eh value per Kramers pair, shared by both partners."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(0.0, sigma, spec.nmo // 2)
+    return np.repeat(d, 2)
+
+
+def scan_point(sp: SynthProblem, lam: float, direction: np.ndarray) -> np.ndarray:
+    """One-electron matrix of scan point lam in [0, 1]: h + lam * diag(direction).  The
+    two-electron integrals of the molecule are unchanged along the scan."""
+    h = sp.h.copy()
+    k = np.arange(sp.spec.nmo)
+    h[k, k] += lam * direction
+    return h

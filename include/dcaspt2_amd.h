@@ -162,6 +162,38 @@ int dc2_memcpy(dc2_context* ctx, void* dst, const void* src, int64_t bytes, int3
  * trip).  where: 0 host, 1 device. */
 int dc2_set_integral_factors(dc2_context* ctx, const double* B, int64_t naux, int32_t where);
 
+/* device pointer and element count of a resident integral block, e.g. to lend it to
+ * another context with dc2_set_integral_block(..., where = 2) */
+int dc2_get_integral_block_device(dc2_context* ctx, const char* name, const double** ptr, int64_t* count);
+
+/* ---- request server: continuous batching on one GPU (no counterpart in the reference,
+ * whose driver runs one problem per process, dcaspt2_run_subprograms.F90:57-62) ------
+ * nslots slots, each with its own context and host thread (lanes_per_slot concurrent
+ * subspace streams per request).  Molecules are registered once (factorised integrals,
+ * as dc2_set_integral_factors; the blocks stay resident in HBM and are shared read-only
+ * by all requests on the molecule).  A request is one CASCI + CASPT2 energy for a
+ * one-electron matrix h (nmo x nmo complex, copied at submit), root and eshift, with
+ * an optional RAS restriction ras[7] (the arguments of dc2_set_ras, or NULL).  A free
+ * slot admits the oldest pending request once the summed device footprint of the running
+ * requests (dc2_request_footprint) plus its own fits mem_budget bytes; a request larger
+ * than the budget is refused at submit.  flags must be 0. */
+typedef struct dc2_server dc2_server;
+int dc2_request_footprint(int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec, int64_t* bytes);
+int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot, dc2_server** out);
+/* finishes every pending request, then stops the slots and frees the molecules */
+int dc2_server_destroy(dc2_server* srv);
+int dc2_server_add_molecule(dc2_server* srv, int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec,
+                            const double* B, int64_t naux, int32_t* mol_id);
+int dc2_server_submit(dc2_server* srv, int32_t mol_id, const double* h, double ecore, int32_t selectroot,
+                      double eshift, const int32_t* ras, int32_t flags, int64_t* req_id);
+/* next completed request, waiting at most timeout_s seconds (*req_id = 0 on timeout):
+ * out[0..7] e2, out[8..15] sumc2, out[16..21] the scalars of dc2_caspt2,
+ * out[22..24] submit/start/end times (s since server creation), out[25] slot;
+ * *req_status is the request's status (its message via dc2_last_error) */
+int dc2_server_wait(dc2_server* srv, double timeout_s, int64_t* req_id, int32_t* req_status, double* out);
+/* out[5] = {pending, running, completed, footprint bytes in use, peak footprint bytes} */
+int dc2_server_stats(dc2_server* srv, int64_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,3 +38,18 @@ def test_no_device_is_reported():
     from dirac_caspt2_amd import _lib
     h = ctypes.c_void_p()
     assert _lib.lib().dc2_context_create(0, ctypes.byref(h)) != 0
+
+
+def test_request_footprint_and_server_errors():
+    """Host-side parts of the request server: the admission estimate grows with the problem,
+    and bad arguments are refused with a message (no GPU needed)."""
+    import ctypes
+    from dirac_caspt2_amd import _lib
+    from dirac_caspt2_amd.api import request_footprint
+    small, big = request_footprint(4, 8, 40, 4), request_footprint(16, 12, 164, 6)
+    assert 0 < small < big
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    assert lib.dc2_server_create(0, 0, 1 << 30, 1, ctypes.byref(h)) == 1
+    assert b"invalid argument" in lib.dc2_last_error()
+    assert lib.dc2_request_footprint(1, 0, 1, 1, ctypes.byref(ctypes.c_int64())) == 1

@@ -1,0 +1,97 @@
+"""Continuous batching (dc2_server_*): every request gives the same energies alone and
+in a concurrent mix, and matches the oracle.
+
+Tolerances: alone vs in the mix 1e-10 Eh per subspace e2/sumc2 and total (the kernels
+and reductions are deterministic; only the overlap differs); against the oracle 1e-9 Eh.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MOLECULES = [
+    dict(spec=(4, 8, 40, 4), seed=20240, ras=None),
+    dict(spec=(2, 6, 24, 2), seed=7, ras=None),
+    dict(spec=(4, 8, 30, 4), seed=11, ras=(2, 4, 2, 1, 1, 0, 0)),   # RASPT2: RAS1/2/3 = 2/4/2, <= 1 hole/particle
+]
+
+
+SHARED = {}
+
+
+@pytest.fixture(scope="module")
+def mix():
+    from dirac_caspt2_amd import synth
+    mols = []
+    for m in MOLECULES:
+        spec = synth.SynthSpec(*m["spec"], seed=m["seed"])
+        sp = synth.generate(spec)
+        mols.append((spec, sp, synth.scan_displacements(spec, m["seed"] + 1), m["ras"]))
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(12):
+        mi = int(rng.integers(0, len(mols)))
+        reqs.append((mi, float(rng.uniform(0, 1)), 1 + (k % 2) * (mi != 2), 0.0 if k % 3 else 0.05))
+    return mols, reqs
+
+
+def _submit(srv, ids, mols, req):
+    mi, lam, root, eshift = req
+    spec, sp, direction, ras = mols[mi]
+    from dirac_caspt2_amd import synth
+    return srv.submit(ids[mi], synth.scan_point(sp, lam, direction), sp.ecore, root, eshift, ras)
+
+
+def test_served_alone_equals_mixed(mix):
+    from dirac_caspt2_amd.api import Server
+    from gpu_helpers import say
+    mols, reqs = mix
+    with Server(0, nslots=4, lanes_per_slot=1) as srv:
+        ids = [srv.add_molecule((s.ninact, s.nact, s.nsec, s.nelec), sp.B) for s, sp, _, _ in mols]
+        say("serve: alone")
+        alone = []
+        for r in reqs:
+            rid = _submit(srv, ids, mols, r)
+            res = srv.wait(60.0)
+            assert res is not None and res.id == rid and res.status == 0, res
+            alone.append(res)
+        say("serve: mixed")
+        sub = {_submit(srv, ids, mols, r): k for k, r in enumerate(reqs)}
+        got = {}
+        while len(got) < len(reqs):
+            res = srv.wait(60.0)
+            assert res is not None, "timed out waiting for a served result"
+            assert res.status == 0, res.error
+            got[sub[res.id]] = res
+        st = srv.stats()
+        say("serve: stats", st, "slots used", sorted({g.slot for g in got.values()}))
+        assert st["pending"] == 0 and st["running"] == 0
+    for k in range(len(reqs)):
+        a, b = alone[k], got[k]
+        for X in "ABCDEFGH":
+            assert abs(a.e2[X] - b.e2[X]) < 1e-10, (k, X)
+            assert abs(a.sumc2[X] - b.sumc2[X]) < 1e-10, (k, X)
+        assert abs(a.total - b.total) < 1e-10, k
+    # concurrency: some requests of the mix overlapped in time
+    spans = sorted((g.t_start, g.t_end) for g in got.values())
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    SHARED["alone"] = alone
+
+
+def test_served_matches_oracle(mix):
+    from dirac_caspt2_amd import synth
+    from oracle import caspt2_ref as R
+    mols, reqs = mix
+    alone = SHARED.get("alone")
+    if alone is None:
+        pytest.skip("needs test_served_alone_equals_mixed")
+    for k in (0, 1, 2, 5):
+        mi, lam, root, eshift = reqs[k]
+        spec, sp, direction, ras = mols[mi]
+        p = R.Problem(spec.ninact, spec.nact, spec.nsec, spec.nelec, synth.scan_point(sp, lam, direction),
+                      R.FactorIntegrals(sp.B), sp.ecore, root, eshift=eshift,
+                      ras=R.RasSpec(*ras) if ras else None)
+        ref = R.run_caspt2(p)
+        for X in "ABCDEFGH":
+            assert abs(alone[k].e2[X] - ref.e2[X]) < 1e-9, (k, X, alone[k].e2[X], ref.e2[X])
+        assert abs(alone[k].total - ref.total) < 1e-9, k
