@@ -24,6 +24,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
+# hardware queues per process for the concurrent slots of the served run (HIP reads this
+# when the runtime starts, i.e. at the first library call)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X dense FP64 (matrix and vector)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
@@ -52,6 +55,58 @@ def cpu_baseline(sp, steps=1):
     return res, {"value": 1.0 / dt, "unit": "energies/s", "cores": int(cores), "kind": "port",
                  "sample": f"{steps} full CASPT2 energy of the same workload with the numpy oracle "
                            f"({dt:.2f} s per energy, BLAS threads = {cores})"}
+
+
+def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_window=0.002):
+    """Scan points of the same molecule through the request server, all submitted at once:
+    energies/s over the whole batch, per-request latency, and the largest |dE2| against the
+    plain FP64 path on the points in plain_points ({index: Caspt2Energies})."""
+    from dirac_caspt2_amd import synth
+    from dirac_caspt2_amd.api import Server
+    direction = synth.scan_displacements(spec, spec.seed + 1)
+    hs = [synth.scan_point(sp, lam, direction) for lam in np.linspace(0.0, 1.0, nreq)]
+    with Server(0, nslots=slots, lanes_per_slot=lanes, eig_batch=eig_batch, eig_window=eig_window) as srv:
+        mid = srv.add_molecule((spec.ninact, spec.nact, spec.nsec, spec.nelec), sp.B)
+        for k in range(slots):                                   # warm every slot
+            srv.submit(mid, hs[k % nreq], sp.ecore, spec.selectroot, spec.eshift)
+        for k in range(slots):
+            r = srv.wait(120.0)
+            assert r is not None and r.status == 0, r
+        t0 = time.perf_counter()
+        ids = {srv.submit(mid, hs[i], sp.ecore, spec.selectroot, spec.eshift): i for i in range(nreq)}
+        res = {}
+        while len(res) < nreq:
+            r = srv.wait(120.0)
+            if r is None:
+                raise RuntimeError("served run timed out")
+            if r.status:
+                raise RuntimeError(f"served request failed: {r.error}")
+            res[ids[r.id]] = r
+        total = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([res[i].latency for i in range(nreq)]) * 1e3
+    svc = np.array([res[i].t_end - res[i].t_start for i in range(nreq)]) * 1e3
+    d = max(abs(res[i].e2[X] - e.e2[X]) for i, e in plain_points.items() for X in "ABCDEFGH")
+    dt = max(abs(res[i].total - e.total) for i, e in plain_points.items())
+    return {"requests": nreq, "slots": slots, "lanes_per_slot": lanes, "energies_per_s": nreq / total,
+            "batch_seconds": total, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "service_p50_ms": float(np.percentile(svc, 50)), "peak_footprint_bytes": st["peak"],
+            "max_abs_dE2_vs_plain_fp64": d, "max_abs_dtotal_vs_plain_fp64": dt,
+            "points_compared": len(plain_points), "precision": "fp64", "eig_batch": eig_batch,
+            "eig_window_s": eig_window, "eig_batches": st["eig_batches"], "eig_jobs": st["eig_jobs"]}
+
+
+def plain_points(ctx, spec, sp, nreq, idx):
+    """Plain FP64 energies of the scan points idx (untimed), for the served-run comparison."""
+    from dirac_caspt2_amd import synth
+    direction = synth.scan_displacements(spec, spec.seed + 1)
+    lams = np.linspace(0.0, 1.0, nreq)
+    out = {}
+    for i in idx:
+        ctx.set_one_electron(synth.scan_point(sp, lams[i], direction), sp.ecore)
+        ctx.casci(spec.selectroot)
+        out[i] = ctx.caspt2(spec.selectroot, spec.eshift)
+    return out
 
 
 def copy_calibration(ctx, mib):
@@ -89,6 +144,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run 3 device-to-device copies of MIB MiB (a plain copy kernel "
                          "of known size, the reference for the HBM counters)")
+    ap.add_argument("--served", type=int, default=96, metavar="N",
+                    help="after the plain steps, N scan points of the same molecule through the request server "
+                         "(0: skip)")
+    ap.add_argument("--slots", type=int, default=32, help="concurrent slots of the served run")
+    ap.add_argument("--slot-lanes", type=int, default=1, help="subspace lanes per slot in the served run")
+    ap.add_argument("--eig-batch", type=int, default=32, help="eigensolves batched across served requests (<=1: off)")
+    ap.add_argument("--eig-window", type=float, default=0.003, help="batching window of the served eigensolves (s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -170,7 +232,15 @@ def main():
     }
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
-    ctx.close()
+    if args.served:
+        idx = sorted({0, args.served // 3, (2 * args.served) // 3, args.served - 1})
+        pts = plain_points(ctx, spec, sp, args.served, idx)
+        ctx.close()
+        srv = served_run(spec, sp, args.served, args.slots, args.slot_lanes, pts, args.eig_batch, args.eig_window)
+        srv["speedup_vs_plain_fp64"] = srv["energies_per_s"] / result["value"]
+        result["served"] = srv
+    else:
+        ctx.close()
     if not args.no_cpu_baseline:
         ref, cb = cpu_baseline(sp)
         result["cpu_baseline"] = cb

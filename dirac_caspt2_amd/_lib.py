@@ -54,6 +54,7 @@ _SIGS = {
     "dc2_get_integral_block_device": (c_int, [P, ctypes.c_char_p, P, P]),
     "dc2_request_footprint": (c_int, [c_int, c_int, c_int, c_int, P]),
     "dc2_server_create": (c_int, [c_int, c_int, c_i64, c_int, P]),
+    "dc2_server_create_batched": (c_int, [c_int, c_int, c_i64, c_int, c_int, ctypes.c_double, P]),
     "dc2_server_destroy": (c_int, [P]),
     "dc2_server_add_molecule": (c_int, [P, c_int, c_int, c_int, c_int, P, c_i64, P]),
     "dc2_server_submit": (c_int, [P, c_int, P, ctypes.c_double, c_int, ctypes.c_double, P, c_int, P]),

@@ -337,11 +337,14 @@ class Server:
     """Continuous-batching request server of the library (dc2_server_*): nslots concurrent
     energies on one GPU, molecules resident in HBM, admission by device-memory footprint."""
 
-    def __init__(self, device: int = 0, nslots: int = 4, mem_budget: int = 200 << 30, lanes_per_slot: int = 1):
+    def __init__(self, device: int = 0, nslots: int = 4, mem_budget: int = 200 << 30, lanes_per_slot: int = 1,
+                 eig_batch: int = 0, eig_window: float = 0.002):
+        """eig_batch > 1 batches the eigensolves of concurrent requests (groups of equal size,
+        flushed when all running requests wait, when full, or eig_window seconds late)."""
         self.lib = _lib.lib()
         h = ctypes.c_void_p()
-        check("dc2_server_create", self.lib.dc2_server_create(device, nslots, int(mem_budget), lanes_per_slot,
-                                                              ctypes.byref(h)))
+        check("dc2_server_create", self.lib.dc2_server_create_batched(device, nslots, int(mem_budget), lanes_per_slot,
+                                                                      eig_batch, float(eig_window), ctypes.byref(h)))
         self.h = h.value
         self.dims: Dict[int, tuple] = {}
 
@@ -378,9 +381,10 @@ class Server:
                             int(out[25]), err)
 
     def stats(self) -> Dict[str, int]:
-        out = np.zeros(5, dtype=np.int64)
+        out = np.zeros(7, dtype=np.int64)
         check("dc2_server_stats", self.lib.dc2_server_stats(self.h, _ptr(out)))
-        return dict(zip(("pending", "running", "completed", "in_use", "peak"), (int(x) for x in out)))
+        return dict(zip(("pending", "running", "completed", "in_use", "peak", "eig_batches", "eig_jobs"),
+                        (int(x) for x in out)))
 
     def close(self):
         if self.h:

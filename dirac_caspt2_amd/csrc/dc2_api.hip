@@ -19,6 +19,7 @@
 #include <string>
 
 #include "../../include/dcaspt2_amd.h"
+#include "dc2_batch.hpp"
 #include "dc2_kernels.hpp"
 
 using namespace dc2;
@@ -136,6 +137,7 @@ struct dc2_context {
   int nlanes = 3;            // concurrent subspace lanes (1 = sequential on the library stream)
   bool schur = true;         // E2 of the determinant-space route through the compressed resolvent
   std::atomic<int64_t> eig_fallbacks{0};
+  dc2::EigBatcher* batcher = nullptr;          // set by a request server: eigensolves batched across requests
   std::unique_ptr<Lane> lane0;                  // the library stream as a lane
   std::vector<std::unique_ptr<Lane>> lanes;     // worker lanes (created on first use)
   GemmStats gstats;        // GEMM launch accounting (dc2_set_profiling)
@@ -309,7 +311,13 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
   const int method = c->eig_method >= 0 ? c->eig_method : (kind == EigKind::rank_deficient ? 2 : 1);
   cplx* keep = L.z("eig_keep", nn);
   DC2_HIP(hipMemcpyAsync(keep, A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
-  int info = run_eig(L, method, n, A, w);
+  int info;
+  if (c->batcher && (method == 1 || method == 2)) {
+    DC2_HIP(hipStreamSynchronize(L.stream));   // the matrix is complete before another stream reads it
+    info = c->batcher->solve(n, method, A, w);
+  } else {
+    info = run_eig(L, method, n, A, w);
+  }
   if (method != 0 && (info != 0 || !all_finite(L.stream, A, nn))) {
     ++c->eig_fallbacks;
     DC2_HIP(hipMemcpyAsync(A, keep, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
@@ -909,6 +917,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
 
 namespace dc2 {
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+void set_batcher(dc2_context* c, EigBatcher* b) { c->batcher = b; }
 }  // namespace dc2
 
 // ===================================================================== C ABI

@@ -22,11 +22,151 @@
 #include <thread>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/dcaspt2_amd.h"
+#include "dc2_batch.hpp"
 
 namespace dc2 {
 void set_last_error(const std::string& msg);   // dc2_api.hip: the message dc2_last_error() returns
+void set_batcher(dc2_context* c, EigBatcher* b);   // dc2_api.hip: route the context's eigensolves
+
+// ---------------------------------------------------------------- EigBatcher
+EigBatcher::EigBatcher(int device, int max_batch, double window_s, int executors)
+    : device_(device), max_batch_(max_batch), window_(window_s) {
+  DC2_HIP(hipSetDevice(device));
+  for (int k = 0; k < executors; ++k) {
+    auto x = std::make_unique<Exec>();
+    DC2_HIP(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
+    if (rocblas_create_handle(&x->blas) != rocblas_status_success) throw Error(3, "rocblas_create_handle failed");
+    rocblas_set_stream(x->blas, x->stream);
+    execs_.push_back(std::move(x));
+  }
+  for (auto& x : execs_) x->th = std::thread(&EigBatcher::run, this, x.get());
 }
+
+EigBatcher::~EigBatcher() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_jobs_.notify_all();
+  for (auto& x : execs_) x->th.join();
+  for (auto& x : execs_) {
+    (void)hipStreamSynchronize(x->stream);
+    x->ptrs.release();
+    x->D.release();
+    x->E.release();
+    x->info.release();
+    rocblas_destroy_handle(x->blas);
+    (void)hipStreamDestroy(x->stream);
+  }
+}
+
+void EigBatcher::enter() {
+  std::lock_guard<std::mutex> lk(mu_);
+  ++participants_;
+}
+
+void EigBatcher::leave() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    --participants_;
+  }
+  cv_jobs_.notify_all();   // the waiting jobs may now be everyone's
+}
+
+void EigBatcher::stats(int64_t* batches, int64_t* jobs) {
+  std::lock_guard<std::mutex> lk(mu_);
+  *batches = batches_;
+  *jobs = solved_;
+}
+
+int EigBatcher::solve(int n, int method, cplx* A, double* w) {
+  Job j;
+  j.n = n;
+  j.method = method;
+  j.A = A;
+  j.w = w;
+  j.t = std::chrono::steady_clock::now();
+  std::unique_lock<std::mutex> lk(mu_);
+  jobs_.push_back(&j);
+  cv_jobs_.notify_all();
+  cv_done_.wait(lk, [&] { return j.done; });
+  return j.info;
+}
+
+void EigBatcher::run(Exec* ex) {
+  (void)hipSetDevice(device_);
+  std::unique_lock<std::mutex> lk(mu_);
+  for (;;) {
+    cv_jobs_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+    if (jobs_.empty()) return;   // stopping and drained
+    // the group of the oldest job, by value: another executor may take it meanwhile.
+    // Flush when full, when every participant waits, or late.
+    const int key_n = jobs_.front()->n, key_m = jobs_.front()->method;
+    const auto deadline = jobs_.front()->t + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                                 std::chrono::duration<double>(window_));
+    auto group_size = [&] {
+      int g = 0;
+      for (const Job* x : jobs_) g += (x->n == key_n && x->method == key_m);
+      return g;
+    };
+    cv_jobs_.wait_until(lk, deadline, [&] {
+      return stop_ || group_size() >= max_batch_ || (int)jobs_.size() >= participants_;
+    });
+    std::vector<Job*> group;
+    for (auto it = jobs_.begin(); it != jobs_.end() && (int)group.size() < max_batch_;) {
+      if ((*it)->n == key_n && (*it)->method == key_m) {
+        group.push_back(*it);
+        it = jobs_.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    if (group.empty()) continue;   // taken by another executor
+    lk.unlock();
+    try {
+      execute(ex, group);
+    } catch (const std::exception&) {
+      for (Job* x : group) x->info = -1;   // reported as a failed solve; the caller falls back
+    }
+    lk.lock();
+    ++batches_;
+    solved_ += (int64_t)group.size();
+    for (Job* x : group) x->done = true;
+    cv_done_.notify_all();
+    cv_jobs_.notify_all();
+  }
+}
+
+void EigBatcher::execute(Exec* x, std::vector<Job*>& group) {
+  const int b = (int)group.size(), n = group[0]->n;
+  std::vector<cplx*> hp(b);
+  for (int k = 0; k < b; ++k) hp[k] = group[k]->A;
+  x->ptrs.ensure(max_batch_);
+  x->D.ensure((size_t)max_batch_ * n);
+  x->E.ensure((size_t)max_batch_ * n);
+  x->info.ensure(max_batch_);
+  DC2_HIP(hipMemcpyAsync(x->ptrs.get(), hp.data(), b * sizeof(cplx*), hipMemcpyHostToDevice, x->stream));
+  rocblas_double_complex* const* A = reinterpret_cast<rocblas_double_complex* const*>(x->ptrs.get());
+  rocblas_status st;
+  if (group[0]->method == 2)
+    st = rocsolver_zheevdj_batched(x->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n, x->D.get(), n,
+                                   x->info.get(), b);
+  else
+    st = rocsolver_zheevd_batched(x->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n, x->D.get(), n,
+                                  x->E.get(), n, x->info.get(), b);
+  if (st != rocblas_status_success) throw Error(3, "batched eigensolver failed to launch");
+  for (int k = 0; k < b; ++k)
+    DC2_HIP(hipMemcpyAsync(group[k]->w, x->D.get() + (size_t)k * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                           x->stream));
+  std::vector<int> info(b);
+  DC2_HIP(hipMemcpyAsync(info.data(), x->info.get(), b * sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  DC2_HIP(hipStreamSynchronize(x->stream));
+  for (int k = 0; k < b; ++k) group[k]->info = info[k];
+}
+}  // namespace dc2
 
 namespace {
 
@@ -86,6 +226,7 @@ int64_t footprint_bytes(int64_t ni, int64_t na, int64_t ns, int64_t ne) {
 
 struct dc2_server {
   int32_t device = 0, nslots = 1, lanes = 1;
+  std::unique_ptr<dc2::EigBatcher> batcher;   // cross-request eigensolve batching (null: off)
   int64_t budget = 0, in_use = 0, peak = 0, completed = 0;
   int running = 0;
   bool stopping = false;
@@ -159,7 +300,9 @@ void slot_main(dc2_server* S, int k) {
       r->t_start = S->now();
       r->slot = k;
     }
+    if (S->batcher) S->batcher->enter();
     run_request(S, k, r);
+    if (S->batcher) S->batcher->leave();
     {
       std::lock_guard<std::mutex> lk(S->mu);
       S->in_use -= r->footprint;
@@ -184,13 +327,26 @@ int dc2_request_footprint(int32_t ninact, int32_t nact, int32_t nsec, int32_t ne
 }
 
 int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot, dc2_server** out) {
-  if (!out || nslots < 1 || nslots > 64 || mem_budget <= 0 || lanes_per_slot < 1 || lanes_per_slot > 3)
+  return dc2_server_create_batched(device, nslots, mem_budget, lanes_per_slot, 0, 0.0, out);
+}
+
+int dc2_server_create_batched(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot,
+                              int32_t eig_batch, double eig_window_s, dc2_server** out) {
+  if (!out || nslots < 1 || nslots > 64 || mem_budget <= 0 || lanes_per_slot < 1 || lanes_per_slot > 3 ||
+      eig_batch < 0 || eig_batch > 64 || eig_window_s < 0.0 || eig_window_s > 1.0)
     return fail(1, "dc2_server_create: invalid argument");
   auto S = std::make_unique<dc2_server>();
   S->device = device;
   S->nslots = nslots;
   S->lanes = lanes_per_slot;
   S->budget = mem_budget;
+  if (eig_batch > 1) {
+    try {
+      S->batcher = std::make_unique<dc2::EigBatcher>(device, eig_batch, eig_window_s);
+    } catch (const std::exception& e) {
+      return fail(2, std::string("dc2_server_create: ") + e.what());
+    }
+  }
   for (int k = 0; k < nslots; ++k) {
     dc2_context* c = nullptr;
     int st = dc2_context_create(device, &c);
@@ -201,6 +357,7 @@ int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_
       for (auto* x : S->slot_ctx) dc2_context_destroy(x);
       return fail(st, "dc2_server_create: " + msg);
     }
+    if (S->batcher) dc2::set_batcher(c, S->batcher.get());
     S->slot_ctx.push_back(c);
     S->slot_mol.push_back(nullptr);
   }
@@ -220,6 +377,7 @@ int dc2_server_destroy(dc2_server* S) {
   for (auto& t : S->threads) t.join();
   for (auto* c : S->slot_ctx) dc2_context_destroy(c);
   for (auto& m : S->mols) dc2_context_destroy(m->ctx);
+  S->batcher.reset();
   for (auto* r : S->done) delete r;
   delete S;
   return 0;
@@ -316,6 +474,8 @@ int dc2_server_stats(dc2_server* S, int64_t* out) {
   out[2] = S->completed;
   out[3] = S->in_use;
   out[4] = S->peak;
+  out[5] = out[6] = 0;
+  if (S->batcher) S->batcher->stats(&out[5], &out[6]);
   return 0;
 }
 

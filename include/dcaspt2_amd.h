@@ -180,6 +180,12 @@ int dc2_get_integral_block_device(dc2_context* ctx, const char* name, const doub
 typedef struct dc2_server dc2_server;
 int dc2_request_footprint(int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec, int64_t* bytes);
 int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot, dc2_server** out);
+/* the same with cross-request eigensolve batching: the Hermitian eigenproblems of the
+ * running requests are grouped by size and solved eig_batch at a time by one batched
+ * rocSOLVER call; a group is flushed when every running request waits for it, when it
+ * is full, or eig_window_s seconds after its oldest problem (eig_batch <= 1: off) */
+int dc2_server_create_batched(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot,
+                              int32_t eig_batch, double eig_window_s, dc2_server** out);
 /* finishes every pending request, then stops the slots and frees the molecules */
 int dc2_server_destroy(dc2_server* srv);
 int dc2_server_add_molecule(dc2_server* srv, int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec,
@@ -191,7 +197,8 @@ int dc2_server_submit(dc2_server* srv, int32_t mol_id, const double* h, double e
  * out[22..24] submit/start/end times (s since server creation), out[25] slot;
  * *req_status is the request's status (its message via dc2_last_error) */
 int dc2_server_wait(dc2_server* srv, double timeout_s, int64_t* req_id, int32_t* req_status, double* out);
-/* out[5] = {pending, running, completed, footprint bytes in use, peak footprint bytes} */
+/* out[7] = {pending, running, completed, footprint bytes in use, peak footprint bytes,
+ *           eigensolve batches, eigenproblems solved in them} */
 int dc2_server_stats(dc2_server* srv, int64_t* out);
 
 #ifdef __cplusplus

@@ -95,3 +95,32 @@ def test_served_matches_oracle(mix):
         for X in "ABCDEFGH":
             assert abs(alone[k].e2[X] - ref.e2[X]) < 1e-9, (k, X, alone[k].e2[X], ref.e2[X])
         assert abs(alone[k].total - ref.total) < 1e-9, k
+
+
+def test_served_with_batched_eigensolves(mix):
+    """The same mix with the eigensolves of concurrent requests batched (groups of equal
+    size solved by one batched rocSOLVER call): every energy within 1e-9 Eh of the request
+    served alone without batching (eigenvectors may differ by rounding-level rotations
+    inside degenerate sets; every E2 here is invariant under them or gauge-fixed)."""
+    from dirac_caspt2_amd.api import Server
+    from gpu_helpers import say
+    mols, reqs = mix
+    alone = SHARED.get("alone")
+    if alone is None:
+        pytest.skip("needs test_served_alone_equals_mixed")
+    with Server(0, nslots=6, lanes_per_slot=1, eig_batch=8, eig_window=0.002) as srv:
+        ids = [srv.add_molecule((s.ninact, s.nact, s.nsec, s.nelec), sp.B) for s, sp, _, _ in mols]
+        sub = {_submit(srv, ids, mols, r): k for k, r in enumerate(reqs)}
+        got = {}
+        while len(got) < len(reqs):
+            res = srv.wait(60.0)
+            assert res is not None, "timed out waiting for a served result"
+            assert res.status == 0, res.error
+            got[sub[res.id]] = res
+        st = srv.stats()
+    say("serve batched: stats", st)
+    assert st["eig_batches"] > 0 and st["eig_jobs"] > st["eig_batches"]
+    for k in range(len(reqs)):
+        for X in "ABCDEFGH":
+            assert abs(alone[k].e2[X] - got[k].e2[X]) < 1e-9, (k, X)
+        assert abs(alone[k].total - got[k].total) < 1e-9, k
