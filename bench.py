@@ -57,7 +57,7 @@ def cpu_baseline(sp, steps=1):
                            f"({dt:.2f} s per energy, BLAS threads = {cores})"}
 
 
-def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_window=0.002):
+def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_window=0.002, split=0):
     """Scan points of the same molecule through the request server, all submitted at once:
     energies/s over the whole batch, per-request latency, and the largest |dE2| against the
     plain FP64 path on the points in plain_points ({index: Caspt2Energies})."""
@@ -68,12 +68,13 @@ def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_wind
     with Server(0, nslots=slots, lanes_per_slot=lanes, eig_batch=eig_batch, eig_window=eig_window) as srv:
         mid = srv.add_molecule((spec.ninact, spec.nact, spec.nsec, spec.nelec), sp.B)
         for k in range(slots):                                   # warm every slot
-            srv.submit(mid, hs[k % nreq], sp.ecore, spec.selectroot, spec.eshift)
+            srv.submit(mid, hs[k % nreq], sp.ecore, spec.selectroot, spec.eshift, split_slices=split)
         for k in range(slots):
             r = srv.wait(120.0)
             assert r is not None and r.status == 0, r
         t0 = time.perf_counter()
-        ids = {srv.submit(mid, hs[i], sp.ecore, spec.selectroot, spec.eshift): i for i in range(nreq)}
+        ids = {srv.submit(mid, hs[i], sp.ecore, spec.selectroot, spec.eshift, split_slices=split): i
+               for i in range(nreq)}
         res = {}
         while len(res) < nreq:
             r = srv.wait(120.0)
@@ -92,8 +93,34 @@ def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_wind
             "batch_seconds": total, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "service_p50_ms": float(np.percentile(svc, 50)), "peak_footprint_bytes": st["peak"],
             "max_abs_dE2_vs_plain_fp64": d, "max_abs_dtotal_vs_plain_fp64": dt,
-            "points_compared": len(plain_points), "precision": "fp64", "eig_batch": eig_batch,
+            "points_compared": len(plain_points),
+            "precision": f"split-precision transforms, {split} INT8 slices" if split else "fp64",
+            "eig_batch": eig_batch,
             "eig_window_s": eig_window, "eig_batches": st["eig_batches"], "eig_jobs": st["eig_jobs"]}
+
+
+def split_steps(ctx, spec, args, fp64_out):
+    """The same energies with split-precision (Ozaki INT8) integral transforms."""
+    ctx.set_option("split_slices", args.split)
+    for _ in range(max(1, args.warmup)):
+        ctx.casci(spec.selectroot)
+        out = ctx.caspt2(spec.selectroot, spec.eshift)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        ctx.casci(spec.selectroot)
+        out = ctx.caspt2(spec.selectroot, spec.eshift)
+        lat.append(time.perf_counter() - t)
+    total = time.perf_counter() - t0
+    fs, fd = ctx.transform_flops()
+    ctx.set_option("split_slices", 0)
+    return {"slices": args.split, "energies_per_s": args.steps / total, "ms_per_step": 1e3 * total / args.steps,
+            "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
+            "transform_seconds_last_step": out.timings[3], "fp64_transform_seconds_last_step": fp64_out.timings[3],
+            "split_flop_share": fs / max(fs + fd, 1.0),
+            "max_abs_dE2_vs_fp64_gpu": max(abs(out.e2[X] - fp64_out.e2[X]) for X in "ABCDEFGH"),
+            "abs_dtotal_vs_fp64_gpu": abs(out.total - fp64_out.total)}
 
 
 def plain_points(ctx, spec, sp, nreq, idx):
@@ -149,6 +176,7 @@ def main():
                          "(0: skip)")
     ap.add_argument("--slots", type=int, default=32, help="concurrent slots of the served run")
     ap.add_argument("--slot-lanes", type=int, default=1, help="subspace lanes per slot in the served run")
+    ap.add_argument("--split", type=int, default=7, help="INT8 slices of the split-precision transforms (0: off)")
     ap.add_argument("--eig-batch", type=int, default=32, help="eigensolves batched across served requests (<=1: off)")
     ap.add_argument("--eig-window", type=float, default=0.003, help="batching window of the served eigensolves (s)")
     args = ap.parse_args()
@@ -198,7 +226,7 @@ def main():
     ach_tr = g_tr["flops"] / g_tr["seconds"] / 1e12 if g_tr["seconds"] > 0 else 0.0
     traffic = measured_traffic(args.workload, dom_op)
     result = {
-        "metric": "CASPT2 energies/s (plain FP64 GPU path)",
+        "metric": "CASPT2 energies/s + p99 latency, split-prec vs FP64 GPU at |\u0394E2|<1e-8 Eh",
         "value": args.steps / total,
         "unit": "energies/s",
         "n_gpus": 1,
@@ -230,13 +258,16 @@ def main():
                      "transform_launches_achieved_tflops": ach_tr},
         "eig_fallbacks": ctx.eig_fallbacks(),
     }
+    if args.split:
+        result["split"] = split_steps(ctx, spec, args, out)
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
     if args.served:
         idx = sorted({0, args.served // 3, (2 * args.served) // 3, args.served - 1})
         pts = plain_points(ctx, spec, sp, args.served, idx)
         ctx.close()
-        srv = served_run(spec, sp, args.served, args.slots, args.slot_lanes, pts, args.eig_batch, args.eig_window)
+        srv = served_run(spec, sp, args.served, args.slots, args.slot_lanes, pts, args.eig_batch, args.eig_window,
+                         args.split)
         srv["speedup_vs_plain_fp64"] = srv["energies_per_s"] / result["value"]
         result["served"] = srv
     else:

@@ -248,9 +248,15 @@ class Caspt2Context:
         check("dc2_get_gemm_op_stats", self.lib.dc2_get_gemm_op_stats(self.h, opA.encode(), opB.encode(), _ptr(out)))
         return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
 
+    def transform_flops(self):
+        """(split-precision, FP64) transform flops of the last caspt2()."""
+        out = np.zeros(2)
+        check("dc2_get_transform_flops", self.lib.dc2_get_transform_flops(self.h, _ptr(out)))
+        return float(out[0]), float(out[1])
+
     OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
                "ac_metric": {"auto": 0, "formula": 1, "determinant": 2},
-               "lanes": {}, "schur": {"off": 0, "on": 1}}
+               "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {}}
 
     def set_option(self, key: str, value):
         """Options: eigensolver auto|qr|dc|dcj|jacobi, ac_metric auto|formula|determinant,
@@ -358,13 +364,13 @@ class Server:
         return mid.value
 
     def submit(self, mol: int, h: np.ndarray, ecore: float, selectroot: int = 1, eshift: float = 0.0,
-               ras=None) -> int:
+               ras=None, split_slices: int = 0) -> int:
         hf = _fortran_complex(h)
         r = None if ras is None else np.ascontiguousarray(ras, dtype=np.int32)
         rid = ctypes.c_int64()
         check("dc2_server_submit", self.lib.dc2_server_submit(self.h, mol, _ptr(hf), float(ecore), selectroot,
-                                                              float(eshift), None if r is None else _ptr(r), 0,
-                                                              ctypes.byref(rid)))
+                                                              float(eshift), None if r is None else _ptr(r),
+                                                              int(split_slices), ctypes.byref(rid)))
         return rid.value
 
     def wait(self, timeout: float = 1.0) -> Optional[ServedResult]:
@@ -396,6 +402,13 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def zgemm_split(ctx: Caspt2Context, slices: int, opB: str, M, N, K, A: DeviceArray, lda, B: DeviceArray, ldb,
+                C: DeviceArray, ldc):
+    """Split-precision C = A^T-view * op(B) (dc2_zgemm_split) on device arrays."""
+    check("dc2_zgemm_split", ctx.lib.dc2_zgemm_split(ctx.h, slices, opB.encode(), M, N, K, A.ptr, lda, B.ptr, ldb,
+                                                     C.ptr, ldc))
 
 
 def request_footprint(ninact: int, nact: int, nsec: int, nelec: int) -> int:

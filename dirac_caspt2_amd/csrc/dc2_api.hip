@@ -136,6 +136,9 @@ struct dc2_context {
   int ac_route = 0;          // A/C metric: 0 automatic, 1 RDM formulas (nact^3), 2 determinant space
   int nlanes = 3;            // concurrent subspace lanes (1 = sequential on the library stream)
   bool schur = true;         // E2 of the determinant-space route through the compressed resolvent
+  int split_slices = 0;      // > 0: transform passes with K >= split_min_k as Ozaki INT8 GEMMs with this many slices
+  int split_min_k = 32;
+  double split_flops = 0.0, fp64_flops = 0.0;   // transform flops of the last dc2_caspt2 by precision
   std::atomic<int64_t> eig_fallbacks{0};
   dc2::EigBatcher* batcher = nullptr;          // set by a request server: eigensolves batched across requests
   std::unique_ptr<Lane> lane0;                  // the library stream as a lane
@@ -435,7 +438,18 @@ void transform(dc2_context* c, const std::string& name) {
     const int off = space_off(c, sp[pass]);
     const cplx* Ub = c->U.get() + off + (int64_t)nmo * off;
     // (i'j'|k'l') = sum conj(U_ii') U_jj' conj(U_kk') U_ll' (ij|kl): conj on the 1st and 3rd index
-    zgemm(c->stream, 'T', (pass % 2 == 0) ? 'R' : 'N', rest, n, n, one, in, n, 0, Ub, nmo, 0, zero, dst, rest, 0, 1);
+    const char opB = (pass % 2 == 0) ? 'R' : 'N';
+    if (c->split_slices > 0 && n >= c->split_min_k) {
+      Lane& L0 = *c->lane0;
+      const int64_t wb = zgemm_ozaki_work_bytes(c->split_slices, rest, n, n);
+      int8_t* work = reinterpret_cast<int8_t*>(L0.l("oz_work", (wb + 7) / 8));
+      int* expo = L0.i("oz_expo", rest + n);
+      zgemm_ozaki(c->stream, c->split_slices, opB, rest, n, n, in, n, Ub, nmo, dst, rest, work, expo);
+      c->split_flops += 8.0 * rest * n * n;
+    } else {
+      zgemm(c->stream, 'T', opB, rest, n, n, one, in, n, 0, Ub, nmo, 0, zero, dst, rest, 0, 1);
+      c->fp64_flops += 8.0 * rest * n * n;
+    }
     in = dst;
     const int64_t d0 = dims[0];
     dims[0] = dims[1]; dims[1] = dims[2]; dims[2] = dims[3]; dims[3] = d0;
@@ -848,6 +862,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   if (doG) need.push_back("G");
   if (doH) need.push_back("H");
   set_gemm_tag(1);
+  c->split_flops = c->fp64_flops = 0.0;
   for (const auto& nm : need) transform(c, nm);
   set_gemm_tag(0);
   c->timings[3] = t3.stop();
@@ -1327,6 +1342,12 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "schur") {
       DC2_CHECK(value == 0 || value == 1, 1, "schur must be 0 or 1");
       c->schur = value != 0;
+    } else if (k == "split_slices") {
+      DC2_CHECK(value == 0 || (value >= 3 && value <= 8), 1, "split_slices must be 0 or 3..8");
+      c->split_slices = (int)value;
+    } else if (k == "split_min_k") {
+      DC2_CHECK(value >= 1 && value <= 4096, 1, "split_min_k must be in 1..4096");
+      c->split_min_k = (int)value;
     } else {
       throw Error(1, "unknown option '" + k + "'");
     }
@@ -1348,6 +1369,29 @@ int dc2_get_gemm_op_stats(dc2_context* c, char opA, char opB, double* out) {
       mx = std::max(mx, r.flops);
     }
     out[0] = n; out[1] = sec; out[2] = fl; out[3] = mx;
+  });
+}
+
+int dc2_zgemm_split(dc2_context* c, int32_t slices, char opB, int64_t M, int64_t N, int64_t K, const double* A,
+                    int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc) {
+  return guarded([&] {
+    DC2_CHECK(c && A && B && C, 1, "null argument");
+    DC2_CHECK(lda >= K && ldb >= K && ldc >= M, 1, "leading dimension too small");
+    Lane& L0 = *c->lane0;
+    const int64_t wb = zgemm_ozaki_work_bytes(slices, M, N, K);
+    int8_t* work = reinterpret_cast<int8_t*>(L0.l("oz_work", (wb + 7) / 8));
+    int* expo = L0.i("oz_expo", M + N);
+    zgemm_ozaki(c->stream, slices, opB, M, N, K, reinterpret_cast<const cplx*>(A), lda,
+                reinterpret_cast<const cplx*>(B), ldb, reinterpret_cast<cplx*>(C), ldc, work, expo);
+    DC2_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int dc2_get_transform_flops(dc2_context* c, double* out) {
+  return guarded([&] {
+    DC2_CHECK(c && out, 1, "null argument");
+    out[0] = c->split_flops;
+    out[1] = c->fp64_flops;
   });
 }
 

@@ -62,6 +62,14 @@ void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int
                const double* eps_act, int64_t m, int64_t* strings, cplx* Phi, double* fdiag);
 void scale_rows(hipStream_t s, int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out);
 
+// ---------------------------------------------------------------- dc2_ozaki.hip
+// Split-precision (Ozaki, INT8 MFMA) complex GEMM C = A^T-view * op(B) of the transform
+// passes: A[m][k] = A[k + lda*m], op(B) = B ('N') or conj(B) ('R'); slices in 3..8, K <= 4096.
+// work: zgemm_ozaki_work_bytes() bytes (16-byte aligned), expo: M + N ints.
+int64_t zgemm_ozaki_work_bytes(int slices, int64_t M, int64_t N, int64_t K);
+void zgemm_ozaki(hipStream_t s, int slices, char opB, int64_t M, int64_t N, int64_t K, const cplx* A, int64_t lda,
+                 const cplx* B, int64_t ldb, cplx* C, int64_t ldc, int8_t* work, int* expo);
+
 // ---------------------------------------------------------------- dc2_pt2.hip
 struct Dens {
   int na;

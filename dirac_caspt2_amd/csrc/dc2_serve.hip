@@ -272,6 +272,7 @@ void run_request(dc2_server* S, int k, Request* r) {
   }
   if (!st) st = dc2_set_ras(c, r->ras[0], r->ras[1], r->ras[2], r->ras[3], r->ras[4], r->ras[5], r->ras[6]);
   if (!st) st = dc2_set_one_electron(c, r->h.data(), r->ecore);
+  if (!st) st = dc2_set_option(c, "split_slices", r->flags);
   std::vector<double> eig(r->root);
   int64_t ndet = 0;
   if (!st) st = dc2_casci(c, r->root, eig.data(), &ndet);
@@ -409,7 +410,8 @@ int dc2_server_add_molecule(dc2_server* S, int32_t ninact, int32_t nact, int32_t
 
 int dc2_server_submit(dc2_server* S, int32_t mol_id, const double* h, double ecore, int32_t selectroot, double eshift,
                       const int32_t* ras, int32_t flags, int64_t* req_id) {
-  if (!S || !h || !req_id || selectroot < 1 || flags != 0) return fail(1, "dc2_server_submit: invalid argument");
+  if (!S || !h || !req_id || selectroot < 1 || !(flags == 0 || (flags >= 3 && flags <= 8)))
+    return fail(1, "dc2_server_submit: invalid argument");
   auto r = std::make_unique<Request>();
   {
     std::lock_guard<std::mutex> lk(S->mu);

@@ -145,8 +145,20 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  solves (default 3: A, C and the rest);
  *   "schur":       1 (default) E2 of the determinant-space route through the compressed
  *                  resolvent when at most 32 directions are cut, 0 always through the
- *                  eigendecomposition of B~. */
+ *                  eigendecomposition of B~;
+ *   "split_slices": 0 (default) FP64 integral transforms, 3..8 split-precision passes
+ *                  (dc2_zgemm_split) with this many slices;
+ *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32). */
 int dc2_set_option(dc2_context* ctx, const char* key, int64_t value);
+/* Split-precision complex GEMM (Ozaki scheme on INT8 matrix cores; no counterpart in
+ * the reference): C[m + ldc*j] = sum_k A[k + lda*m] * op(B)[k + ldb*j] on device memory,
+ * op(B) = B ('N') or conj(B) ('R'), every row of A and column of B cut into `slices`
+ * 7-bit digits (3..8) after power-of-two scaling; K <= 4096.  The integral transforms
+ * use it for passes with K >= "split_min_k" when the option "split_slices" is non-zero. */
+int dc2_zgemm_split(dc2_context* ctx, int32_t slices, char opB, int64_t M, int64_t N, int64_t K, const double* A,
+                    int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc);
+/* transform flops of the last dc2_caspt2: out[0] in split precision, out[1] in FP64 */
+int dc2_get_transform_flops(dc2_context* ctx, double* out);
 /* number of eigensolves that fell back to the tridiagonal QR algorithm */
 int dc2_get_eig_fallbacks(dc2_context* ctx, int64_t* n);
 
@@ -176,7 +188,8 @@ int dc2_get_integral_block_device(dc2_context* ctx, const char* name, const doub
  * an optional RAS restriction ras[7] (the arguments of dc2_set_ras, or NULL).  A free
  * slot admits the oldest pending request once the summed device footprint of the running
  * requests (dc2_request_footprint) plus its own fits mem_budget bytes; a request larger
- * than the budget is refused at submit.  flags must be 0. */
+ * than the budget is refused at submit.  flags: 0 FP64 integral transforms, 3..8 split-
+ * precision transforms with that many slices (the option "split_slices"). */
 typedef struct dc2_server dc2_server;
 int dc2_request_footprint(int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec, int64_t* bytes);
 int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot, dc2_server** out);
