@@ -120,7 +120,8 @@ def split_steps(ctx, spec, args, fp64_out):
             "transform_seconds_last_step": out.timings[3], "fp64_transform_seconds_last_step": fp64_out.timings[3],
             "split_flop_share": fs / max(fs + fd, 1.0),
             "max_abs_dE2_vs_fp64_gpu": max(abs(out.e2[X] - fp64_out.e2[X]) for X in "ABCDEFGH"),
-            "abs_dtotal_vs_fp64_gpu": abs(out.total - fp64_out.total)}
+            "abs_dtotal_vs_fp64_gpu": abs(out.total - fp64_out.total),
+            "stage_seconds_last_step": list(out.timings), "_e2": dict(out.e2), "_total": out.total}
 
 
 def plain_points(ctx, spec, sp, nreq, idx):
@@ -161,12 +162,182 @@ def measured_traffic(workload, op):
     return None
 
 
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def generate_all(specs, threads=8):
+    from concurrent.futures import ThreadPoolExecutor
+
+    from dirac_caspt2_amd import synth
+    with ThreadPoolExecutor(threads) as ex:
+        out = []
+        for k, sp in enumerate(ex.map(synth.generate, specs)):
+            out.append(sp)
+            if k % 8 == 7:
+                log(f"generated {k + 1}/{len(specs)}")
+    return out
+
+
+def batch_workload(args):
+    """64 independent seeded instances (nsec ~ U[40,300], nact in {8,10,12} with nact/2
+    electrons, ninact ~ U[4,24]) solved (a) one after another by the plain FP64 path and
+    (b) all at once through the request server with cross-request eigensolve batching."""
+    from dirac_caspt2_amd import synth
+    from dirac_caspt2_amd.api import Caspt2Context, Server
+    from dirac_caspt2_amd.problem import load_factors
+    specs = synth.batch_instances(args.instances)
+    sps = generate_all(specs)
+    plain = []
+    t_plain = 0.0
+    for k, (s, sp) in enumerate(zip(specs, sps)):
+        ctx = Caspt2Context(0)
+        load_factors(ctx, (s.ninact, s.nact, s.nsec, s.nelec), sp.h, sp.B, sp.ecore, on_device=True)
+        if k == 0:                                  # warm the kernels once
+            ctx.casci(1)
+            ctx.caspt2(1, 0.0)
+        t = time.perf_counter()
+        ctx.casci(s.selectroot)
+        plain.append(ctx.caspt2(s.selectroot, s.eshift))
+        t_plain += time.perf_counter() - t
+        ctx.close()
+        if k % 16 == 15:
+            log(f"plain {k + 1}/{len(specs)}")
+    with Server(0, nslots=args.slots, lanes_per_slot=args.slot_lanes, eig_batch=args.eig_batch,
+                eig_window=args.eig_window) as srv:
+        ids = [srv.add_molecule((s.ninact, s.nact, s.nsec, s.nelec), sp.B) for s, sp in zip(specs, sps)]
+        log("molecules resident")
+        for k in range(min(args.slots, len(specs))):      # warm the slots
+            srv.submit(ids[k], sps[k].h, sps[k].ecore, 1, 0.0, split_slices=args.split)
+        for k in range(min(args.slots, len(specs))):
+            srv.wait(300.0)
+        t0 = time.perf_counter()
+        sub = {srv.submit(ids[k], sp.h, sp.ecore, s.selectroot, s.eshift, split_slices=args.split): k
+               for k, (s, sp) in enumerate(zip(specs, sps))}
+        res = {}
+        while len(res) < len(specs):
+            r = srv.wait(300.0)
+            if r is None or r.status:
+                raise RuntimeError(f"batched request failed: {None if r is None else r.error}")
+            res[sub[r.id]] = r
+        t_batch = time.perf_counter() - t0
+        st = srv.stats()
+    even = [k for k, s in enumerate(specs) if s.nelec % 2 == 0]
+    d_even = max(abs(res[k].e2[X] - plain[k].e2[X]) for k in even for X in "ABCDEFGH")
+    d_odd = max((abs(res[k].e2[X] - plain[k].e2[X]) for k in range(len(specs)) if k not in even
+                 for X in "ABCDEFGH"), default=0.0)
+    lat = np.array([r.latency for r in res.values()]) * 1e3
+    return {"metric": "CASPT2 energies/s, 64 independent instances batched vs sequential plain solves",
+            "value": len(specs) / t_batch, "unit": "energies/s", "n_gpus": 1, "higher_is_better": True,
+            "sequential_plain_energies_per_s": len(specs) / t_plain, "speedup": t_plain / t_batch,
+            "batch_seconds": t_batch, "sequential_seconds": t_plain,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "slots": args.slots, "eig_batch": args.eig_batch, "eig_batches": st["eig_batches"],
+            "eig_jobs": st["eig_jobs"], "split_slices": args.split,
+            "max_abs_dE2_even_electron_instances": d_even,
+            "max_abs_dE2_odd_electron_instances": d_odd,
+            "note": "odd-electron instances (nact 10, 5 electrons) have Kramers-degenerate CASCI roots; "
+                    "their energies depend on the eigenvector chosen inside the pair, as in the reference",
+            "dtype": "f64 (complex128)", "data": "synthetic (seeded)",
+            "config": {"workload": f"batch: {len(specs)} instances, seed 20243"}}
+
+
+def serve_workload(args):
+    """Poisson stream of CASPT2 requests over resident molecules (scan points), continuous
+    batching with cross-request eigensolve batching and split-precision transforms."""
+    import threading
+
+    from dirac_caspt2_amd import synth
+    from dirac_caspt2_amd.api import Caspt2Context, Server
+    mols = synth.serve_molecules(args.molecules)
+    sps = generate_all([s for s, _ in mols])
+    dirs = [synth.scan_displacements(s, s.seed + 1) for s, _ in mols]
+    rng = np.random.default_rng(20245)
+    n = args.requests
+    which = rng.integers(0, len(mols), n)
+    lams = rng.uniform(0.0, 1.0, n)
+    gaps = rng.exponential(1.0 / args.rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    with Server(0, nslots=args.slots, lanes_per_slot=args.slot_lanes, eig_batch=args.eig_batch,
+                eig_window=args.eig_window) as srv:
+        ids = [srv.add_molecule((s.ninact, s.nact, s.nsec, s.nelec), sp.B) for (s, _), sp in zip(mols, sps)]
+        for k, ((s, ras), sp) in enumerate(zip(mols, sps)):          # warm every molecule once
+            srv.submit(ids[k], sp.h, sp.ecore, 1, 0.0, ras, split_slices=args.split)
+        for _ in mols:
+            r = srv.wait(300.0)
+            if r is None or r.status:
+                raise RuntimeError(f"warm-up request failed: {None if r is None else r.error}")
+        log("molecules resident and warm")
+        sub = {}
+        lock = threading.Lock()
+
+        def feeder():
+            t0 = time.perf_counter()
+            for i in range(n):
+                dt = arrivals[i] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                m = int(which[i])
+                s, ras = mols[m]
+                h = synth.scan_point(sps[m], float(lams[i]), dirs[m])
+                rid = srv.submit(ids[m], h, sps[m].ecore, 1, 0.0, ras, split_slices=args.split)
+                with lock:
+                    sub[rid] = i
+        th = threading.Thread(target=feeder)
+        t_start = time.perf_counter()
+        th.start()
+        res = {}
+        last = time.perf_counter()
+        while len(res) < n:
+            r = srv.wait(300.0)
+            if r is None or r.status:
+                raise RuntimeError(f"served request failed: {None if r is None else r.error}")
+            with lock:
+                res[sub[r.id]] = r
+            if time.perf_counter() - last > 30:
+                log(f"served {len(res)}/{n}")
+                last = time.perf_counter()
+        th.join()
+        t_total = time.perf_counter() - t_start
+        st = srv.stats()
+    lat = np.array([res[i].latency for i in range(n)]) * 1e3
+    # spot check: a few requests recomputed alone by the plain FP64 path
+    from dirac_caspt2_amd.problem import load_factors
+    dmax = 0.0
+    for i in range(0, n, max(1, n // 6))[:6]:
+        m = int(which[i])
+        s, ras = mols[m]
+        ctx = Caspt2Context(0)
+        load_factors(ctx, (s.ninact, s.nact, s.nsec, s.nelec), synth.scan_point(sps[m], float(lams[i]), dirs[m]),
+                     sps[m].B, sps[m].ecore, on_device=True)
+        if ras:
+            ctx.set_ras(*ras)
+        ctx.casci(1)
+        e = ctx.caspt2(1, 0.0)
+        ctx.close()
+        dmax = max(dmax, max(abs(e.e2[X] - res[i].e2[X]) for X in "ABCDEFGH"))
+    return {"metric": "served CASPT2 energies/s + p99 latency (Poisson arrivals)", "value": n / t_total,
+            "unit": "energies/s", "n_gpus": 1, "higher_is_better": True,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "offered_rate_per_s": args.rate, "requests": n, "molecules": len(mols),
+            "ras_share": float(np.mean([mols[int(w)][1] is not None for w in which])),
+            "slots": args.slots, "eig_batch": args.eig_batch, "eig_batches": st["eig_batches"],
+            "eig_jobs": st["eig_jobs"], "split_slices": args.split, "peak_footprint_bytes": st["peak"],
+            "spot_check_max_abs_dE2_vs_plain_fp64": dmax, "dtype": "f64 (complex128)",
+            "data": "synthetic (seeded scan points)",
+            "config": {"workload": f"serve: {n} requests at {args.rate}/s over {len(mols)} molecules"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="fp64", choices=["small", "fp64", "split"])
+    ap.add_argument("--workload", default="fp64", choices=["small", "fp64", "split", "batch", "serve"])
+    ap.add_argument("--instances", type=int, default=64, help="batch workload: independent instances")
+    ap.add_argument("--requests", type=int, default=2000, help="serve workload: requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve workload: Poisson arrival rate (1/s)")
+    ap.add_argument("--molecules", type=int, default=12, help="serve workload: resident molecules")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run 3 device-to-device copies of MIB MiB (a plain copy kernel "
@@ -185,6 +356,12 @@ def main():
               "(the sharding design is documented in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch":
+        print(json.dumps(batch_workload(args)))
+        return
+    if args.workload == "serve":
+        print(json.dumps(serve_workload(args)))
+        return
     from dirac_caspt2_amd import synth
     from dirac_caspt2_amd.api import Caspt2Context
     from dirac_caspt2_amd.problem import load_factors
@@ -257,9 +434,20 @@ def main():
                      "all_gemm_achieved_tflops": ach_all,
                      "transform_launches_achieved_tflops": ach_tr},
         "eig_fallbacks": ctx.eig_fallbacks(),
+        "casci_ndet_roots_davidson_iterations": ctx.casci_info(),
     }
     if args.split:
         result["split"] = split_steps(ctx, spec, args, out)
+        s_e2, s_total = result["split"].pop("_e2"), result["split"].pop("_total")
+        gold = os.path.join(HERE, "tests", "golden", f"synthetic_{args.workload}.json")
+        if os.path.exists(gold):     # the oracle's energies of this seeded workload
+            g = json.load(open(gold))
+            result["vs_oracle"] = {
+                "fp64_gpu_abs_dE2": {X: abs(out.e2[X] - g["e2"][X]) for X in "ABCDEFGH"},
+                "split_abs_dE2": {X: abs(s_e2[X] - g["e2"][X]) for X in "ABCDEFGH"},
+                "fp64_gpu_abs_dtotal": abs(out.total - g["total"]),
+                "split_abs_dtotal": abs(s_total - g["total"]),
+                "oracle_seconds_8_cores": g.get("oracle_seconds")}
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
     if args.served:

@@ -248,6 +248,12 @@ class Caspt2Context:
         check("dc2_get_gemm_op_stats", self.lib.dc2_get_gemm_op_stats(self.h, opA.encode(), opB.encode(), _ptr(out)))
         return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
 
+    def casci_info(self):
+        """(ndet, roots held, Davidson iterations of the last casci; 0 = dense solver)."""
+        out = np.zeros(3, dtype=np.int64)
+        check("dc2_get_casci_info", self.lib.dc2_get_casci_info(self.h, _ptr(out)))
+        return tuple(int(x) for x in out)
+
     def transform_flops(self):
         """(split-precision, FP64) transform flops of the last caspt2()."""
         out = np.zeros(2)
@@ -256,7 +262,8 @@ class Caspt2Context:
 
     OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
                "ac_metric": {"auto": 0, "formula": 1, "determinant": 2},
-               "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {}}
+               "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {},
+               "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}}
 
     def set_option(self, key: str, value):
         """Options: eigensolver auto|qr|dc|dcj|jacobi, ac_metric auto|formula|determinant,
@@ -409,6 +416,15 @@ def zgemm_split(ctx: Caspt2Context, slices: int, opB: str, M, N, K, A: DeviceArr
     """Split-precision C = A^T-view * op(B) (dc2_zgemm_split) on device arrays."""
     check("dc2_zgemm_split", ctx.lib.dc2_zgemm_split(ctx.h, slices, opB.encode(), M, N, K, A.ptr, lda, B.ptr, ldb,
                                                      C.ptr, ldc))
+
+
+def host_heev(a: np.ndarray):
+    """Host Hermitian eigensolver of the library (the Davidson subspace solver): (w, V)."""
+    A = np.asfortranarray(np.asarray(a, dtype=np.complex128)).copy(order="F")
+    n = A.shape[0]
+    w = np.zeros(n)
+    check("dc2_host_heev", _lib.lib().dc2_host_heev(n, _ptr(A), _ptr(w)))
+    return w, A
 
 
 def request_footprint(ninact: int, nact: int, nsec: int, nelec: int) -> int:

@@ -20,6 +20,7 @@
 
 #include "../../include/dcaspt2_amd.h"
 #include "dc2_batch.hpp"
+#include "dc2_hostla.hpp"
 #include "dc2_kernels.hpp"
 
 using namespace dc2;
@@ -113,8 +114,11 @@ struct dc2_context {
   int64_t nfull = 0, ndet = 0;
   DevBuf<int64_t> full_dets, cas_dets;
   DevBuf<int> rank_full, rank_cas, cas2full, exc;
-  DevBuf<cplx> civec;
+  DevBuf<cplx> civec, cimat;               // eigenvectors (ndet x nroots_avail), the CI matrix
   std::vector<double> ecas;
+  int64_t nroots_avail = 0;                 // eigenpairs held in civec / ecas
+  int casci_solver = 0;                     // 0 automatic, 1 dense eigensolver, 2 block Davidson
+  int davidson_iters = 0;
   bool have_casci = false;
   // CASPT2 state
   DevBuf<cplx> fock, U, hp, cnew, c_full, cF, Y1, Y1F, G1, G1F, G2, G2F, G3, G3F;
@@ -368,7 +372,140 @@ void upload_ints(hipStream_t s, DevBuf<int>& dst, const std::vector<int>& v) {
 }
 
 // ---------------------------------------------------------------------------
-void run_casci(dc2_context* c) {
+// Lowest nroot eigenpairs of the Hermitian CI matrix H (n x n, device) by block Davidson with
+// the diagonal preconditioner: block of b >= nroot + 3 vectors (Kramers-degenerate pairs stay
+// inside the block), subspace of at most maxsub vectors (restart on the Ritz vectors), converged
+// when every requested residual |H x - theta x| is below 1e-11.  Subspace eigenproblems and the
+// block orthonormalisation run on the host (herm_jacobi).  Writes c->civec (n x nroot) and
+// c->ecas; returns false (no result) when it does not converge, and the caller falls back to
+// the dense eigensolver.
+bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
+  Lane& L = *c->lane0;
+  hipStream_t s = L.stream;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0), mone = mk(-1.0, 0.0);
+  const int b = (int)std::min<int64_t>(n, std::max(nroot + 3, 6));
+  const int maxsub = (int)std::min<int64_t>(n, std::max(6 * b, 48));
+  if (maxsub < 2 * b) return false;
+  cplx* V = L.z("dv_V", n * maxsub);
+  cplx* W = L.z("dv_W", n * maxsub);
+  cplx* X = L.z("dv_X", n * b);
+  cplx* AX = L.z("dv_AX", n * b);
+  cplx* R = L.z("dv_R", n * b);
+  cplx* T = L.z("dv_T", n * b);
+  cplx* R2 = L.z("dv_R2", n * b);
+  cplx* Sm = L.z("dv_small", (int64_t)maxsub * maxsub);
+  cplx* Sm2 = L.z("dv_small2", (int64_t)maxsub * b);
+  double* d = L.d("dv_d", n);
+  double* th = L.d("dv_th", b);
+  int* idx = L.i("dv_idx", b);
+  auto to_host = [&](void* dst, const void* src, size_t bytes) {
+    DC2_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    DC2_HIP(hipStreamSynchronize(s));
+  };
+  auto to_dev = [&](void* dst, const void* src, size_t bytes) {
+    DC2_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    DC2_HIP(hipStreamSynchronize(s));   // the host source is a local vector
+  };
+  // start: unit vectors at the b smallest diagonal elements
+  davidson_diag(s, n, H, d);
+  std::vector<double> dh(n);
+  to_host(dh.data(), d, n * sizeof(double));
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::partial_sort(order.begin(), order.begin() + b, order.end(), [&](int x, int y) { return dh[x] < dh[y]; });
+  to_dev(idx, order.data(), b * sizeof(int));
+  davidson_unit(s, n, b, idx, V);
+  zgemm(s, 'N', 'N', n, b, n, one, H, n, 0, V, n, 0, zero, W, n, 0, 1);
+  int m = b;
+  const double tol = 1e-11;
+  std::vector<zc> Th, Yh, G, Q;
+  std::vector<double> theta, gw;
+  for (int it = 1; it <= 200; ++it) {
+    c->davidson_iters = it;
+    // Rayleigh-Ritz on span(V)
+    zgemm(s, 'C', 'N', m, m, n, one, V, n, 0, W, n, 0, zero, Sm, m, 0, 1);
+    Th.resize((size_t)m * m);
+    to_host(Th.data(), Sm, (size_t)m * m * sizeof(cplx));
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < j; ++i) {
+        const zc x = 0.5 * (Th[i + (size_t)m * j] + std::conj(Th[j + (size_t)m * i]));
+        Th[i + (size_t)m * j] = x;
+        Th[j + (size_t)m * i] = std::conj(x);
+      }
+    theta.resize(m);
+    Yh.resize((size_t)m * m);
+    herm_jacobi(m, Th.data(), theta.data(), Yh.data());
+    to_dev(Sm2, Yh.data(), (size_t)m * b * sizeof(cplx));       // first b Ritz vectors
+    to_dev(th, theta.data(), b * sizeof(double));
+    zgemm(s, 'N', 'N', n, b, m, one, V, n, 0, Sm2, m, 0, zero, X, n, 0, 1);
+    zgemm(s, 'N', 'N', n, b, m, one, W, n, 0, Sm2, m, 0, zero, AX, n, 0, 1);
+    davidson_residual(s, n, b, AX, X, th, R);
+    zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, Sm, b, 0, 1);
+    std::vector<zc> rr((size_t)b * b);
+    to_host(rr.data(), Sm, rr.size() * sizeof(cplx));
+    double worst = 0.0;
+    std::vector<int> sel;
+    for (int j = 0; j < b; ++j) {
+      const double res = std::sqrt(std::fabs(rr[j + (size_t)b * j].real()));
+      if (j < nroot) worst = std::max(worst, res);
+      if (res > 0.1 * tol) sel.push_back(j);
+    }
+    if (worst < tol) {
+      c->civec.ensure(n * nroot);
+      DC2_HIP(hipMemcpyAsync(c->civec.get(), X, n * nroot * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+      DC2_HIP(hipStreamSynchronize(s));
+      c->ecas.assign(theta.begin(), theta.begin() + nroot);
+      c->nroots_avail = nroot;
+      return true;
+    }
+    if (sel.empty()) return false;
+    const int ns = (int)sel.size();
+    to_dev(idx, sel.data(), ns * sizeof(int));
+    davidson_precond(s, n, ns, idx, R, th, d, R2);
+    {   // unit columns, so that the Gram matrix after projection measures new directions only
+      zgemm(s, 'C', 'N', ns, ns, n, one, R2, n, 0, R2, n, 0, zero, Sm, ns, 0, 1);
+      std::vector<zc> g((size_t)ns * ns), dg((size_t)ns * ns, 0.0);
+      to_host(g.data(), Sm, g.size() * sizeof(cplx));
+      for (int k = 0; k < ns; ++k) {
+        const double nrm = std::sqrt(std::fabs(g[k + (size_t)ns * k].real()));
+        dg[k + (size_t)ns * k] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+      }
+      to_dev(Sm2, dg.data(), dg.size() * sizeof(cplx));
+      zgemm(s, 'N', 'N', n, ns, ns, one, R2, n, 0, Sm2, ns, 0, zero, T, n, 0, 1);
+    }
+    if (m + ns > maxsub) {   // restart on the b Ritz vectors (orthonormal)
+      DC2_HIP(hipMemcpyAsync(V, X, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+      DC2_HIP(hipMemcpyAsync(W, AX, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+      m = b;
+    }
+    // two classical Gram-Schmidt passes against V, then an orthonormal basis of the block
+    for (int pass = 0; pass < 2; ++pass) {
+      zgemm(s, 'C', 'N', m, ns, n, one, V, n, 0, T, n, 0, zero, Sm, m, 0, 1);
+      zgemm(s, 'N', 'N', n, ns, m, mone, V, n, 0, Sm, m, 0, one, T, n, 0, 1);
+    }
+    zgemm(s, 'C', 'N', ns, ns, n, one, T, n, 0, T, n, 0, zero, Sm, ns, 0, 1);
+    G.resize((size_t)ns * ns);
+    to_host(G.data(), Sm, G.size() * sizeof(cplx));
+    Q.resize(G.size());
+    gw.resize(ns);
+    herm_jacobi(ns, G.data(), gw.data(), Q.data());
+    std::vector<zc> Sc;
+    int nk = 0;
+    for (int k = ns - 1; k >= 0; --k) {
+      if (!(gw[k] > 1e-12) || m + nk >= maxsub) continue;   // |projected unit vector|^2
+      for (int i = 0; i < ns; ++i) Sc.push_back(Q[i + (size_t)ns * k] / std::sqrt(gw[k]));
+      ++nk;
+    }
+    if (nk == 0) return false;
+    to_dev(Sm2, Sc.data(), Sc.size() * sizeof(cplx));
+    zgemm(s, 'N', 'N', n, nk, ns, one, T, n, 0, Sm2, ns, 0, zero, V + n * m, n, 0, 1);
+    zgemm(s, 'N', 'N', n, nk, n, one, H, n, 0, V + n * m, n, 0, zero, W + n * m, n, 0, 1);
+    m += nk;
+  }
+  return false;
+}
+
+void run_casci(dc2_context* c, int nroot) {
   need_spaces(c);
   DC2_CHECK(c->h.get() != nullptr, 1, "dc2_set_one_electron has not been called");
   const cplx* O = get_block(c, "OOOO");
@@ -406,14 +543,21 @@ void run_casci(dc2_context* c) {
   c->cas2full.ensure(nd);
   enumerate_dets(s, p, ff, fc, c->rank_full.get(), c->rank_cas.get(), c->full_dets.get(), c->cas_dets.get(),
                  c->cas2full.get(), &nf, &nd, counts);
-  c->civec.ensure(nd * nd);
+  c->cimat.ensure(nd * nd);
   casmat(s, c->ni, na, c->ne, nd, c->cas_dets.get(), c->rank_cas.get(), O, c->nmoc(), c->h.get(), c->nmo(),
-         c->civec.get());
-  double* w = L0.d("ci_w", nd);
-  zheevd(c, (int)nd, c->civec.get(), w, "the CASCI matrix");
-  c->ecas.resize(nd);
-  DC2_HIP(hipMemcpyAsync(c->ecas.data(), w, nd * sizeof(double), hipMemcpyDeviceToHost, s));
-  DC2_HIP(hipStreamSynchronize(s));
+         c->cimat.get());
+  const bool dav = c->casci_solver == 2 || (c->casci_solver == 0 && nd >= 1500 && nroot <= 16);
+  c->davidson_iters = 0;
+  if (!dav || !davidson(c, nd, c->cimat.get(), std::max(nroot, 1))) {
+    c->civec.ensure(nd * nd);
+    DC2_HIP(hipMemcpyAsync(c->civec.get(), c->cimat.get(), nd * nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+    double* w = L0.d("ci_w", nd);
+    zheevd(c, (int)nd, c->civec.get(), w, "the CASCI matrix");
+    c->ecas.resize(nd);
+    DC2_HIP(hipMemcpyAsync(c->ecas.data(), w, nd * sizeof(double), hipMemcpyDeviceToHost, s));
+    DC2_HIP(hipStreamSynchronize(s));
+    c->nroots_avail = nd;
+  }
   c->have_casci = true;
   c->have_pt2 = false;
   c->timings[0] = tm.stop();
@@ -717,7 +861,8 @@ Lane& worker_lane(dc2_context* c, int i) {
 
 void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, double* sumc2, double* scal) {
   DC2_CHECK(c->have_casci, 1, "dc2_casci must run before dc2_caspt2");
-  DC2_CHECK(selectroot >= 1 && selectroot <= c->ndet, 1, "selectroot out of range");
+  DC2_CHECK(selectroot >= 1 && selectroot <= c->nroots_avail, 1,
+            "selectroot out of range (dc2_casci must compute at least selectroot roots)");
   DC2_CHECK(!(c->ni == 0 && c->ns == 0), 1, "The CASPT2 energy cannot be defined when ninact = 0 and nsec = 0.");
   hipStream_t s = c->stream;
   const int ni = c->ni, na = c->na, ns = c->ns, nmo = c->nmo();
@@ -1082,9 +1227,9 @@ int dc2_casci(dc2_context* c, int32_t nroot, double* eigen, int64_t* ndet_out) {
   return guarded([&] {
     DC2_CHECK(c, 1, "null context");
     DC2_HIP(hipSetDevice(c->device));
-    run_casci(c);
+    run_casci(c, nroot);
     if (ndet_out) *ndet_out = c->ndet;
-    for (int k = 0; k < nroot && k < c->ndet; ++k) eigen[k] = c->ecas[k] + c->ecore;
+    for (int k = 0; k < nroot && k < c->nroots_avail; ++k) eigen[k] = c->ecas[k] + c->ecore;
   });
 }
 
@@ -1098,7 +1243,7 @@ int dc2_get_dets(dc2_context* c, int64_t* dets) {
 int dc2_get_ci_vector(dc2_context* c, int32_t root, double* v) {
   return guarded([&] {
     DC2_CHECK(c && c->have_casci, 1, "no CASCI result");
-    DC2_CHECK(root >= 1 && root <= c->ndet, 1, "root out of range");
+    DC2_CHECK(root >= 1 && root <= c->nroots_avail, 1, "root out of range (not computed by dc2_casci)");
     DC2_HIP(hipMemcpy(v, c->civec.get() + (int64_t)(root - 1) * c->ndet, c->ndet * sizeof(cplx),
                       hipMemcpyDeviceToHost));
   });
@@ -1175,10 +1320,7 @@ int dc2_get_timings(dc2_context* c, double* t, int32_t n) {
 int dc2_casmat(dc2_context* c, double* mat) {
   return guarded([&] {
     DC2_CHECK(c && c->have_casci && mat, 1, "no CASCI result");
-    DevBuf<cplx> m(c->ndet * c->ndet);
-    casmat(c->stream, c->ni, c->na, c->ne, c->ndet, c->cas_dets.get(), c->rank_cas.get(), get_block(c, "OOOO"),
-           c->nmoc(), c->h.get(), c->nmo(), m.get());
-    DC2_HIP(hipMemcpyAsync(mat, m.get(), c->ndet * c->ndet * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
+    DC2_HIP(hipMemcpyAsync(mat, c->cimat.get(), c->ndet * c->ndet * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
     DC2_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -1342,6 +1484,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "schur") {
       DC2_CHECK(value == 0 || value == 1, 1, "schur must be 0 or 1");
       c->schur = value != 0;
+    } else if (k == "casci_solver") {
+      DC2_CHECK(value >= 0 && value <= 2, 1, "casci_solver must be 0..2");
+      c->casci_solver = (int)value;
     } else if (k == "split_slices") {
       DC2_CHECK(value == 0 || (value >= 3 && value <= 8), 1, "split_slices must be 0 or 3..8");
       c->split_slices = (int)value;
@@ -1392,6 +1537,24 @@ int dc2_get_transform_flops(dc2_context* c, double* out) {
     DC2_CHECK(c && out, 1, "null argument");
     out[0] = c->split_flops;
     out[1] = c->fp64_flops;
+  });
+}
+
+int dc2_host_heev(int32_t n, double* a, double* w) {
+  return guarded([&] {
+    DC2_CHECK(a && w && n >= 1 && n <= 4096, 1, "invalid argument");
+    std::vector<zc> A(reinterpret_cast<zc*>(a), reinterpret_cast<zc*>(a) + (size_t)n * n), V((size_t)n * n);
+    herm_jacobi(n, A.data(), w, V.data());
+    std::copy(V.begin(), V.end(), reinterpret_cast<zc*>(a));
+  });
+}
+
+int dc2_get_casci_info(dc2_context* c, int64_t* out) {
+  return guarded([&] {
+    DC2_CHECK(c && out, 1, "null argument");
+    out[0] = c->ndet;
+    out[1] = c->nroots_avail;
+    out[2] = c->davidson_iters;
   });
 }
 

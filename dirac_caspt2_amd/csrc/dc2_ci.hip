@@ -500,6 +500,35 @@ __global__ void k_fdet(int na, int64_t m, const int64_t* strings, const double* 
   f[r] = s;
 }
 
+// ---- block Davidson helpers
+__global__ void k_diag_real(int64_t n, const cplx* H, double* d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = H[i + n * i].x;
+}
+
+__global__ void k_unit_cols(int64_t n, int ncol, const int* rows, cplx* V) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n) V[i + n * j] = mk(i == rows[j] ? 1.0 : 0.0, 0.0);
+}
+
+// R[:, j] = AX[:, j] - theta_j X[:, j]
+__global__ void k_ritz_residual(int64_t n, const cplx* AX, const cplx* X, const double* theta, cplx* R) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n) R[i + n * j] = csub(AX[i + n * j], cscale(X[i + n * j], theta[j]));
+}
+
+// T[:, c] = R[:, sel_c] / (theta_{sel_c} - d), denominators kept at least 1e-8 in magnitude
+__global__ void k_precond(int64_t n, const int* sel, const cplx* R, const double* theta, const double* d, cplx* T) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y, j = sel[c];
+  if (i >= n) return;
+  double den = theta[j] - d[i];
+  if (fabs(den) < 1e-8) den = den < 0 ? -1e-8 : 1e-8;
+  T[i + n * c] = cscale(R[i + n * j], 1.0 / den);
+}
+
 __global__ void k_scale_rows(int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (r >= m) return;
@@ -641,6 +670,25 @@ void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int
   hipLaunchKernelGGL(k_phi, dim3(cdiv(m, 256), na, na * na), dim3(256), 0, s, na, create ? 1 : 0, m, strings, nf,
                      rank_full, Y1, Phi);
   hipLaunchKernelGGL(k_fdet, dim3(cdiv(m, 256)), dim3(256), 0, s, na, m, strings, eps_act, fdiag);
+  DC2_HIP(hipGetLastError());
+}
+
+void davidson_diag(hipStream_t s, int64_t n, const cplx* H, double* d) {
+  hipLaunchKernelGGL(k_diag_real, dim3(cdiv(n, 256)), dim3(256), 0, s, n, H, d);
+  DC2_HIP(hipGetLastError());
+}
+void davidson_unit(hipStream_t s, int64_t n, int ncol, const int* rows, cplx* V) {
+  hipLaunchKernelGGL(k_unit_cols, dim3(cdiv(n, 256), ncol), dim3(256), 0, s, n, ncol, rows, V);
+  DC2_HIP(hipGetLastError());
+}
+void davidson_residual(hipStream_t s, int64_t n, int ncol, const cplx* AX, const cplx* X, const double* theta,
+                       cplx* R) {
+  hipLaunchKernelGGL(k_ritz_residual, dim3(cdiv(n, 256), ncol), dim3(256), 0, s, n, AX, X, theta, R);
+  DC2_HIP(hipGetLastError());
+}
+void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const cplx* R, const double* theta,
+                      const double* d, cplx* T) {
+  hipLaunchKernelGGL(k_precond, dim3(cdiv(n, 256), ncol), dim3(256), 0, s, n, sel, R, theta, d, T);
   DC2_HIP(hipGetLastError());
 }
 

@@ -9,6 +9,8 @@
 // Used for: the four one-index integral transforms (module_intra.F90), the
 // RDM contractions (density_matrix.F90), the V contractions and the
 // (U L^-1/2)^H B (U L^-1/2) products of every solve_X_subspace.
+#include <algorithm>
+#include <map>
 #include <mutex>
 
 #include "dc2_common.hpp"
@@ -35,16 +37,21 @@ __device__ inline cplx loadB(const cplx* __restrict__ B, int64_t ldb, int64_t k,
   return OPB == 'C' ? cconj(v) : v;
 }
 
+// blockIdx.z = batch index * sk + K-split index; split kz covers K in [kz*kchunk, (kz+1)*kchunk)
+// and writes its partial product to C + kz * sC (the split-K workspace) when sk > 1.
 template <char OPA, char OPB>
-__global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t K, cplx alpha,
+__global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t Kfull, cplx alpha,
                                                const cplx* __restrict__ A, int64_t lda, int64_t sA,
                                                const cplx* __restrict__ B, int64_t ldb, int64_t sB,
-                                               cplx beta, cplx* __restrict__ C, int64_t ldc, int64_t sC) {
+                                               cplx beta, cplx* __restrict__ C, int64_t ldc, int64_t sC, int sk,
+                                               int64_t kchunk) {
   __shared__ double Ar[BK][LDS_STRIDE], Ai[BK][LDS_STRIDE], Br[BK][LDS_STRIDE], Bi[BK][LDS_STRIDE];
-  const int64_t bz = blockIdx.z;
+  const int64_t bz = blockIdx.z / sk, kz = blockIdx.z % sk;
   A += bz * sA;
   B += bz * sB;
-  C += bz * sC;
+  C += (sk > 1 ? kz : bz) * sC;
+  const int64_t kbeg = kz * kchunk;
+  const int64_t K = (kbeg + kchunk < Kfull) ? kbeg + kchunk : Kfull;   // loop end
   const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -57,7 +64,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t K, 
       acci[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+  for (int64_t k0 = kbeg; k0 < K; k0 += BK) {
     // stage A (BM x BK) and B (BK x BN) tiles; the index order follows the
     // contiguous direction of the operand in memory
 #pragma unroll
@@ -124,6 +131,29 @@ __global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t K, 
       }
 }
 
+// C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order)
+__global__ void k_splitk_reduce(int64_t M, int64_t N, int sk, const cplx* __restrict__ W, cplx alpha, cplx beta,
+                                cplx* __restrict__ C, int64_t ldc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * N) return;
+  const int64_t m = i % M, n = i / M;
+  cplx acc = mk(0.0, 0.0);
+  for (int z = 0; z < sk; ++z) acc = cadd(acc, W[i + M * N * z]);
+  cplx out = cmul(alpha, acc);
+  if (beta.x != 0.0 || beta.y != 0.0) cfma(out, beta, C[m + ldc * n]);
+  C[m + ldc * n] = out;
+}
+
+// grow-only split-K workspaces, one per stream (kernels on one stream run in order)
+std::mutex g_ws_mu;
+std::map<hipStream_t, DevBuf<cplx>> g_ws;
+cplx* splitk_workspace(hipStream_t s, size_t n) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  DevBuf<cplx>& b = g_ws[s];
+  b.ensure(n);
+  return b.get();
+}
+
 GemmStats* g_stats = nullptr;
 std::mutex g_stats_mu;
 thread_local int g_tag = 0;
@@ -131,7 +161,17 @@ thread_local int g_tag = 0;
 template <char OPA, char OPB>
 void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda, int64_t sA,
             const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC, int64_t batch) {
-  dim3 grid(cdiv(M, BM), cdiv(N, BN), (unsigned)batch);
+  // split K over workgroups when the output has few tiles and K is long
+  const int64_t tiles = (int64_t)cdiv(M, BM) * cdiv(N, BN) * batch;
+  int sk = 1;
+  int64_t kchunk = K;
+  if (batch == 1 && tiles < 128 && K >= 1024) {
+    int64_t want = std::min<int64_t>(K / 512, (256 + tiles - 1) / tiles);
+    want = std::max<int64_t>(2, std::min<int64_t>(want, 32));
+    kchunk = ((K + want - 1) / want + BK - 1) / BK * BK;
+    sk = (int)((K + kchunk - 1) / kchunk);
+  }
+  dim3 grid(cdiv(M, BM), cdiv(N, BN), (unsigned)(batch * sk));
   GemmRecord r{};
   const bool acct = g_stats && g_stats->on;
   if (acct) {
@@ -139,8 +179,15 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     DC2_HIP(hipEventCreate(&r.b));
     DC2_HIP(hipEventRecord(r.a, s));
   }
-  hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                     C, ldc, sC);
+  if (sk > 1) {
+    cplx* W = splitk_workspace(s, (size_t)sk * M * N);
+    hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, mk(1.0, 0.0), A, lda, sA, B, ldb, sB,
+                       mk(0.0, 0.0), W, M, M * N, sk, kchunk);
+    hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, s, M, N, sk, W, alpha, beta, C, ldc);
+  } else {
+    hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                       C, ldc, sC, 1, K);
+  }
   DC2_HIP(hipGetLastError());
   if (acct) {
     DC2_HIP(hipEventRecord(r.b, s));

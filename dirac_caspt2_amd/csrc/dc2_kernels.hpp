@@ -61,6 +61,14 @@ int64_t binom_host(int n, int k);
 void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int* rank_full, const cplx* Y1,
                const double* eps_act, int64_t m, int64_t* strings, cplx* Phi, double* fdiag);
 void scale_rows(hipStream_t s, int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out);
+// block Davidson helpers: diagonal of H, unit start vectors e_rows[j], Ritz residuals
+// R = AX - X diag(theta), diagonal preconditioner T[:, c] = R[:, sel_c] / (theta_sel_c - d)
+void davidson_diag(hipStream_t s, int64_t n, const cplx* H, double* d);
+void davidson_unit(hipStream_t s, int64_t n, int ncol, const int* rows, cplx* V);
+void davidson_residual(hipStream_t s, int64_t n, int ncol, const cplx* AX, const cplx* X, const double* theta,
+                       cplx* R);
+void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const cplx* R, const double* theta,
+                      const double* d, cplx* T);
 
 // ---------------------------------------------------------------- dc2_ozaki.hip
 // Split-precision (Ozaki, INT8 MFMA) complex GEMM C = A^T-view * op(B) of the transform

@@ -143,3 +143,36 @@ def scan_point(sp: SynthProblem, lam: float, direction: np.ndarray) -> np.ndarra
     k = np.arange(sp.spec.nmo)
     h[k, k] += lam * direction
     return h
+
+
+def batch_instances(n: int = 64, seed: int = 20243):
+    """Independent instances of the batching workload, one seed each: nsec ~ U[40, 300],
+    nact in {8, 10, 12} with nact/2 electrons, ninact ~ U[4, 24] (all even: Kramers pairs)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nact = int(rng.choice([8, 10, 12]))
+        ninact = 2 * int(rng.integers(2, 13))
+        nsec = 2 * int(rng.integers(20, 151))
+        out.append(SynthSpec(ninact=ninact, nact=nact, nsec=nsec, nelec=nact // 2, seed=seed * 1000 + k))
+    return out
+
+
+# RAS restriction of the RASPT2 share of the serving workload: RAS1/RAS2/RAS3 of 4 spinors
+# each (nact = 12, 6 electrons), at most 2 holes in RAS1 and 2 electrons in RAS3
+SERVE_RAS = (4, 4, 4, 2, 2, 0, 0)
+
+
+def serve_molecules(n: int = 12, seed: int = 20244):
+    """Molecules of the serving workload: nsec ~ U[40, 280], nact in {8, 12, 14} with nact/2
+    electrons, ninact ~ U[4, 24]; every third molecule is a RASPT2 one (SERVE_RAS)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        ras = k % 3 == 2
+        nact = 12 if ras else int(rng.choice([8, 12, 14]))
+        ninact = 2 * int(rng.integers(2, 13))
+        nsec = 2 * int(rng.integers(20, 141))
+        out.append((SynthSpec(ninact=ninact, nact=nact, nsec=nsec, nelec=nact // 2, seed=seed * 1000 + k),
+                    SERVE_RAS if ras else None))
+    return out

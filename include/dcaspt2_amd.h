@@ -81,7 +81,8 @@ int dc2_set_integral_block(dc2_context* ctx, const char* name, const double* dat
 
 /* ---- CASCI (casci.F90:31-72) --------------------------------------------------
  * nroot energies eigen(k) = ecas(k) + ecore are written to eigen[0..nroot-1]
- * (fewer if ndet < nroot; *ndet_out receives ndet). */
+ * (fewer if ndet < nroot; *ndet_out receives ndet).  At least selectroot roots must be
+ * requested before dc2_caspt2 (the Davidson solver computes only the requested ones). */
 int dc2_casci(dc2_context* ctx, int32_t nroot, double* eigen, int64_t* ndet_out);
 /* determinant bit strings (ndet int64) and the CI vector of a root (ndet complex) */
 int dc2_get_dets(dc2_context* ctx, int64_t* dets);
@@ -148,7 +149,10 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  eigendecomposition of B~;
  *   "split_slices": 0 (default) FP64 integral transforms, 3..8 split-precision passes
  *                  (dc2_zgemm_split) with this many slices;
- *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32). */
+ *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32);
+ *   "casci_solver": 0 automatic (default: block Davidson for ndet >= 1500 and nroot <= 16,
+ *                  the dense eigensolver otherwise), 1 dense (all roots, as cdiag), 2 Davidson
+ *                  (the nroot lowest roots; falls back to dense when it does not converge). */
 int dc2_set_option(dc2_context* ctx, const char* key, int64_t value);
 /* Split-precision complex GEMM (Ozaki scheme on INT8 matrix cores; no counterpart in
  * the reference): C[m + ldc*j] = sum_k A[k + lda*m] * op(B)[k + ldb*j] on device memory,
@@ -159,6 +163,12 @@ int dc2_zgemm_split(dc2_context* ctx, int32_t slices, char opB, int64_t M, int64
                     int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc);
 /* transform flops of the last dc2_caspt2: out[0] in split precision, out[1] in FP64 */
 int dc2_get_transform_flops(dc2_context* ctx, double* out);
+/* out[3] = {ndet, roots held by the last dc2_casci, Davidson iterations (0: dense solver)} */
+int dc2_get_casci_info(dc2_context* ctx, int64_t* out);
+/* Hermitian eigendecomposition on the host (cyclic Jacobi; the Davidson subspace solver):
+ * a (n x n complex, column-major) is overwritten by the eigenvectors, w by the ascending
+ * eigenvalues.  No GPU needed. */
+int dc2_host_heev(int32_t n, double* a, double* w);
 /* number of eigensolves that fell back to the tridiagonal QR algorithm */
 int dc2_get_eig_fallbacks(dc2_context* ctx, int64_t* n);
 

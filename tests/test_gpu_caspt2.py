@@ -119,3 +119,43 @@ def test_medium_caspt2(medium, metric, schur, lanes):
             blocks = ref.e2_blocks[X]
             if blocks:
                 assert out.info[X][1] == sum(b.dimm for _, b in blocks), X   # (dimn, dimm, npert, ran)
+
+
+def test_small_davidson(small):
+    """Block Davidson CASCI (forced at 70 determinants) against the oracle: the 5 lowest
+    CASCI energies within 1e-10 Eh, and CASPT2 for roots 1 and 2 within 1e-9 Eh."""
+    from gpu_helpers import assert_energies, context_from_synth, say
+    from oracle import caspt2_ref as R
+    sp, p, cas, ref = small
+    with context_from_synth(sp) as ctx:
+        ctx.set_option("casci_solver", "davidson")
+        eig = ctx.casci(5)
+        nd, nr, iters = ctx.casci_info()
+        say("davidson: ndet", nd, "roots", nr, "iterations", iters)
+        assert nr == 5 and iters > 0
+        np.testing.assert_allclose(eig, cas[3][:5], atol=1e-10, rtol=0)
+        out = ctx.caspt2(1, 0.0)
+        assert_energies(out, ref, 1e-9, "davidson root 1")
+        p2 = R.Problem(p.ninact, p.nact, p.nsec, p.nelec, p.h, p.eri, p.ecore, selectroot=2, eshift=0.1)
+        out2 = ctx.caspt2(2, 0.1)
+        assert_energies(out2, R.run_caspt2(p2, cas), 1e-9, "davidson root 2")
+
+
+def test_medium_davidson_vs_dense(medium):
+    """192 spinors: Davidson and the dense solver give the same CASCI energies (1e-10 Eh)
+    and CASPT2 energies (1e-9 Eh)."""
+    from gpu_helpers import assert_energies, context_from_synth, say
+    sp, ref = medium
+    with context_from_synth(sp) as ctx:
+        ctx.set_option("casci_solver", "davidson")
+        e_auto = ctx.casci(3)
+        assert ctx.casci_info()[2] > 0
+        out_auto = ctx.caspt2(1, 0.0)
+        ctx.set_option("casci_solver", "dense")
+        e_dense = ctx.casci(3)
+        assert ctx.casci_info()[2] == 0
+        out_dense = ctx.caspt2(1, 0.0)
+    say("davidson vs dense casci", np.abs(e_auto - e_dense).max())
+    np.testing.assert_allclose(e_auto, e_dense, atol=1e-10, rtol=0)
+    assert_energies(out_auto, ref, 1e-9, "davidson")
+    assert_energies(out_dense, ref, 1e-9, "dense")
