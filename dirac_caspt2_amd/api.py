@@ -261,7 +261,7 @@ class Caspt2Context:
         return float(out[0]), float(out[1])
 
     OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
-               "ac_metric": {"auto": 0, "formula": 1, "determinant": 2},
+               "ac_metric": {"auto": 0, "formula": 1, "determinant": 2, "antisymmetric": 3},
                "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {},
                "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}}
 

@@ -635,28 +635,30 @@ void finish_block(dc2_context* c, Lane& L, int64_t dimm, int64_t npert, cplx* bc
 
 // count of retained eigenvalues (w >= 1e-8, cutoff.F90:64-69) and the distance of the
 // closest eigenvalue to the threshold
-int64_t count_retained(Lane& L, int64_t n, const double* w_dev, double& margin) {
+// closest eigenvalue to the threshold; scale: the factor between these eigenvalues and those
+// of the reference's S matrix (2 for the antisymmetric reduction of A)
+int64_t count_retained(Lane& L, int64_t n, const double* w_dev, double& margin, double scale = 1.0) {
   std::vector<double> w(n);
   DC2_HIP(hipMemcpyAsync(w.data(), w_dev, n * sizeof(double), hipMemcpyDeviceToHost, L.stream));
   DC2_HIP(hipStreamSynchronize(L.stream));
   const double thr = 1.0e-8;
   int64_t dimm = 0;
   for (int64_t k = 0; k < n; ++k) {
-    if (w[k] >= thr) ++dimm;
-    margin = std::min(margin, std::fabs(w[k] - thr));
+    if (scale * w[k] >= thr) ++dimm;
+    margin = std::min(margin, std::fabs(scale * w[k] - thr));
   }
   return dimm;
 }
 
 // S -> cdiag(1e-8) -> ccutoff -> ulambda_s_half -> B~ -> cdiag -> E2 (solve_A_subspace.F90:150-266)
 SolveOut solve_block(dc2_context* c, Lane& L, int64_t n, int64_t npert, cplx* S, cplx* B, const cplx* V,
-                     const double* alpha, EigKind kind) {
+                     const double* alpha, EigKind kind, double scale = 1.0) {
   SolveOut r;
   if (n == 0) return r;
   hipStream_t s = L.stream;
   double* ws = L.d("sb_ws", n);
   zheevd(c, L, (int)n, S, ws, "an S matrix", kind);
-  const int64_t dimm = count_retained(L, n, ws, r.margin);
+  const int64_t dimm = count_retained(L, n, ws, r.margin, scale);
   r.dimm = dimm;
   if (dimm == 0 || npert == 0) return r;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
@@ -777,15 +779,37 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
       case 'F': n = ntp; npert = nsp; break;
       case 'G': n = na; npert = nsp * ni; break;
     }
-    // A and C: determinant-space metric when the (nelec+-1)-electron space is smaller than nact^3
-    const bool gram = (X == 'A' || X == 'C') && c->ac_route != 1 &&
-                      (c->ac_route == 2 || binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) < n);
-    cplx* S = gram ? nullptr : L.z("S", n * n);
-    cplx* B = gram ? nullptr : L.z("B", n * n);
+    // A and C: determinant-space metric when the (nelec+-1)-electron space is the smallest
+    // basis; A: the antisymmetric reduction (tuples t < u) when that is smaller still
+    const int64_t mdet = (X == 'A' || X == 'C') ? binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) : 0;
+    const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+    bool gram = false, reduced = false;
+    if (X == 'A' || X == 'C') {
+      if (c->ac_route == 2) gram = true;
+      else if (c->ac_route == 3) reduced = X == 'A';
+      else if (c->ac_route == 0) {
+        if (X == 'A' && nred < mdet && nred < n) reduced = true;
+        else gram = mdet < n;
+      }
+    }
+    const int64_t nb = reduced ? nred : n;   // dimension of the S/B matrices built
+    cplx* S = gram ? nullptr : L.z("S", nb * nb);
+    cplx* B = gram ? nullptr : L.z("B", nb * nb);
     cplx* V = L.z("V", n * npert);
     double* alpha = L.d("alpha", npert);
     const int* pairs = (X == 'B' || X == 'F') ? c->tpairs.get() : nullptr;
-    if (n > 0 && !gram) build_SB(s, X, d, eps_act, c->e0.get(), n, pairs, S, B);
+    int* amap = nullptr;
+    if (reduced) {   // row k of the reduced matrices is tuple (t, u, v), t < u, of the formulas
+      std::vector<int> hm;
+      hm.reserve(nred);
+      for (int t = 0; t < na; ++t)
+        for (int u = t + 1; u < na; ++u)
+          for (int v = 0; v < na; ++v) hm.push_back((t * na + u) * na + v);
+      amap = L.i("A_map", nred);
+      DC2_HIP(hipMemcpyAsync(amap, hm.data(), nred * sizeof(int), hipMemcpyHostToDevice, s));
+      DC2_HIP(hipStreamSynchronize(s));
+    }
+    if (n > 0 && !gram) build_SB(s, X, d, eps_act, c->e0.get(), nb, pairs, S, B, amap);
     if (n > 0 && npert > 0) {
       switch (X) {
         case 'A': {
@@ -827,14 +851,24 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
     }
     const std::string tag(1, X);
     if (!gram) {
-      check_finite(c, s, ("S" + tag).c_str(), S, n * n);
-      check_finite(c, s, ("B" + tag).c_str(), B, n * n);
+      check_finite(c, s, ("S" + tag).c_str(), S, nb * nb);
+      check_finite(c, s, ("B" + tag).c_str(), B, nb * nb);
     }
     check_finite(c, s, ("V" + tag).c_str(), V, n * npert);
     check_finite(c, s, ("alpha" + tag).c_str(), alpha, npert);
-    const EigKind kind = (X == 'A' || X == 'C') ? EigKind::rank_deficient : EigKind::general;
-    r = gram ? solve_block_gram(c, L, X == 'A', n, npert, V, alpha)
-             : solve_block(c, L, n, npert, S, B, V, alpha, kind);
+    if (gram) {
+      r = solve_block_gram(c, L, X == 'A', n, npert, V, alpha);
+    } else if (reduced) {
+      // S_full = K^H S_red K with K K^H = 2 I (a+_u a+_t = -a+_t a+_u, a+_t a+_t = 0), so the
+      // nonzero eigenvalues of the reference's S are twice those of S_red: cutoff at 0.5e-8
+      cplx* Vr = L.z("A_Vred", nred * npert);
+      gather_rows(s, nred, npert, amap, V, n, Vr);
+      r = solve_block(c, L, nred, npert, S, B, Vr, alpha,
+                      nred > mdet ? EigKind::rank_deficient : EigKind::general, 2.0);
+    } else {
+      const EigKind kind = (X == 'A' || X == 'C') ? EigKind::rank_deficient : EigKind::general;
+      r = solve_block(c, L, n, npert, S, B, V, alpha, kind);
+    }
   }
   e2[k] = r.e2;
   sumc2[k] = r.sumc2;
@@ -1476,7 +1510,7 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
       DC2_CHECK(value >= -1 && value <= 3, 1, "eigensolver must be -1..3");
       c->eig_method = (int)value;
     } else if (k == "ac_metric") {
-      DC2_CHECK(value >= 0 && value <= 2, 1, "ac_metric must be 0..2");
+      DC2_CHECK(value >= 0 && value <= 3, 1, "ac_metric must be 0..3");
       c->ac_route = (int)value;
     } else if (k == "lanes") {
       DC2_CHECK(value >= 1 && value <= 3, 1, "lanes must be 1..3");

@@ -83,8 +83,11 @@ struct Dens {
   int na;
   const cplx *G1, *G1F, *G2, *G2F, *G3, *G3F;  // see dc2_pt2.hip for the index conventions
 };
+// map (optional, increasing): build the rows/columns map[0..n) of the formula matrices only
 void build_SB(hipStream_t s, char X, const Dens& d, const double* eps_act, const double* e0_dev, int64_t n,
-              const int* pairs, cplx* S, cplx* B);
+              const int* pairs, cplx* S, cplx* B, const int* map = nullptr);
+// out[i + nr*c] = in[map[i] + ldin*c]
+void gather_rows(hipStream_t s, int64_t nr, int64_t ncol, const int* map, const cplx* in, int64_t ldin, cplx* out);
 // effective one-electron terms of A, C, D (tramo1 + integral sums)
 void effh_A(hipStream_t s, int ni, int na, const cplx* hp, int ldh, const cplx* A2, cplx* effh);
 void effh_C(hipStream_t s, int ni, int na, int ns, const cplx* hp, int ldh, const cplx* C1, const cplx* C2,

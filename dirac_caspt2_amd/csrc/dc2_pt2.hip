@@ -88,18 +88,20 @@ __device__ void sb_formula(const DV& d, const double* e, double e0, const int* p
   }
 }
 
+// map (optional, increasing): row/column k of the built matrices is tuple map[k] of the formula
 template <char X>
-__global__ void k_build_SB(DV d, const double* eps_act, const double* e0p, int64_t n, const int* pairs, cplx* S,
-                           cplx* B) {
+__global__ void k_build_SB(DV d, const double* eps_act, const double* e0p, int64_t n, const int* pairs,
+                           const int* map, cplx* S, cplx* B) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= n) return;
   const double e0 = *e0p;
+  const int64_t fi = map ? map[i] : i, fj = map ? map[j] : j;
   cplx s, b;
   if (i <= j) {
-    sb_formula<X>(d, eps_act, e0, pairs, i, j, s, b);
+    sb_formula<X>(d, eps_act, e0, pairs, fi, fj, s, b);
     if (i == j) { s = cconj(s); b = cconj(b); }   // sc(j,i) = DCONJG(sc(i,j)) with i == j
   } else {
-    sb_formula<X>(d, eps_act, e0, pairs, j, i, s, b);
+    sb_formula<X>(d, eps_act, e0, pairs, fj, fi, s, b);
     s = cconj(s);
     b = cconj(b);
   }
@@ -434,6 +436,11 @@ __global__ __launch_bounds__(RB) void k_final2(int64_t np, const double* partial
   if (threadIdx.x == 0) { out2[0] = e; out2[1] = s; }
 }
 
+__global__ void k_gather_rows(int64_t nr, int64_t ncol, const int* map, const cplx* in, int64_t ldin, cplx* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (i < nr) out[i + nr * c] = in[map[i] + ldin * c];
+}
+
 }  // namespace
 
 int64_t e2_partials_needed(int64_t nwork) {
@@ -444,20 +451,28 @@ int64_t e2_partials_needed(int64_t nwork) {
 }
 
 void build_SB(hipStream_t s, char X, const Dens& d, const double* eps_act, const double* e0_dev, int64_t n,
-              const int* pairs, cplx* S, cplx* B) {
+              const int* pairs, cplx* S, cplx* B, const int* map) {
   DC2_CHECK(n < 65536, 1, "build_SB: dimension too large");
   dim3 g(cdiv(n, 128), (unsigned)n);
   DV v = dv(d);
   switch (X) {
-    case 'A': hipLaunchKernelGGL(k_build_SB<'A'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
-    case 'B': hipLaunchKernelGGL(k_build_SB<'B'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
-    case 'C': hipLaunchKernelGGL(k_build_SB<'C'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
-    case 'D': hipLaunchKernelGGL(k_build_SB<'D'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
-    case 'E': hipLaunchKernelGGL(k_build_SB<'E'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
-    case 'F': hipLaunchKernelGGL(k_build_SB<'F'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
-    case 'G': hipLaunchKernelGGL(k_build_SB<'G'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, S, B); break;
+    case 'A': hipLaunchKernelGGL(k_build_SB<'A'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, map, S, B); break;
+    case 'B': hipLaunchKernelGGL(k_build_SB<'B'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, map, S, B); break;
+    case 'C': hipLaunchKernelGGL(k_build_SB<'C'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, map, S, B); break;
+    case 'D': hipLaunchKernelGGL(k_build_SB<'D'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, map, S, B); break;
+    case 'E': hipLaunchKernelGGL(k_build_SB<'E'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, map, S, B); break;
+    case 'F': hipLaunchKernelGGL(k_build_SB<'F'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, map, S, B); break;
+    case 'G': hipLaunchKernelGGL(k_build_SB<'G'>, g, dim3(128), 0, s, v, eps_act, e0_dev, n, pairs, map, S, B); break;
     default: throw Error(1, "build_SB: bad subspace");
   }
+  DC2_HIP(hipGetLastError());
+}
+
+void gather_rows(hipStream_t s, int64_t nr, int64_t ncol, const int* map, const cplx* in, int64_t ldin, cplx* out) {
+  if (nr <= 0 || ncol <= 0) return;
+  DC2_CHECK(ncol < 65536, 1, "gather_rows: too many columns");
+  hipLaunchKernelGGL(k_gather_rows, dim3(cdiv(nr, 256), (unsigned)ncol), dim3(256), 0, s, nr, ncol, map, in, ldin,
+                     out);
   DC2_HIP(hipGetLastError());
 }
 

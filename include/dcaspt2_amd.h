@@ -140,8 +140,10 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *   "eigensolver": -1 automatic (default: divide and conquer, with Jacobi leaves for the
  *                  rank-deficient nact^3 S matrices), 0 tridiagonal QR (the ZHEEV
  *                  algorithm), 1 divide and conquer, 2 D&C with Jacobi leaves, 3 Jacobi;
- *   "ac_metric":   0 automatic (default), 1 the RDM-formula S/B matrices of size nact^3,
- *                  2 the determinant-space factorisation of the A and C metrics;
+ *   "ac_metric":   0 automatic (default: the smallest exact basis), 1 the RDM-formula S/B
+ *                  matrices of size nact^3, 2 the determinant-space factorisation of the A and
+ *                  C metrics, 3 for A the formula matrices on the tuples t < u (a+_t a+_u is
+ *                  antisymmetric, so these span the same space and S's eigenvalues halve);
  *   "lanes":       1..3 concurrent streams (each with a host thread) for the subspace
  *                  solves (default 3: A, C and the rest);
  *   "schur":       1 (default) E2 of the determinant-space route through the compressed

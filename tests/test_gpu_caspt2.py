@@ -36,7 +36,7 @@ def test_small_casci_and_dets(small):
         np.testing.assert_allclose(M, R.casmat(p, cas[0]), atol=1e-12, rtol=0)
 
 
-METRICS = ["formula", "determinant"]
+METRICS = ["formula", "determinant", "antisymmetric"]
 
 
 @pytest.mark.parametrize("metric", METRICS)
@@ -98,7 +98,8 @@ def medium():
 
 
 @pytest.mark.parametrize("metric,schur,lanes", [("formula", "on", 3), ("determinant", "on", 3),
-                                                 ("determinant", "off", 1), ("determinant", "on", 1)])
+                                                 ("determinant", "off", 1), ("determinant", "on", 1),
+                                                 ("antisymmetric", "on", 3)])
 def test_medium_caspt2(medium, metric, schur, lanes):
     """192 spinors (16/12/164, 6 electrons, 924 determinants); the retained dimensions of
     every subspace agree with the oracle for both A/C metric routes, with E2 of the
