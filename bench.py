@@ -424,7 +424,8 @@ def main():
                                     "transforms": stage[3],
                                     **{f"subspace_{X}": stage[4 + k] for k, X in enumerate("ABCDEFGH")}},
         "energies_last_step": {"total": out.total, "e2": out.e2},
-        "roofline": {"kernel": f"k_zgemm<'{dom_op[0]}','{dom_op[1]}'> (MFMA FP64 complex GEMM)", "bound": "mfma",
+        "roofline": {"kernel": f"k_zgemm2<'{dom_op[0]}','{dom_op[1]}',16,1> (MFMA FP64 complex GEMM)",
+                     "bound": "mfma",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "launches_per_step": dom["launches"] / args.steps,

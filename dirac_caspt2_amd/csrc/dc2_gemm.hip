@@ -10,6 +10,7 @@
 // RDM contractions (density_matrix.F90), the V contractions and the
 // (U L^-1/2)^H B (U L^-1/2) products of every solve_X_subspace.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 
@@ -131,6 +132,115 @@ __global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t Kfu
       }
 }
 
+// Software-pipelined variant: the next K tile is fetched from global memory into registers
+// while the current one is multiplied out of LDS, so load latency hides behind the MFMAs.
+// KB: K per tile (16 or 32); PAD: LDS row padding in doubles (bank spread of the
+// transposed-operand stores and of the MFMA operand reads).
+template <char OPA, char OPB, int KB, int PAD>
+__global__ __launch_bounds__(256) void k_zgemm2(int64_t M, int64_t N, int64_t Kfull, cplx alpha,
+                                                const cplx* __restrict__ A, int64_t lda, int64_t sA,
+                                                const cplx* __restrict__ B, int64_t ldb, int64_t sB,
+                                                cplx beta, cplx* __restrict__ C, int64_t ldc, int64_t sC, int sk,
+                                                int64_t kchunk) {
+  constexpr int LS = BM + PAD;               // BM == BN
+  constexpr int NE = BM * KB / 256;          // elements of each operand tile per thread
+  __shared__ double Ar[KB][LS], Ai[KB][LS], Br[KB][LS], Bi[KB][LS];
+  const int64_t bz = blockIdx.z / sk, kz = blockIdx.z % sk;
+  A += bz * sA;
+  B += bz * sB;
+  C += (sk > 1 ? kz : bz) * sC;
+  const int64_t kbeg = kz * kchunk;
+  const int64_t kend = (kbeg + kchunk < Kfull) ? kbeg + kchunk : Kfull;
+  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  d4 accr[2][2], acci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accr[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      acci[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  cplx ra[NE], rb[NE];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+      const int e = tid + 256 * r;
+      int mm, kk, nn, kb;
+      if (OPA == 'N') { mm = e % BM; kk = e / BM; } else { kk = e % KB; mm = e / KB; }
+      if (OPB == 'N' || OPB == 'R') { kb = e % KB; nn = e / KB; } else { nn = e % BN; kb = e / BN; }
+      const int64_t gm = m0 + mm, gk = k0 + kk, gn = n0 + nn, gkb = k0 + kb;
+      ra[r] = (gm < M && gk < kend) ? loadA<OPA>(A, lda, gm, gk) : mk(0.0, 0.0);
+      rb[r] = (gn < N && gkb < kend) ? loadB<OPB>(B, ldb, gkb, gn) : mk(0.0, 0.0);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+      const int e = tid + 256 * r;
+      int mm, kk, nn, kb;
+      if (OPA == 'N') { mm = e % BM; kk = e / BM; } else { kk = e % KB; mm = e / KB; }
+      if (OPB == 'N' || OPB == 'R') { kb = e % KB; nn = e / KB; } else { nn = e % BN; kb = e / BN; }
+      Ar[kk][mm] = ra[r].x;
+      Ai[kk][mm] = ra[r].y;
+      Br[kb][nn] = rb[r].x;
+      Bi[kb][nn] = rb[r].y;
+    }
+  };
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore();
+  }
+  __syncthreads();
+  for (int64_t k0 = kbeg; k0 < kend; k0 += KB) {
+    const bool more = k0 + KB < kend;
+    if (more) gload(k0 + KB);
+#pragma unroll
+    for (int kk = 0; kk < KB; kk += 4) {
+      const int kl = kk + (lane >> 4);
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Ar[kl][wm + 16 * i + (lane & 15)];
+        ai[i] = Ai[kl][wm + 16 * i + (lane & 15)];
+        br[i] = Br[kl][wn + 16 * i + (lane & 15)];
+        bi[i] = Bi[kl][wn + 16 * i + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accr[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    if (more) {
+      sstore();
+      __syncthreads();
+    }
+  }
+  const bool beta_zero = (beta.x == 0.0 && beta.y == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+        const int64_t col = n0 + wn + 16 * j + (lane & 15);
+        if (row < M && col < N) {
+          cplx acc = mk(accr[i][j][r], acci[i][j][r]);
+          cplx out = cmul(alpha, acc);
+          if (!beta_zero) cfma(out, beta, C[row + ldc * col]);
+          C[row + ldc * col] = out;
+        }
+      }
+}
+
 // C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order)
 __global__ void k_splitk_reduce(int64_t M, int64_t N, int sk, const cplx* __restrict__ W, cplx alpha, cplx beta,
                                 cplx* __restrict__ C, int64_t ldc) {
@@ -155,6 +265,12 @@ cplx* splitk_workspace(hipStream_t s, size_t n) {
 }
 
 GemmStats* g_stats = nullptr;
+// kernel variant: 0 k_zgemm, 1..4 k_zgemm2 with (KB, PAD) = (16,1), (16,16), (32,1), (32,16).
+// Default 1: measured fastest on every shape of the path (e.g. 61 vs 46 TF/s on 2048^3).
+int g_variant = [] {
+  const char* v = std::getenv("DC2_GEMM_VARIANT");
+  return v ? std::atoi(v) : 1;
+}();
 std::mutex g_stats_mu;
 thread_local int g_tag = 0;
 
@@ -179,15 +295,39 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     DC2_HIP(hipEventCreate(&r.b));
     DC2_HIP(hipEventRecord(r.a, s));
   }
+  cplx* Cw = C;
+  int64_t ldw = ldc, sw = sC;
+  cplx a1 = alpha, b1 = beta;
   if (sk > 1) {
-    cplx* W = splitk_workspace(s, (size_t)sk * M * N);
-    hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, mk(1.0, 0.0), A, lda, sA, B, ldb, sB,
-                       mk(0.0, 0.0), W, M, M * N, sk, kchunk);
-    hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, s, M, N, sk, W, alpha, beta, C, ldc);
-  } else {
-    hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                       C, ldc, sC, 1, K);
+    Cw = splitk_workspace(s, (size_t)sk * M * N);
+    ldw = M;
+    sw = M * N;
+    a1 = mk(1.0, 0.0);
+    b1 = mk(0.0, 0.0);
   }
+  switch (g_variant) {
+    case 1:
+      hipLaunchKernelGGL((k_zgemm2<OPA, OPB, 16, 1>), grid, dim3(256), 0, s, M, N, K, a1, A, lda, sA, B, ldb, sB, b1,
+                         Cw, ldw, sw, sk, kchunk);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_zgemm2<OPA, OPB, 16, 16>), grid, dim3(256), 0, s, M, N, K, a1, A, lda, sA, B, ldb, sB,
+                         b1, Cw, ldw, sw, sk, kchunk);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_zgemm2<OPA, OPB, 32, 1>), grid, dim3(256), 0, s, M, N, K, a1, A, lda, sA, B, ldb, sB, b1,
+                         Cw, ldw, sw, sk, kchunk);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_zgemm2<OPA, OPB, 32, 16>), grid, dim3(256), 0, s, M, N, K, a1, A, lda, sA, B, ldb, sB,
+                         b1, Cw, ldw, sw, sk, kchunk);
+      break;
+    default:
+      hipLaunchKernelGGL((k_zgemm<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, a1, A, lda, sA, B, ldb, sB, b1, Cw,
+                         ldw, sw, sk, kchunk);
+  }
+  if (sk > 1)
+    hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, s, M, N, sk, Cw, alpha, beta, C, ldc);
   DC2_HIP(hipGetLastError());
   if (acct) {
     DC2_HIP(hipEventRecord(r.b, s));
