@@ -392,19 +392,20 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   cplx* AX = L.z("dv_AX", n * b);
   cplx* R = L.z("dv_R", n * b);
   cplx* T = L.z("dv_T", n * b);
-  cplx* R2 = L.z("dv_R2", n * b);
   cplx* Sm = L.z("dv_small", (int64_t)maxsub * maxsub);
-  cplx* Sm2 = L.z("dv_small2", (int64_t)maxsub * b);
+  cplx* Sm2 = L.z("dv_small2", (int64_t)maxsub * b + b);   // Ritz coefficients, then theta
+  cplx* Sm3 = L.z("dv_small3", (int64_t)b * b);
   double* d = L.d("dv_d", n);
-  double* th = L.d("dv_th", b);
   int* idx = L.i("dv_idx", b);
+  // host sources of the asynchronous uploads stay alive until the next synchronising download
+  std::vector<zc> up_y((size_t)maxsub * b + b), up_sc((size_t)b * b);
+  std::vector<int> up_idx(b);
   auto to_host = [&](void* dst, const void* src, size_t bytes) {
     DC2_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     DC2_HIP(hipStreamSynchronize(s));
   };
   auto to_dev = [&](void* dst, const void* src, size_t bytes) {
     DC2_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    DC2_HIP(hipStreamSynchronize(s));   // the host source is a local vector
   };
   // start: unit vectors at the b smallest diagonal elements
   davidson_diag(s, n, H, d);
@@ -413,13 +414,15 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::partial_sort(order.begin(), order.begin() + b, order.end(), [&](int x, int y) { return dh[x] < dh[y]; });
-  to_dev(idx, order.data(), b * sizeof(int));
+  std::copy(order.begin(), order.begin() + b, up_idx.begin());
+  to_dev(idx, up_idx.data(), b * sizeof(int));
   davidson_unit(s, n, b, idx, V);
   zgemm(s, 'N', 'N', n, b, n, one, H, n, 0, V, n, 0, zero, W, n, 0, 1);
   int m = b;
   const double tol = 1e-11;
-  std::vector<zc> Th, Yh, G, Q;
+  std::vector<zc> Th, Yh, G, Q, rr((size_t)b * b);
   std::vector<double> theta, gw;
+  double* th = reinterpret_cast<double*>(Sm2 + (size_t)maxsub * b);
   for (int it = 1; it <= 200; ++it) {
     c->davidson_iters = it;
     // Rayleigh-Ritz on span(V)
@@ -435,21 +438,22 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     theta.resize(m);
     Yh.resize((size_t)m * m);
     herm_jacobi(m, Th.data(), theta.data(), Yh.data());
-    to_dev(Sm2, Yh.data(), (size_t)m * b * sizeof(cplx));       // first b Ritz vectors
-    to_dev(th, theta.data(), b * sizeof(double));
+    std::copy(Yh.begin(), Yh.begin() + (size_t)m * b, up_y.begin());     // first b Ritz vectors
+    std::memcpy(up_y.data() + (size_t)maxsub * b, theta.data(), b * sizeof(double));
+    to_dev(Sm2, up_y.data(), (size_t)m * b * sizeof(cplx));
+    to_dev(th, up_y.data() + (size_t)maxsub * b, b * sizeof(double));
     zgemm(s, 'N', 'N', n, b, m, one, V, n, 0, Sm2, m, 0, zero, X, n, 0, 1);
     zgemm(s, 'N', 'N', n, b, m, one, W, n, 0, Sm2, m, 0, zero, AX, n, 0, 1);
     davidson_residual(s, n, b, AX, X, th, R);
-    zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, Sm, b, 0, 1);
-    std::vector<zc> rr((size_t)b * b);
-    to_host(rr.data(), Sm, rr.size() * sizeof(cplx));
+    // residual norms; every correction is kept (converged ones vanish in the projection)
+    zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, Sm3, b, 0, 1);
+    for (int j = 0; j < b; ++j) up_idx[j] = j;
+    to_dev(idx, up_idx.data(), b * sizeof(int));
+    davidson_precond(s, n, b, idx, R, th, d, T);
+    davidson_normalize(s, n, b, T);
+    to_host(rr.data(), Sm3, rr.size() * sizeof(cplx));
     double worst = 0.0;
-    std::vector<int> sel;
-    for (int j = 0; j < b; ++j) {
-      const double res = std::sqrt(std::fabs(rr[j + (size_t)b * j].real()));
-      if (j < nroot) worst = std::max(worst, res);
-      if (res > 0.1 * tol) sel.push_back(j);
-    }
+    for (int j = 0; j < nroot; ++j) worst = std::max(worst, std::sqrt(std::fabs(rr[j + (size_t)b * j].real())));
     if (worst < tol) {
       c->civec.ensure(n * nroot);
       DC2_HIP(hipMemcpyAsync(c->civec.get(), X, n * nroot * sizeof(cplx), hipMemcpyDeviceToDevice, s));
@@ -458,21 +462,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       c->nroots_avail = nroot;
       return true;
     }
-    if (sel.empty()) return false;
-    const int ns = (int)sel.size();
-    to_dev(idx, sel.data(), ns * sizeof(int));
-    davidson_precond(s, n, ns, idx, R, th, d, R2);
-    {   // unit columns, so that the Gram matrix after projection measures new directions only
-      zgemm(s, 'C', 'N', ns, ns, n, one, R2, n, 0, R2, n, 0, zero, Sm, ns, 0, 1);
-      std::vector<zc> g((size_t)ns * ns), dg((size_t)ns * ns, 0.0);
-      to_host(g.data(), Sm, g.size() * sizeof(cplx));
-      for (int k = 0; k < ns; ++k) {
-        const double nrm = std::sqrt(std::fabs(g[k + (size_t)ns * k].real()));
-        dg[k + (size_t)ns * k] = nrm > 0.0 ? 1.0 / nrm : 0.0;
-      }
-      to_dev(Sm2, dg.data(), dg.size() * sizeof(cplx));
-      zgemm(s, 'N', 'N', n, ns, ns, one, R2, n, 0, Sm2, ns, 0, zero, T, n, 0, 1);
-    }
+    const int ns = b;
     if (m + ns > maxsub) {   // restart on the b Ritz vectors (orthonormal)
       DC2_HIP(hipMemcpyAsync(V, X, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
       DC2_HIP(hipMemcpyAsync(W, AX, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
@@ -489,16 +479,15 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     Q.resize(G.size());
     gw.resize(ns);
     herm_jacobi(ns, G.data(), gw.data(), Q.data());
-    std::vector<zc> Sc;
     int nk = 0;
     for (int k = ns - 1; k >= 0; --k) {
       if (!(gw[k] > 1e-12) || m + nk >= maxsub) continue;   // |projected unit vector|^2
-      for (int i = 0; i < ns; ++i) Sc.push_back(Q[i + (size_t)ns * k] / std::sqrt(gw[k]));
+      for (int i = 0; i < ns; ++i) up_sc[i + (size_t)ns * nk] = Q[i + (size_t)ns * k] / std::sqrt(gw[k]);
       ++nk;
     }
     if (nk == 0) return false;
-    to_dev(Sm2, Sc.data(), Sc.size() * sizeof(cplx));
-    zgemm(s, 'N', 'N', n, nk, ns, one, T, n, 0, Sm2, ns, 0, zero, V + n * m, n, 0, 1);
+    to_dev(Sm3, up_sc.data(), (size_t)ns * nk * sizeof(cplx));
+    zgemm(s, 'N', 'N', n, nk, ns, one, T, n, 0, Sm3, ns, 0, zero, V + n * m, n, 0, 1);
     zgemm(s, 'N', 'N', n, nk, n, one, H, n, 0, V + n * m, n, 0, zero, W + n * m, n, 0, 1);
     m += nk;
   }
@@ -546,7 +535,7 @@ void run_casci(dc2_context* c, int nroot) {
   c->cimat.ensure(nd * nd);
   casmat(s, c->ni, na, c->ne, nd, c->cas_dets.get(), c->rank_cas.get(), O, c->nmoc(), c->h.get(), c->nmo(),
          c->cimat.get());
-  const bool dav = c->casci_solver == 2 || (c->casci_solver == 0 && nd >= 1500 && nroot <= 16);
+  const bool dav = c->casci_solver == 2 || (c->casci_solver == 0 && nd >= 300 && nroot <= 16);
   c->davidson_iters = 0;
   if (!dav || !davidson(c, nd, c->cimat.get(), std::max(nroot, 1))) {
     c->civec.ensure(nd * nd);

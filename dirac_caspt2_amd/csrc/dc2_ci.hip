@@ -529,6 +529,23 @@ __global__ void k_precond(int64_t n, const int* sel, const cplx* R, const double
   T[i + n * c] = cscale(R[i + n * j], 1.0 / den);
 }
 
+// one workgroup per column: T[:, c] /= |T[:, c]| (deterministic tree reduction)
+__global__ __launch_bounds__(256) void k_col_normalize(int64_t n, cplx* T) {
+  __shared__ double sh[256];
+  const int64_t c = blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) s += cabs2(T[i + n * c]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double nrm = sqrt(sh[0]);
+  const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) T[i + n * c] = cscale(T[i + n * c], inv);
+}
+
 __global__ void k_scale_rows(int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
   if (r >= m) return;
@@ -689,6 +706,12 @@ void davidson_residual(hipStream_t s, int64_t n, int ncol, const cplx* AX, const
 void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const cplx* R, const double* theta,
                       const double* d, cplx* T) {
   hipLaunchKernelGGL(k_precond, dim3(cdiv(n, 256), ncol), dim3(256), 0, s, n, sel, R, theta, d, T);
+  DC2_HIP(hipGetLastError());
+}
+
+void davidson_normalize(hipStream_t s, int64_t n, int ncol, cplx* T) {
+  if (ncol <= 0) return;
+  hipLaunchKernelGGL(k_col_normalize, dim3(ncol), dim3(256), 0, s, n, T);
   DC2_HIP(hipGetLastError());
 }
 

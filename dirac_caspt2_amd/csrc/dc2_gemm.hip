@@ -281,9 +281,9 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
   const int64_t tiles = (int64_t)cdiv(M, BM) * cdiv(N, BN) * batch;
   int sk = 1;
   int64_t kchunk = K;
-  if (batch == 1 && tiles < 128 && K >= 1024) {
-    int64_t want = std::min<int64_t>(K / 512, (256 + tiles - 1) / tiles);
-    want = std::max<int64_t>(2, std::min<int64_t>(want, 32));
+  if (batch == 1 && tiles < 128 && K >= 256) {
+    int64_t want = std::min<int64_t>(K / 128, (256 + tiles - 1) / tiles);
+    want = std::max<int64_t>(2, std::min<int64_t>(want, 64));
     kchunk = ((K + want - 1) / want + BK - 1) / BK * BK;
     sk = (int)((K + kchunk - 1) / kchunk);
   }

@@ -69,6 +69,7 @@ void davidson_residual(hipStream_t s, int64_t n, int ncol, const cplx* AX, const
                        cplx* R);
 void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const cplx* R, const double* theta,
                       const double* d, cplx* T);
+void davidson_normalize(hipStream_t s, int64_t n, int ncol, cplx* T);
 
 // ---------------------------------------------------------------- dc2_ozaki.hip
 // Split-precision (Ozaki, INT8 MFMA) complex GEMM C = A^T-view * op(B) of the transform

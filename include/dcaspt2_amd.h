@@ -152,7 +152,7 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *   "split_slices": 0 (default) FP64 integral transforms, 3..8 split-precision passes
  *                  (dc2_zgemm_split) with this many slices;
  *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32);
- *   "casci_solver": 0 automatic (default: block Davidson for ndet >= 1500 and nroot <= 16,
+ *   "casci_solver": 0 automatic (default: block Davidson for ndet >= 300 and nroot <= 16,
  *                  the dense eigensolver otherwise), 1 dense (all roots, as cdiag), 2 Davidson
  *                  (the nroot lowest roots; falls back to dense when it does not converge). */
 int dc2_set_option(dc2_context* ctx, const char* key, int64_t value);

@@ -27,7 +27,7 @@ def _op(x, op):
 @pytest.mark.parametrize("opA", ["N", "T", "C"])
 @pytest.mark.parametrize("opB", ["N", "R", "T", "C"])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 97), (12, 200, 280),
-                                   (40, 24, 3001), (8, 3, 2500)])   # the last two split K over workgroups
+                                   (40, 24, 3001), (8, 3, 2500), (48, 48, 924)])   # the last three split K
 def test_zgemm_ops(ctx, opA, opB, shape):
     from dirac_caspt2_amd.api import DeviceArray, zgemm
     from gpu_helpers import say
