@@ -315,7 +315,7 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
             EigKind kind = EigKind::general) {
   if (n <= 0) return;
   const int64_t nn = (int64_t)n * n;
-  const int method = c->eig_method >= 0 ? c->eig_method : (kind == EigKind::rank_deficient ? 2 : 1);
+  int method = c->eig_method >= 0 ? c->eig_method : (kind == EigKind::rank_deficient ? 2 : 1);
   cplx* keep = L.z("eig_keep", nn);
   DC2_HIP(hipMemcpyAsync(keep, A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
   int info;
@@ -325,10 +325,14 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
   } else {
     info = run_eig(L, method, n, A, w);
   }
-  if (method != 0 && (info != 0 || !all_finite(L.stream, A, nn))) {
+  // fallbacks: divide and conquer -> D&C with Jacobi leaves -> tridiagonal QR
+  for (int next : {2, 0}) {
+    if (method == 0 || (info == 0 && all_finite(L.stream, A, nn))) break;
+    if (next == 2 && method != 1) continue;   // only plain D&C falls back to D&C with Jacobi leaves
     ++c->eig_fallbacks;
     DC2_HIP(hipMemcpyAsync(A, keep, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
-    info = run_eig(L, 0, n, A, w);
+    info = run_eig(L, next, n, A, w);
+    method = next;
   }
   check_finite(c, L.stream, what, A, nn);
   if (info != 0)
@@ -697,7 +701,10 @@ SolveOut solve_block_gram(dc2_context* c, Lane& L, bool create, int64_t n, int64
   zgemm(s, 'N', 'C', m, m, n, one, Phi, m, 0, Phi, m, 0, zero, G, m, 0, 1);
   set_gemm_tag(0);
   check_finite(c, s, "Phi Phi^H", G, m * m);
-  zheevd(c, L, (int)m, G, lam, "a determinant-space metric");
+  // a RAS-restricted CI vector leaves directions of the (nelec+-1) space unreached: G is then
+  // rank-deficient, where plain divide and conquer can return NaN eigenvectors
+  const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
+  zheevd(c, L, (int)m, G, lam, "a determinant-space metric", ras ? EigKind::rank_deficient : EigKind::general);
   const int64_t dimm = count_retained(L, m, lam, r.margin);
   r.dimm = dimm;
   if (dimm == 0 || npert == 0) return r;
