@@ -155,6 +155,24 @@ class Caspt2Context:
         n = int(np.prod(arr.shape)) if isinstance(arr, np.ndarray) else arr.nbytes // 16
         check("dc2_set_integral_block", self.lib.dc2_set_integral_block(self.h, name.encode(), _ptr(arr), n, where))
 
+    def set_integral_records(self, idx: np.ndarray, val: np.ndarray):
+        """MDCINTNEW-equivalent records: idx (n, 4) 1-based CAS-order labels of (ij|kl), val (n,)
+        complex; every block is filled on the device from their 16 symmetry images."""
+        ijkl = np.ascontiguousarray(idx, dtype=np.int32)
+        v = np.ascontiguousarray(np.asarray(val, dtype=np.complex128))
+        check("dc2_set_integral_records", self.lib.dc2_set_integral_records(self.h, ijkl.shape[0], _ptr(ijkl),
+                                                                            _ptr(v), 0))
+
+    def integral_block(self, name: str) -> np.ndarray:
+        """Copy of a resident original-basis integral block (column-major, its four extents)."""
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        check("dc2_get_integral_block_device", self.lib.dc2_get_integral_block_device(self.h, name.encode(),
+                                                                                      ctypes.byref(p), ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.complex128)
+        check("dc2_memcpy", self.lib.dc2_memcpy(self.h, out.ctypes.data, p.value, 16 * n.value, 2))
+        return out.reshape(block_shape(self.dims, name), order="F")
+
     def set_integral_factors(self, B: np.ndarray):
         """(pq|rs) = sum_P B[P,p,q] B[P,r,s]: every block is formed on the device."""
         Bf = np.asfortranarray(np.asarray(B, dtype=np.complex128))

@@ -7,6 +7,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <exception>
 #include <thread>
@@ -1484,6 +1485,59 @@ int dc2_get_gemm_stats(dc2_context* c, int32_t tag, double* out) {
       mx = std::max(mx, r.flops);
     }
     out[0] = n; out[1] = sec; out[2] = fl; out[3] = mx;
+  });
+}
+
+int dc2_set_integral_records(dc2_context* c, int64_t n, const int32_t* ijkl, const double* values, int32_t where) {
+  return guarded([&] {
+    DC2_CHECK(c && (n == 0 || (ijkl && values)), 1, "null argument");
+    DC2_CHECK(where == 0 || where == 1, 1, "where must be 0 (host) or 1 (device)");
+    need_spaces(c);
+    hipStream_t s = c->stream;
+    std::vector<std::string> names;
+    std::vector<std::array<int, 4>> spaces, dims;
+    std::vector<int64_t> counts;
+    for (const auto& sh : kShapes) {
+      const int64_t cnt = block_count(c, sh.sp);
+      if (cnt <= 0) continue;
+      std::array<int, 4> sp, dm;
+      for (int k = 0; k < 4; ++k) {
+        const char ch = sh.sp[k];
+        sp[k] = ch == 'i' ? 0 : ch == 'a' ? 1 : ch == 's' ? 2 : 3;
+        dm[k] = space_dim(c, ch);
+      }
+      names.push_back(sh.name);
+      spaces.push_back(sp);
+      dims.push_back(dm);
+      counts.push_back(cnt);
+    }
+    std::vector<DevBuf<unsigned long long>> owner(names.size());
+    std::vector<cplx*> data(names.size());
+    std::vector<unsigned long long*> own(names.size());
+    std::vector<const int*> psp(names.size()), pdm(names.size());
+    for (size_t b = 0; b < names.size(); ++b) {
+      Block& blk = c->blocks[names[b]];
+      blk.derived = false;
+      blk.count = counts[b];
+      blk.own.alloc(counts[b]);
+      blk.ptr = blk.own.get();
+      owner[b].alloc(counts[b]);
+      data[b] = blk.own.get();
+      own[b] = owner[b].get();
+      psp[b] = spaces[b].data();
+      pdm[b] = dims[b].data();
+    }
+    DevBuf<int> dijkl(std::max<int64_t>(4 * n, 1));
+    DevBuf<double> dval(std::max<int64_t>(2 * n, 1));
+    const hipMemcpyKind kind = where == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    if (n > 0) {
+      DC2_HIP(hipMemcpyAsync(dijkl.get(), ijkl, 4 * n * sizeof(int), kind, s));
+      DC2_HIP(hipMemcpyAsync(dval.get(), values, 2 * n * sizeof(double), kind, s));
+    }
+    ingest_records(s, c->ni, c->na, c->ns, (int)names.size(), psp.data(), pdm.data(), data.data(), own.data(),
+                   counts.data(), n, dijkl.get(), dval.get());
+    DC2_HIP(hipStreamSynchronize(s));
+    c->have_casci = c->have_pt2 = false;
   });
 }
 

@@ -1,0 +1,135 @@
+// Integral ingest on the GPU: MDCINTNEW-equivalent records -> the dense blocks of the path.
+//
+// A record (i, j, k, l, v) holds one symmetry-unique two-electron integral (ij|kl) with
+// 1-based CAS-order spinor labels, as create_newmdcint.F90:173-176 writes them.  Its 16
+// images follow module_2integrals.F90:395-434: with t(p) the Kramers partner (p+1 for odd
+// p, p-1 for even p), s_ij = +1 when i+j is even and -1 otherwise,
+//   (ij|kl) = v, (t(j)t(i)|kl) = s_ij v, (ij|t(l)t(k)) = s_kl v, (t(j)t(i)|t(l)t(k)) = s_ij s_kl v,
+//   (ji|lk) = conj v and its three partners, (kl|ij) = v and its three partners,
+//   (lk|ji) = conj v and its three partners.
+// Every image is written into each block whose space pattern it matches (OOOO, F1, F2,
+// A1 ... H), which replaces the reference's dense fill plus the subspace files written
+// by divide_2_elec_integral_into_subspaces.F90.  Where two records reach the same element
+// the later record wins (then the later image), as in a sequential fill: a first pass
+// takes the maximum key (record * 16 + image) per element with an atomic max, a second
+// pass lets only the winner write.
+#include "dc2_kernels.hpp"
+
+namespace dc2 {
+namespace {
+
+struct IngestBlock {
+  cplx* data;
+  unsigned long long* owner;
+  int space[4];      // 0 inactive, 1 active, 2 secondary, 3 inactive+active ("o")
+  int n[4];
+};
+
+constexpr int MAX_BLOCKS = 16;
+
+struct IngestParams {
+  int ni, na, ns, nblocks;
+  IngestBlock b[MAX_BLOCKS];
+};
+
+__device__ inline int space_of(int p0, int ni, int na) {   // p0: 0-based spinor
+  return p0 < ni ? 0 : (p0 < ni + na ? 1 : 2);
+}
+
+__device__ inline bool match(int want, int sp) { return want == 3 ? sp <= 1 : want == sp; }
+
+__device__ inline int local_index(int want, int p0, int ni, int na) {
+  return (want == 0 || want == 3) ? p0 : (want == 1 ? p0 - ni : p0 - ni - na);
+}
+
+__device__ inline int partner(int p) { return (p & 1) ? p + 1 : p - 1; }   // 1-based labels
+
+// image g (0..15) of record (i, j, k, l, v): 1-based labels and value
+__device__ inline void image(int g, int i, int j, int k, int l, cplx v, int* q, cplx& x) {
+  const double sij = ((i + j) & 1) ? -1.0 : 1.0, skl = ((k + l) & 1) ? -1.0 : 1.0;
+  const int it = partner(i), jt = partner(j), kt = partner(k), lt = partner(l);
+  const cplx vc = cconj(v);
+  switch (g) {
+    case 0: q[0] = i; q[1] = j; q[2] = k; q[3] = l; x = v; break;
+    case 1: q[0] = jt; q[1] = it; q[2] = k; q[3] = l; x = cscale(v, sij); break;
+    case 2: q[0] = i; q[1] = j; q[2] = lt; q[3] = kt; x = cscale(v, skl); break;
+    case 3: q[0] = jt; q[1] = it; q[2] = lt; q[3] = kt; x = cscale(v, sij * skl); break;
+    case 4: q[0] = j; q[1] = i; q[2] = l; q[3] = k; x = vc; break;
+    case 5: q[0] = it; q[1] = jt; q[2] = l; q[3] = k; x = cscale(vc, sij); break;
+    case 6: q[0] = j; q[1] = i; q[2] = kt; q[3] = lt; x = cscale(vc, skl); break;
+    case 7: q[0] = it; q[1] = jt; q[2] = kt; q[3] = lt; x = cscale(vc, sij * skl); break;
+    case 8: q[0] = k; q[1] = l; q[2] = i; q[3] = j; x = v; break;
+    case 9: q[0] = lt; q[1] = kt; q[2] = i; q[3] = j; x = cscale(v, skl); break;
+    case 10: q[0] = k; q[1] = l; q[2] = jt; q[3] = it; x = cscale(v, sij); break;
+    case 11: q[0] = lt; q[1] = kt; q[2] = jt; q[3] = it; x = cscale(v, sij * skl); break;
+    case 12: q[0] = l; q[1] = k; q[2] = j; q[3] = i; x = vc; break;
+    case 13: q[0] = kt; q[1] = lt; q[2] = j; q[3] = i; x = cscale(vc, skl); break;
+    case 14: q[0] = l; q[1] = k; q[2] = it; q[3] = jt; x = cscale(vc, sij); break;
+    default: q[0] = kt; q[1] = lt; q[2] = it; q[3] = jt; x = cscale(vc, sij * skl); break;
+  }
+}
+
+template <int PASS>
+__global__ void k_ingest(IngestParams P, int64_t nrec, const int* __restrict__ ijkl, const double* __restrict__ val) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrec * 16) return;
+  const int64_t r = t >> 4;
+  const int g = (int)(t & 15);
+  const int nmo = P.ni + P.na + P.ns;
+  const int i = ijkl[4 * r], j = ijkl[4 * r + 1], k = ijkl[4 * r + 2], l = ijkl[4 * r + 3];
+  if (i < 1 || j < 1 || k < 1 || l < 1 || i > nmo || j > nmo || k > nmo || l > nmo) return;
+  int q[4];
+  cplx x;
+  image(g, i, j, k, l, mk(val[2 * r], val[2 * r + 1]), q, x);
+  int sp[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (q[a] < 1 || q[a] > nmo) return;   // a partner label beyond the spinor set
+    q[a] -= 1;
+    sp[a] = space_of(q[a], P.ni, P.na);
+  }
+  const unsigned long long key = (unsigned long long)t;   // record * 16 + image
+  for (int b = 0; b < P.nblocks; ++b) {
+    const IngestBlock& B = P.b[b];
+    if (!(match(B.space[0], sp[0]) && match(B.space[1], sp[1]) && match(B.space[2], sp[2]) &&
+          match(B.space[3], sp[3])))
+      continue;
+    const int64_t e = (int64_t)local_index(B.space[0], q[0], P.ni, P.na) +
+                      (int64_t)B.n[0] * (local_index(B.space[1], q[1], P.ni, P.na) +
+                                         (int64_t)B.n[1] * (local_index(B.space[2], q[2], P.ni, P.na) +
+                                                            (int64_t)B.n[2] * local_index(B.space[3], q[3], P.ni, P.na)));
+    if (PASS == 0) atomicMax(&B.owner[e], key + 1);
+    else if (B.owner[e] == key + 1) B.data[e] = x;
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+
+void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const int* const* spaces, const int* const* dims,
+                    cplx* const* data, unsigned long long* const* owner, const int64_t* counts, int64_t nrec,
+                    const int* ijkl, const double* val) {
+  DC2_CHECK(nblocks >= 1 && nblocks <= MAX_BLOCKS, 1, "ingest: bad block count");
+  DC2_CHECK(nrec >= 0 && cdiv(nrec * 16, 256) < (1ll << 31), 1, "ingest: too many records for one launch");
+  IngestParams P;
+  P.ni = ni; P.na = na; P.ns = ns; P.nblocks = nblocks;
+  for (int b = 0; b < nblocks; ++b) {
+    P.b[b].data = data[b];
+    P.b[b].owner = owner[b];
+    for (int a = 0; a < 4; ++a) { P.b[b].space[a] = spaces[b][a]; P.b[b].n[a] = dims[b][a]; }
+    fill_zero(s, data[b], counts[b]);
+    if (counts[b] > 0) hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(counts[b], 256)), dim3(256), 0, s, owner[b], counts[b]);
+  }
+  if (nrec > 0) {
+    const int64_t nt = nrec * 16;
+    hipLaunchKernelGGL(k_ingest<0>, dim3(cdiv(nt, 256)), dim3(256), 0, s, P, nrec, ijkl, val);
+    hipLaunchKernelGGL(k_ingest<1>, dim3(cdiv(nt, 256)), dim3(256), 0, s, P, nrec, ijkl, val);
+  }
+  DC2_HIP(hipGetLastError());
+}
+
+}  // namespace dc2

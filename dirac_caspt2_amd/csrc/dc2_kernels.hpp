@@ -79,6 +79,14 @@ int64_t zgemm_ozaki_work_bytes(int slices, int64_t M, int64_t N, int64_t K);
 void zgemm_ozaki(hipStream_t s, int slices, char opB, int64_t M, int64_t N, int64_t K, const cplx* A, int64_t lda,
                  const cplx* B, int64_t ldb, cplx* C, int64_t ldc, int8_t* work, int* expo);
 
+// ---------------------------------------------------------------- dc2_ingest.hip
+// Expand MDCINTNEW-equivalent records (1-based CAS-order i, j, k, l and (re, im)) into the
+// dense blocks: spaces[b][4] in {0 inactive, 1 active, 2 secondary, 3 inactive+active},
+// dims[b][4] the block extents, owner[b] a scratch key array of counts[b] elements.
+void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const int* const* spaces, const int* const* dims,
+                    cplx* const* data, unsigned long long* const* owner, const int64_t* counts, int64_t nrec,
+                    const int* ijkl, const double* val);
+
 // ---------------------------------------------------------------- dc2_pt2.hip
 struct Dens {
   int na;

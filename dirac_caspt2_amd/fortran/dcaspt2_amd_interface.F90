@@ -18,7 +18,8 @@ module dcaspt2_amd
     private
     public :: c_ptr, c_null_ptr
     public :: dc2_context_create, dc2_context_destroy, dc2_version, dc2_set_spaces, dc2_set_symmetry, dc2_set_ras, &
-              dc2_set_one_electron, dc2_set_integral_block, dc2_set_integral_factors, dc2_casci, dc2_caspt2, &
+              dc2_set_one_electron, dc2_set_integral_block, dc2_set_integral_records, dc2_set_integral_factors, &
+              dc2_casci, dc2_caspt2, &
               dc2_get_subspace_info, dc2_cdiag, dc2_casmat, dc2_synchronize
     public :: dcaspt2_amd_check, dcaspt2_amd_last_error, dcaspt2_amd_open, dcaspt2_amd_close, dcaspt2_amd_casci, &
               dcaspt2_amd_casmat, dcaspt2_amd_cdiag, dcaspt2_amd_r4dcaspt2_tra
@@ -70,6 +71,15 @@ module dcaspt2_amd
             character(kind=c_char), intent(in) :: name(*)
             complex(c_double_complex), intent(in) :: data(*)
             integer(c_int64_t), value :: count
+            integer(c_int32_t), value :: where
+        end function
+        integer(c_int) function dc2_set_integral_records(ctx, n, ijkl, values, where) &
+            bind(C, name="dc2_set_integral_records")
+            import :: c_int, c_int32_t, c_int64_t, c_double_complex, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: n
+            integer(c_int32_t), intent(in) :: ijkl(*)
+            complex(c_double_complex), intent(in) :: values(*)
             integer(c_int32_t), value :: where
         end function
         integer(c_int) function dc2_set_integral_factors(ctx, b, naux, where) bind(C, name="dc2_set_integral_factors")

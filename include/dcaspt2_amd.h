@@ -186,6 +186,14 @@ int dc2_memcpy(dc2_context* ctx, void* dst, const void* src, int64_t bytes, int3
  * trip).  where: 0 host, 1 device. */
 int dc2_set_integral_factors(dc2_context* ctx, const double* B, int64_t naux, int32_t where);
 
+/* Integral ingest on the GPU: n MDCINTNEW-equivalent records (create_newmdcint.F90:173-176),
+ * ijkl[4n] = 1-based CAS-order spinor labels (i, j, k, l) of (ij|kl), values[2n] = (re, im)
+ * as written there.  Every record's 16 symmetry images (module_2integrals.F90:395-434) are
+ * scattered on the device into every block of dc2_set_integral_block (OOOO, F1, F2, A1 ... H)
+ * whose space pattern they match; where two records reach the same element the later one
+ * wins.  This replaces the dense fill and the subspace files of
+ * divide_2_elec_integral_into_subspaces.F90.  where: 0 host arrays, 1 device arrays. */
+int dc2_set_integral_records(dc2_context* ctx, int64_t n, const int32_t* ijkl, const double* values, int32_t where);
 /* device pointer and element count of a resident integral block, e.g. to lend it to
  * another context with dc2_set_integral_block(..., where = 2) */
 int dc2_get_integral_block_device(dc2_context* ctx, const char* name, const double** ptr, int64_t* count);
