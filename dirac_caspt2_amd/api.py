@@ -436,12 +436,16 @@ def zgemm_split(ctx: Caspt2Context, slices: int, opB: str, M, N, K, A: DeviceArr
                                                      C.ptr, ldc))
 
 
-def host_heev(a: np.ndarray):
-    """Host Hermitian eigensolver of the library (the Davidson subspace solver): (w, V)."""
+HOST_HEEV_METHODS = {"auto": 0, "jacobi": 1, "ql": 2}
+
+
+def host_heev(a: np.ndarray, method: str = "auto"):
+    """Host Hermitian eigensolver of the library (the Davidson subspace solver): (w, V).
+    method: "auto" (QL, Jacobi if QL fails), "jacobi" or "ql"."""
     A = np.asfortranarray(np.asarray(a, dtype=np.complex128)).copy(order="F")
     n = A.shape[0]
     w = np.zeros(n)
-    check("dc2_host_heev", _lib.lib().dc2_host_heev(n, _ptr(A), _ptr(w)))
+    check("dc2_host_heev", _lib.lib().dc2_host_heev(n, _ptr(A), _ptr(w), HOST_HEEV_METHODS[method]))
     return w, A
 
 

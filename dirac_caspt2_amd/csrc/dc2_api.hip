@@ -10,6 +10,7 @@
 #include <array>
 #include <atomic>
 #include <exception>
+#include <future>
 #include <thread>
 #include <chrono>
 #include <cmath>
@@ -381,7 +382,7 @@ void upload_ints(hipStream_t s, DevBuf<int>& dst, const std::vector<int>& v) {
 // the diagonal preconditioner: block of b >= nroot + 3 vectors (Kramers-degenerate pairs stay
 // inside the block), subspace of at most maxsub vectors (restart on the Ritz vectors), converged
 // when every requested residual |H x - theta x| is below 1e-11.  Subspace eigenproblems and the
-// block orthonormalisation run on the host (herm_jacobi).  Writes c->civec (n x nroot) and
+// block orthonormalisation run on the host (herm_eig).  Writes c->civec (n x nroot) and
 // c->ecas; returns false (no result) when it does not converge, and the caller falls back to
 // the dense eigensolver.
 bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
@@ -442,7 +443,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       }
     theta.resize(m);
     Yh.resize((size_t)m * m);
-    herm_jacobi(m, Th.data(), theta.data(), Yh.data());
+    herm_eig(m, Th.data(), theta.data(), Yh.data());
     std::copy(Yh.begin(), Yh.begin() + (size_t)m * b, up_y.begin());     // first b Ritz vectors
     std::memcpy(up_y.data() + (size_t)maxsub * b, theta.data(), b * sizeof(double));
     to_dev(Sm2, up_y.data(), (size_t)m * b * sizeof(cplx));
@@ -483,7 +484,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     to_host(G.data(), Sm, G.size() * sizeof(cplx));
     Q.resize(G.size());
     gw.resize(ns);
-    herm_jacobi(ns, G.data(), gw.data(), Q.data());
+    herm_eig(ns, G.data(), gw.data(), Q.data());
     int nk = 0;
     for (int k = ns - 1; k >= 0; --k) {
       if (!(gw[k] > 1e-12) || m + nk >= maxsub) continue;   // |projected unit vector|^2
@@ -681,21 +682,34 @@ SolveOut solve_block(dc2_context* c, Lane& L, int64_t n, int64_t npert, cplx* S,
 // (U L^-1/2)^H V = L^-1 G_k^H (Phi V).  Used when m < n = nact^3.  When at most
 // e2_schur_max_dropped() directions are dropped, E2 comes from the compressed resolvent
 // of diag(F + alpha) (k_e2_schur) instead of an eigendecomposition of B~.
-SolveOut solve_block_gram(dc2_context* c, Lane& L, bool create, int64_t n, int64_t npert, const cplx* V,
-                          const double* alpha) {
-  SolveOut r;
+struct GramMetric {
+  bool valid = false;
+  int64_t m = 0, dimm = 0;
+  double margin = INFINITY;
+  double seconds = 0.0;   // time spent on the lane before the subspace itself ran
+};
+
+// Phi, G = Phi Phi^H, its eigendecomposition and the retained set: everything of the A/C
+// determinant-space solve that does not depend on V (only on the rotated CI vector and the
+// active orbital energies), so it can run while the integral transforms are still going.
+GramMetric gram_metric(dc2_context* c, Lane& L, char X, int64_t n) {
+  GramMetric g;
   hipStream_t s = L.stream;
+  const bool create = X == 'A';
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
   const int na = c->na;
   const int64_t m = binom_host(na, create ? c->ne + 1 : c->ne - 1);
   const double thr = 1.0e-8;
-  if (n > m) r.margin = thr;   // the n - m eigenvalues of S that vanish identically
-  if (m == 0) return r;
+  g.valid = true;
+  g.m = m;
+  if (n > m) g.margin = thr;   // the n - m eigenvalues of S that vanish identically
+  if (m == 0) return g;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  int64_t* strings = L.l("g_strings", m);
-  cplx* Phi = L.z("g_Phi", m * n);
-  cplx* G = L.z("g_G", m * m);
-  double* f = L.d("g_f", m);
-  double* lam = L.d("g_lam", m);
+  int64_t* strings = L.l(nm("_strings").c_str(), m);
+  cplx* Phi = L.z(nm("_Phi").c_str(), m * n);
+  cplx* G = L.z(nm("_G").c_str(), m * m);
+  double* f = L.d(nm("_f").c_str(), m);
+  double* lam = L.d(nm("_lam").c_str(), m);
   build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), c->Y1.get(), c->eps.get() + c->ni, m, strings, Phi,
             f);
   set_gemm_tag(4);
@@ -706,21 +720,50 @@ SolveOut solve_block_gram(dc2_context* c, Lane& L, bool create, int64_t n, int64
   // rank-deficient, where plain divide and conquer can return NaN eigenvectors
   const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
   zheevd(c, L, (int)m, G, lam, "a determinant-space metric", ras ? EigKind::rank_deficient : EigKind::general);
-  const int64_t dimm = count_retained(L, m, lam, r.margin);
+  g.dimm = count_retained(L, m, lam, g.margin);
+  if (g.dimm > 0) {
+    cplx* GL = L.z(nm("_GL").c_str(), m * g.dimm);
+    scale_cols_tail(s, m, g.dimm, G, lam, GL, true);
+  }
+  return g;
+}
+
+// A and C through their determinant-space factor.  With Phi the (m x n) matrix of the
+// vectors a+_t E_uv|0> (A) or a_t E_uv|0> (C) over the full (nelec+-1)-electron string
+// space, the formula matrices of solve_A/C_subspace.F90 are S = Phi^H Phi and
+// B = Phi^H F Phi, F = diag(sum_w eps_w n_w) (F commutes past the ladder operators).
+// The nonzero spectrum of S is that of G = Phi Phi^H (m x m): for G g = l g, the S
+// eigenvector is u = Phi^H g / sqrt(l), so the retained set (l >= 1e-8) and dimm are those
+// of the reference, U L^-1/2 = Phi^H G_k L^-1, B~ = G_k^H F G_k and
+// (U L^-1/2)^H V = L^-1 G_k^H (Phi V).  Used when m < n = nact^3.  When at most
+// e2_schur_max_dropped() directions are dropped, E2 comes from the compressed resolvent
+// of diag(F + alpha) (k_e2_schur) instead of an eigendecomposition of B~.  g: the metric
+// of gram_metric on the same lane; lane buffers are named per subspace X (A or C), so both
+// metrics can be held on one lane.
+SolveOut solve_block_gram(dc2_context* c, Lane& L, char X, const GramMetric& g, int64_t n, int64_t npert,
+                          const cplx* V, const double* alpha) {
+  SolveOut r;
+  hipStream_t s = L.stream;
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  const int64_t m = g.m, dimm = g.dimm;
+  r.margin = g.margin;
   r.dimm = dimm;
-  if (dimm == 0 || npert == 0) return r;
+  if (m == 0 || dimm == 0 || npert == 0) return r;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const cplx* Phi = L.z(nm("_Phi").c_str(), m * n);
+  cplx* G = L.z(nm("_G").c_str(), m * m);
+  const double* f = L.d(nm("_f").c_str(), m);
+  const cplx* GL = L.z(nm("_GL").c_str(), m * dimm);
   const int64_t nd = m - dimm;
   const cplx* Gk = G + m * nd;
-  cplx* GL = L.z("g_GL", m * dimm);
-  cplx* P = L.z("g_P", m * npert);
-  cplx* Y = L.z("g_Y", dimm * npert);
-  scale_cols_tail(s, m, dimm, G, lam, GL, true);
+  cplx* P = L.z(nm("_P").c_str(), m * npert);
+  cplx* Y = L.z(nm("_Y").c_str(), dimm * npert);
   set_gemm_tag(4);
   zgemm(s, 'N', 'N', m, npert, n, one, Phi, m, 0, V, n, 0, zero, P, m, 0, 1);
   zgemm(s, 'C', 'N', dimm, npert, m, one, GL, m, 0, P, m, 0, zero, Y, dimm, 0, 1);
   set_gemm_tag(0);
   if (c->schur && nd <= e2_schur_max_dropped()) {
-    cplx* Z = L.z("g_Z", m * npert);
+    cplx* Z = L.z(nm("_Z").c_str(), m * npert);
     set_gemm_tag(4);
     zgemm(s, 'N', 'N', m, npert, dimm, one, Gk, m, 0, Y, dimm, 0, zero, Z, m, 0, 1);
     set_gemm_tag(0);
@@ -731,8 +774,8 @@ SolveOut solve_block_gram(dc2_context* c, Lane& L, bool create, int64_t n, int64
     if (std::isfinite(r.e2) && std::isfinite(r.sumc2)) return r;
     // a singular compressed system: fall through to the eigendecomposition of B~
   }
-  cplx* FG = L.z("g_FG", m * dimm);
-  cplx* bc1 = L.z("g_bc1", dimm * dimm);
+  cplx* FG = L.z(nm("_FG").c_str(), m * dimm);
+  cplx* bc1 = L.z(nm("_bc1").c_str(), dimm * dimm);
   scale_rows(s, m, dimm, f, Gk, FG);
   set_gemm_tag(4);
   zgemm(s, 'C', 'N', dimm, dimm, m, one, Gk, m, 0, FG, m, 0, zero, bc1, dimm, 0, 1);
@@ -741,8 +784,31 @@ SolveOut solve_block_gram(dc2_context* c, Lane& L, bool create, int64_t n, int64
   return r;
 }
 
+struct AcRoute {
+  bool gram = false, reduced = false;
+};
+
+// A and C: determinant-space metric when the (nelec+-1)-electron space is the smallest
+// basis; A: the antisymmetric reduction (tuples t < u) when that is smaller still
+AcRoute ac_route_of(const dc2_context* c, char X) {
+  AcRoute r;
+  if (X != 'A' && X != 'C') return r;
+  const int na = c->na;
+  const int64_t n = (int64_t)na * na * na;
+  const int64_t mdet = binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1);
+  const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+  if (c->ac_route == 2) r.gram = true;
+  else if (c->ac_route == 3) r.reduced = X == 'A';
+  else if (c->ac_route == 0) {
+    if (X == 'A' && nred < mdet && nred < n) r.reduced = true;
+    else r.gram = mdet < n;
+  }
+  return r;
+}
+
 // one subspace k (0..7 = A..H) on lane L: S/B/V/alpha assembly and the solve
-void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, double* sumc2) {
+void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, double* sumc2,
+                  const GramMetric* early = nullptr) {
   hipStream_t s = L.stream;
   const int ni = c->ni, na = c->na, ns = c->ns, nmo = c->nmo();
   const int64_t na2 = (int64_t)na * na, na3 = na2 * na;
@@ -776,19 +842,10 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
       case 'F': n = ntp; npert = nsp; break;
       case 'G': n = na; npert = nsp * ni; break;
     }
-    // A and C: determinant-space metric when the (nelec+-1)-electron space is the smallest
-    // basis; A: the antisymmetric reduction (tuples t < u) when that is smaller still
     const int64_t mdet = (X == 'A' || X == 'C') ? binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) : 0;
     const int64_t nred = (int64_t)na * na * (na - 1) / 2;
-    bool gram = false, reduced = false;
-    if (X == 'A' || X == 'C') {
-      if (c->ac_route == 2) gram = true;
-      else if (c->ac_route == 3) reduced = X == 'A';
-      else if (c->ac_route == 0) {
-        if (X == 'A' && nred < mdet && nred < n) reduced = true;
-        else gram = mdet < n;
-      }
-    }
+    const AcRoute route = ac_route_of(c, X);
+    const bool gram = route.gram, reduced = route.reduced;
     const int64_t nb = reduced ? nred : n;   // dimension of the S/B matrices built
     cplx* S = gram ? nullptr : L.z("S", nb * nb);
     cplx* B = gram ? nullptr : L.z("B", nb * nb);
@@ -854,7 +911,8 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
     check_finite(c, s, ("V" + tag).c_str(), V, n * npert);
     check_finite(c, s, ("alpha" + tag).c_str(), alpha, npert);
     if (gram) {
-      r = solve_block_gram(c, L, X == 'A', n, npert, V, alpha);
+      const GramMetric g = (early && early->valid) ? *early : gram_metric(c, L, X, n);
+      r = solve_block_gram(c, L, X, g, n, npert, V, alpha);
     } else if (reduced) {
       // S_full = K^H S_red K with K K^H = 2 I (a+_u a+_t = -a+_t a+_u, a+_t a+_t = 0), so the
       // nonzero eigenvalues of the reference's S are twice those of S_red: cutoff at 0.5e-8
@@ -874,7 +932,7 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
   c->sub_info[4 * k + 2] = npert;
   c->sub_info[4 * k + 3] = 1;
   c->sub_margin[k] = r.margin;
-  c->timings[4 + k] = tx.stop();
+  c->timings[4 + k] = tx.stop() + (early && early->valid ? early->seconds : 0.0);
 }
 
 // worker lane i (1-based), created on first use
@@ -929,7 +987,11 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   check_finite(c, "the Fock matrix", c->fock.get(), (int64_t)nmo * nmo);
   c->U.ensure((int64_t)nmo * nmo);
   fill_zero(s, c->U.get(), (int64_t)nmo * nmo);
-  for (int sp = 0; sp < 3; ++sp) {
+  // blocks of at most kHostFockEig spinors are diagonalised on the host (a rocSOLVER call
+  // costs about a millisecond at any size); the canonical gauge makes the eigenvectors
+  // independent of the solver
+  constexpr int kHostFockEig = 64;
+  auto diag_space = [&](int sp) {
     const int lo = sp == 0 ? 0 : (sp == 1 ? ni : ni + na), hi = sp == 0 ? ni : (sp == 1 ? ni + na : nmo);
     for (int isym = 1; isym <= c->nsym; ++isym) {
       std::vector<int> ind;
@@ -943,13 +1005,33 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       cplx* scr = L0.z("fd_scr", (int64_t)n * n);
       double* w = L0.d("fd_w", n);
       gather_block(s, n, dind, c->fock.get(), nmo, blk);
-      zheevd(c, n, blk, w, "a Fock block");
+      std::vector<zc> hA, hV;   // alive until the synchronisation at the end of the block
+      std::vector<double> hw;
+      if (n <= kHostFockEig) {
+        hA.resize((size_t)n * n);
+        hV.resize(hA.size());
+        hw.resize(n);
+        DC2_HIP(hipMemcpyAsync(hA.data(), blk, hA.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
+        DC2_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < n; ++j) {   // the upper triangle, as the device solver reads it
+          hA[j + (size_t)n * j] = hA[j + (size_t)n * j].real();
+          for (int i = 0; i < j; ++i) hA[j + (size_t)n * i] = std::conj(hA[i + (size_t)n * j]);
+        }
+        herm_eig(n, hA.data(), hw.data(), hV.data());
+        DC2_HIP(hipMemcpyAsync(blk, hV.data(), hV.size() * sizeof(cplx), hipMemcpyHostToDevice, s));
+        DC2_HIP(hipMemcpyAsync(w, hw.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+      } else {
+        zheevd(c, n, blk, w, "a Fock block");
+      }
       canonical_gauge(s, n, w, blk, scr);
       scatter_block(s, n, dind, blk, w, c->U.get(), nmo, c->eps.get());
       DC2_HIP(hipStreamSynchronize(s));
     }
-  }
+  };
+  diag_space(0);
+  diag_space(1);
   // ---- tracic: ds c~ = c (trac.F90:76-156)
+  const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
   {
     cplx* Ua = L0.z("tr_Ua", na2);
     cplx* ds = L0.z("tr_ds", nd * nd);
@@ -958,115 +1040,57 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     extract4(s, c->U.get(), ld, off, nn, Ua);
     tracic_ds(s, na, c->ne, nd, c->cas_dets.get(), Ua, ds);
     c->cnew.ensure(nd);
-    DC2_HIP(hipMemcpyAsync(c->cnew.get(), c0, nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-    int* ipiv = L0.i("tr_ipiv", nd);
-    c->info.ensure(1);
-    DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds),
-                                   (int)nd, ipiv, c->info.get()));
-    int info = 0;
-    DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    DC2_HIP(hipStreamSynchronize(s));
-    if (info != 0) throw Error(100 + info, "zgetrf (tracic) failed, info = " + std::to_string(info));
-    DC2_ROCSOLVER(rocsolver_zgetrs(c->blas, rocblas_operation_none, (int)nd, 1,
-                                   reinterpret_cast<rocblas_double_complex*>(ds), (int)nd, ipiv,
-                                   reinterpret_cast<rocblas_double_complex*>(c->cnew.get()), (int)nd));
+    if (!ras) {
+      // without RAS limits the CASCI space of one total symmetry is closed under the active
+      // rotation (U_act mixes spinors of one irrep only), so ds is the unitary image of U_act
+      // on that space and ds^-1 = ds^H
+      zgemm(s, 'C', 'N', nd, 1, nd, one, ds, nd, 0, c0, nd, 0, zero, c->cnew.get(), nd, 0, 1);
+    } else {
+      DC2_HIP(hipMemcpyAsync(c->cnew.get(), c0, nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+      int* ipiv = L0.i("tr_ipiv", nd);
+      c->info.ensure(1);
+      DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds),
+                                     (int)nd, ipiv, c->info.get()));
+      int info = 0;
+      DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+      DC2_HIP(hipStreamSynchronize(s));
+      if (info != 0) throw Error(100 + info, "zgetrf (tracic) failed, info = " + std::to_string(info));
+      DC2_ROCSOLVER(rocsolver_zgetrs(c->blas, rocblas_operation_none, (int)nd, 1,
+                                     reinterpret_cast<rocblas_double_complex*>(ds), (int)nd, ipiv,
+                                     reinterpret_cast<rocblas_double_complex*>(c->cnew.get()), (int)nd));
+    }
   }
-  c->timings[1] = t1.stop();
-  check_finite(c, "U", c->U.get(), (int64_t)nmo * nmo);
-  check_finite(c, "eps", c->eps.get(), nmo);
   check_finite(c, "the rotated CI vector", c->cnew.get(), nd);
-
-  // ---- densities of the rotated state (density_matrix.F90)
-  Timer t2(s);
-  set_gemm_tag(2);
+  // ---- E_uv|0> of the rotated state: what the A/C determinant-space metrics need
   const double* eps_act = c->eps.get() + ni;
   embed_ci(s, na, nf, nd, c->cas2full.get(), c->cnew.get(), c->full_dets.get(), eps_act, c->c_full.get(),
            c->cF.get());
-  c->Y1F.ensure(nf * na2);
   apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
-  apply_E(s, na, nf, 1, c->exc.get(), c->cF.get(), c->Y1F.get());
-  c->G1F.ensure(na2);
-  c->G2.ensure(na2 * na2);
-  c->G2F.ensure(na2 * na2);
-  zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
-  zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1F.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1F.get(), na2, 0, 1);
-  calce0(s, na, eps_act, c->G1.get(), c->e0.get());
-  zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1.get(), nf, 0, zero, c->G2.get(), na2, 0, 1);
-  zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1F.get(), nf, 0, zero, c->G2F.get(), na2, 0, 1);
+
+  // subspaces: A and C (the large metrics) on their own lanes, the rest on the library
+  // stream.  The lanes start now with the metrics of the A/C determinant-space route (they
+  // need only E_uv|0> and the active orbital energies) and take the rest of their subspaces
+  // once the transforms and densities below are done.
   const bool doA = ni > 0, doB = ni > 0, doC = ns > 0, doD = ni > 0 && ns > 0, doF = ns > 0;
   const bool doE = doD, doG = doD, doH = doD;
   const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
-  if (doA || doC) {
-    const int64_t na4 = na2 * na2;
-    c->G3.ensure(na2 * na4);
-    c->G3F.ensure(na2 * na4);
-    cplx* Y2 = L0.z("Y2", nf * na4);
-    apply_E(s, na, nf, na2, c->exc.get(), c->Y1.get(), Y2);
-    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3.get(), na2, 0, 1);
-    apply_E(s, na, nf, na2, c->exc.get(), c->Y1F.get(), Y2);
-    zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3F.get(), na2, 0, 1);
-  }
-  // h' = U^H h U (tramo1)
-  c->hp.ensure((int64_t)nmo * nmo);
-  {
-    cplx* T = L0.z("hT", (int64_t)nmo * nmo);
-    zgemm(s, 'C', 'N', nmo, nmo, nmo, one, c->U.get(), nmo, 0, c->h.get(), nmo, 0, zero, T, nmo, 0, 1);
-    zgemm(s, 'N', 'N', nmo, nmo, nmo, one, T, nmo, 0, c->U.get(), nmo, 0, zero, c->hp.get(), nmo, 0, 1);
-  }
-  set_gemm_tag(0);
-  c->timings[2] = t2.stop();
-  check_finite(c, "e0", c->e0.get(), 1);
-  check_finite(c, "G1", c->G1.get(), na2);
-  check_finite(c, "G1F", c->G1F.get(), na2);
-  check_finite(c, "G2", c->G2.get(), na2 * na2);
-  check_finite(c, "G2F", c->G2F.get(), na2 * na2);
-  if (doA || doC) {
-    check_finite(c, "G3", c->G3.get(), na2 * na2 * na2);
-    check_finite(c, "G3F", c->G3F.get(), na2 * na2 * na2);
-  }
-  check_finite(c, "h'", c->hp.get(), (int64_t)nmo * nmo);
-
-  // ---- transforms of every needed subspace block (module_intra.F90)
-  Timer t3(s);
-  std::vector<std::string> need;
-  if (doA) { need.push_back("A1"); need.push_back("A2"); }
-  if (doB) need.push_back("B");
-  if (doC) { need.push_back("C1"); if (ni) { need.push_back("C2"); need.push_back("C3"); } }
-  if (doD) { need.push_back("D1"); need.push_back("D2"); need.push_back("D3"); }
-  if (doE) need.push_back("E");
-  if (doF) need.push_back("F");
-  if (doG) need.push_back("G");
-  if (doH) need.push_back("H");
-  set_gemm_tag(1);
-  c->split_flops = c->fp64_flops = 0.0;
-  for (const auto& nm : need) transform(c, nm);
-  set_gemm_tag(0);
-  c->timings[3] = t3.stop();
-  for (const auto& nm : need) check_finite(c, ("transformed " + nm).c_str(), c->tblk[nm].get(), block_count(c, shape_of(nm)));
-
-  upload_ints(s, c->tpairs, pair_list(na));
-  upload_ints(s, c->ipairs, pair_list(ni));
-  upload_ints(s, c->spairs, pair_list(ns));
-  for (int k = 0; k < 8; ++k) { e2[k] = 0.0; sumc2[k] = 0.0; }
-
-  // subspaces: A and C (the large metrics) on their own lanes, the rest on the library
-  // stream; every lane waits for the transforms and densities above
   std::vector<std::vector<int>> plan(std::max(1, std::min(c->nlanes, 3)));
   for (int k = 0; k < 8; ++k) {
     if (!do_sub[k]) continue;
     const int lane = plan.size() == 1 ? 0 : (k == 0 ? 1 : (k == 2 ? (int)plan.size() - 1 : 0));
     plan[lane].push_back(k);
   }
-  hipEvent_t ready = nullptr;
+  hipEvent_t y1_ready = nullptr, ready = nullptr;
+  DC2_HIP(hipEventCreateWithFlags(&y1_ready, hipEventDisableTiming));
   DC2_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-  DC2_HIP(hipEventRecord(ready, s));
+  DC2_HIP(hipEventRecord(y1_ready, s));
   std::vector<std::exception_ptr> errs(plan.size());
   std::vector<Lane*> lane_of(plan.size(), c->lane0.get());
-  for (size_t w = 1; w < plan.size(); ++w) {   // all lanes exist before any thread starts
-    if (plan[w].empty()) continue;
-    lane_of[w] = &worker_lane(c, (int)w);
-    DC2_HIP(hipStreamWaitEvent(lane_of[w]->stream, ready, 0));
-  }
+  for (size_t w = 1; w < plan.size(); ++w)   // all lanes exist before any thread starts
+    if (!plan[w].empty()) lane_of[w] = &worker_lane(c, (int)w);
+  GramMetric early[8];
+  std::promise<bool> go;   // true: the shared stages succeeded and `ready` is recorded
+  std::shared_future<bool> go_f = go.get_future().share();
   std::vector<std::thread> threads;
   for (size_t w = 1; w < plan.size(); ++w) {
     if (plan[w].empty()) continue;
@@ -1074,20 +1098,112 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       try {
         DC2_HIP(hipSetDevice(c->device));
         Lane& Lw = *lane_of[w];
-        for (int k : plan[w]) run_subspace(c, Lw, k, eshift, e2, sumc2);
+        DC2_HIP(hipStreamWaitEvent(Lw.stream, y1_ready, 0));
+        for (int k : plan[w]) {
+          if ((k != 0 && k != 2) || !ac_route_of(c, kSub[k]).gram) continue;
+          Timer tm(Lw.stream);
+          early[k] = gram_metric(c, Lw, kSub[k], (int64_t)na * na2);
+          early[k].seconds = tm.stop();
+        }
+        if (!go_f.get()) return;
+        DC2_HIP(hipStreamWaitEvent(Lw.stream, ready, 0));
+        for (int k : plan[w]) run_subspace(c, Lw, k, eshift, e2, sumc2, &early[k]);
         DC2_HIP(hipStreamSynchronize(Lw.stream));
       } catch (...) {
         errs[w] = std::current_exception();
       }
     });
   }
+  auto finish_lanes = [&]() {
+    for (auto& t : threads) t.join();
+    (void)hipEventDestroy(y1_ready);
+    (void)hipEventDestroy(ready);
+  };
   try {
-    for (int k : plan[0]) run_subspace(c, *c->lane0, k, eshift, e2, sumc2);
+    diag_space(2);
+    c->timings[1] = t1.stop();
+    check_finite(c, "U", c->U.get(), (int64_t)nmo * nmo);
+    check_finite(c, "eps", c->eps.get(), nmo);
+
+    // ---- densities of the rotated state (density_matrix.F90)
+    Timer t2(s);
+    set_gemm_tag(2);
+    c->Y1F.ensure(nf * na2);
+    apply_E(s, na, nf, 1, c->exc.get(), c->cF.get(), c->Y1F.get());
+    c->G1F.ensure(na2);
+    c->G2.ensure(na2 * na2);
+    c->G2F.ensure(na2 * na2);
+    zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
+    zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1F.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1F.get(), na2, 0, 1);
+    calce0(s, na, eps_act, c->G1.get(), c->e0.get());
+    zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1.get(), nf, 0, zero, c->G2.get(), na2, 0, 1);
+    zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1F.get(), nf, 0, zero, c->G2F.get(), na2, 0, 1);
+    if (doA || doC) {
+      const int64_t na4 = na2 * na2;
+      c->G3.ensure(na2 * na4);
+      c->G3F.ensure(na2 * na4);
+      cplx* Y2 = L0.z("Y2", nf * na4);
+      apply_E(s, na, nf, na2, c->exc.get(), c->Y1.get(), Y2);
+      zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3.get(), na2, 0, 1);
+      apply_E(s, na, nf, na2, c->exc.get(), c->Y1F.get(), Y2);
+      zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3F.get(), na2, 0, 1);
+    }
+    // h' = U^H h U (tramo1)
+    c->hp.ensure((int64_t)nmo * nmo);
+    {
+      cplx* T = L0.z("hT", (int64_t)nmo * nmo);
+      zgemm(s, 'C', 'N', nmo, nmo, nmo, one, c->U.get(), nmo, 0, c->h.get(), nmo, 0, zero, T, nmo, 0, 1);
+      zgemm(s, 'N', 'N', nmo, nmo, nmo, one, T, nmo, 0, c->U.get(), nmo, 0, zero, c->hp.get(), nmo, 0, 1);
+    }
+    set_gemm_tag(0);
+    c->timings[2] = t2.stop();
+    check_finite(c, "e0", c->e0.get(), 1);
+    check_finite(c, "G1", c->G1.get(), na2);
+    check_finite(c, "G1F", c->G1F.get(), na2);
+    check_finite(c, "G2", c->G2.get(), na2 * na2);
+    check_finite(c, "G2F", c->G2F.get(), na2 * na2);
+    if (doA || doC) {
+      check_finite(c, "G3", c->G3.get(), na2 * na2 * na2);
+      check_finite(c, "G3F", c->G3F.get(), na2 * na2 * na2);
+    }
+    check_finite(c, "h'", c->hp.get(), (int64_t)nmo * nmo);
+
+    // ---- transforms of every needed subspace block (module_intra.F90)
+    Timer t3(s);
+    std::vector<std::string> need;
+    if (doA) { need.push_back("A1"); need.push_back("A2"); }
+    if (doB) need.push_back("B");
+    if (doC) { need.push_back("C1"); if (ni) { need.push_back("C2"); need.push_back("C3"); } }
+    if (doD) { need.push_back("D1"); need.push_back("D2"); need.push_back("D3"); }
+    if (doE) need.push_back("E");
+    if (doF) need.push_back("F");
+    if (doG) need.push_back("G");
+    if (doH) need.push_back("H");
+    set_gemm_tag(1);
+    c->split_flops = c->fp64_flops = 0.0;
+    for (const auto& nm : need) transform(c, nm);
+    set_gemm_tag(0);
+    c->timings[3] = t3.stop();
+    for (const auto& nm : need) check_finite(c, ("transformed " + nm).c_str(), c->tblk[nm].get(), block_count(c, shape_of(nm)));
+
+    upload_ints(s, c->tpairs, pair_list(na));
+    upload_ints(s, c->ipairs, pair_list(ni));
+    upload_ints(s, c->spairs, pair_list(ns));
+    for (int k = 0; k < 8; ++k) { e2[k] = 0.0; sumc2[k] = 0.0; }
+
+    DC2_HIP(hipEventRecord(ready, s));
+  } catch (...) {
+    go.set_value(false);
+    finish_lanes();
+    throw;
+  }
+  go.set_value(true);
+  try {
+    for (int k : plan[0]) run_subspace(c, *c->lane0, k, eshift, e2, sumc2, &early[k]);
   } catch (...) {
     errs[0] = std::current_exception();
   }
-  for (auto& t : threads) t.join();
-  (void)hipEventDestroy(ready);
+  finish_lanes();
   for (auto& e : errs)
     if (e) std::rethrow_exception(e);
   double e0h = 0.0;
@@ -1624,11 +1740,13 @@ int dc2_get_transform_flops(dc2_context* c, double* out) {
   });
 }
 
-int dc2_host_heev(int32_t n, double* a, double* w) {
+int dc2_host_heev(int32_t n, double* a, double* w, int32_t method) {
   return guarded([&] {
-    DC2_CHECK(a && w && n >= 1 && n <= 4096, 1, "invalid argument");
+    DC2_CHECK(a && w && n >= 1 && n <= 4096 && method >= 0 && method <= 2, 1, "invalid argument");
     std::vector<zc> A(reinterpret_cast<zc*>(a), reinterpret_cast<zc*>(a) + (size_t)n * n), V((size_t)n * n);
-    herm_jacobi(n, A.data(), w, V.data());
+    if (method == 0) herm_eig(n, A.data(), w, V.data());
+    else if (method == 1) herm_jacobi(n, A.data(), w, V.data());
+    else DC2_CHECK(herm_tql(n, A.data(), w, V.data()), 3, "QL iteration did not converge");
     std::copy(V.begin(), V.end(), reinterpret_cast<zc*>(a));
   });
 }

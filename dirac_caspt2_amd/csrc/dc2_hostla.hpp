@@ -17,61 +17,223 @@ typedef std::complex<double> zc;
 // [[a_pp, b], [conj(b), a_qq]], b = |b| e^{i phi}, by U = P R with
 // P = diag(1, e^{-i phi}) and the real rotation R of the symmetric block
 // [[a_pp, |b|], [|b|, a_qq]]; sweeps run until every off-diagonal element is below
-// eps * sqrt(|a_pp a_qq|) or 1e-300 in magnitude.
+// eps * sqrt(|a_pp a_qq|) or 1e-300 in magnitude.  A <- A U is applied to the columns;
+// rows p and q outside the 2x2 block then follow from hermiticity, the block itself is
+// rotated explicitly.  Complex products are written out in real arithmetic (the library
+// complex multiply carries NaN/Inf recovery that costs more than the rotation itself).
 inline void herm_jacobi(int n, zc* A, double* w, zc* V) {
-  auto a = [&](int i, int j) -> zc& { return A[i + (size_t)n * j]; };
-  auto v = [&](int i, int j) -> zc& { return V[i + (size_t)n * j]; };
+  double* a = reinterpret_cast<double*>(A);   // a[2*(i + n*j)] = Re A_ij, [.. + 1] = Im
+  double* v = reinterpret_cast<double*>(V);
+  const size_t ld = 2 * (size_t)n;
   for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) v(i, j) = (i == j) ? 1.0 : 0.0;
-  for (int i = 0; i < n; ++i) a(i, i) = a(i, i).real();
+    for (int i = 0; i < n; ++i) {
+      v[2 * i + ld * j] = (i == j) ? 1.0 : 0.0;
+      v[2 * i + ld * j + 1] = 0.0;
+    }
+  for (int i = 0; i < n; ++i) a[2 * i + ld * i + 1] = 0.0;
   const double eps = 2.2e-16;
+  // columns p and q of M (n rows): (x_p, x_q) <- (c x_p - s e x_q, s x_p + c e x_q), e = er + i ei
+  auto rotate_cols = [n](double* xp, double* xq, double c, double s, double er, double ei) {
+    for (int r = 0; r < n; ++r) {
+      const double pr = xp[2 * r], pi = xp[2 * r + 1], qr = xq[2 * r], qi = xq[2 * r + 1];
+      const double tr = er * qr - ei * qi, ti = er * qi + ei * qr;
+      xp[2 * r] = c * pr - s * tr;
+      xp[2 * r + 1] = c * pi - s * ti;
+      xq[2 * r] = s * pr + c * tr;
+      xq[2 * r + 1] = s * pi + c * ti;
+    }
+  };
   for (int sweep = 0; sweep < 100; ++sweep) {
     bool rotated = false;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
-        const zc b = a(p, q);
-        const double ab = std::abs(b);
-        const double app = a(p, p).real(), aqq = a(q, q).real();
+        double* cp = a + ld * p;
+        double* cq = a + ld * q;
+        const double br = cq[2 * p], bi = cq[2 * p + 1];   // b = A_pq
+        const double ab = std::hypot(br, bi);
+        const double app = cp[2 * p], aqq = cq[2 * q];
         if (ab <= 1e-300 || ab <= eps * std::sqrt(std::fabs(app * aqq)) * 0.5) continue;
         rotated = true;
-        const zc ph = b / ab;                       // e^{i phi}
+        const double phr = br / ab, phi = bi / ab;           // e^{i phi}
         const double theta = (aqq - app) / (2.0 * ab);
         const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
         const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        // columns: A <- A U, U = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
-        const zc eq = std::conj(ph);
+        // A <- A U and V <- V U with U = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
+        rotate_cols(cp, cq, c, s, phr, -phi);
+        rotate_cols(v + ld * p, v + ld * q, c, s, phr, -phi);
+        // rows p, q outside the block: A_pr = conj(A_rp), A_qr = conj(A_rq)
         for (int r = 0; r < n; ++r) {
-          const zc xp = a(r, p), xq = a(r, q);
-          a(r, p) = c * xp - s * eq * xq;
-          a(r, q) = s * xp + c * eq * xq;
-          const zc vp = v(r, p), vq = v(r, q);
-          v(r, p) = c * vp - s * eq * vq;
-          v(r, q) = s * vp + c * eq * vq;
+          if (r == p || r == q) continue;
+          double* cr = a + ld * r;
+          cr[2 * p] = cp[2 * r];
+          cr[2 * p + 1] = -cp[2 * r + 1];
+          cr[2 * q] = cq[2 * r];
+          cr[2 * q + 1] = -cq[2 * r + 1];
         }
-        // rows: A <- U^H A
-        for (int r = 0; r < n; ++r) {
-          const zc xp = a(p, r), xq = a(q, r);
-          a(p, r) = c * xp - s * ph * xq;
-          a(q, r) = s * xp + c * ph * xq;
-        }
-        a(p, q) = 0.0;
-        a(q, p) = 0.0;
-        a(p, p) = a(p, p).real();
-        a(q, q) = a(q, q).real();
+        // the block: the diagonalised values of the real rotation of [[app, |b|], [|b|, aqq]]
+        cp[2 * p] = app - t * ab;
+        cp[2 * p + 1] = 0.0;
+        cq[2 * q] = aqq + t * ab;
+        cq[2 * q + 1] = 0.0;
+        cp[2 * q] = cp[2 * q + 1] = 0.0;
+        cq[2 * p] = cq[2 * p + 1] = 0.0;
       }
     if (!rotated) break;
   }
   std::vector<int> idx(n);
   std::iota(idx.begin(), idx.end(), 0);
   std::vector<double> d(n);
-  for (int i = 0; i < n; ++i) d[i] = a(i, i).real();
+  for (int i = 0; i < n; ++i) d[i] = a[2 * i + ld * i];
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
   std::vector<zc> Vs((size_t)n * n);
   for (int k = 0; k < n; ++k) {
     w[k] = d[idx[k]];
-    for (int i = 0; i < n; ++i) Vs[i + (size_t)n * k] = v(i, idx[k]);
+    for (int i = 0; i < n; ++i) Vs[i + (size_t)n * k] = V[i + (size_t)n * idx[k]];
   }
   std::copy(Vs.begin(), Vs.end(), V);
+}
+
+// Hermitian eigendecomposition by Householder reduction to a real symmetric tridiagonal
+// matrix and the implicit QL iteration with Wilkinson shifts (the tql2 scheme), for the
+// small projected problems of the Davidson solver, where it needs about a fifth of the
+// flops of Jacobi.  Same contract as herm_jacobi: A (column-major, overwritten), w ascending,
+// V orthonormal eigenvectors.  Returns false if an eigenvalue needs more than 90 QL steps.
+//
+// Reduction: for column k the reflector H_k = I - tau v v^H (tau = 2 / v^H v, real) maps
+// x = A[k+1:, k] to alpha e_1 with alpha = -e^{i arg x_0} |x|; the trailing block becomes
+// B - v q^H - q v^H with p = tau B v, q = p - (tau/2)(v^H p) v.  The complex subdiagonal
+// e_k = alpha is made real by D = diag(delta), delta_{k+1} = delta_k e_k / |e_k|, so that
+// A = Q D T D^H Q^H with T real; eigenvectors are Q D Z for the QL rotations Z.
+inline bool herm_tql(int n, zc* A, double* w, zc* V) {
+  auto at = [&](int i, int j) -> zc& { return A[i + (size_t)n * j]; };
+  auto mul = [](zc x, zc y) { return zc(x.real() * y.real() - x.imag() * y.imag(), x.real() * y.imag() + x.imag() * y.real()); };
+  auto cmulc = [](zc x, zc y) {   // conj(x) * y
+    return zc(x.real() * y.real() + x.imag() * y.imag(), x.real() * y.imag() - x.imag() * y.real());
+  };
+  std::vector<double> d(n), e(n, 0.0), tau(n, 0.0);
+  std::vector<zc> sub(n, zc(0.0, 0.0)), p(n), q(n);
+  for (int k = 0; k + 1 < n; ++k) {
+    const int m = n - k - 1;
+    zc* x = &at(k + 1, k);
+    double xn2 = 0.0, tail2 = 0.0;
+    for (int i = 0; i < m; ++i) xn2 += std::norm(x[i]);
+    tail2 = xn2 - std::norm(x[0]);
+    if (m == 1 || tail2 <= 0.0) {   // already tridiagonal in this column
+      sub[k] = x[0];
+      tau[k] = 0.0;
+      continue;
+    }
+    const double xn = std::sqrt(xn2), ax0 = std::abs(x[0]);
+    const zc ph = ax0 > 0.0 ? x[0] / ax0 : zc(1.0, 0.0);
+    const zc alpha = -xn * ph;
+    x[0] -= alpha;                    // v = x - alpha e_1, kept in place of x
+    const double t = 2.0 / (2.0 * xn2 + 2.0 * ax0 * xn);
+    tau[k] = t;
+    sub[k] = alpha;
+    // p = tau B v over the trailing block B = A[k+1:, k+1:] (full Hermitian storage)
+    for (int i = 0; i < m; ++i) p[i] = zc(0.0, 0.0);
+    for (int j = 0; j < m; ++j) {
+      const zc vj = x[j];
+      const zc* col = &at(k + 1, k + 1 + j);
+      for (int i = 0; i < m; ++i) p[i] += mul(col[i], vj);
+    }
+    zc vhp(0.0, 0.0);
+    for (int i = 0; i < m; ++i) {
+      p[i] *= t;
+      vhp += cmulc(x[i], p[i]);
+    }
+    const double K = 0.5 * t * vhp.real();
+    for (int i = 0; i < m; ++i) q[i] = p[i] - K * x[i];
+    for (int j = 0; j < m; ++j) {
+      zc* col = &at(k + 1, k + 1 + j);
+      const zc cqj = std::conj(q[j]), cvj = std::conj(x[j]);
+      for (int i = 0; i < m; ++i) col[i] -= mul(x[i], cqj) + mul(q[i], cvj);
+    }
+  }
+  for (int i = 0; i < n; ++i) d[i] = at(i, i).real();
+  std::vector<zc> delta(n, zc(1.0, 0.0));
+  for (int k = 0; k + 1 < n; ++k) {
+    const double ae = std::abs(sub[k]);
+    e[k] = ae;
+    delta[k + 1] = ae > 0.0 ? mul(delta[k], sub[k] / ae) : delta[k];
+  }
+  // implicit QL on (d, e): e[i] couples i and i+1
+  std::vector<double> z((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) z[i + (size_t)n * i] = 1.0;
+  const double eps = 2.2e-16;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0, m;
+    do {
+      for (m = l; m + 1 < n; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= eps * dd) break;
+      }
+      if (m != l) {
+        if (++iter > 90) return false;
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+        double s = 1.0, c = 1.0, pp = 0.0;
+        int i;
+        bool underflow = false;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i], b = c * e[i];
+          r = std::hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= pp;
+            e[m] = 0.0;
+            underflow = true;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - pp;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          pp = s * r;
+          d[i + 1] = g + pp;
+          g = c * r - b;
+          double* z0 = &z[(size_t)n * i];
+          double* z1 = &z[(size_t)n * (i + 1)];
+          for (int k = 0; k < n; ++k) {
+            const double zf = z1[k];
+            z1[k] = s * z0[k] + c * zf;
+            z0[k] = c * z0[k] - s * zf;
+          }
+        }
+        if (underflow) continue;
+        d[l] -= pp;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  // eigenvectors: X = Q D Z, Q = H_0 H_1 ... applied from the last reflector
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+  for (int c2 = 0; c2 < n; ++c2) {
+    const int src = idx[c2];
+    w[c2] = d[src];
+    zc* col = V + (size_t)n * c2;
+    for (int i = 0; i < n; ++i) col[i] = delta[i] * z[i + (size_t)n * src];
+    for (int k = n - 2; k >= 0; --k) {
+      if (tau[k] == 0.0) continue;
+      const int m = n - k - 1;
+      const zc* v = &at(k + 1, k);
+      zc s(0.0, 0.0);
+      for (int i = 0; i < m; ++i) s += cmulc(v[i], col[k + 1 + i]);
+      s *= tau[k];
+      for (int i = 0; i < m; ++i) col[k + 1 + i] -= mul(v[i], s);
+    }
+  }
+  return true;
+}
+
+// herm_tql, or herm_jacobi on a copy of A if QL does not converge
+inline void herm_eig(int n, zc* A, double* w, zc* V) {
+  std::vector<zc> keep(A, A + (size_t)n * n);
+  if (!herm_tql(n, A, w, V)) herm_jacobi(n, keep.data(), w, V);
 }
 
 }  // namespace dc2

@@ -167,10 +167,12 @@ int dc2_zgemm_split(dc2_context* ctx, int32_t slices, char opB, int64_t M, int64
 int dc2_get_transform_flops(dc2_context* ctx, double* out);
 /* out[3] = {ndet, roots held by the last dc2_casci, Davidson iterations (0: dense solver)} */
 int dc2_get_casci_info(dc2_context* ctx, int64_t* out);
-/* Hermitian eigendecomposition on the host (cyclic Jacobi; the Davidson subspace solver):
- * a (n x n complex, column-major) is overwritten by the eigenvectors, w by the ascending
- * eigenvalues.  No GPU needed. */
-int dc2_host_heev(int32_t n, double* a, double* w);
+/* Hermitian eigendecomposition on the host (the Davidson subspace solver): a (n x n
+ * complex, column-major) is overwritten by the eigenvectors, w by the ascending eigenvalues.
+ * method 0: Householder tridiagonalisation + implicit QL, Jacobi if QL does not converge
+ * (what the Davidson solver uses); 1: cyclic Jacobi; 2: QL only (status 3 if it does not
+ * converge).  No GPU needed. */
+int dc2_host_heev(int32_t n, double* a, double* w, int32_t method);
 /* number of eigensolves that fell back to the tridiagonal QR algorithm */
 int dc2_get_eig_fallbacks(dc2_context* ctx, int64_t* n);
 
