@@ -278,13 +278,13 @@ class Caspt2Context:
         check("dc2_get_transform_flops", self.lib.dc2_get_transform_flops(self.h, _ptr(out)))
         return float(out[0]), float(out[1])
 
-    OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3},
+    OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3, "native": 4},
                "ac_metric": {"auto": 0, "formula": 1, "determinant": 2, "antisymmetric": 3},
                "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {},
                "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}}
 
     def set_option(self, key: str, value):
-        """Options: eigensolver auto|qr|dc|dcj|jacobi, ac_metric auto|formula|determinant,
+        """Options: eigensolver auto|qr|dc|dcj|jacobi|native, ac_metric auto|formula|determinant,
         lanes 1..3 (concurrent subspace streams), schur on|off."""
         v = self.OPTIONS[key][value] if isinstance(value, str) else int(value)
         check("dc2_set_option", self.lib.dc2_set_option(self.h, key.encode(), v))

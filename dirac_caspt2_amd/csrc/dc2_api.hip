@@ -93,7 +93,10 @@ struct Lane {
     info.release();
     if (owned) {
       if (blas) rocblas_destroy_handle(blas);
-      if (stream) (void)hipStreamDestroy(stream);
+      if (stream) {
+        release_stream_workspace(stream);
+        (void)hipStreamDestroy(stream);
+      }
     }
   }
 };
@@ -317,11 +320,32 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
             EigKind kind = EigKind::general) {
   if (n <= 0) return;
   const int64_t nn = (int64_t)n * n;
-  int method = c->eig_method >= 0 ? c->eig_method : (kind == EigKind::rank_deficient ? 2 : 1);
+  const bool native_ok = n >= 3 && n <= 1024;
+  int method = c->eig_method >= 0 ? c->eig_method
+                                  : (kind == EigKind::rank_deficient ? 2 : (native_ok && !c->batcher ? 4 : 1));
+  if (method == 4 && !native_ok) method = 1;
   cplx* keep = L.z("eig_keep", nn);
   DC2_HIP(hipMemcpyAsync(keep, A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
-  int info;
-  if (c->batcher && (method == 1 || method == 2)) {
+  int info = 0;
+  if (method == 4) {
+    // tridiagonalisation of this library + rocSOLVER dstedc; rocSOLVER D&C if it cannot run
+    // or its result is not usable
+    void* work = L.d("eig_native", heev_native_work_bytes(n) / 8 + 1);
+    L.info.ensure(2);
+    bool done = heev_native(L.stream, L.blas, n, A, n, w, work, L.info.get());
+    if (done) {
+      DC2_HIP(hipMemcpyAsync(&info, L.info.get(), sizeof(int), hipMemcpyDeviceToHost, L.stream));
+      DC2_HIP(hipStreamSynchronize(L.stream));
+      done = info == 0 && all_finite(L.stream, A, nn);
+    }
+    if (!done) {
+      ++c->eig_fallbacks;
+      DC2_HIP(hipMemcpyAsync(A, keep, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
+      method = 1;
+    }
+  }
+  if (method == 4) {
+  } else if (c->batcher && (method == 1 || method == 2)) {
     DC2_HIP(hipStreamSynchronize(L.stream));   // the matrix is complete before another stream reads it
     info = c->batcher->solve(n, method, A, w);
   } else {
@@ -329,7 +353,7 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
   }
   // fallbacks: divide and conquer -> D&C with Jacobi leaves -> tridiagonal QR
   for (int next : {2, 0}) {
-    if (method == 0 || (info == 0 && all_finite(L.stream, A, nn))) break;
+    if (method == 0 || method == 4 || (info == 0 && all_finite(L.stream, A, nn))) break;
     if (next == 2 && method != 1) continue;   // only plain D&C falls back to D&C with Jacobi leaves
     ++c->eig_fallbacks;
     DC2_HIP(hipMemcpyAsync(A, keep, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
@@ -1242,7 +1266,7 @@ int dc2_context_create(int32_t device, dc2_context** out) {
     c->debug = dbg && dbg[0] && dbg[0] != '0';
     if (const char* em = std::getenv("DC2_EIGEN")) {
       const std::string m(em);
-      c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : -1;
+      c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : m == "native" ? 4 : -1;
     }
     if (const char* r = std::getenv("DC2_AC_METRIC")) {
       const std::string m(r);
@@ -1270,7 +1294,10 @@ int dc2_context_destroy(dc2_context* ctx) {
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     hipStream_t s = ctx->stream;
     delete ctx;
-    if (s) (void)hipStreamDestroy(s);
+    if (s) {
+      release_stream_workspace(s);
+      (void)hipStreamDestroy(s);
+    }
   });
 }
 
@@ -1673,7 +1700,7 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     DC2_CHECK(c && key, 1, "null argument");
     const std::string k(key);
     if (k == "eigensolver") {
-      DC2_CHECK(value >= -1 && value <= 3, 1, "eigensolver must be -1..3");
+      DC2_CHECK(value >= -1 && value <= 4, 1, "eigensolver must be -1..4");
       c->eig_method = (int)value;
     } else if (k == "ac_metric") {
       DC2_CHECK(value >= 0 && value <= 3, 1, "ac_metric must be 0..3");

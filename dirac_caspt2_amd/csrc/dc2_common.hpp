@@ -98,6 +98,8 @@ struct GemmStats {
 void set_gemm_stats(GemmStats* st);
 // tag of the GEMM launches issued by the calling host thread
 void set_gemm_tag(int tag);
+// free the split-K workspace of stream s (call before destroying the stream)
+void release_stream_workspace(hipStream_t s);
 
 // C[m,n] = alpha * sum_k op(A)[m,k] op(B)[k,n] + beta * C[m,n], batched by strides.
 // opA: 'N' A[m + lda*k], 'T' A[k + lda*m], 'C' conj(A[k + lda*m])

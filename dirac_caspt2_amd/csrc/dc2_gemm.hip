@@ -254,12 +254,15 @@ __global__ void k_splitk_reduce(int64_t M, int64_t N, int sk, const cplx* __rest
   C[m + ldc * n] = out;
 }
 
-// grow-only split-K workspaces, one per stream (kernels on one stream run in order)
+// grow-only split-K workspaces, one per stream (kernels on one stream run in order).  The
+// owner of a stream releases its workspace before destroying the stream; the map itself is
+// never destroyed, so no device memory is freed by static destructors at process exit,
+// when the HIP runtime may already be gone.
 std::mutex g_ws_mu;
-std::map<hipStream_t, DevBuf<cplx>> g_ws;
+std::map<hipStream_t, DevBuf<cplx>>* g_ws = new std::map<hipStream_t, DevBuf<cplx>>;
 cplx* splitk_workspace(hipStream_t s, size_t n) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  DevBuf<cplx>& b = g_ws[s];
+  DevBuf<cplx>& b = (*g_ws)[s];
   b.ensure(n);
   return b.get();
 }
@@ -359,6 +362,11 @@ void set_gemm_stats(GemmStats* st) {
   g_stats = st;
 }
 void set_gemm_tag(int tag) { g_tag = tag; }
+
+void release_stream_workspace(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  g_ws->erase(s);
+}
 
 void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A,
            int64_t lda, int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc,

@@ -1,0 +1,470 @@
+// Hermitian eigensolver for 3 <= n <= 1024 (the matrix its upper triangle defines, as
+// rocSOLVER's fill_upper): Householder tridiagonalisation in one persistent
+// kernel, divide and conquer on the real tridiagonal (rocSOLVER dstedc), back-transformation
+// with one wave per eigenvector.
+//
+// Tridiagonalisation (k_hetrd).  Step k (k = 0 .. n-3) applies the reflector
+// H_k = I - tau_k v_k v_k^H, v_k supported on rows k+1 .. n-1, to the trailing block
+// B = A[k+1:, k+1:]:  p = tau_k B v_k,  K = (tau_k / 2) Re(v_k^H p),  q = p - K v_k,
+// B <- B - v_k q^H - q v_k^H.  The reflector of column k maps x = A[k+1:, k] to
+// alpha_k e_1 with alpha_k = -e^{i arg x_0} |x| (tau_k = 2 / v^H v is real); alpha_k is the
+// complex subdiagonal.  A = Q T' Q^H with T' Hermitian tridiagonal, Q = H_0 H_1 ... H_{n-3};
+// D = diag(delta), delta_0 = 1, delta_{k+1} = delta_k alpha_k / |alpha_k|, makes
+// T = D^H T' D real symmetric with subdiagonal |alpha_k|, so the eigenvectors are
+// X = Q D Z for the eigenvectors Z of T.
+//
+// Layout: workgroup w of nwg owns the columns j = w + nwg * l (l = wave + 8 c, c < 2), held
+// in registers for the whole reduction; lane L of a wave holds rows L + 64 s.  Per step:
+//   (a) every workgroup forms p_j for its columns (a wave reduction per column) and a partial
+//       of Re(v^H p); the owner of column k+1 publishes that column as it stands;
+//   -- one grid barrier --
+//   (b) every workgroup reads p, the partials and column k+1, updates its columns, applies
+//       the same update to its copy of column k+1 and derives reflector k+1 from it.
+// Every workgroup computes reflector k+1 from identical data with identical code, so all
+// hold the same v; workgroup 0 writes d, alpha, tau, delta and the reflectors.  Exchanged
+// data (p, partials, column k+1) are double-buffered by the parity of k and read with
+// device-coherent loads after the barrier.  All workgroups must be resident together (see
+// CoopSlot); each barrier wait has a spin limit after which the kernel raises a flag and
+// every workgroup leaves, and the host then uses rocSOLVER instead.
+#include "dc2_kernels.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <condition_variable>
+#include <mutex>
+
+namespace dc2 {
+namespace {
+
+constexpr int HT_WAVES = 8;
+constexpr int HT_THREADS = 64 * HT_WAVES;
+constexpr int HT_CPW = 2;                       // columns per wave
+constexpr int HT_COLS = HT_WAVES * HT_CPW;      // columns per workgroup
+constexpr int HT_NMAX = 1024;
+
+struct HetrdWork {
+  cplx* P;          // [2][n] p of the current step
+  cplx* NC;         // [2][n] column k+1 as it stands before step k's update
+  double* S;        // [2][nwg] partial sums of Re(v^H p)
+  unsigned* bar;    // barrier counter, zero on entry
+  int* err;         // 1: a barrier wait hit its spin limit
+};
+
+// exchanged data are written and read with device-scope (coherent) accesses, which do not
+// go through the non-coherent part of the per-XCD L2
+__device__ inline double ld_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline cplx ld_coherent(const cplx* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return mk(ld_coherent(q), ld_coherent(q + 1));
+}
+__device__ inline void st_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_coherent(cplx* p, cplx v) {
+  double* q = reinterpret_cast<double*>(p);
+  st_coherent(q, v.x);
+  st_coherent(q + 1, v.y);
+}
+
+// Grid barrier over all workgroups (all resident); false if the wait was abandoned.  The
+// workgroup barrier orders this step's stores of every wave before thread 0's release
+// increment; thread 0 first stores the workgroup's partial (the sum of red[0..7]) to
+// part_dst when that is not null.  The waiting loop polls with relaxed loads and one
+// acquire fence follows.
+__device__ bool grid_sync(unsigned* bar, unsigned target, int* err, double* part_dst, const double* red) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    if (part_dst) {
+      double ps = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < HT_WAVES; ++ww) ps += red[ww];
+      st_coherent(part_dst, ps);
+    }
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    long spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1l << 22)) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+__device__ inline cplx wave_sum(cplx x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    x.x += __shfl_xor(x.x, o, 64);
+    x.y += __shfl_xor(x.y, o, 64);
+  }
+  return x;
+}
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// sum over the workgroup of one double per thread (same order in every workgroup)
+__device__ double block_sum(double x, double* red) {
+  x = wave_sum(x);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < HT_WAVES; ++w) s += red[w];
+  return s;
+}
+
+// Reflector of column k from sc (the column as updated, rows k .. n-1): d_k, alpha_k, tau_k,
+// v_k into sv (zero at rows <= k).  Workgroup 0 writes d, alpha, tau, delta and v_k.
+__device__ void reflector(int n, int k, const cplx* sc, cplx* sv, double* red, double* s_tau, cplx* s_delta,
+                          double* d, cplx* sub, double* tau, cplx* delta, cplx* Vh) {
+  double x2 = 0.0;
+  for (int i = k + 1 + threadIdx.x; i < n; i += HT_THREADS) x2 += cabs2(sc[i]);
+  const double xn2 = block_sum(x2, red);
+  const cplx x0 = sc[k + 1];
+  const double ax0 = sqrt(cabs2(x0));
+  const double tail = xn2 - ax0 * ax0;
+  double t = 0.0;
+  cplx alpha = x0;
+  cplx ph = mk(1.0, 0.0);
+  if (n - k - 1 > 1 && tail > 0.0) {
+    const double xn = sqrt(xn2);
+    if (ax0 > 0.0) ph = mk(x0.x / ax0, x0.y / ax0);
+    alpha = mk(-xn * ph.x, -xn * ph.y);
+    t = 2.0 / (2.0 * xn2 + 2.0 * ax0 * xn);
+  }
+  for (int i = threadIdx.x; i < n; i += HT_THREADS) {
+    cplx v = mk(0.0, 0.0);
+    if (t != 0.0 && i > k) v = (i == k + 1) ? csub(sc[i], alpha) : sc[i];
+    sv[i] = v;
+    if (blockIdx.x == 0) Vh[i + (int64_t)n * k] = v;
+  }
+  if (threadIdx.x == 0) {
+    *s_tau = t;
+    if (blockIdx.x == 0) {
+      d[k] = sc[k].x;
+      sub[k] = alpha;
+      tau[k] = t;
+      const double aa = sqrt(cabs2(alpha));
+      const cplx dk = *s_delta;
+      const cplx dn = aa > 0.0 ? cmul(dk, mk(alpha.x / aa, alpha.y / aa)) : dk;
+      delta[k + 1] = dn;
+      *s_delta = dn;
+    }
+  }
+  __syncthreads();
+}
+
+template <int S>
+__global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restrict__ A, int64_t lda, HetrdWork w,
+                                                      double* __restrict__ d, cplx* __restrict__ sub,
+                                                      double* __restrict__ tau, cplx* __restrict__ delta,
+                                                      cplx* __restrict__ Vh) {
+  __shared__ cplx sv[HT_NMAX], sq[HT_NMAX], sc[HT_NMAX];
+  __shared__ double red[HT_WAVES];
+  __shared__ double s_tau;
+  __shared__ cplx s_delta;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwg = gridDim.x, wg = blockIdx.x;
+  int col[HT_CPW];
+  cplx a[HT_CPW][S];
+#pragma unroll
+  for (int c = 0; c < HT_CPW; ++c) {
+    col[c] = wg + nwg * (wave + HT_WAVES * c);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int i = lane + 64 * s;
+      a[c][s] = (col[c] < n && i < n) ? A[i + lda * col[c]] : mk(0.0, 0.0);
+    }
+  }
+  if (threadIdx.x == 0) {
+    s_delta = mk(1.0, 0.0);
+    if (wg == 0) delta[0] = s_delta;
+  }
+  for (int i = threadIdx.x; i < n; i += HT_THREADS) sc[i] = A[i];   // column 0
+  __syncthreads();
+  reflector(n, 0, sc, sv, red, &s_tau, &s_delta, d, sub, tau, delta, Vh);
+  for (int k = 0; k <= n - 3; ++k) {
+    const int par = k & 1;
+    const double t = s_tau;
+    cplx* P = w.P + (int64_t)par * n;
+    cplx* NC = w.NC + (int64_t)par * n;
+    double* Sp = w.S + (int64_t)par * nwg;
+    // (a) p_j = t sum_i conj(A_ij) v_i for the owned columns j >= k+1
+    double part = 0.0;
+    if (t != 0.0) {
+#pragma unroll
+      for (int c = 0; c < HT_CPW; ++c) {
+        const int j = col[c];
+        if (j >= n || j <= k) continue;   // uniform over the wave
+        cplx acc = mk(0.0, 0.0);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int i = lane + 64 * s;
+          if (i > k && i < n) cfma(acc, cconj(a[c][s]), sv[i]);
+        }
+        acc = wave_sum(acc);
+        const cplx pj = mk(t * acc.x, t * acc.y);
+        if (lane == 0) st_coherent(P + j, pj);
+        const cplx vj = sv[j];
+        part += vj.x * pj.x + vj.y * pj.y;   // Re(conj(v_j) p_j)
+      }
+      if (lane == 0) red[wave] = part;
+    }
+    // the owner publishes column k+1
+#pragma unroll
+    for (int c = 0; c < HT_CPW; ++c)
+      if (col[c] == k + 1) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int i = lane + 64 * s;
+          if (i > k && i < n) st_coherent(NC + i, a[c][s]);
+        }
+      }
+    if (!grid_sync(w.bar, (unsigned)(k + 1) * (unsigned)nwg, w.err, t != 0.0 ? Sp + wg : nullptr, red)) return;
+    // (b) one round of coherent loads: column k+1, p and the partials (nwg <= 64, one per
+    // thread, summed in the same order in every workgroup); q = p - K v is formed where used
+    double kp = 0.0;
+    if (t != 0.0 && threadIdx.x < nwg) kp = ld_coherent(Sp + threadIdx.x);
+    for (int i = k + 1 + threadIdx.x; i < n; i += HT_THREADS) {
+      sc[i] = ld_coherent(NC + i);
+      if (t != 0.0) sq[i] = ld_coherent(P + i);
+    }
+    if (t != 0.0) {
+      const double K = 0.5 * t * block_sum(kp, red);   // its barriers also publish sc and sq
+      auto qat = [&](int i) {
+        const cplx p = sq[i], v = sv[i];
+        return mk(p.x - K * v.x, p.y - K * v.y);
+      };
+#pragma unroll
+      for (int c = 0; c < HT_CPW; ++c) {
+        const int j = col[c];
+        if (j >= n || j <= k) continue;
+        const cplx cqj = cconj(qat(j)), cvj = cconj(sv[j]);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int i = lane + 64 * s;
+          if (i > k && i < n) {
+            cplx x = a[c][s];
+            x = csub(x, cmul(sv[i], cqj));
+            x = csub(x, cmul(qat(i), cvj));
+            a[c][s] = x;
+          }
+        }
+      }
+      const cplx cq1 = cconj(qat(k + 1)), cv1 = cconj(sv[k + 1]);
+      for (int i = k + 1 + threadIdx.x; i < n; i += HT_THREADS) {
+        cplx x = sc[i];
+        x = csub(x, cmul(sv[i], cq1));
+        x = csub(x, cmul(qat(i), cv1));
+        sc[i] = x;
+      }
+    }
+    __syncthreads();
+    reflector(n, k + 1, sc, sv, red, &s_tau, &s_delta, d, sub, tau, delta, Vh);
+  }
+  // the last diagonal element, as updated by the last step
+#pragma unroll
+  for (int c = 0; c < HT_CPW; ++c)
+    if (col[c] == n - 1) {
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (lane + 64 * s == n - 1) d[n - 1] = a[c][s].x;
+    }
+}
+
+// the Hermitian matrix the upper triangle defines (what rocSOLVER's fill_upper reads):
+// A_ij = conj(A_ji) below the diagonal, real diagonal
+__global__ void k_herm_from_upper(int n, cplx* __restrict__ A, int64_t lda) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  const int i = (int)(t % n), j = (int)(t / n);
+  if (i > j) A[i + lda * j] = cconj(A[j + lda * i]);
+  else if (i == j) A[i + lda * i].y = 0.0;
+}
+
+// e_k = |alpha_k| (the real subdiagonal); Z <- identity (dstedc reads it)
+__global__ void k_tri_prep(int n, const cplx* __restrict__ sub, double* __restrict__ e, double* __restrict__ Z) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n - 1) e[t] = sqrt(cabs2(sub[t]));
+  if (t < (int64_t)n * n) Z[t] = (t % (n + 1) == 0) ? 1.0 : 0.0;
+}
+
+// X[:, c] = H_0 H_1 ... H_{n-3} (delta .* Z[:, c]); one wave per column, 8 columns per
+// workgroup; the reflectors are staged through LDS in chunks of BT_CH (coalesced loads)
+constexpr int BT_CH = 8;
+template <int S>
+__global__ __launch_bounds__(512) void k_hetrd_back(int n, const double* __restrict__ Z, const cplx* __restrict__ delta,
+                                                   const double* __restrict__ tau, const cplx* __restrict__ Vh,
+                                                   cplx* __restrict__ X, int64_t ldx) {
+  __shared__ cplx vs[BT_CH][HT_NMAX];
+  __shared__ double ts[BT_CH];
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 8 + (threadIdx.x >> 6);
+  const bool active = c < n;   // every wave takes part in the chunk loads
+  cplx x[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int i = lane + 64 * s;
+    x[s] = (active && i < n) ? cscale(delta[i], Z[i + (int64_t)n * c]) : mk(0.0, 0.0);
+  }
+  for (int k0 = n - 3; k0 >= 0; k0 -= BT_CH) {
+    const int nk = min(BT_CH, k0 + 1);   // reflectors k0, k0-1, ..., k0-nk+1
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nk * n; idx += 512) {
+      const int kk = idx / n, i = idx - kk * n, k = k0 - kk;
+      vs[kk][i] = i > k ? Vh[i + (int64_t)n * k] : mk(0.0, 0.0);
+    }
+    if (threadIdx.x < nk) ts[threadIdx.x] = tau[k0 - threadIdx.x];
+    __syncthreads();
+    if (!active) continue;
+    for (int kk = 0; kk < nk; ++kk) {
+      const double t = ts[kk];
+      if (t == 0.0) continue;
+      cplx dot = mk(0.0, 0.0);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int i = lane + 64 * s;
+        if (i < n) cfma(dot, cconj(vs[kk][i]), x[s]);
+      }
+      dot = wave_sum(dot);
+      const cplx f = mk(t * dot.x, t * dot.y);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int i = lane + 64 * s;
+        if (i < n) x[s] = csub(x[s], cmul(vs[kk][i], f));
+      }
+    }
+  }
+  if (!active) return;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int i = lane + 64 * s;
+    if (i < n) X[i + ldx * c] = x[s];
+  }
+}
+
+// The workgroups of one k_hetrd wait for each other, so all of them must be resident at
+// once.  Each one fits a compute unit on its own (one workgroup of 8 waves at <= 256 VGPRs
+// and 49 KB LDS), and the reductions in flight in this process hold at most one workgroup
+// per compute unit between them; other kernels sharing the device always finish, so every
+// workgroup of every reduction in flight is eventually scheduled.
+std::mutex g_coop_mu;
+std::condition_variable g_coop_cv;
+int g_coop_units = -1;   // workgroups still available (the device's compute units)
+
+struct CoopSlot {
+  int units;
+  CoopSlot(int u, int cus) : units(u) {
+    std::unique_lock<std::mutex> lk(g_coop_mu);
+    if (g_coop_units < 0) g_coop_units = cus;
+    g_coop_cv.wait(lk, [&] { return g_coop_units >= units; });
+    g_coop_units -= units;
+  }
+  ~CoopSlot() {
+    {
+      std::lock_guard<std::mutex> lk(g_coop_mu);
+      g_coop_units += units;
+    }
+    g_coop_cv.notify_all();
+  }
+};
+
+template <int S>
+hipError_t launch_hetrd(hipStream_t s, int nwg, int n, const cplx* A, int64_t lda, HetrdWork w, double* d, cplx* sub,
+                        double* tau, cplx* delta, cplx* Vh) {
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_hetrd<S>),
+                                                              HT_THREADS, 0);
+  if (e != hipSuccess || per_cu < 1) return e != hipSuccess ? e : hipErrorLaunchOutOfResources;
+  hipLaunchKernelGGL((k_hetrd<S>), dim3(nwg), dim3(HT_THREADS), 0, s, n, A, lda, w, d, sub, tau, delta, Vh);
+  return hipGetLastError();
+}
+
+template <int S>
+void launch_back(hipStream_t s, int n, const double* Z, const cplx* delta, const double* tau, const cplx* Vh, cplx* X,
+                 int64_t ldx) {
+  hipLaunchKernelGGL((k_hetrd_back<S>), dim3(cdiv(n, 8)), dim3(512), 0, s, n, Z, delta, tau, Vh, X, ldx);
+}
+
+}  // namespace
+
+int64_t heev_native_work_bytes(int n) {
+  const int64_t nn = (int64_t)n * n;
+  const int nwg = (int)cdiv(n, HT_COLS);
+  return nn * 16 + nn * 8 + (int64_t)n * (16 * 6 + 8 * 3) + 16 * nwg + 16 + 10 * 256;   // 10 pieces, 256-aligned
+}
+
+bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev) {
+  DC2_CHECK(n >= 3 && n <= HT_NMAX, 1, "heev_native: n must be in 3..1024");
+  const int nwg = (int)cdiv(n, HT_COLS);
+  const int64_t nn = (int64_t)n * n;
+  char* p = reinterpret_cast<char*>(work);
+  auto take = [&](int64_t bytes) {
+    char* q = p;
+    p += (bytes + 255) / 256 * 256;
+    return q;
+  };
+  cplx* Vh = reinterpret_cast<cplx*>(take(nn * 16));
+  double* Z = reinterpret_cast<double*>(take(nn * 8));
+  cplx* sub = reinterpret_cast<cplx*>(take(n * 16));
+  cplx* delta = reinterpret_cast<cplx*>(take(n * 16));
+  cplx* P = reinterpret_cast<cplx*>(take(2 * n * 16));
+  cplx* NC = reinterpret_cast<cplx*>(take(2 * n * 16));
+  double* tau = reinterpret_cast<double*>(take(n * 8));
+  double* e = reinterpret_cast<double*>(take(n * 8));
+  double* S = reinterpret_cast<double*>(take(2 * nwg * 8));
+  unsigned* flags = reinterpret_cast<unsigned*>(take(16));
+  HetrdWork hw{P, NC, S, flags, reinterpret_cast<int*>(flags + 1)};
+  DC2_HIP(hipMemsetAsync(flags, 0, 16, s));
+  hipLaunchKernelGGL(k_herm_from_upper, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, A, lda);
+  DC2_HIP(hipGetLastError());
+  int dev = 0, cus = 0;
+  DC2_HIP(hipGetDevice(&dev));
+  DC2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (nwg > cus) return false;
+  hipError_t st;
+  {
+    CoopSlot slot(nwg, cus);
+    if (n <= 128) st = launch_hetrd<2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 256) st = launch_hetrd<4>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 512) st = launch_hetrd<8>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 768) st = launch_hetrd<12>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    if (st != hipSuccess) {
+      (void)hipGetLastError();
+      return false;   // e.g. the workgroup does not fit a compute unit: the caller uses rocSOLVER
+    }
+    int flag[2] = {0, 0};
+    DC2_HIP(hipMemcpyAsync(flag, flags, sizeof(flag), hipMemcpyDeviceToHost, s));
+    DC2_HIP(hipStreamSynchronize(s));
+    if (flag[1] != 0) return false;
+  }
+  hipLaunchKernelGGL(k_tri_prep, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, sub, e, Z);
+  DC2_HIP(hipGetLastError());
+  const rocblas_status rs = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, e, Z, n, info_dev);
+  if (rs != rocblas_status_success) return false;
+  if (n <= 128) launch_back<2>(s, n, Z, delta, tau, Vh, A, lda);
+  else if (n <= 256) launch_back<4>(s, n, Z, delta, tau, Vh, A, lda);
+  else if (n <= 512) launch_back<8>(s, n, Z, delta, tau, Vh, A, lda);
+  else if (n <= 768) launch_back<12>(s, n, Z, delta, tau, Vh, A, lda);
+  else launch_back<16>(s, n, Z, delta, tau, Vh, A, lda);
+  DC2_HIP(hipGetLastError());
+  return true;
+}
+
+}  // namespace dc2

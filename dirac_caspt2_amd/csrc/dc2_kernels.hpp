@@ -3,6 +3,8 @@
 
 #include "dc2_common.hpp"
 
+#include <rocblas/rocblas.h>
+
 namespace dc2 {
 
 // ---------------------------------------------------------------- dc2_ci.hip
@@ -86,6 +88,14 @@ void zgemm_ozaki(hipStream_t s, int slices, char opB, int64_t M, int64_t N, int6
 void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const int* const* spaces, const int* const* dims,
                     cplx* const* data, unsigned long long* const* owner, const int64_t* counts, int64_t nrec,
                     const int* ijkl, const double* val);
+
+// ---------------------------------------------------------------- dc2_heev.hip
+// Hermitian eigendecomposition of A (n x n, 3 <= n <= 1024, the matrix its upper triangle
+// defines): eigenvectors over A, ascending eigenvalues in w; dstedc's info in info_dev.  false:
+// nothing usable was computed (the caller falls back to rocSOLVER).  work: at least
+// heev_native_work_bytes(n) bytes of device memory.
+int64_t heev_native_work_bytes(int n);
+bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev);
 
 // ---------------------------------------------------------------- dc2_pt2.hip
 struct Dens {

@@ -59,6 +59,7 @@ EigBatcher::~EigBatcher() {
     x->E.release();
     x->info.release();
     rocblas_destroy_handle(x->blas);
+    release_stream_workspace(x->stream);
     (void)hipStreamDestroy(x->stream);
   }
 }

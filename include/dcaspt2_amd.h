@@ -137,9 +137,12 @@ int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
 /* the same figures for the launches of one kernel instantiation k_zgemm<opA, opB> */
 int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
 /* Numerical options (no counterpart in the reference):
- *   "eigensolver": -1 automatic (default: divide and conquer, with Jacobi leaves for the
+ *   "eigensolver": -1 automatic (default: 4 for 3 <= n <= 1024 outside the batched server,
+ *                  rocSOLVER divide and conquer otherwise, with Jacobi leaves for the
  *                  rank-deficient nact^3 S matrices), 0 tridiagonal QR (the ZHEEV
- *                  algorithm), 1 divide and conquer, 2 D&C with Jacobi leaves, 3 Jacobi;
+ *                  algorithm), 1 divide and conquer, 2 D&C with Jacobi leaves, 3 Jacobi,
+ *                  4 this library's Householder tridiagonalisation (one persistent kernel)
+ *                  + rocSOLVER dstedc + its back-transformation (n <= 1024; D&C otherwise);
  *   "ac_metric":   0 automatic (default: the smallest exact basis), 1 the RDM-formula S/B
  *                  matrices of size nact^3, 2 the determinant-space factorisation of the A and
  *                  C metrics, 3 for A the formula matrices on the tuples t < u (a+_t a+_u is
