@@ -126,7 +126,7 @@ struct dc2_context {
   int davidson_iters = 0;
   bool have_casci = false;
   // CASPT2 state
-  DevBuf<cplx> fock, U, hp, cnew, c_full, cF, Y1, Y1F, G1, G1F, G2, G2F, G3, G3F;
+  DevBuf<cplx> fock, U, hp, cnew, c_full, cF, Y1, Y1o, Y1F, G1, G1F, G2, G2F, G3, G3F;
   DevBuf<double> eps, e0, out2, partial;
   std::map<std::string, DevBuf<cplx>> tblk;
   DevBuf<int> tpairs, ipairs, spairs;
@@ -143,7 +143,7 @@ struct dc2_context {
   // rank-deficient nact^3 S matrices), else 0 QR (zheev), 1 D&C, 2 D&C + Jacobi, 3 Jacobi
   int eig_method = -1;
   int ac_route = 0;          // A/C metric: 0 automatic, 1 RDM formulas (nact^3), 2 determinant space
-  int nlanes = 3;            // concurrent subspace lanes (1 = sequential on the library stream)
+  int nlanes = 4;            // concurrent subspace lanes (1 = sequential on the library stream)
   bool schur = true;         // E2 of the determinant-space route through the compressed resolvent
   int split_slices = 0;      // > 0: transform passes with K >= split_min_k as Ozaki INT8 GEMMs with this many slices
   int split_min_k = 32;
@@ -404,9 +404,11 @@ void upload_ints(hipStream_t s, DevBuf<int>& dst, const std::vector<int>& v) {
 // ---------------------------------------------------------------------------
 // Lowest nroot eigenpairs of the Hermitian CI matrix H (n x n, device) by block Davidson with
 // the diagonal preconditioner: block of b >= nroot + 3 vectors (Kramers-degenerate pairs stay
-// inside the block), subspace of at most maxsub vectors (restart on the Ritz vectors), converged
-// when every requested residual |H x - theta x| is below 1e-11.  Subspace eigenproblems and the
-// block orthonormalisation run on the host (herm_eig).  Writes c->civec (n x nroot) and
+// inside the block), subspace of at most maxsub vectors (thick restart on the Ritz vectors),
+// converged when every requested residual |H x - theta x| is below 1e-11.  The Rayleigh
+// matrix V^H H V is kept on the host and extended by the new block only (after a restart it
+// is diag(theta)); its eigenproblem and the block orthonormalisation run on the host
+// (herm_eig), with two host round trips per iteration.  Writes c->civec (n x nroot) and
 // c->ecas; returns false (no result) when it does not converge, and the caller falls back to
 // the dense eigensolver.
 bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
@@ -422,9 +424,11 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   cplx* AX = L.z("dv_AX", n * b);
   cplx* R = L.z("dv_R", n * b);
   cplx* T = L.z("dv_T", n * b);
-  cplx* Sm = L.z("dv_small", (int64_t)maxsub * maxsub);
-  cplx* Sm2 = L.z("dv_small2", (int64_t)maxsub * b + b);   // Ritz coefficients, then theta
-  cplx* Sm3 = L.z("dv_small3", (int64_t)b * b);
+  cplx* Cn = L.z("dv_C", (int64_t)maxsub * b);           // V^H W_new
+  cplx* Sm = L.z("dv_small", (int64_t)maxsub * b);       // Gram-Schmidt coefficients
+  cplx* Sy = L.z("dv_y", (int64_t)maxsub * b + b);       // Ritz coefficients, then theta
+  cplx* RG = L.z("dv_rg", 2 * (int64_t)b * b);           // R^H R and T^H T, one download
+  cplx* Sc = L.z("dv_sc", (int64_t)b * b);
   double* d = L.d("dv_d", n);
   int* idx = L.i("dv_idx", b);
   // host sources of the asynchronous uploads stay alive until the next synchronising download
@@ -448,42 +452,64 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   to_dev(idx, up_idx.data(), b * sizeof(int));
   davidson_unit(s, n, b, idx, V);
   zgemm(s, 'N', 'N', n, b, n, one, H, n, 0, V, n, 0, zero, W, n, 0, 1);
-  int m = b;
+  std::vector<int> seq(b);
+  std::iota(seq.begin(), seq.end(), 0);
+  int* sel = L.i("dv_sel", b);
+  to_dev(sel, seq.data(), b * sizeof(int));
+  int m = b, nk = b;   // subspace size; the last nk columns are new
   const double tol = 1e-11;
-  std::vector<zc> Th, Yh, G, Q, rr((size_t)b * b);
+  std::vector<zc> Th((size_t)maxsub * maxsub), Tm, Yh, G, Q, Ch((size_t)maxsub * b), rg(2 * (size_t)b * b);
   std::vector<double> theta, gw;
-  double* th = reinterpret_cast<double*>(Sm2 + (size_t)maxsub * b);
+  double* th = reinterpret_cast<double*>(Sy + (size_t)maxsub * b);
   for (int it = 1; it <= 200; ++it) {
     c->davidson_iters = it;
-    // Rayleigh-Ritz on span(V)
-    zgemm(s, 'C', 'N', m, m, n, one, V, n, 0, W, n, 0, zero, Sm, m, 0, 1);
-    Th.resize((size_t)m * m);
-    to_host(Th.data(), Sm, (size_t)m * m * sizeof(cplx));
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < j; ++i) {
-        const zc x = 0.5 * (Th[i + (size_t)m * j] + std::conj(Th[j + (size_t)m * i]));
-        Th[i + (size_t)m * j] = x;
-        Th[j + (size_t)m * i] = std::conj(x);
+    // Rayleigh matrix: the new columns V^H W[:, m-nk:m]
+    zgemm(s, 'C', 'N', m, nk, n, one, V, n, 0, W + n * (m - nk), n, 0, zero, Cn, m, 0, 1);
+    to_host(Ch.data(), Cn, (size_t)m * nk * sizeof(cplx));
+    for (int j = 0; j < nk; ++j)
+      for (int i = 0; i < m; ++i) {
+        const zc x = Ch[i + (size_t)m * j];
+        const int jj = m - nk + j;
+        if (i < m - nk) {
+          Th[i + (size_t)maxsub * jj] = x;
+          Th[jj + (size_t)maxsub * i] = std::conj(x);
+        } else if (i <= jj) {   // the new diagonal block: its upper triangle, mirrored
+          const zc y = (i == jj) ? zc(x.real(), 0.0) : 0.5 * (x + std::conj(Ch[jj + (size_t)m * (i - (m - nk))]));
+          Th[i + (size_t)maxsub * jj] = y;
+          Th[jj + (size_t)maxsub * i] = std::conj(y);
+        }
       }
+    Tm.resize((size_t)m * m);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) Tm[i + (size_t)m * j] = Th[i + (size_t)maxsub * j];
     theta.resize(m);
     Yh.resize((size_t)m * m);
-    herm_eig(m, Th.data(), theta.data(), Yh.data());
+    herm_eig(m, Tm.data(), theta.data(), Yh.data());
     std::copy(Yh.begin(), Yh.begin() + (size_t)m * b, up_y.begin());     // first b Ritz vectors
     std::memcpy(up_y.data() + (size_t)maxsub * b, theta.data(), b * sizeof(double));
-    to_dev(Sm2, up_y.data(), (size_t)m * b * sizeof(cplx));
+    to_dev(Sy, up_y.data(), (size_t)m * b * sizeof(cplx));
     to_dev(th, up_y.data() + (size_t)maxsub * b, b * sizeof(double));
-    zgemm(s, 'N', 'N', n, b, m, one, V, n, 0, Sm2, m, 0, zero, X, n, 0, 1);
-    zgemm(s, 'N', 'N', n, b, m, one, W, n, 0, Sm2, m, 0, zero, AX, n, 0, 1);
+    zgemm(s, 'N', 'N', n, b, m, one, V, n, 0, Sy, m, 0, zero, X, n, 0, 1);
+    zgemm(s, 'N', 'N', n, b, m, one, W, n, 0, Sy, m, 0, zero, AX, n, 0, 1);
     davidson_residual(s, n, b, AX, X, th, R);
-    // residual norms; every correction is kept (converged ones vanish in the projection)
-    zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, Sm3, b, 0, 1);
-    for (int j = 0; j < b; ++j) up_idx[j] = j;
-    to_dev(idx, up_idx.data(), b * sizeof(int));
-    davidson_precond(s, n, b, idx, R, th, d, T);
+    zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, RG, b, 0, 1);   // residual norms
+    davidson_precond(s, n, b, sel, R, th, d, T);
     davidson_normalize(s, n, b, T);
-    to_host(rr.data(), Sm3, rr.size() * sizeof(cplx));
+    const bool restart = m + b > maxsub;
+    if (restart) {   // thick restart on the b Ritz vectors: V^H H V = diag(theta)
+      DC2_HIP(hipMemcpyAsync(V, X, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+      DC2_HIP(hipMemcpyAsync(W, AX, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+      m = b;
+    }
+    // two classical Gram-Schmidt passes against V, then the Gram matrix of the block
+    for (int pass = 0; pass < 2; ++pass) {
+      zgemm(s, 'C', 'N', m, b, n, one, V, n, 0, T, n, 0, zero, Sm, m, 0, 1);
+      zgemm(s, 'N', 'N', n, b, m, mone, V, n, 0, Sm, m, 0, one, T, n, 0, 1);
+    }
+    zgemm(s, 'C', 'N', b, b, n, one, T, n, 0, T, n, 0, zero, RG + (size_t)b * b, b, 0, 1);
+    to_host(rg.data(), RG, rg.size() * sizeof(cplx));
     double worst = 0.0;
-    for (int j = 0; j < nroot; ++j) worst = std::max(worst, std::sqrt(std::fabs(rr[j + (size_t)b * j].real())));
+    for (int j = 0; j < nroot; ++j) worst = std::max(worst, std::sqrt(std::fabs(rg[j + (size_t)b * j].real())));
     if (worst < tol) {
       c->civec.ensure(n * nroot);
       DC2_HIP(hipMemcpyAsync(c->civec.get(), X, n * nroot * sizeof(cplx), hipMemcpyDeviceToDevice, s));
@@ -492,32 +518,23 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       c->nroots_avail = nroot;
       return true;
     }
-    const int ns = b;
-    if (m + ns > maxsub) {   // restart on the b Ritz vectors (orthonormal)
-      DC2_HIP(hipMemcpyAsync(V, X, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-      DC2_HIP(hipMemcpyAsync(W, AX, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-      m = b;
+    if (restart) {
+      for (int j = 0; j < b; ++j)
+        for (int i = 0; i < b; ++i) Th[i + (size_t)maxsub * j] = (i == j) ? zc(theta[j], 0.0) : zc(0.0, 0.0);
     }
-    // two classical Gram-Schmidt passes against V, then an orthonormal basis of the block
-    for (int pass = 0; pass < 2; ++pass) {
-      zgemm(s, 'C', 'N', m, ns, n, one, V, n, 0, T, n, 0, zero, Sm, m, 0, 1);
-      zgemm(s, 'N', 'N', n, ns, m, mone, V, n, 0, Sm, m, 0, one, T, n, 0, 1);
-    }
-    zgemm(s, 'C', 'N', ns, ns, n, one, T, n, 0, T, n, 0, zero, Sm, ns, 0, 1);
-    G.resize((size_t)ns * ns);
-    to_host(G.data(), Sm, G.size() * sizeof(cplx));
+    G.assign(rg.begin() + (size_t)b * b, rg.end());
     Q.resize(G.size());
-    gw.resize(ns);
-    herm_eig(ns, G.data(), gw.data(), Q.data());
-    int nk = 0;
-    for (int k = ns - 1; k >= 0; --k) {
+    gw.resize(b);
+    herm_eig(b, G.data(), gw.data(), Q.data());
+    nk = 0;
+    for (int k = b - 1; k >= 0; --k) {
       if (!(gw[k] > 1e-12) || m + nk >= maxsub) continue;   // |projected unit vector|^2
-      for (int i = 0; i < ns; ++i) up_sc[i + (size_t)ns * nk] = Q[i + (size_t)ns * k] / std::sqrt(gw[k]);
+      for (int i = 0; i < b; ++i) up_sc[i + (size_t)b * nk] = Q[i + (size_t)b * k] / std::sqrt(gw[k]);
       ++nk;
     }
     if (nk == 0) return false;
-    to_dev(Sm3, up_sc.data(), (size_t)ns * nk * sizeof(cplx));
-    zgemm(s, 'N', 'N', n, nk, ns, one, T, n, 0, Sm3, ns, 0, zero, V + n * m, n, 0, 1);
+    to_dev(Sc, up_sc.data(), (size_t)b * nk * sizeof(cplx));
+    zgemm(s, 'N', 'N', n, nk, b, one, T, n, 0, Sc, b, 0, zero, V + n * m, n, 0, 1);
     zgemm(s, 'N', 'N', n, nk, n, one, H, n, 0, V + n * m, n, 0, zero, W + n * m, n, 0, 1);
     m += nk;
   }
@@ -669,43 +686,70 @@ int64_t count_retained(Lane& L, int64_t n, const double* w_dev, double& margin, 
   return dimm;
 }
 
-// S -> cdiag(1e-8) -> ccutoff -> ulambda_s_half -> B~ -> cdiag -> E2 (solve_A_subspace.F90:150-266)
-SolveOut solve_block(dc2_context* c, Lane& L, int64_t n, int64_t npert, cplx* S, cplx* B, const cplx* V,
-                     const double* alpha, EigKind kind, double scale = 1.0) {
-  SolveOut r;
-  if (n == 0) return r;
+// The V-independent half of a subspace solve (solve_A_subspace.F90:150-200): S -> cdiag(1e-8)
+// -> ccutoff -> ulambda_s_half -> B~ -> cdiag.  Leaves U L^-1/2 (X_uc), the eigenvectors of
+// B~ (X_bc1) and its eigenvalues (X_wb) in lane buffers named for subspace X; S and B are
+// overwritten.  scale: the factor between these S eigenvalues and those of the reference's S
+// matrix (2 for the antisymmetric reduction of A).
+struct BlockPrep {
+  int64_t n = 0, dimm = 0;
+  double margin = INFINITY;
+};
+BlockPrep solve_prep(dc2_context* c, Lane& L, char X, int64_t n, cplx* S, cplx* B, EigKind kind, double scale = 1.0) {
+  BlockPrep p;
+  p.n = n;
+  if (n == 0) return p;
   hipStream_t s = L.stream;
-  double* ws = L.d("sb_ws", n);
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  double* ws = L.d(nm("_ws").c_str(), n);
   zheevd(c, L, (int)n, S, ws, "an S matrix", kind);
-  const int64_t dimm = count_retained(L, n, ws, r.margin, scale);
-  r.dimm = dimm;
-  if (dimm == 0 || npert == 0) return r;
+  const int64_t dimm = count_retained(L, n, ws, p.margin, scale);
+  p.dimm = dimm;
+  if (dimm == 0) return p;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  cplx* uc = L.z("sb_uc", n * dimm);
-  cplx* T1 = L.z("sb_T1", dimm * n);
-  cplx* bc1 = L.z("sb_bc1", dimm * dimm);
-  cplx* Y = L.z("sb_Y", dimm * npert);
+  cplx* uc = L.z(nm("_uc").c_str(), n * dimm);
+  cplx* T1 = L.z(nm("_T1").c_str(), dimm * n);
+  cplx* bc1 = L.z(nm("_bc1").c_str(), dimm * dimm);
+  double* wb = L.d(nm("_wb").c_str(), dimm);
   check_finite(c, s, "S eigenvectors", S, n * n);
   scale_cols_tail(s, n, dimm, S, ws, uc);
   set_gemm_tag(4);
   zgemm(s, 'C', 'N', dimm, n, n, one, uc, n, 0, B, n, 0, zero, T1, dimm, 0, 1);
   zgemm(s, 'N', 'N', dimm, dimm, n, one, T1, dimm, 0, uc, n, 0, zero, bc1, dimm, 0, 1);
-  zgemm(s, 'C', 'N', dimm, npert, n, one, uc, n, 0, V, n, 0, zero, Y, dimm, 0, 1);
   set_gemm_tag(0);
-  finish_block(c, L, dimm, npert, bc1, Y, alpha, r);
+  check_finite(c, s, "B~", bc1, dimm * dimm);
+  zheevd(c, L, (int)dimm, bc1, wb, "a B~ matrix");
+  return p;
+}
+
+// The V half (solve_A_subspace.F90:197-257): Y = (U L^-1/2)^H V, X = W^H Y, E2 and sumc2
+SolveOut solve_finish(dc2_context* c, Lane& L, char X, const BlockPrep& p, int64_t npert, const cplx* V,
+                      const double* alpha) {
+  SolveOut r;
+  r.margin = p.margin;
+  r.dimm = p.dimm;
+  if (p.n == 0 || p.dimm == 0 || npert == 0) return r;
+  hipStream_t s = L.stream;
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  const int64_t n = p.n, dimm = p.dimm;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const cplx* uc = L.z(nm("_uc").c_str(), n * dimm);
+  const cplx* bc1 = L.z(nm("_bc1").c_str(), dimm * dimm);
+  const double* wb = L.d(nm("_wb").c_str(), dimm);
+  cplx* Y = L.z(nm("_Y").c_str(), dimm * npert);
+  cplx* Xm = L.z(nm("_X").c_str(), dimm * npert);
+  set_gemm_tag(4);
+  zgemm(s, 'C', 'N', dimm, npert, n, one, uc, n, 0, V, n, 0, zero, Y, dimm, 0, 1);
+  zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1, dimm, 0, Y, dimm, 0, zero, Xm, dimm, 0, 1);
+  set_gemm_tag(0);
+  const int64_t np = e2_partials_needed(dimm * npert);
+  double* partial = L.d(nm("_partial").c_str(), 2 * np);
+  double* out2 = L.d(nm("_out2").c_str(), 2);
+  e2_reduce(s, dimm, npert, Xm, wb, alpha, partial, np, out2);
+  read_out2(L, out2, r);
   return r;
 }
 
-// A and C through their determinant-space factor.  With Phi the (m x n) matrix of the
-// vectors a+_t E_uv|0> (A) or a_t E_uv|0> (C) over the full (nelec+-1)-electron string
-// space, the formula matrices of solve_A/C_subspace.F90 are S = Phi^H Phi and
-// B = Phi^H F Phi, F = diag(sum_w eps_w n_w) (F commutes past the ladder operators).
-// The nonzero spectrum of S is that of G = Phi Phi^H (m x m): for G g = l g, the S
-// eigenvector is u = Phi^H g / sqrt(l), so the retained set (l >= 1e-8) and dimm are those
-// of the reference, U L^-1/2 = Phi^H G_k L^-1, B~ = G_k^H F G_k and
-// (U L^-1/2)^H V = L^-1 G_k^H (Phi V).  Used when m < n = nact^3.  When at most
-// e2_schur_max_dropped() directions are dropped, E2 comes from the compressed resolvent
-// of diag(F + alpha) (k_e2_schur) instead of an eigendecomposition of B~.
 struct GramMetric {
   bool valid = false;
   int64_t m = 0, dimm = 0;
@@ -713,10 +757,10 @@ struct GramMetric {
   double seconds = 0.0;   // time spent on the lane before the subspace itself ran
 };
 
-// Phi, G = Phi Phi^H, its eigendecomposition and the retained set: everything of the A/C
-// determinant-space solve that does not depend on V (only on the rotated CI vector and the
-// active orbital energies), so it can run while the integral transforms are still going.
-GramMetric gram_metric(dc2_context* c, Lane& L, char X, int64_t n) {
+// Phi from the E_uv|0> vectors Y of the CASCI state in one orbital basis, G = Phi Phi^H,
+// its eigendecomposition (lane buffers X_G, X_lam) and the retained count (w >= 1e-8,
+// cutoff.F90:64-69).  eps_act: the orbital energies of f (zeros when f is not used).
+GramMetric gram_eig(dc2_context* c, Lane& L, char X, int64_t n, const cplx* Y, const double* eps_act) {
   GramMetric g;
   hipStream_t s = L.stream;
   const bool create = X == 'A';
@@ -734,8 +778,7 @@ GramMetric gram_metric(dc2_context* c, Lane& L, char X, int64_t n) {
   cplx* G = L.z(nm("_G").c_str(), m * m);
   double* f = L.d(nm("_f").c_str(), m);
   double* lam = L.d(nm("_lam").c_str(), m);
-  build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), c->Y1.get(), c->eps.get() + c->ni, m, strings, Phi,
-            f);
+  build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), Y, eps_act, m, strings, Phi, f);
   set_gemm_tag(4);
   zgemm(s, 'N', 'C', m, m, n, one, Phi, m, 0, Phi, m, 0, zero, G, m, 0, 1);
   set_gemm_tag(0);
@@ -745,10 +788,56 @@ GramMetric gram_metric(dc2_context* c, Lane& L, char X, int64_t n) {
   const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
   zheevd(c, L, (int)m, G, lam, "a determinant-space metric", ras ? EigKind::rank_deficient : EigKind::general);
   g.dimm = count_retained(L, m, lam, g.margin);
-  if (g.dimm > 0) {
-    cplx* GL = L.z(nm("_GL").c_str(), m * g.dimm);
-    scale_cols_tail(s, m, g.dimm, G, lam, GL, true);
-  }
+  return g;
+}
+
+// Eigenvectors of G computed for the CASCI vector in the original orbitals, carried to the
+// canonical (rotated) orbitals.  G is the same operator in both bases (the sums over t, u, v
+// of a+_t E_uv|0><0|E_vu a_t do not depend on the orbital basis of the active space), so its
+// eigenvalues, hence the retained set, are those of the rotated G, and an eigenvector g in
+// the original string basis is ds^H g in the rotated one, with ds(k, l) = det(U_act[occ_k,
+// occ_l]) over the (nelec+-1)-strings: tracic's ds (trac.F90:105-141) on that space, whose
+// coefficients transform as c = ds c~.
+void gram_rotate(dc2_context* c, Lane& L, char X, const GramMetric& g) {
+  if (g.m == 0) return;
+  hipStream_t s = L.stream;
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  const int na = c->na, nmo = c->nmo();
+  const int64_t m = g.m;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const int64_t* strings = L.l(nm("_strings").c_str(), m);
+  cplx* G = L.z(nm("_G").c_str(), m * m);
+  cplx* Ua = L.z(nm("_Ua").c_str(), (int64_t)na * na);
+  cplx* M = L.z(nm("_M").c_str(), m * m);
+  cplx* T = L.z(nm("_T").c_str(), m * m);
+  int64_t ld[3] = {nmo, 1, 1};
+  int off[4] = {c->ni, c->ni, 0, 0}, nn[4] = {na, na, 1, 1};
+  extract4(s, c->U.get(), ld, off, nn, Ua);
+  tracic_ds(s, na, X == 'A' ? c->ne + 1 : c->ne - 1, m, strings, Ua, M);
+  set_gemm_tag(4);
+  zgemm(s, 'C', 'N', m, m, m, one, M, m, 0, G, m, 0, zero, T, m, 0, 1);
+  set_gemm_tag(0);
+  DC2_HIP(hipMemcpyAsync(G, T, m * m * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+}
+
+// Phi and f of the rotated state (unless the metric was computed from it) and GL = G_k L^-1
+void gram_rotated_parts(dc2_context* c, Lane& L, char X, int64_t n, const GramMetric& g, bool phi_current) {
+  if (g.m == 0) return;
+  hipStream_t s = L.stream;
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  const int64_t m = g.m;
+  if (!phi_current)
+    build_phi(s, c->na, c->ne, X == 'A', c->nfull, c->rank_full.get(), c->Y1.get(), c->eps.get() + c->ni, m,
+              L.l(nm("_strings").c_str(), m), L.z(nm("_Phi").c_str(), m * n), L.d(nm("_f").c_str(), m));
+  if (g.dimm > 0)
+    scale_cols_tail(s, m, g.dimm, L.z(nm("_G").c_str(), m * m), L.d(nm("_lam").c_str(), m),
+                    L.z(nm("_GL").c_str(), m * g.dimm), true);
+}
+
+// the whole metric from the rotated state (when it was not started early)
+GramMetric gram_metric(dc2_context* c, Lane& L, char X, int64_t n) {
+  GramMetric g = gram_eig(c, L, X, n, c->Y1.get(), c->eps.get() + c->ni);
+  gram_rotated_parts(c, L, X, n, g, true);
   return g;
 }
 
@@ -830,64 +919,119 @@ AcRoute ac_route_of(const dc2_context* c, char X) {
   return r;
 }
 
-// one subspace k (0..7 = A..H) on lane L: S/B/V/alpha assembly and the solve
+// sizes of subspace X: the S/B dimension n (the formula matrices) and the perturbers
+void subspace_dims(const dc2_context* c, char X, int64_t& n, int64_t& npert) {
+  const int64_t ni = c->ni, na = c->na, ns = c->ns;
+  const int64_t ntp = na * (na - 1) / 2, nip = ni * (ni - 1) / 2, nsp = ns * (ns - 1) / 2;
+  n = 0;
+  npert = 0;
+  switch (X) {
+    case 'A': n = na * na * na; npert = ni; break;
+    case 'B': n = ntp; npert = nip; break;
+    case 'C': n = na * na * na; npert = ns; break;
+    case 'D': n = na * na; npert = ns * ni; break;
+    case 'E': n = na; npert = nip * ns; break;
+    case 'F': n = ntp; npert = nsp; break;
+    case 'G': n = na; npert = nsp * ni; break;
+    default: npert = nsp * nip; break;   // H
+  }
+}
+
+// the V-independent half of subspace k: the S/B matrices from the densities (or the
+// determinant-space metric) and their eigendecompositions
+struct SubPrep {
+  bool valid = false;
+  BlockPrep bp;     // formula and antisymmetric routes
+  GramMetric g;     // determinant route
+  double seconds = 0.0;
+};
+
+SubPrep subspace_prep(dc2_context* c, Lane& L, int k) {
+  SubPrep p;
+  p.valid = true;
+  const char X = kSub[k];
+  if (X == 'H') return p;
+  hipStream_t s = L.stream;
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  Timer tx(s);
+  const int na = c->na;
+  int64_t n, npert;
+  subspace_dims(c, X, n, npert);
+  const AcRoute route = ac_route_of(c, X);
+  if (route.gram) {
+    p.g = gram_metric(c, L, X, n);
+  } else if (n > 0) {
+    Dens d{na, c->G1.get(), c->G1F.get(), c->G2.get(), c->G2F.get(), c->G3.get(), c->G3F.get()};
+    const int64_t mdet = (X == 'A' || X == 'C') ? binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) : 0;
+    const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+    const int64_t nb = route.reduced ? nred : n;   // dimension of the S/B matrices built
+    cplx* S = L.z(nm("_S").c_str(), nb * nb);
+    cplx* B = L.z(nm("_B").c_str(), nb * nb);
+    const int* pairs = (X == 'B' || X == 'F') ? c->tpairs.get() : nullptr;
+    int* amap = nullptr;
+    if (route.reduced) {   // row k of the reduced matrices is tuple (t, u, v), t < u, of the formulas
+      std::vector<int> hm;
+      hm.reserve(nred);
+      for (int t = 0; t < na; ++t)
+        for (int u = t + 1; u < na; ++u)
+          for (int v = 0; v < na; ++v) hm.push_back((t * na + u) * na + v);
+      amap = L.i(nm("_map").c_str(), nred);
+      DC2_HIP(hipMemcpyAsync(amap, hm.data(), nred * sizeof(int), hipMemcpyHostToDevice, s));
+      DC2_HIP(hipStreamSynchronize(s));
+    }
+    build_SB(s, X, d, c->eps.get() + c->ni, c->e0.get(), nb, pairs, S, B, amap);
+    check_finite(c, s, (std::string("S") + X).c_str(), S, nb * nb);
+    check_finite(c, s, (std::string("B") + X).c_str(), B, nb * nb);
+    if (route.reduced) {
+      // S_full = K^H S_red K with K K^H = 2 I (a+_u a+_t = -a+_t a+_u, a+_t a+_t = 0), so the
+      // nonzero eigenvalues of the reference's S are twice those of S_red: cutoff at 0.5e-8
+      p.bp = solve_prep(c, L, X, nred, S, B, nred > mdet ? EigKind::rank_deficient : EigKind::general, 2.0);
+    } else {
+      const EigKind kind = (X == 'A' || X == 'C') ? EigKind::rank_deficient : EigKind::general;
+      p.bp = solve_prep(c, L, X, n, S, B, kind);
+    }
+  }
+  p.seconds = tx.stop();
+  return p;
+}
+
+// one subspace k (0..7 = A..H) on lane L: V and alpha, then the solve with the prepared
+// S/B half (prepared here if prep is null or not valid)
 void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, double* sumc2,
-                  const GramMetric* early = nullptr) {
+                  const SubPrep* prep = nullptr) {
   hipStream_t s = L.stream;
   const int ni = c->ni, na = c->na, ns = c->ns, nmo = c->nmo();
   const int64_t na2 = (int64_t)na * na, na3 = na2 * na;
-  const int64_t ntp = (int64_t)na * (na - 1) / 2, nip = (int64_t)ni * (ni - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
+  const int64_t nip = (int64_t)ni * (ni - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  const double* eps_act = c->eps.get() + ni;
   Dens d{na, c->G1.get(), c->G1F.get(), c->G2.get(), c->G2F.get(), c->G3.get(), c->G3F.get()};
   auto blk = [&](const char* name) { return c->tblk.at(name).get(); };
   const char X = kSub[k];
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  SubPrep local;
+  if (!prep || !prep->valid) {
+    local = subspace_prep(c, L, k);
+    prep = &local;
+  }
   Timer tx(s);
   SolveOut r;
   int64_t n = 0, npert = 0;
+  subspace_dims(c, X, n, npert);
   if (X == 'H') {
     // eps laid out [secondary..., inactive...] for the kernel
     double* ep = L.d("H_eps", ns + ni);
     DC2_HIP(hipMemcpyAsync(ep, c->eps.get() + ni + na, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
     DC2_HIP(hipMemcpyAsync(ep + ns, c->eps.get(), ni * sizeof(double), hipMemcpyDeviceToDevice, s));
     const int64_t np = e2_partials_needed(nsp * nip);
-    double* partial = L.d("partial", 2 * np);
-    double* out2 = L.d("out2", 2);
+    double* partial = L.d("H_partial", 2 * np);
+    double* out2 = L.d("H_out2", 2);
     e2_H(s, ni, ns, blk("H"), ep, eshift, c->ipairs.get(), c->spairs.get(), partial, np, out2);
     read_out2(L, out2, r);
-    npert = nsp * nip;
+    n = 0;
   } else {
-    switch (X) {
-      case 'A': n = na3; npert = ni; break;
-      case 'B': n = ntp; npert = nip; break;
-      case 'C': n = na3; npert = ns; break;
-      case 'D': n = na2; npert = (int64_t)ns * ni; break;
-      case 'E': n = na; npert = nip * ns; break;
-      case 'F': n = ntp; npert = nsp; break;
-      case 'G': n = na; npert = nsp * ni; break;
-    }
-    const int64_t mdet = (X == 'A' || X == 'C') ? binom_host(na, X == 'A' ? c->ne + 1 : c->ne - 1) : 0;
-    const int64_t nred = (int64_t)na * na * (na - 1) / 2;
     const AcRoute route = ac_route_of(c, X);
-    const bool gram = route.gram, reduced = route.reduced;
-    const int64_t nb = reduced ? nred : n;   // dimension of the S/B matrices built
-    cplx* S = gram ? nullptr : L.z("S", nb * nb);
-    cplx* B = gram ? nullptr : L.z("B", nb * nb);
-    cplx* V = L.z("V", n * npert);
-    double* alpha = L.d("alpha", npert);
-    const int* pairs = (X == 'B' || X == 'F') ? c->tpairs.get() : nullptr;
-    int* amap = nullptr;
-    if (reduced) {   // row k of the reduced matrices is tuple (t, u, v), t < u, of the formulas
-      std::vector<int> hm;
-      hm.reserve(nred);
-      for (int t = 0; t < na; ++t)
-        for (int u = t + 1; u < na; ++u)
-          for (int v = 0; v < na; ++v) hm.push_back((t * na + u) * na + v);
-      amap = L.i("A_map", nred);
-      DC2_HIP(hipMemcpyAsync(amap, hm.data(), nred * sizeof(int), hipMemcpyHostToDevice, s));
-      DC2_HIP(hipStreamSynchronize(s));
-    }
-    if (n > 0 && !gram) build_SB(s, X, d, eps_act, c->e0.get(), nb, pairs, S, B, amap);
+    cplx* V = L.z(nm("_V").c_str(), n * npert);
+    double* alpha = L.d(nm("_alpha").c_str(), npert);
     if (n > 0 && npert > 0) {
       switch (X) {
         case 'A': {
@@ -927,26 +1071,17 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
       }
       alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert, alpha);
     }
-    const std::string tag(1, X);
-    if (!gram) {
-      check_finite(c, s, ("S" + tag).c_str(), S, nb * nb);
-      check_finite(c, s, ("B" + tag).c_str(), B, nb * nb);
-    }
-    check_finite(c, s, ("V" + tag).c_str(), V, n * npert);
-    check_finite(c, s, ("alpha" + tag).c_str(), alpha, npert);
-    if (gram) {
-      const GramMetric g = (early && early->valid) ? *early : gram_metric(c, L, X, n);
-      r = solve_block_gram(c, L, X, g, n, npert, V, alpha);
-    } else if (reduced) {
-      // S_full = K^H S_red K with K K^H = 2 I (a+_u a+_t = -a+_t a+_u, a+_t a+_t = 0), so the
-      // nonzero eigenvalues of the reference's S are twice those of S_red: cutoff at 0.5e-8
+    check_finite(c, s, nm("V").c_str(), V, n * npert);
+    check_finite(c, s, nm("alpha").c_str(), alpha, npert);
+    if (route.gram) {
+      r = solve_block_gram(c, L, X, prep->g, n, npert, V, alpha);
+    } else if (route.reduced) {
+      const int64_t nred = (int64_t)na * na * (na - 1) / 2;
       cplx* Vr = L.z("A_Vred", nred * npert);
-      gather_rows(s, nred, npert, amap, V, n, Vr);
-      r = solve_block(c, L, nred, npert, S, B, Vr, alpha,
-                      nred > mdet ? EigKind::rank_deficient : EigKind::general, 2.0);
+      gather_rows(s, nred, npert, L.i(nm("_map").c_str(), nred), V, n, Vr);
+      r = solve_finish(c, L, X, prep->bp, npert, Vr, alpha);
     } else {
-      const EigKind kind = (X == 'A' || X == 'C') ? EigKind::rank_deficient : EigKind::general;
-      r = solve_block(c, L, n, npert, S, B, V, alpha, kind);
+      r = solve_finish(c, L, X, prep->bp, npert, V, alpha);
     }
   }
   e2[k] = r.e2;
@@ -956,7 +1091,7 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
   c->sub_info[4 * k + 2] = npert;
   c->sub_info[4 * k + 3] = 1;
   c->sub_margin[k] = r.margin;
-  c->timings[4 + k] = tx.stop() + (early && early->valid ? early->seconds : 0.0);
+  c->timings[4 + k] = tx.stop() + prep->seconds;
 }
 
 // worker lane i (1-based), created on first use
@@ -995,22 +1130,96 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   c->c_full.ensure(nf);
   c->cF.ensure(nf);
   c->Y1.ensure(nf * na2);
+  c->Y1o.ensure(nf * na2);
   c->G1.ensure(na2);
   double* zeros_act = L0.d("zeros_act", std::max(na, 1));
   DC2_HIP(hipMemsetAsync(zeros_act, 0, std::max(na, 1) * sizeof(double), s));
   const cplx* c0 = c->civec.get() + (int64_t)(selectroot - 1) * nd;
   embed_ci(s, na, nf, nd, c->cas2full.get(), c0, c->full_dets.get(), zeros_act, c->c_full.get(), c->cF.get());
-  apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
-  zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
+  apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1o.get());
+  zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1o.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
   check_finite(c, "the CASCI 1-RDM", c->G1.get(), na2);
-  // ---- Fock matrix (fockcasci.F90) and its diagonalisation per space and irrep (fockdiag.F90)
-  c->fock.ensure((int64_t)nmo * nmo);
-  const cplx* F1 = ns ? get_block(c, "F1") : nullptr;
-  const cplx* F2 = ns ? get_block(c, "F2") : nullptr;
-  fock_build(s, ni, na, ns, c->h.get(), nmo, get_block(c, "OOOO"), F1, F2, c->G1.get(), c->fock.get());
-  check_finite(c, "the Fock matrix", c->fock.get(), (int64_t)nmo * nmo);
-  c->U.ensure((int64_t)nmo * nmo);
-  fill_zero(s, c->U.get(), (int64_t)nmo * nmo);
+
+  // Subspaces: A and C (the large metrics) on their own lanes, the rest on the library
+  // stream.  The lanes start now with the eigendecompositions of the A/C determinant-space
+  // metrics, which need only E_uv|0> of the CASCI state in the original orbitals (the
+  // metric is the same operator in every orbital basis of the active space), carry the
+  // eigenvectors to the canonical orbitals once the active rotation is known, and take the
+  // rest of their subspaces once the transforms and densities below are done.
+  const bool doA = ni > 0, doB = ni > 0, doC = ns > 0, doD = ni > 0 && ns > 0, doF = ns > 0;
+  const bool doE = doD, doG = doD, doH = doD;
+  const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
+  // lanes: 1 all on the library stream; 2: A and C on lane 1; 3: A on 1, C on 2; 4: D and F
+  // (the next largest) on lane 3 as well
+  std::vector<std::vector<int>> plan(std::max(1, std::min(c->nlanes, 4)));
+  for (int k = 0; k < 8; ++k) {
+    if (!do_sub[k]) continue;
+    int lane = 0;
+    if (plan.size() >= 2 && (k == 0 || k == 2)) lane = k == 0 ? 1 : std::min<int>(2, (int)plan.size() - 1);
+    if (plan.size() >= 4 && (k == 3 || k == 5)) lane = 3;
+    plan[lane].push_back(k);
+  }
+  hipEvent_t y1o_ready = nullptr, rot_ready = nullptr, dens_ready = nullptr, ready = nullptr;
+  DC2_HIP(hipEventCreateWithFlags(&y1o_ready, hipEventDisableTiming));
+  DC2_HIP(hipEventCreateWithFlags(&rot_ready, hipEventDisableTiming));
+  DC2_HIP(hipEventCreateWithFlags(&dens_ready, hipEventDisableTiming));
+  DC2_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  DC2_HIP(hipEventRecord(y1o_ready, s));
+  std::vector<std::exception_ptr> errs(plan.size());
+  std::vector<Lane*> lane_of(plan.size(), c->lane0.get());
+  for (size_t w = 1; w < plan.size(); ++w)   // all lanes exist before any thread starts
+    if (!plan[w].empty()) lane_of[w] = &worker_lane(c, (int)w);
+  SubPrep prep[8];
+  // gates, each true when the stages before it succeeded and its event is recorded:
+  // rotation known (rot_ready), densities (dens_ready), transforms (ready)
+  std::promise<bool> go_rot, go_dens, go;
+  std::shared_future<bool> go_rot_f = go_rot.get_future().share(), go_dens_f = go_dens.get_future().share(),
+                           go_f = go.get_future().share();
+  std::vector<std::thread> threads;
+  for (size_t w = 1; w < plan.size(); ++w) {
+    if (plan[w].empty()) continue;
+    threads.emplace_back([&, w]() {
+      try {
+        DC2_HIP(hipSetDevice(c->device));
+        Lane& Lw = *lane_of[w];
+        DC2_HIP(hipStreamWaitEvent(Lw.stream, y1o_ready, 0));
+        const int64_t n3 = (int64_t)na * na2;
+        for (int k : plan[w]) {
+          if ((k != 0 && k != 2) || !ac_route_of(c, kSub[k]).gram) continue;
+          Timer tm(Lw.stream);
+          prep[k].g = gram_eig(c, Lw, kSub[k], n3, c->Y1o.get(), zeros_act);
+          prep[k].seconds = tm.stop();
+        }
+        if (!go_rot_f.get()) return;
+        DC2_HIP(hipStreamWaitEvent(Lw.stream, rot_ready, 0));
+        for (int k : plan[w]) {
+          if (!prep[k].g.valid) continue;
+          Timer tm(Lw.stream);
+          gram_rotate(c, Lw, kSub[k], prep[k].g);
+          gram_rotated_parts(c, Lw, kSub[k], n3, prep[k].g, false);
+          prep[k].seconds += tm.stop();
+          prep[k].valid = true;
+        }
+        if (!go_dens_f.get()) return;
+        DC2_HIP(hipStreamWaitEvent(Lw.stream, dens_ready, 0));
+        for (int k : plan[w])
+          if (!prep[k].valid) prep[k] = subspace_prep(c, Lw, k);
+        if (!go_f.get()) return;
+        DC2_HIP(hipStreamWaitEvent(Lw.stream, ready, 0));
+        for (int k : plan[w]) run_subspace(c, Lw, k, eshift, e2, sumc2, &prep[k]);
+        DC2_HIP(hipStreamSynchronize(Lw.stream));
+      } catch (...) {
+        errs[w] = std::current_exception();
+      }
+    });
+  }
+  auto finish_lanes = [&]() {
+    for (auto& t : threads) t.join();
+    (void)hipEventDestroy(y1o_ready);
+    (void)hipEventDestroy(rot_ready);
+    (void)hipEventDestroy(dens_ready);
+    (void)hipEventDestroy(ready);
+  };
   // blocks of at most kHostFockEig spinors are diagonalised on the host (a rocSOLVER call
   // costs about a millisecond at any size); the canonical gauge makes the eigenvectors
   // independent of the solver
@@ -1052,97 +1261,63 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       DC2_HIP(hipStreamSynchronize(s));
     }
   };
-  diag_space(0);
-  diag_space(1);
-  // ---- tracic: ds c~ = c (trac.F90:76-156)
-  const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
-  {
-    cplx* Ua = L0.z("tr_Ua", na2);
-    cplx* ds = L0.z("tr_ds", nd * nd);
-    int64_t ld[3] = {nmo, 1, 1};
-    int off[4] = {ni, ni, 0, 0}, nn[4] = {na, na, 1, 1};
-    extract4(s, c->U.get(), ld, off, nn, Ua);
-    tracic_ds(s, na, c->ne, nd, c->cas_dets.get(), Ua, ds);
-    c->cnew.ensure(nd);
-    if (!ras) {
-      // without RAS limits the CASCI space of one total symmetry is closed under the active
-      // rotation (U_act mixes spinors of one irrep only), so ds is the unitary image of U_act
-      // on that space and ds^-1 = ds^H
-      zgemm(s, 'C', 'N', nd, 1, nd, one, ds, nd, 0, c0, nd, 0, zero, c->cnew.get(), nd, 0, 1);
-    } else {
-      DC2_HIP(hipMemcpyAsync(c->cnew.get(), c0, nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-      int* ipiv = L0.i("tr_ipiv", nd);
-      c->info.ensure(1);
-      DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds),
-                                     (int)nd, ipiv, c->info.get()));
-      int info = 0;
-      DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-      DC2_HIP(hipStreamSynchronize(s));
-      if (info != 0) throw Error(100 + info, "zgetrf (tracic) failed, info = " + std::to_string(info));
-      DC2_ROCSOLVER(rocsolver_zgetrs(c->blas, rocblas_operation_none, (int)nd, 1,
-                                     reinterpret_cast<rocblas_double_complex*>(ds), (int)nd, ipiv,
-                                     reinterpret_cast<rocblas_double_complex*>(c->cnew.get()), (int)nd));
-    }
-  }
-  check_finite(c, "the rotated CI vector", c->cnew.get(), nd);
-  // ---- E_uv|0> of the rotated state: what the A/C determinant-space metrics need
   const double* eps_act = c->eps.get() + ni;
-  embed_ci(s, na, nf, nd, c->cas2full.get(), c->cnew.get(), c->full_dets.get(), eps_act, c->c_full.get(),
-           c->cF.get());
-  apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
-
-  // subspaces: A and C (the large metrics) on their own lanes, the rest on the library
-  // stream.  The lanes start now with the metrics of the A/C determinant-space route (they
-  // need only E_uv|0> and the active orbital energies) and take the rest of their subspaces
-  // once the transforms and densities below are done.
-  const bool doA = ni > 0, doB = ni > 0, doC = ns > 0, doD = ni > 0 && ns > 0, doF = ns > 0;
-  const bool doE = doD, doG = doD, doH = doD;
-  const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
-  std::vector<std::vector<int>> plan(std::max(1, std::min(c->nlanes, 3)));
-  for (int k = 0; k < 8; ++k) {
-    if (!do_sub[k]) continue;
-    const int lane = plan.size() == 1 ? 0 : (k == 0 ? 1 : (k == 2 ? (int)plan.size() - 1 : 0));
-    plan[lane].push_back(k);
-  }
-  hipEvent_t y1_ready = nullptr, ready = nullptr;
-  DC2_HIP(hipEventCreateWithFlags(&y1_ready, hipEventDisableTiming));
-  DC2_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-  DC2_HIP(hipEventRecord(y1_ready, s));
-  std::vector<std::exception_ptr> errs(plan.size());
-  std::vector<Lane*> lane_of(plan.size(), c->lane0.get());
-  for (size_t w = 1; w < plan.size(); ++w)   // all lanes exist before any thread starts
-    if (!plan[w].empty()) lane_of[w] = &worker_lane(c, (int)w);
-  GramMetric early[8];
-  std::promise<bool> go;   // true: the shared stages succeeded and `ready` is recorded
-  std::shared_future<bool> go_f = go.get_future().share();
-  std::vector<std::thread> threads;
-  for (size_t w = 1; w < plan.size(); ++w) {
-    if (plan[w].empty()) continue;
-    threads.emplace_back([&, w]() {
-      try {
-        DC2_HIP(hipSetDevice(c->device));
-        Lane& Lw = *lane_of[w];
-        DC2_HIP(hipStreamWaitEvent(Lw.stream, y1_ready, 0));
-        for (int k : plan[w]) {
-          if ((k != 0 && k != 2) || !ac_route_of(c, kSub[k]).gram) continue;
-          Timer tm(Lw.stream);
-          early[k] = gram_metric(c, Lw, kSub[k], (int64_t)na * na2);
-          early[k].seconds = tm.stop();
-        }
-        if (!go_f.get()) return;
-        DC2_HIP(hipStreamWaitEvent(Lw.stream, ready, 0));
-        for (int k : plan[w]) run_subspace(c, Lw, k, eshift, e2, sumc2, &early[k]);
-        DC2_HIP(hipStreamSynchronize(Lw.stream));
-      } catch (...) {
-        errs[w] = std::current_exception();
+  try {
+    // ---- Fock matrix (fockcasci.F90) and its diagonalisation per space and irrep (fockdiag.F90)
+    c->fock.ensure((int64_t)nmo * nmo);
+    const cplx* F1 = ns ? get_block(c, "F1") : nullptr;
+    const cplx* F2 = ns ? get_block(c, "F2") : nullptr;
+    fock_build(s, ni, na, ns, c->h.get(), nmo, get_block(c, "OOOO"), F1, F2, c->G1.get(), c->fock.get());
+    check_finite(c, "the Fock matrix", c->fock.get(), (int64_t)nmo * nmo);
+    c->U.ensure((int64_t)nmo * nmo);
+    fill_zero(s, c->U.get(), (int64_t)nmo * nmo);
+    diag_space(0);
+    diag_space(1);
+    // ---- tracic: ds c~ = c (trac.F90:76-156)
+    const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
+    {
+      cplx* Ua = L0.z("tr_Ua", na2);
+      cplx* ds = L0.z("tr_ds", nd * nd);
+      int64_t ld[3] = {nmo, 1, 1};
+      int off[4] = {ni, ni, 0, 0}, nn[4] = {na, na, 1, 1};
+      extract4(s, c->U.get(), ld, off, nn, Ua);
+      tracic_ds(s, na, c->ne, nd, c->cas_dets.get(), Ua, ds);
+      c->cnew.ensure(nd);
+      if (!ras) {
+        // without RAS limits the CASCI space of one total symmetry is closed under the active
+        // rotation (U_act mixes spinors of one irrep only), so ds is the unitary image of U_act
+        // on that space and ds^-1 = ds^H
+        zgemm(s, 'C', 'N', nd, 1, nd, one, ds, nd, 0, c0, nd, 0, zero, c->cnew.get(), nd, 0, 1);
+      } else {
+        DC2_HIP(hipMemcpyAsync(c->cnew.get(), c0, nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+        int* ipiv = L0.i("tr_ipiv", nd);
+        c->info.ensure(1);
+        DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds),
+                                       (int)nd, ipiv, c->info.get()));
+        int info = 0;
+        DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        DC2_HIP(hipStreamSynchronize(s));
+        if (info != 0) throw Error(100 + info, "zgetrf (tracic) failed, info = " + std::to_string(info));
+        DC2_ROCSOLVER(rocsolver_zgetrs(c->blas, rocblas_operation_none, (int)nd, 1,
+                                       reinterpret_cast<rocblas_double_complex*>(ds), (int)nd, ipiv,
+                                       reinterpret_cast<rocblas_double_complex*>(c->cnew.get()), (int)nd));
       }
-    });
+    }
+    check_finite(c, "the rotated CI vector", c->cnew.get(), nd);
+    // ---- E_uv|0> of the rotated state
+    embed_ci(s, na, nf, nd, c->cas2full.get(), c->cnew.get(), c->full_dets.get(), eps_act, c->c_full.get(),
+             c->cF.get());
+    apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1.get());
+
+    DC2_HIP(hipEventRecord(rot_ready, s));
+  } catch (...) {
+    go_rot.set_value(false);
+    go_dens.set_value(false);
+    go.set_value(false);
+    finish_lanes();
+    throw;
   }
-  auto finish_lanes = [&]() {
-    for (auto& t : threads) t.join();
-    (void)hipEventDestroy(y1_ready);
-    (void)hipEventDestroy(ready);
-  };
+  go_rot.set_value(true);
   try {
     diag_space(2);
     c->timings[1] = t1.stop();
@@ -1191,6 +1366,18 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       check_finite(c, "G3F", c->G3F.get(), na2 * na2 * na2);
     }
     check_finite(c, "h'", c->hp.get(), (int64_t)nmo * nmo);
+    upload_ints(s, c->tpairs, pair_list(na));
+    upload_ints(s, c->ipairs, pair_list(ni));
+    upload_ints(s, c->spairs, pair_list(ns));
+    DC2_HIP(hipEventRecord(dens_ready, s));
+  } catch (...) {
+    go_dens.set_value(false);
+    go.set_value(false);
+    finish_lanes();
+    throw;
+  }
+  go_dens.set_value(true);
+  try {
 
     // ---- transforms of every needed subspace block (module_intra.F90)
     Timer t3(s);
@@ -1210,9 +1397,6 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     c->timings[3] = t3.stop();
     for (const auto& nm : need) check_finite(c, ("transformed " + nm).c_str(), c->tblk[nm].get(), block_count(c, shape_of(nm)));
 
-    upload_ints(s, c->tpairs, pair_list(na));
-    upload_ints(s, c->ipairs, pair_list(ni));
-    upload_ints(s, c->spairs, pair_list(ns));
     for (int k = 0; k < 8; ++k) { e2[k] = 0.0; sumc2[k] = 0.0; }
 
     DC2_HIP(hipEventRecord(ready, s));
@@ -1223,7 +1407,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   }
   go.set_value(true);
   try {
-    for (int k : plan[0]) run_subspace(c, *c->lane0, k, eshift, e2, sumc2, &early[k]);
+    for (int k : plan[0]) run_subspace(c, *c->lane0, k, eshift, e2, sumc2, &prep[k]);
   } catch (...) {
     errs[0] = std::current_exception();
   }
@@ -1279,7 +1463,7 @@ int dc2_context_create(int32_t device, dc2_context** out) {
     c->lane0 = std::make_unique<Lane>();
     c->lane0->stream = c->stream;
     c->lane0->blas = c->blas;
-    if (const char* nl = std::getenv("DC2_LANES")) c->nlanes = std::max(1, std::min(3, std::atoi(nl)));
+    if (const char* nl = std::getenv("DC2_LANES")) c->nlanes = std::max(1, std::min(4, std::atoi(nl)));
     *out = c.release();
   });
 }
@@ -1706,7 +1890,7 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
       DC2_CHECK(value >= 0 && value <= 3, 1, "ac_metric must be 0..3");
       c->ac_route = (int)value;
     } else if (k == "lanes") {
-      DC2_CHECK(value >= 1 && value <= 3, 1, "lanes must be 1..3");
+      DC2_CHECK(value >= 1 && value <= 4, 1, "lanes must be 1..4");
       c->nlanes = (int)value;
     } else if (k == "schur") {
       DC2_CHECK(value == 0 || value == 1, 1, "schur must be 0 or 1");
