@@ -57,6 +57,7 @@ _SIGS = {
     "dc2_get_transform_flops": (c_int, [P, P]),
     "dc2_get_casci_info": (c_int, [P, P]),
     "dc2_host_heev": (c_int, [c_int, P, P, c_int]),
+    "dc2_get_hetrd_stats": (c_int, [c_int, P]),
     "dc2_request_footprint": (c_int, [c_int, c_int, c_int, c_int, P]),
     "dc2_server_create": (c_int, [c_int, c_int, c_i64, c_int, P]),
     "dc2_server_create_batched": (c_int, [c_int, c_int, c_i64, c_int, c_int, ctypes.c_double, P]),

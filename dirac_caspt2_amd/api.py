@@ -436,6 +436,14 @@ def zgemm_split(ctx: Caspt2Context, slices: int, opB: str, M, N, K, A: DeviceArr
                                                      C.ptr, ldc))
 
 
+def hetrd_stats(n: int):
+    """(calls, summed device seconds) of the tridiagonalisation kernel for n x n matrices in
+    this process; n = 0 resets the counts."""
+    out = np.zeros(2)
+    check("dc2_get_hetrd_stats", _lib.lib().dc2_get_hetrd_stats(n, _ptr(out)))
+    return int(out[0]), float(out[1])
+
+
 HOST_HEEV_METHODS = {"auto": 0, "jacobi": 1, "ql": 2}
 
 

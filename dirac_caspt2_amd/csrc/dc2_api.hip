@@ -1140,12 +1140,13 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1o.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
   check_finite(c, "the CASCI 1-RDM", c->G1.get(), na2);
 
-  // Subspaces: A and C (the large metrics) on their own lanes, the rest on the library
-  // stream.  The lanes start now with the eigendecompositions of the A/C determinant-space
-  // metrics, which need only E_uv|0> of the CASCI state in the original orbitals (the
-  // metric is the same operator in every orbital basis of the active space), carry the
-  // eigenvectors to the canonical orbitals once the active rotation is known, and take the
-  // rest of their subspaces once the transforms and densities below are done.
+  // Subspaces: A and C (the large metrics) on their own lanes, D and F on a third, the rest
+  // on the library stream.  Without RAS limits the A/C lanes start now with the
+  // eigendecompositions of their determinant-space metrics, which need only E_uv|0> of the
+  // CASCI state in the original orbitals (the metric is the same operator in every orbital
+  // basis of the active space), and carry the eigenvectors to the canonical orbitals once
+  // the active rotation is known.  Every lane prepares the S/B halves of its subspaces once
+  // the densities exist and takes the V halves once the transforms are done.
   const bool doA = ni > 0, doB = ni > 0, doC = ns > 0, doD = ni > 0 && ns > 0, doF = ns > 0;
   const bool doE = doD, doG = doD, doH = doD;
   const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
@@ -1159,6 +1160,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     if (plan.size() >= 4 && (k == 3 || k == 5)) lane = 3;
     plan[lane].push_back(k);
   }
+  const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
   hipEvent_t y1o_ready = nullptr, rot_ready = nullptr, dens_ready = nullptr, ready = nullptr;
   DC2_HIP(hipEventCreateWithFlags(&y1o_ready, hipEventDisableTiming));
   DC2_HIP(hipEventCreateWithFlags(&rot_ready, hipEventDisableTiming));
@@ -1184,19 +1186,28 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
         Lane& Lw = *lane_of[w];
         DC2_HIP(hipStreamWaitEvent(Lw.stream, y1o_ready, 0));
         const int64_t n3 = (int64_t)na * na2;
-        for (int k : plan[w]) {
-          if ((k != 0 && k != 2) || !ac_route_of(c, kSub[k]).gram) continue;
-          Timer tm(Lw.stream);
-          prep[k].g = gram_eig(c, Lw, kSub[k], n3, c->Y1o.get(), zeros_act);
-          prep[k].seconds = tm.stop();
-        }
+        auto gram_k = [&](int k) { return (k == 0 || k == 2) && ac_route_of(c, kSub[k]).gram; };
+        // the original-basis metric describes the rotated state only when the CASCI space is
+        // closed under the active rotation (no RAS limits); with RAS limits tracic projects
+        // the rotated state back onto the restricted space, and the metric waits for it
+        if (!ras)
+          for (int k : plan[w]) {
+            if (!gram_k(k)) continue;
+            Timer tm(Lw.stream);
+            prep[k].g = gram_eig(c, Lw, kSub[k], n3, c->Y1o.get(), zeros_act);
+            prep[k].seconds = tm.stop();
+          }
         if (!go_rot_f.get()) return;
         DC2_HIP(hipStreamWaitEvent(Lw.stream, rot_ready, 0));
         for (int k : plan[w]) {
-          if (!prep[k].g.valid) continue;
+          if (!gram_k(k)) continue;
           Timer tm(Lw.stream);
-          gram_rotate(c, Lw, kSub[k], prep[k].g);
-          gram_rotated_parts(c, Lw, kSub[k], n3, prep[k].g, false);
+          if (prep[k].g.valid) {
+            gram_rotate(c, Lw, kSub[k], prep[k].g);
+            gram_rotated_parts(c, Lw, kSub[k], n3, prep[k].g, false);
+          } else {
+            prep[k].g = gram_metric(c, Lw, kSub[k], n3);
+          }
           prep[k].seconds += tm.stop();
           prep[k].valid = true;
         }
@@ -1274,7 +1285,6 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     diag_space(0);
     diag_space(1);
     // ---- tracic: ds c~ = c (trac.F90:76-156)
-    const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
     {
       cplx* Ua = L0.z("tr_Ua", na2);
       cplx* ds = L0.z("tr_ds", nd * nd);
@@ -1959,6 +1969,14 @@ int dc2_host_heev(int32_t n, double* a, double* w, int32_t method) {
     else if (method == 1) herm_jacobi(n, A.data(), w, V.data());
     else DC2_CHECK(herm_tql(n, A.data(), w, V.data()), 3, "QL iteration did not converge");
     std::copy(V.begin(), V.end(), reinterpret_cast<zc*>(a));
+  });
+}
+
+int dc2_get_hetrd_stats(int32_t n, double* out) {
+  return guarded([&] {
+    DC2_CHECK(out && n >= 0, 1, "invalid argument");
+    if (n == 0) hetrd_stats_reset();
+    else hetrd_stats(n, out);
   });
 }
 

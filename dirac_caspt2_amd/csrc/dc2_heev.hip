@@ -32,6 +32,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <condition_variable>
+#include <map>
 #include <mutex>
 
 namespace dc2 {
@@ -401,7 +402,23 @@ void launch_back(hipStream_t s, int n, const double* Z, const cplx* delta, const
   hipLaunchKernelGGL((k_hetrd_back<S>), dim3(cdiv(n, 8)), dim3(512), 0, s, n, Z, delta, tau, Vh, X, ldx);
 }
 
+// device time of every tridiagonalisation, by matrix size (HIP events around the launch)
+std::mutex g_hetrd_mu;
+std::map<int, std::pair<int64_t, double>> g_hetrd_time;
+
 }  // namespace
+
+void hetrd_stats(int n, double* out) {
+  std::lock_guard<std::mutex> lk(g_hetrd_mu);
+  auto it = g_hetrd_time.find(n);
+  out[0] = it == g_hetrd_time.end() ? 0.0 : (double)it->second.first;
+  out[1] = it == g_hetrd_time.end() ? 0.0 : it->second.second;
+}
+
+void hetrd_stats_reset() {
+  std::lock_guard<std::mutex> lk(g_hetrd_mu);
+  g_hetrd_time.clear();
+}
 
 int64_t heev_native_work_bytes(int n) {
   const int64_t nn = (int64_t)n * n;
@@ -438,8 +455,12 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
   DC2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   if (nwg > cus) return false;
   hipError_t st;
+  thread_local hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto& e : ev)
+    if (!e) DC2_HIP(hipEventCreate(&e));
   {
     CoopSlot slot(nwg, cus);
+    DC2_HIP(hipEventRecord(ev[0], s));
     if (n <= 128) st = launch_hetrd<2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 256) st = launch_hetrd<4>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 512) st = launch_hetrd<8>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
@@ -449,10 +470,17 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
       (void)hipGetLastError();
       return false;   // e.g. the workgroup does not fit a compute unit: the caller uses rocSOLVER
     }
+    DC2_HIP(hipEventRecord(ev[1], s));
     int flag[2] = {0, 0};
     DC2_HIP(hipMemcpyAsync(flag, flags, sizeof(flag), hipMemcpyDeviceToHost, s));
     DC2_HIP(hipStreamSynchronize(s));
     if (flag[1] != 0) return false;
+    float ms = 0.0f;
+    DC2_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    std::lock_guard<std::mutex> lk(g_hetrd_mu);
+    auto& e = g_hetrd_time[n];
+    e.first += 1;
+    e.second += 1e-3 * ms;
   }
   hipLaunchKernelGGL(k_tri_prep, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, sub, e, Z);
   DC2_HIP(hipGetLastError());

@@ -96,6 +96,9 @@ void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const in
 // heev_native_work_bytes(n) bytes of device memory.
 int64_t heev_native_work_bytes(int n);
 bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev);
+// {calls, summed device seconds} of the tridiagonalisation kernel for size n in this process
+void hetrd_stats(int n, double* out);
+void hetrd_stats_reset();
 
 // ---------------------------------------------------------------- dc2_pt2.hip
 struct Dens {

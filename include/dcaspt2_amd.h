@@ -168,6 +168,10 @@ int dc2_zgemm_split(dc2_context* ctx, int32_t slices, char opB, int64_t M, int64
                     int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc);
 /* transform flops of the last dc2_caspt2: out[0] in split precision, out[1] in FP64 */
 int dc2_get_transform_flops(dc2_context* ctx, double* out);
+/* Measurement, no counterpart in the reference: out[2] = {calls, summed device seconds} of
+ * this library's tridiagonalisation kernel for n x n matrices in this process (HIP events
+ * around each launch); n = 0 resets the counts. */
+int dc2_get_hetrd_stats(int32_t n, double* out);
 /* out[3] = {ndet, roots held by the last dc2_casci, Davidson iterations (0: dense solver)} */
 int dc2_get_casci_info(dc2_context* ctx, int64_t* out);
 /* Hermitian eigendecomposition on the host (the Davidson subspace solver): a (n x n

@@ -161,3 +161,32 @@ def test_medium_davidson_vs_dense(medium):
     np.testing.assert_allclose(e_auto, e_dense, atol=1e-10, rtol=0)
     assert_energies(out_auto, ref, 1e-9, "davidson")
     assert_energies(out_dense, ref, 1e-9, "dense")
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 3, 4])
+def test_ras_determinant_metric_on_lanes(lanes):
+    """RASPT2 with the A/C determinant-space route on every lane layout against the oracle.
+    With RAS limits tracic projects the rotated CI vector onto the restricted space, so the
+    A/C metrics must come from the rotated state (with more than one lane the metrics of
+    unrestricted problems are computed from the original orbitals ahead of the rotation)."""
+    from dirac_caspt2_amd import synth
+    from dirac_caspt2_amd.api import Caspt2Context
+    from dirac_caspt2_amd.problem import load_factors
+    from gpu_helpers import assert_energies, say
+    from oracle import caspt2_ref as R
+    say("RAS determinant metric, lanes", lanes)
+    spec = synth.SynthSpec(ninact=4, nact=8, nsec=30, nelec=4, seed=11)
+    ras = (2, 4, 2, 1, 1, 0, 0)
+    sp = synth.generate(spec)
+    h = synth.scan_point(sp, 0.3, synth.scan_displacements(spec, spec.seed + 1))
+    p = R.Problem(spec.ninact, spec.nact, spec.nsec, spec.nelec, h, R.FactorIntegrals(sp.B), sp.ecore, 1,
+                  ras=R.RasSpec(*ras))
+    ref = R.run_caspt2(p)
+    with Caspt2Context(0) as ctx:
+        load_factors(ctx, (spec.ninact, spec.nact, spec.nsec, spec.nelec), h, sp.B, sp.ecore)
+        ctx.set_ras(*ras)
+        ctx.set_option("ac_metric", "determinant")
+        ctx.set_option("lanes", lanes)
+        ctx.casci(1)
+        out = ctx.caspt2(1, 0.0)
+        assert_energies(out, ref, 1e-9, f"RAS lanes {lanes}")
