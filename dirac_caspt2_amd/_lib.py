@@ -58,6 +58,8 @@ _SIGS = {
     "dc2_get_casci_info": (c_int, [P, P]),
     "dc2_host_heev": (c_int, [c_int, P, P, c_int]),
     "dc2_get_hetrd_stats": (c_int, [c_int, P]),
+    "dc2_set_casci_result": (c_int, [P, c_i64, c_int, P, P, P]),
+    "dc2_caspt2_restart": (c_int, [P, c_int, ctypes.c_double, c_int, P, P, P, P, P]),
     "dc2_request_footprint": (c_int, [c_int, c_int, c_int, c_int, P]),
     "dc2_server_create": (c_int, [c_int, c_int, c_i64, c_int, P]),
     "dc2_server_create_batched": (c_int, [c_int, c_int, c_i64, c_int, c_int, ctypes.c_double, P]),

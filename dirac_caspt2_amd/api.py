@@ -207,6 +207,33 @@ class Caspt2Context:
         s2 = np.zeros(8)
         sc = np.zeros(6)
         check("dc2_caspt2", self.lib.dc2_caspt2(self.h, selectroot, float(eshift), _ptr(e2), _ptr(s2), _ptr(sc)))
+        return self._energies(e2, s2, sc)
+
+    def caspt2_restart(self, selectroot: int, eshift: float, next_subspace: str, e2_file: Dict[str, float],
+                       sumc2_file: Dict[str, float]) -> Caspt2Energies:
+        """CASPT2 resumed from a caspt2_restart file: subspaces before next_subspace ('A'..'H', or
+        'I' for none) are skipped; e2_file/sumc2_file hold the file's values (absent = 0)."""
+        first = "ABCDEFGHI".index(next_subspace)
+        fe = np.array([e2_file.get(X, 0.0) for X in SUBSPACES])
+        fs = np.array([sumc2_file.get(X, 0.0) for X in SUBSPACES])
+        e2 = np.zeros(8)
+        s2 = np.zeros(8)
+        sc = np.zeros(6)
+        check("dc2_caspt2_restart", self.lib.dc2_caspt2_restart(self.h, selectroot, float(eshift), first, _ptr(fe),
+                                                                _ptr(fs), _ptr(e2), _ptr(s2), _ptr(sc)))
+        return self._energies(e2, s2, sc)
+
+    def set_casci_result(self, dets, coeffs: np.ndarray, ecas):
+        """Use a CASCI result computed elsewhere (determinants in order, complex CI vectors
+        ndet x nroot, energies without ecore) instead of casci()."""
+        d = np.ascontiguousarray(np.asarray(dets, dtype=np.int64))
+        c = np.asfortranarray(np.asarray(coeffs, dtype=np.complex128).reshape(len(d), -1))
+        e = np.ascontiguousarray(np.asarray(ecas, dtype=np.float64))
+        check("dc2_set_casci_result", self.lib.dc2_set_casci_result(self.h, len(d), c.shape[1], _ptr(d), _ptr(c),
+                                                                    _ptr(e)))
+        self.ndet = len(d)
+
+    def _energies(self, e2, s2, sc) -> Caspt2Energies:
         info = np.zeros(32, dtype=np.int64)
         margin = np.zeros(8)
         check("dc2_get_subspace_info", self.lib.dc2_get_subspace_info(self.h, _ptr(info), _ptr(margin)))

@@ -541,12 +541,11 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   return false;
 }
 
-void run_casci(dc2_context* c, int nroot) {
+// the CASCI determinants (search_cas_configuration.F90) and the full string space of the
+// densities: c->ndet, cas_dets, full_dets, rank tables, cas2full
+void enumerate_space(dc2_context* c) {
   need_spaces(c);
-  DC2_CHECK(c->h.get() != nullptr, 1, "dc2_set_one_electron has not been called");
-  const cplx* O = get_block(c, "OOOO");
   hipStream_t s = c->stream;
-  Timer tm(s);
   const int na = c->na;
   const int64_t N = 1ll << na;
   // scratch from the library lane's grow-only buffers: a hipFree would synchronise every
@@ -579,6 +578,18 @@ void run_casci(dc2_context* c, int nroot) {
   c->cas2full.ensure(nd);
   enumerate_dets(s, p, ff, fc, c->rank_full.get(), c->rank_cas.get(), c->full_dets.get(), c->cas_dets.get(),
                  c->cas2full.get(), &nf, &nd, counts);
+}
+
+void run_casci(dc2_context* c, int nroot) {
+  need_spaces(c);
+  DC2_CHECK(c->h.get() != nullptr, 1, "dc2_set_one_electron has not been called");
+  const cplx* O = get_block(c, "OOOO");
+  hipStream_t s = c->stream;
+  Timer tm(s);
+  const int na = c->na;
+  Lane& L0 = *c->lane0;
+  enumerate_space(c);
+  const int64_t nd = c->ndet;
   c->cimat.ensure(nd * nd);
   casmat(s, c->ni, na, c->ne, nd, c->cas_dets.get(), c->rank_cas.get(), O, c->nmoc(), c->h.get(), c->nmo(),
          c->cimat.get());
@@ -1107,7 +1118,13 @@ Lane& worker_lane(dc2_context* c, int i) {
   return *c->lanes[i - 1];
 }
 
-void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, double* sumc2, double* scal) {
+// first: subspaces before it (0 = A .. 8 = none computed) are skipped, as with a caspt2_restart
+// file whose next subspace is `first` (module_restart_file.F90:15-24); e2_file/sumc2_file: the
+// values read from that file (zeros where it lists nothing, may be null).  As in
+// r4dcaspt2_tra.F90:168 and solve_X_subspace, the totals start from the sums of the file's
+// values and every subspace computed adds its own.
+void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, double* sumc2, double* scal,
+                int first = 0, const double* e2_file = nullptr, const double* sumc2_file = nullptr) {
   DC2_CHECK(c->have_casci, 1, "dc2_casci must run before dc2_caspt2");
   DC2_CHECK(selectroot >= 1 && selectroot <= c->nroots_avail, 1,
             "selectroot out of range (dc2_casci must compute at least selectroot roots)");
@@ -1147,8 +1164,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   // basis of the active space), and carry the eigenvectors to the canonical orbitals once
   // the active rotation is known.  Every lane prepares the S/B halves of its subspaces once
   // the densities exist and takes the V halves once the transforms are done.
-  const bool doA = ni > 0, doB = ni > 0, doC = ns > 0, doD = ni > 0 && ns > 0, doF = ns > 0;
-  const bool doE = doD, doG = doD, doH = doD;
+  const bool doA = ni > 0 && first <= 0, doB = ni > 0 && first <= 1, doC = ns > 0 && first <= 2;
+  const bool doD = ni > 0 && ns > 0 && first <= 3, doE = ni > 0 && ns > 0 && first <= 4;
+  const bool doF = ns > 0 && first <= 5, doG = ni > 0 && ns > 0 && first <= 6, doH = ni > 0 && ns > 0 && first <= 7;
   const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
   // lanes: 1 all on the library stream; 2: A and C on lane 1; 3: A on 1, C on 2; 4: D and F
   // (the next largest) on lane 3 as well
@@ -1428,7 +1446,18 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   DC2_HIP(hipMemcpyAsync(&e0h, c->e0.get(), sizeof(double), hipMemcpyDeviceToHost, s));
   DC2_HIP(hipStreamSynchronize(s));
   double e2all = 0.0, s2 = 0.0;
-  for (int k = 0; k < 8; ++k) { e2all += e2[k]; s2 += sumc2[k]; }
+  for (int k = 0; k < 8; ++k) {
+    const double fe = e2_file ? e2_file[k] : 0.0, fs = sumc2_file ? sumc2_file[k] : 0.0;
+    e2all += fe;
+    s2 += fs;
+    if (do_sub[k]) {
+      e2all += e2[k];
+      s2 += sumc2[k];
+    } else {
+      e2[k] = fe;
+      sumc2[k] = fs;
+    }
+  }
   scal[0] = e0h;
   scal[1] = e2all;
   scal[2] = s2;
@@ -1622,6 +1651,41 @@ int dc2_caspt2(dc2_context* c, int32_t selectroot, double eshift, double* e2, do
     DC2_CHECK(c && e2 && sumc2 && scalars, 1, "null argument");
     DC2_HIP(hipSetDevice(c->device));
     run_caspt2(c, selectroot, eshift, e2, sumc2, scalars);
+  });
+}
+
+int dc2_set_casci_result(dc2_context* c, int64_t ndet, int32_t nroot, const int64_t* dets, const double* coeffs,
+                         const double* ecas) {
+  return guarded([&] {
+    DC2_CHECK(c && dets && coeffs && ecas && ndet > 0 && nroot >= 1, 1, "invalid argument");
+    DC2_HIP(hipSetDevice(c->device));
+    enumerate_space(c);
+    DC2_CHECK(c->ndet == ndet, 1,
+              "ndet of the CASCI result (" + std::to_string(ndet) + ") differs from this problem's (" +
+                  std::to_string(c->ndet) + ")");
+    std::vector<int64_t> mine(ndet);
+    DC2_HIP(hipMemcpyAsync(mine.data(), c->cas_dets.get(), ndet * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    DC2_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < ndet; ++k)
+      DC2_CHECK(mine[k] == dets[k], 1, "determinant " + std::to_string(k + 1) + " of the CASCI result differs");
+    c->civec.ensure(ndet * nroot);
+    DC2_HIP(hipMemcpyAsync(c->civec.get(), coeffs, ndet * nroot * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+    DC2_HIP(hipStreamSynchronize(c->stream));
+    c->ecas.assign(ecas, ecas + nroot);
+    c->nroots_avail = nroot;
+    c->davidson_iters = 0;
+    c->have_casci = true;
+    c->have_pt2 = false;
+  });
+}
+
+int dc2_caspt2_restart(dc2_context* c, int32_t selectroot, double eshift, int32_t first, const double* e2_file,
+                       const double* sumc2_file, double* e2, double* sumc2, double* scalars) {
+  return guarded([&] {
+    DC2_CHECK(c && e2 && sumc2 && scalars && e2_file && sumc2_file, 1, "null argument");
+    DC2_CHECK(first >= 0 && first <= 8, 1, "first must be 0 (A) .. 8 (none computed)");
+    DC2_HIP(hipSetDevice(c->device));
+    run_caspt2(c, selectroot, eshift, e2, sumc2, scalars, first, e2_file, sumc2_file);
   });
 }
 

@@ -19,7 +19,7 @@ module dcaspt2_amd
     public :: c_ptr, c_null_ptr
     public :: dc2_context_create, dc2_context_destroy, dc2_version, dc2_set_spaces, dc2_set_symmetry, dc2_set_ras, &
               dc2_set_one_electron, dc2_set_integral_block, dc2_set_integral_records, dc2_set_integral_factors, &
-              dc2_casci, dc2_caspt2, &
+              dc2_casci, dc2_caspt2, dc2_set_casci_result, dc2_caspt2_restart, &
               dc2_get_subspace_info, dc2_cdiag, dc2_casmat, dc2_synchronize
     public :: dcaspt2_amd_check, dcaspt2_amd_last_error, dcaspt2_amd_open, dcaspt2_amd_close, dcaspt2_amd_casci, &
               dcaspt2_amd_casmat, dcaspt2_amd_cdiag, dcaspt2_amd_r4dcaspt2_tra
@@ -81,6 +81,25 @@ module dcaspt2_amd
             integer(c_int32_t), intent(in) :: ijkl(*)
             complex(c_double_complex), intent(in) :: values(*)
             integer(c_int32_t), value :: where
+        end function
+        integer(c_int) function dc2_set_casci_result(ctx, ndet, nroot, dets, coeffs, ecas) &
+            bind(C, name="dc2_set_casci_result")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: ndet
+            integer(c_int32_t), value :: nroot
+            integer(c_int64_t), intent(in) :: dets(*)
+            complex(c_double_complex), intent(in) :: coeffs(*)
+            real(c_double), intent(in) :: ecas(*)
+        end function
+        integer(c_int) function dc2_caspt2_restart(ctx, selectroot, eshift, first, e2_file, sumc2_file, e2, sumc2, &
+                                                   scalars) bind(C, name="dc2_caspt2_restart")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: selectroot, first
+            real(c_double), value :: eshift
+            real(c_double), intent(in) :: e2_file(8), sumc2_file(8)
+            real(c_double), intent(out) :: e2(8), sumc2(8), scalars(6)
         end function
         integer(c_int) function dc2_set_integral_factors(ctx, b, naux, where) bind(C, name="dc2_set_integral_factors")
             import :: c_int, c_int32_t, c_int64_t, c_double_complex, c_ptr

@@ -195,6 +195,21 @@ int dc2_memcpy(dc2_context* ctx, void* dst, const void* src, int64_t bytes, int3
  * trip).  where: 0 host, 1 device. */
 int dc2_set_integral_factors(dc2_context* ctx, const double* B, int64_t naux, int32_t where);
 
+/* CASCI result computed elsewhere, e.g. read from a CIDATA_sym<totsym> file (casci.F90:117-160,
+ * read_cidata.F90): ndet determinants (bit strings of occupied active spinors, the values of
+ * dict_cas_idx, which must equal this problem's determinants in order), nroot CI vectors
+ * (complex, ndet x nroot, column-major) and the CASCI energies ecas without ecore.  Replaces
+ * dc2_casci before dc2_caspt2. */
+int dc2_set_casci_result(dc2_context* ctx, int64_t ndet, int32_t nroot, const int64_t* dets, const double* coeffs,
+                         const double* ecas);
+/* dc2_caspt2 resumed from a caspt2_restart_<totsym>_<root> file (module_restart_file.F90:55-172):
+ * subspaces before `first` (0 = A .. 7 = H, 8 = none computed) are skipped, e2_file[8] and
+ * sumc2_file[8] hold the file's values (zero where it lists nothing).  As the reference
+ * (r4dcaspt2_tra.F90:168, solve_X_subspace), e2all and sumc2 start from the sums of the file's
+ * values and every computed subspace adds its own; e2[k] and sumc2[k] of a skipped subspace
+ * are the file's. */
+int dc2_caspt2_restart(dc2_context* ctx, int32_t selectroot, double eshift, int32_t first, const double* e2_file,
+                       const double* sumc2_file, double* e2_subspace, double* sumc2_subspace, double* scalars);
 /* Integral ingest on the GPU: n MDCINTNEW-equivalent records (create_newmdcint.F90:173-176),
  * ijkl[4n] = 1-based CAS-order spinor labels (i, j, k, l) of (ij|kl), values[2n] = (re, im)
  * as written there.  Every record's 16 symmetry images (module_2integrals.F90:395-434) are
