@@ -344,13 +344,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run 3 device-to-device copies of MIB MiB (a plain copy kernel "
                          "of known size, the reference for the HBM counters)")
-    ap.add_argument("--served", type=int, default=96, metavar="N",
+    ap.add_argument("--served", type=int, default=128, metavar="N",
                     help="after the plain steps, N scan points of the same molecule through the request server "
                          "(0: skip)")
-    ap.add_argument("--slots", type=int, default=32, help="concurrent slots of the served run")
+    ap.add_argument("--slots", type=int, default=64, help="concurrent slots of the served run")
     ap.add_argument("--slot-lanes", type=int, default=1, help="subspace lanes per slot in the served run")
     ap.add_argument("--split", type=int, default=7, help="INT8 slices of the split-precision transforms (0: off)")
-    ap.add_argument("--eig-batch", type=int, default=32, help="eigensolves batched across served requests (<=1: off)")
+    ap.add_argument("--eig-batch", type=int, default=64, help="eigensolves batched across served requests (<=1: off)")
     ap.add_argument("--eig-window", type=float, default=0.003, help="batching window of the served eigensolves (s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -334,8 +334,8 @@ int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_
 
 int dc2_server_create_batched(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot,
                               int32_t eig_batch, double eig_window_s, dc2_server** out) {
-  if (!out || nslots < 1 || nslots > 64 || mem_budget <= 0 || lanes_per_slot < 1 || lanes_per_slot > 3 ||
-      eig_batch < 0 || eig_batch > 64 || eig_window_s < 0.0 || eig_window_s > 1.0)
+  if (!out || nslots < 1 || nslots > 256 || mem_budget <= 0 || lanes_per_slot < 1 || lanes_per_slot > 4 ||
+      eig_batch < 0 || eig_batch > 256 || eig_window_s < 0.0 || eig_window_s > 1.0)
     return fail(1, "dc2_server_create: invalid argument");
   auto S = std::make_unique<dc2_server>();
   S->device = device;

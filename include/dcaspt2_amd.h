@@ -224,8 +224,8 @@ int dc2_get_integral_block_device(dc2_context* ctx, const char* name, const doub
 
 /* ---- request server: continuous batching on one GPU (no counterpart in the reference,
  * whose driver runs one problem per process, dcaspt2_run_subprograms.F90:57-62) ------
- * nslots slots, each with its own context and host thread (lanes_per_slot concurrent
- * subspace streams per request).  Molecules are registered once (factorised integrals,
+ * nslots (1..256) slots, each with its own context and host thread (lanes_per_slot, 1..4,
+ * concurrent subspace streams per request).  Molecules are registered once (factorised integrals,
  * as dc2_set_integral_factors; the blocks stay resident in HBM and are shared read-only
  * by all requests on the molecule).  A request is one CASCI + CASPT2 energy for a
  * one-electron matrix h (nmo x nmo complex, copied at submit), root and eshift, with
@@ -239,8 +239,9 @@ int dc2_request_footprint(int32_t ninact, int32_t nact, int32_t nsec, int32_t ne
 int dc2_server_create(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot, dc2_server** out);
 /* the same with cross-request eigensolve batching: the Hermitian eigenproblems of the
  * running requests are grouped by size and solved eig_batch at a time by one batched
- * rocSOLVER call; a group is flushed when every running request waits for it, when it
- * is full, or eig_window_s seconds after its oldest problem (eig_batch <= 1: off) */
+ * rocSOLVER call (eig_batch <= 256); a group is flushed when every running request waits
+ * for it, when it is full, or eig_window_s seconds after its oldest problem (eig_batch <= 1:
+ * off) */
 int dc2_server_create_batched(int32_t device, int32_t nslots, int64_t mem_budget, int32_t lanes_per_slot,
                               int32_t eig_batch, double eig_window_s, dc2_server** out);
 /* finishes every pending request, then stops the slots and frees the molecules */
