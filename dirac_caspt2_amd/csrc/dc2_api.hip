@@ -1990,7 +1990,7 @@ int dc2_get_gemm_op_stats(dc2_context* c, char opA, char opB, double* out) {
     DC2_HIP(hipStreamSynchronize(c->stream));
     double n = 0, sec = 0, fl = 0, mx = 0;
     for (auto& r : c->gstats.rec) {
-      if (r.opA != opA || r.opB != opB) continue;
+      if (r.opA != opA || r.opB != opB || !r.mfma) continue;
       float ms = 0;
       DC2_HIP(hipEventElapsedTime(&ms, r.a, r.b));
       n += 1;

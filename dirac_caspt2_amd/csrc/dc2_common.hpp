@@ -90,6 +90,7 @@ struct GemmRecord {
   double flops;
   int tag;
   char opA, opB;   // the kernel instantiation k_zgemm<opA, opB>
+  bool mfma;        // false: one of the kernels for skinny shapes
 };
 struct GemmStats {
   bool on = false;

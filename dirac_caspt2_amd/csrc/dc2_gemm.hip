@@ -8,7 +8,10 @@
 //
 // Used for: the four one-index integral transforms (module_intra.F90), the
 // RDM contractions (density_matrix.F90), the V contractions and the
-// (U L^-1/2)^H B (U L^-1/2) products of every solve_X_subspace.
+// (U L^-1/2)^H B (U L^-1/2) products of every solve_X_subspace.  Single products
+// whose output has too few 64x64 tiles (the block Davidson of the CI matrix) go to
+// plain FP64 kernels instead: one wave per output row over a long K, one thread per
+// output over a short K, one thread per row for at most 8 columns over a long K.
 #include <algorithm>
 #include <cstdlib>
 #include <map>
@@ -241,6 +244,114 @@ __global__ __launch_bounds__(256) void k_zgemm2(int64_t M, int64_t N, int64_t Kf
       }
 }
 
+// ---- shapes with too few output tiles for the MFMA kernel (the block Davidson of the CI
+// matrix: Rayleigh-matrix columns, Gram-Schmidt coefficients, subspace updates, H x)
+
+__device__ inline cplx out_value(cplx acc, cplx alpha, cplx beta, const cplx* c) {
+  cplx out = cmul(alpha, acc);
+  if (beta.x != 0.0 || beta.y != 0.0) cfma(out, beta, *c);
+  return out;
+}
+
+// op(A) = A^H or A^T, op(B) = B or conj(B) with a small output (M x N): both operands are read
+// along K, contiguous.  One wave per row m and up to 4 columns; the lanes stride over K and
+// their partial sums meet in a fixed butterfly, so the result does not depend on timing.
+constexpr int DOT_NB = 4;
+template <char OPA, char OPB>
+__global__ __launch_bounds__(256) void k_zgemm_dot(int64_t M, int64_t N, int64_t K, cplx alpha,
+                                                   const cplx* __restrict__ A, int64_t lda,
+                                                   const cplx* __restrict__ B, int64_t ldb, cplx beta,
+                                                   cplx* __restrict__ C, int64_t ldc) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);   // wave-uniform
+  const int64_t nb = (N + DOT_NB - 1) / DOT_NB;
+  if (w >= M * nb) return;
+  const int64_t m = w % M, n0 = (w / M) * DOT_NB;
+  const int nn = (int)(N - n0 < DOT_NB ? N - n0 : DOT_NB);
+  cplx acc[DOT_NB];
+#pragma unroll
+  for (int j = 0; j < DOT_NB; ++j) acc[j] = mk(0.0, 0.0);
+  const cplx* a = A + lda * m;
+  const cplx* b = B + ldb * n0;
+  for (int64_t k = lane; k < K; k += 64) {
+    cplx x = a[k];
+    if (OPA == 'C') x = cconj(x);
+#pragma unroll
+    for (int j = 0; j < DOT_NB; ++j)
+      if (j < nn) {
+        cplx y = b[k + ldb * j];
+        if (OPB == 'R') y = cconj(y);
+        cfma(acc[j], x, y);
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < DOT_NB; ++j) {
+    double re = acc[j].x, im = acc[j].y;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      re += __shfl_xor(re, off);
+      im += __shfl_xor(im, off);
+    }
+    acc[j] = mk(re, im);
+  }
+  if (lane < nn) {
+    cplx s = acc[0];
+#pragma unroll
+    for (int j = 1; j < DOT_NB; ++j)
+      if (lane == j) s = acc[j];
+    cplx* c = C + m + ldc * (n0 + lane);
+    *c = out_value(s, alpha, beta, c);
+  }
+}
+
+// op(A) = A, op(B) = B or conj(B) with a short K: one thread per output element
+template <char OPB>
+__global__ __launch_bounds__(256) void k_zgemm_smallk(int64_t M, int64_t N, int64_t K, cplx alpha,
+                                                      const cplx* __restrict__ A, int64_t lda,
+                                                      const cplx* __restrict__ B, int64_t ldb, cplx beta,
+                                                      cplx* __restrict__ C, int64_t ldc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * N) return;
+  const int64_t m = i % M, n = i / M;
+  cplx acc = mk(0.0, 0.0);
+  for (int64_t k = 0; k < K; ++k) {
+    cplx y = B[k + ldb * n];
+    if (OPB == 'R') y = cconj(y);
+    cfma(acc, A[m + lda * k], y);
+  }
+  cplx* c = C + m + ldc * n;
+  *c = out_value(acc, alpha, beta, c);
+}
+
+// op(A) = A, op(B) = B or conj(B) with at most TS_NMAX columns and a long K: thread = row m,
+// all columns; blockIdx.y = K-split index, partial products to W[kz][n][m] (summed by
+// k_splitk_reduce in a fixed order)
+constexpr int TS_NMAX = 8;
+template <char OPB>
+__global__ __launch_bounds__(256) void k_zgemm_tall(int64_t M, int64_t N, int64_t K, const cplx* __restrict__ A,
+                                                    int64_t lda, const cplx* __restrict__ B, int64_t ldb,
+                                                    cplx* __restrict__ W, int64_t kchunk) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t kz = blockIdx.y, k0 = kz * kchunk;
+  const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  if (m >= M) return;
+  cplx acc[TS_NMAX];
+#pragma unroll
+  for (int j = 0; j < TS_NMAX; ++j) acc[j] = mk(0.0, 0.0);
+  for (int64_t k = k0; k < k1; ++k) {
+    const cplx x = A[m + lda * k];
+#pragma unroll
+    for (int j = 0; j < TS_NMAX; ++j)
+      if (j < N) {
+        cplx y = B[k + ldb * j];
+        if (OPB == 'R') y = cconj(y);
+        cfma(acc[j], x, y);
+      }
+  }
+  cplx* w = W + M * N * kz + m;
+  for (int j = 0; j < N; ++j) w[M * j] = acc[j];
+}
+
 // C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order)
 __global__ void k_splitk_reduce(int64_t M, int64_t N, int sk, const cplx* __restrict__ W, cplx alpha, cplx beta,
                                 cplx* __restrict__ C, int64_t ldc) {
@@ -277,9 +388,65 @@ int g_variant = [] {
 std::mutex g_stats_mu;
 thread_local int g_tag = 0;
 
+// Kernels for the shapes the 64x64 MFMA tiles serve badly (single products only): small
+// outputs with A read along K (k_zgemm_dot), a short K (k_zgemm_smallk), at most 8 columns
+// and a long K (k_zgemm_tall).  Returns false when the MFMA kernel is the one to use.
+template <char OPA, char OPB>
+bool launch_skinny(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda,
+                   const cplx* B, int64_t ldb, cplx beta, cplx* C, int64_t ldc) {
+  constexpr bool b_along_k = OPB == 'N' || OPB == 'R';
+  if constexpr ((OPA == 'C' || OPA == 'T') && b_along_k) {
+    if (M * N > 2048) return false;
+    const int64_t waves = M * ((N + DOT_NB - 1) / DOT_NB);
+    hipLaunchKernelGGL((k_zgemm_dot<OPA, OPB>), dim3((unsigned)cdiv(waves, 4)), dim3(256), 0, s, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc);
+    return true;
+  } else if constexpr (OPA == 'N' && b_along_k) {
+    if (K <= 64 && M * N <= 65536) {
+      hipLaunchKernelGGL((k_zgemm_smallk<OPB>), dim3((unsigned)cdiv(M * N, 256)), dim3(256), 0, s, M, N, K, alpha, A,
+                         lda, B, ldb, beta, C, ldc);
+      return true;
+    }
+    if (N > TS_NMAX || K <= 64 || M > 65536) return false;
+    const int64_t rb = cdiv(M, 256);
+    const int64_t want = std::max<int64_t>(1, std::min<int64_t>({cdiv(256, rb), 32, cdiv(K, 16)}));
+    const int64_t kchunk = cdiv(K, want), sk = cdiv(K, kchunk);
+    cplx* W = splitk_workspace(s, (size_t)sk * M * N);
+    hipLaunchKernelGGL((k_zgemm_tall<OPB>), dim3((unsigned)rb, (unsigned)sk), dim3(256), 0, s, M, N, K, A, lda, B,
+                       ldb, W, kchunk);
+    hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, s, M, N, (int)sk, W, alpha, beta, C,
+                       ldc);
+    return true;
+  }
+  return false;
+}
+
 template <char OPA, char OPB>
 void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda, int64_t sA,
             const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC, int64_t batch) {
+  GemmRecord r{};
+  const bool acct = g_stats && g_stats->on;
+  if (acct) {
+    DC2_HIP(hipEventCreate(&r.a));
+    DC2_HIP(hipEventCreate(&r.b));
+    DC2_HIP(hipEventRecord(r.a, s));
+  }
+  auto record = [&](bool mfma) {
+    if (!acct) return;
+    r.mfma = mfma;
+    DC2_HIP(hipEventRecord(r.b, s));
+    r.flops = 8.0 * (double)M * (double)N * (double)K * (double)batch;
+    r.tag = g_tag;
+    r.opA = OPA;
+    r.opB = OPB;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    g_stats->rec.push_back(r);
+  };
+  if (batch == 1 && g_variant != 0 && launch_skinny<OPA, OPB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)) {
+    DC2_HIP(hipGetLastError());
+    record(false);
+    return;
+  }
   // split K over workgroups when the output has few tiles and K is long
   const int64_t tiles = (int64_t)cdiv(M, BM) * cdiv(N, BN) * batch;
   int sk = 1;
@@ -291,13 +458,6 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     sk = (int)((K + kchunk - 1) / kchunk);
   }
   dim3 grid(cdiv(M, BM), cdiv(N, BN), (unsigned)(batch * sk));
-  GemmRecord r{};
-  const bool acct = g_stats && g_stats->on;
-  if (acct) {
-    DC2_HIP(hipEventCreate(&r.a));
-    DC2_HIP(hipEventCreate(&r.b));
-    DC2_HIP(hipEventRecord(r.a, s));
-  }
   cplx* Cw = C;
   int64_t ldw = ldc, sw = sC;
   cplx a1 = alpha, b1 = beta;
@@ -332,15 +492,7 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
   if (sk > 1)
     hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, s, M, N, sk, Cw, alpha, beta, C, ldc);
   DC2_HIP(hipGetLastError());
-  if (acct) {
-    DC2_HIP(hipEventRecord(r.b, s));
-    r.flops = 8.0 * (double)M * (double)N * (double)K * (double)batch;
-    r.tag = g_tag;
-    r.opA = OPA;
-    r.opB = OPB;
-    std::lock_guard<std::mutex> lk(g_stats_mu);
-    g_stats->rec.push_back(r);
-  }
+  record(true);
 }
 
 template <char OPA>

@@ -134,7 +134,8 @@ int dc2_synchronize(dc2_context* ctx);
  * 4 S^-1/2 projections, 5 integral blocks from factors, 0 other */
 int dc2_set_profiling(dc2_context* ctx, int32_t on);
 int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
-/* the same figures for the launches of one kernel instantiation k_zgemm<opA, opB> */
+/* the same figures for the launches of one MFMA kernel instantiation k_zgemm<opA, opB>
+ * (products routed to the kernels for skinny shapes are counted by dc2_get_gemm_stats only) */
 int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
 /* Numerical options (no counterpart in the reference):
  *   "eigensolver": -1 automatic (default: 4 for 3 <= n <= 1024 outside the batched server,

@@ -27,7 +27,12 @@ def _op(x, op):
 @pytest.mark.parametrize("opA", ["N", "T", "C"])
 @pytest.mark.parametrize("opB", ["N", "R", "T", "C"])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 97), (12, 200, 280),
-                                   (40, 24, 3001), (8, 3, 2500), (48, 48, 924)])   # the last three split K
+                                   (40, 24, 3001), (8, 3, 2500), (48, 48, 924),   # the last three split K
+                                   # the block-Davidson shapes: small outputs over a long K, a short K,
+                                   # at most 8 columns over a long K, and their edges
+                                   (48, 6, 924), (6, 6, 924), (924, 1, 924), (45, 7, 63), (2048, 1, 3),
+                                   (924, 6, 48), (924, 6, 64), (255, 8, 65), (924, 6, 924), (3000, 5, 4097),
+                                   (33, 9, 700)])
 def test_zgemm_ops(ctx, opA, opB, shape):
     from dirac_caspt2_amd.api import DeviceArray, zgemm
     from gpu_helpers import say
