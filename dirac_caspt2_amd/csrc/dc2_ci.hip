@@ -348,6 +348,68 @@ __global__ __launch_bounds__(256) void k_gauge(int n, const double* w, cplx* V, 
   }
 }
 
+// ds(k, kt) for exactly NE electrons with every index static, so the NE x NE minor stays in
+// registers: the same LU with partial pivoting, in the same order, as k_tracic_ds below.
+template <int NE>
+__global__ __launch_bounds__(128) void k_tracic_ds_reg(int na, int64_t ndet, const int64_t* dets,
+                                                       const cplx* __restrict__ Ua, cplx* ds) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t kt = blockIdx.y;
+  if (k >= ndet) return;
+  int ok[NE], on[NE];
+  {
+    uint64_t a = (uint64_t)dets[k], b = (uint64_t)dets[kt];
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {   // the r-th occupied orbital: lowest set bit, then clear it
+      ok[r] = __ffsll((unsigned long long)a) - 1;
+      on[r] = __ffsll((unsigned long long)b) - 1;
+      a &= a - 1;
+      b &= b - 1;
+    }
+  }
+  cplx A[NE][NE];
+#pragma unroll
+  for (int r = 0; r < NE; ++r)
+#pragma unroll
+    for (int c = 0; c < NE; ++c) A[r][c] = Ua[ok[r] + (int64_t)na * on[c]];
+  int swaps = 0;
+#pragma unroll
+  for (int c = 0; c < NE; ++c) {
+    int piv = c;
+    double best = fabs(A[c][c].x) + fabs(A[c][c].y);
+#pragma unroll
+    for (int r = c + 1; r < NE; ++r) {
+      const double v = fabs(A[r][c].x) + fabs(A[r][c].y);
+      if (v > best) { best = v; piv = r; }
+    }
+    if (piv != c) {
+      ++swaps;
+#pragma unroll
+      for (int r = c + 1; r < NE; ++r)
+        if (r == piv) {
+#pragma unroll
+          for (int q = 0; q < NE; ++q) { const cplx tmp = A[c][q]; A[c][q] = A[r][q]; A[r][q] = tmp; }
+        }
+    }
+    const cplx d = A[c][c];
+    const double den = cabs2(d);
+    if (den == 0.0) continue;
+    const cplx dinv = mk(d.x / den, -d.y / den);
+#pragma unroll
+    for (int r = c + 1; r < NE; ++r) {
+      const cplx l = cmul(A[r][c], dinv);
+#pragma unroll
+      for (int q = c + 1; q < NE; ++q) A[r][q] = csub(A[r][q], cmul(l, A[c][q]));
+    }
+  }
+  cplx det = mk(1.0, 0.0);
+#pragma unroll
+  for (int r = 0; r < NE; ++r) det = cmul(A[r][r], det);
+  if (swaps & 1) det = mk(-det.x, -det.y);
+  ds[k + ndet * kt] = det;
+}
+
+// any ne <= NE (the minor in private memory)
 template <int NE>
 __global__ void k_tracic_ds(int na, int ne, int64_t ndet, const int64_t* dets, const cplx* __restrict__ Ua,
                             cplx* ds) {
@@ -627,6 +689,21 @@ void canonical_gauge(hipStream_t s, int n, const double* w, cplx* V, cplx* scrat
 void tracic_ds(hipStream_t s, int na, int ne, int64_t ndet, const int64_t* cas_dets, const cplx* Uact, cplx* ds) {
   DC2_CHECK(ndet < 65536, 1, "tracic: more than 65535 determinants");
   dim3 g(cdiv(ndet, 128), (unsigned)ndet);
+  switch (ne) {   // exact sizes with the minor in registers
+    case 1: hipLaunchKernelGGL(k_tracic_ds_reg<1>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    case 2: hipLaunchKernelGGL(k_tracic_ds_reg<2>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    case 3: hipLaunchKernelGGL(k_tracic_ds_reg<3>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    case 4: hipLaunchKernelGGL(k_tracic_ds_reg<4>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    case 5: hipLaunchKernelGGL(k_tracic_ds_reg<5>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    case 6: hipLaunchKernelGGL(k_tracic_ds_reg<6>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    case 7: hipLaunchKernelGGL(k_tracic_ds_reg<7>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    case 8: hipLaunchKernelGGL(k_tracic_ds_reg<8>, g, dim3(128), 0, s, na, ndet, cas_dets, Uact, ds); break;
+    default: break;
+  }
+  if (ne >= 1 && ne <= 8) {
+    DC2_HIP(hipGetLastError());
+    return;
+  }
   if (ne <= 4) hipLaunchKernelGGL(k_tracic_ds<4>, g, dim3(128), 0, s, na, ne, ndet, cas_dets, Uact, ds);
   else if (ne <= 8) hipLaunchKernelGGL(k_tracic_ds<8>, g, dim3(128), 0, s, na, ne, ndet, cas_dets, Uact, ds);
   else if (ne <= 16) hipLaunchKernelGGL(k_tracic_ds<16>, g, dim3(128), 0, s, na, ne, ndet, cas_dets, Uact, ds);

@@ -256,7 +256,7 @@ __device__ inline cplx out_value(cplx acc, cplx alpha, cplx beta, const cplx* c)
 // op(A) = A^H or A^T, op(B) = B or conj(B) with a small output (M x N): both operands are read
 // along K, contiguous.  One wave per row m and up to 4 columns; the lanes stride over K and
 // their partial sums meet in a fixed butterfly, so the result does not depend on timing.
-constexpr int DOT_NB = 4;
+constexpr int DOT_NB = 4, DOT_U = 4;
 template <char OPA, char OPB>
 __global__ __launch_bounds__(256) void k_zgemm_dot(int64_t M, int64_t N, int64_t K, cplx alpha,
                                                    const cplx* __restrict__ A, int64_t lda,
@@ -273,16 +273,24 @@ __global__ __launch_bounds__(256) void k_zgemm_dot(int64_t M, int64_t N, int64_t
   for (int j = 0; j < DOT_NB; ++j) acc[j] = mk(0.0, 0.0);
   const cplx* a = A + lda * m;
   const cplx* b = B + ldb * n0;
-  for (int64_t k = lane; k < K; k += 64) {
-    cplx x = a[k];
-    if (OPA == 'C') x = cconj(x);
+  // DOT_U strides of K per round: all their loads are issued before the first multiply
+  for (int64_t k0 = lane; k0 < K; k0 += 64 * DOT_U) {
+    cplx x[DOT_U], y[DOT_U][DOT_NB];
 #pragma unroll
-    for (int j = 0; j < DOT_NB; ++j)
-      if (j < nn) {
-        cplx y = b[k + ldb * j];
-        if (OPB == 'R') y = cconj(y);
-        cfma(acc[j], x, y);
-      }
+    for (int u = 0; u < DOT_U; ++u) {
+      const int64_t k = k0 + 64 * u;
+      x[u] = k < K ? a[k] : mk(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < DOT_NB; ++j) y[u][j] = (k < K && j < nn) ? b[k + ldb * j] : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u) {
+      if (k0 + 64 * u >= K) break;
+      const cplx xv = OPA == 'C' ? cconj(x[u]) : x[u];
+#pragma unroll
+      for (int j = 0; j < DOT_NB; ++j)
+        if (j < nn) cfma(acc[j], xv, OPB == 'R' ? cconj(y[u][j]) : y[u][j]);
+    }
   }
 #pragma unroll
   for (int j = 0; j < DOT_NB; ++j) {
@@ -326,30 +334,43 @@ __global__ __launch_bounds__(256) void k_zgemm_smallk(int64_t M, int64_t N, int6
 // op(A) = A, op(B) = B or conj(B) with at most TS_NMAX columns and a long K: thread = row m,
 // all columns; blockIdx.y = K-split index, partial products to W[kz][n][m] (summed by
 // k_splitk_reduce in a fixed order)
-constexpr int TS_NMAX = 8;
+constexpr int TS_NMAX = 8, TS_U = 8, TS_KMAX = 64;
 template <char OPB>
 __global__ __launch_bounds__(256) void k_zgemm_tall(int64_t M, int64_t N, int64_t K, const cplx* __restrict__ A,
                                                     int64_t lda, const cplx* __restrict__ B, int64_t ldb,
                                                     cplx* __restrict__ W, int64_t kchunk) {
+  __shared__ cplx Bs[TS_KMAX][TS_NMAX];   // this split's rows of op(B), read as broadcasts
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t kz = blockIdx.y, k0 = kz * kchunk;
-  const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  const int kc = (int)((k0 + kchunk < K ? k0 + kchunk : K) - k0);   // <= TS_KMAX (host-checked)
+  for (int e = threadIdx.x; e < kc * N; e += blockDim.x) {
+    const int kk = e % kc, j = e / kc;
+    const cplx y = B[k0 + kk + ldb * j];
+    Bs[kk][j] = OPB == 'R' ? cconj(y) : y;
+  }
+  __syncthreads();
   if (m >= M) return;
   cplx acc[TS_NMAX];
 #pragma unroll
   for (int j = 0; j < TS_NMAX; ++j) acc[j] = mk(0.0, 0.0);
-  for (int64_t k = k0; k < k1; ++k) {
-    const cplx x = A[m + lda * k];
+  const cplx* a = A + m + lda * k0;
+  // TS_U rows of K per round: their loads of A are issued before the first multiply
+  for (int kb = 0; kb < kc; kb += TS_U) {
+    cplx x[TS_U];
 #pragma unroll
-    for (int j = 0; j < TS_NMAX; ++j)
-      if (j < N) {
-        cplx y = B[k + ldb * j];
-        if (OPB == 'R') y = cconj(y);
-        cfma(acc[j], x, y);
-      }
+    for (int u = 0; u < TS_U; ++u) x[u] = kb + u < kc ? a[lda * (kb + u)] : mk(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < TS_U; ++u) {
+      if (kb + u >= kc) break;
+#pragma unroll
+      for (int j = 0; j < TS_NMAX; ++j)
+        if (j < N) cfma(acc[j], x[u], Bs[kb + u][j]);
+    }
   }
   cplx* w = W + M * N * kz + m;
-  for (int j = 0; j < N; ++j) w[M * j] = acc[j];
+#pragma unroll
+  for (int j = 0; j < TS_NMAX; ++j)   // static indices: acc stays in registers
+    if (j < N) w[M * j] = acc[j];
 }
 
 // C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order)
@@ -359,7 +380,15 @@ __global__ void k_splitk_reduce(int64_t M, int64_t N, int sk, const cplx* __rest
   if (i >= M * N) return;
   const int64_t m = i % M, n = i / M;
   cplx acc = mk(0.0, 0.0);
-  for (int z = 0; z < sk; ++z) acc = cadd(acc, W[i + M * N * z]);
+  int z = 0;
+  for (; z + 8 <= sk; z += 8) {   // eight loads in flight, summed in order
+    cplx v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = W[i + M * N * (z + u)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = cadd(acc, v[u]);
+  }
+  for (; z < sk; ++z) acc = cadd(acc, W[i + M * N * z]);
   cplx out = cmul(alpha, acc);
   if (beta.x != 0.0 || beta.y != 0.0) cfma(out, beta, C[m + ldc * n]);
   C[m + ldc * n] = out;
@@ -409,8 +438,9 @@ bool launch_skinny(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, c
     }
     if (N > TS_NMAX || K <= 64 || M > 65536) return false;
     const int64_t rb = cdiv(M, 256);
-    const int64_t want = std::max<int64_t>(1, std::min<int64_t>({cdiv(256, rb), 32, cdiv(K, 16)}));
+    const int64_t want = std::max<int64_t>(cdiv(K, TS_KMAX), std::min<int64_t>({cdiv(256, rb), 32, cdiv(K, 16)}));
     const int64_t kchunk = cdiv(K, want), sk = cdiv(K, kchunk);
+    if (kchunk > TS_KMAX || sk >= 65536) return false;
     cplx* W = splitk_workspace(s, (size_t)sk * M * N);
     hipLaunchKernelGGL((k_zgemm_tall<OPB>), dim3((unsigned)rb, (unsigned)sk), dim3(256), 0, s, M, N, K, A, lda, B,
                        ldb, W, kchunk);
