@@ -316,9 +316,37 @@ enum class EigKind { general, rank_deficient };
 // divide-and-conquer variants returned NaN eigenvectors with info = 0 for
 // rank-deficient S matrices with hundreds of zero eigenvalues) falls back to the
 // tridiagonal QR algorithm on a saved copy.
+// Eigenpairs of the Hermitian matrix whose upper triangle A (n x n, device) holds, as the
+// device solvers read it, on the host (herm_eig): eigenvectors overwrite A, ascending
+// eigenvalues go to w.  Synchronises the stream.
+void host_heev_upper(hipStream_t s, int n, cplx* A, double* w) {
+  std::vector<zc> hA((size_t)n * n), hV((size_t)n * n);
+  std::vector<double> hw(n);
+  DC2_HIP(hipMemcpyAsync(hA.data(), A, hA.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < n; ++j) {
+    hA[j + (size_t)n * j] = hA[j + (size_t)n * j].real();
+    for (int i = 0; i < j; ++i) hA[j + (size_t)n * i] = std::conj(hA[i + (size_t)n * j]);
+  }
+  herm_eig(n, hA.data(), hw.data(), hV.data());
+  DC2_HIP(hipMemcpyAsync(A, hV.data(), hV.size() * sizeof(cplx), hipMemcpyHostToDevice, s));
+  DC2_HIP(hipMemcpyAsync(w, hw.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  DC2_HIP(hipStreamSynchronize(s));   // the host arrays are the sources of the copies
+}
+
+// matrices up to this size are diagonalised on the host by the automatic eigensolver choice:
+// a device solve is a chain of dependent launches (about 0.3-0.5 ms at any size), the host
+// solve of a 32 x 32 matrix takes about 0.1 ms
+constexpr int kHostEigMax = 32;
+
 void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what = "",
             EigKind kind = EigKind::general) {
   if (n <= 0) return;
+  if (c->eig_method < 0 && n <= kHostEigMax) {
+    host_heev_upper(L.stream, n, A, w);
+    check_finite(c, L.stream, what, A, (int64_t)n * n);
+    return;
+  }
   const int64_t nn = (int64_t)n * n;
   const bool native_ok = n >= 3 && n <= 1024;
   int method = c->eig_method >= 0 ? c->eig_method
@@ -1267,25 +1295,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       cplx* scr = L0.z("fd_scr", (int64_t)n * n);
       double* w = L0.d("fd_w", n);
       gather_block(s, n, dind, c->fock.get(), nmo, blk);
-      std::vector<zc> hA, hV;   // alive until the synchronisation at the end of the block
-      std::vector<double> hw;
-      if (n <= kHostFockEig) {
-        hA.resize((size_t)n * n);
-        hV.resize(hA.size());
-        hw.resize(n);
-        DC2_HIP(hipMemcpyAsync(hA.data(), blk, hA.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
-        DC2_HIP(hipStreamSynchronize(s));
-        for (int j = 0; j < n; ++j) {   // the upper triangle, as the device solver reads it
-          hA[j + (size_t)n * j] = hA[j + (size_t)n * j].real();
-          for (int i = 0; i < j; ++i) hA[j + (size_t)n * i] = std::conj(hA[i + (size_t)n * j]);
-        }
-        herm_eig(n, hA.data(), hw.data(), hV.data());
-        DC2_HIP(hipMemcpyAsync(blk, hV.data(), hV.size() * sizeof(cplx), hipMemcpyHostToDevice, s));
-        DC2_HIP(hipMemcpyAsync(w, hw.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-      } else {
-        zheevd(c, n, blk, w, "a Fock block");
-      }
-      canonical_gauge(s, n, w, blk, scr);
+      if (n <= kHostFockEig) host_heev_upper(s, n, blk, w);
+      else zheevd(c, n, blk, w, "a Fock block");
+      canonical_gauge(s, n, w, blk, scr, L0.i("fd_starts", n + 2));
       scatter_block(s, n, dind, blk, w, c->U.get(), nmo, c->eps.get());
       DC2_HIP(hipStreamSynchronize(s));
     }

@@ -292,17 +292,35 @@ __global__ void k_scatter_block(int n, const int* ind, const cplx* V, const doub
   if (i == 0) eps[ind[j]] = w[j];
 }
 
-// Canonical gauge of Hermitian eigenvectors; one workgroup (see the oracle's canonical_gauge)
-__global__ __launch_bounds__(256) void k_gauge(int n, const double* w, cplx* V, cplx* Ub) {
+// Canonical gauge of Hermitian eigenvectors (the oracle's canonical_gauge).  The groups of
+// (near-)degenerate eigenvalues: a group starts at i0 and takes every following j with
+// |w_j - w_i0| <= 1e-8 max(1, |w_i0|); starts[g] is the first column of group g, starts[ng] = n.
+__global__ void k_gauge_groups(int n, const double* w, int* starts) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int g = 0, i0 = 0;
+  while (i0 < n) {
+    starts[g++] = i0;
+    int i1 = i0 + 1;
+    while (i1 < n && fabs(w[i1] - w[i0]) <= 1.0e-8 * fmax(1.0, fabs(w[i0]))) ++i1;
+    i0 = i1;
+  }
+  starts[g] = n;
+  starts[n + 1] = g;
+}
+
+// One workgroup per group (blockIdx.x >= the group count: nothing to do); group g works in
+// the columns starts[g] .. starts[g+1]-1 of V and of the scratch Ub (n x n).
+__global__ __launch_bounds__(256) void k_gauge(int n, const int* starts, cplx* V, cplx* Ub) {
   __shared__ double dsh[4096];
   __shared__ double rmax[256];
   __shared__ int rmin[256];
   const int t = threadIdx.x;
-  int i0 = 0;
-  while (i0 < n) {
-    int i1 = i0 + 1;
-    while (i1 < n && fabs(w[i1] - w[i0]) <= 1.0e-8 * fmax(1.0, fabs(w[i0]))) ++i1;
+  const int g = blockIdx.x;
+  if (g >= starts[n + 1]) return;   // uniform over the workgroup
+  {
+    const int i0 = starts[g], i1 = starts[g + 1];
     const int m = i1 - i0;
+    Ub += (int64_t)n * i0;
     for (int k = 0; k < m; ++k) {
       double lmax = -1.0;
       for (int r = t; r < n; r += 256) {
@@ -343,8 +361,6 @@ __global__ __launch_bounds__(256) void k_gauge(int n, const double* w, cplx* V, 
     }
     for (int k = 0; k < m; ++k)
       for (int r = t; r < n; r += 256) V[r + (int64_t)n * (i0 + k)] = Ub[r + (int64_t)n * k];
-    __syncthreads();
-    i0 = i1;
   }
 }
 
@@ -680,9 +696,11 @@ void scatter_block(hipStream_t s, int n, const int* ind, const cplx* V, const do
   DC2_HIP(hipGetLastError());
 }
 
-void canonical_gauge(hipStream_t s, int n, const double* w, cplx* V, cplx* scratch) {
+void canonical_gauge(hipStream_t s, int n, const double* w, cplx* V, cplx* scratch, int* starts) {
   DC2_CHECK(n <= 4096, 1, "canonical_gauge: block larger than 4096");
-  hipLaunchKernelGGL(k_gauge, dim3(1), dim3(256), 0, s, n, w, V, scratch);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gauge_groups, dim3(1), dim3(64), 0, s, n, w, starts);
+  hipLaunchKernelGGL(k_gauge, dim3(n), dim3(256), 0, s, n, starts, V, scratch);
   DC2_HIP(hipGetLastError());
 }
 

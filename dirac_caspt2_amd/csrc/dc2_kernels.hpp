@@ -41,8 +41,9 @@ void fock_build(hipStream_t s, int ni, int na, int ns, const cplx* h, int ldh, c
 void gather_block(hipStream_t s, int n, const int* ind, const cplx* fock, int ldf, cplx* out);
 void scatter_block(hipStream_t s, int n, const int* ind, const cplx* V, const double* w, cplx* U, int ldu,
                    double* eps);
-// canonical eigenvector gauge (see canonical_gauge in the oracle); scratch >= n*n
-void canonical_gauge(hipStream_t s, int n, const double* w, cplx* V, cplx* scratch);
+// canonical eigenvector gauge (see canonical_gauge in the oracle); scratch >= n*n complex,
+// starts >= n + 2 ints
+void canonical_gauge(hipStream_t s, int n, const double* w, cplx* V, cplx* scratch, int* starts);
 // ds(k,kt) = det(Uact[occ_k, occ_kt]) with partial pivoting (trac.F90:105-131, sdet.F90:87-108)
 void tracic_ds(hipStream_t s, int na, int ne, int64_t ndet, const int64_t* cas_dets, const cplx* Uact, cplx* ds);
 // e0 = sum_w eps_w Re D1(w,w) (calce0.F90)
