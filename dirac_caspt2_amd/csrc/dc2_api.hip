@@ -443,8 +443,10 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   Lane& L = *c->lane0;
   hipStream_t s = L.stream;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0), mone = mk(-1.0, 0.0);
-  const int b = (int)std::min<int64_t>(n, std::max(nroot + 3, 6));
-  const int maxsub = (int)std::min<int64_t>(n, std::max(6 * b, 48));
+  // b = 4, maxsub = 24 for one root: measured fastest on the 924-determinant CASCI (20
+  // iterations, against 19 with 6 / 48, whose larger host eigenproblems cost more)
+  const int b = (int)std::min<int64_t>(n, std::max(nroot + 3, 4));
+  const int maxsub = (int)std::min<int64_t>(n, std::max(6 * b, 24));
   if (maxsub < 2 * b) return false;
   cplx* V = L.z("dv_V", n * maxsub);
   cplx* W = L.z("dv_W", n * maxsub);

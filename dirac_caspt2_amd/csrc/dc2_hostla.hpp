@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdint>
 #include <numeric>
 #include <vector>
 
@@ -93,6 +94,78 @@ inline void herm_jacobi(int n, zc* A, double* w, zc* V) {
   std::copy(Vs.begin(), Vs.end(), V);
 }
 
+// The implicit QL iteration with Wilkinson shifts (the tql2 scheme) on the real symmetric
+// tridiagonal (d, e), e[i] coupling i and i+1, without eigenvectors: d becomes the
+// eigenvalues (unsorted, e is destroyed) and every sweep's Givens rotations are recorded,
+// sweeps[3k .. 3k+2] = (m, istop, first): rotation i = m-1 .. istop of sweep k acts on
+// (i, i+1) with cs/sn[first + m-1-i], so that the eigenvectors are accumulated elsewhere
+// (on the host by herm_tql, on the device by the eigensolver).  e_m is negligible below eps
+// times the running max of |d_l| + |e_l|.  False if an eigenvalue needs more than 90 sweeps
+// or the rotations exceed max_rot.
+inline bool tql_rotations(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>& cs,
+                          std::vector<double>& sn, std::vector<int>& sweeps, size_t max_rot = SIZE_MAX) {
+  // sqrt(a^2 + b^2), the scaled std::hypot only where the squares could leave the range
+  auto hyp = [](double a, double b) {
+    const double m = std::max(std::fabs(a), std::fabs(b));
+    return (m < 1e150 && m > 1e-150) || m == 0.0 ? std::sqrt(a * a + b * b) : std::hypot(a, b);
+  };
+  cs.clear();
+  sn.clear();
+  sweeps.clear();
+  const double eps = 2.2e-16;
+  double tst1 = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0, m;
+    tst1 = std::max(tst1, std::fabs(d[l]) + std::fabs(e[l]));
+    do {
+      for (m = l; m + 1 < n; ++m)
+        if (std::fabs(e[m]) <= eps * tst1) break;
+      if (m != l) {
+        if (++iter > 90) return false;
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = hyp(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+        double s = 1.0, c = 1.0, pp = 0.0;
+        const int first = (int)cs.size();
+        int i;
+        bool underflow = false;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i], b = c * e[i];
+          r = hyp(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= pp;
+            e[m] = 0.0;
+            underflow = true;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - pp;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          pp = s * r;
+          d[i + 1] = g + pp;
+          g = c * r - b;
+          cs.push_back(c);
+          sn.push_back(s);
+        }
+        const int istop = underflow ? i + 1 : l;
+        if (istop < m) {
+          sweeps.push_back(m);
+          sweeps.push_back(istop);
+          sweeps.push_back(first);
+        }
+        if (cs.size() > max_rot) return false;
+        if (underflow) continue;
+        d[l] -= pp;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  return true;
+}
+
 // Hermitian eigendecomposition by Householder reduction to a real symmetric tridiagonal
 // matrix and the implicit QL iteration with Wilkinson shifts (the tql2 scheme), for the
 // small projected problems of the Davidson solver, where it needs about a fifth of the
@@ -157,56 +230,24 @@ inline bool herm_tql(int n, zc* A, double* w, zc* V) {
     e[k] = ae;
     delta[k + 1] = ae > 0.0 ? mul(delta[k], sub[k] / ae) : delta[k];
   }
-  // implicit QL on (d, e): e[i] couples i and i+1
+  // implicit QL on (d, e), its rotations accumulated into the real Z
+  std::vector<double> cs, sn;
+  std::vector<int> sweeps;
+  if (!tql_rotations(n, d, e, cs, sn, sweeps)) return false;
   std::vector<double> z((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) z[i + (size_t)n * i] = 1.0;
-  const double eps = 2.2e-16;
-  for (int l = 0; l < n; ++l) {
-    int iter = 0, m;
-    do {
-      for (m = l; m + 1 < n; ++m) {
-        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-        if (std::fabs(e[m]) <= eps * dd) break;
+  for (size_t k = 0; k < sweeps.size(); k += 3) {
+    const int m = sweeps[k], istop = sweeps[k + 1], first = sweeps[k + 2];
+    for (int i = m - 1; i >= istop; --i) {
+      const double c = cs[first + m - 1 - i], s = sn[first + m - 1 - i];
+      double* z0 = &z[(size_t)n * i];
+      double* z1 = &z[(size_t)n * (i + 1)];
+      for (int r = 0; r < n; ++r) {
+        const double zf = z1[r];
+        z1[r] = s * z0[r] + c * zf;
+        z0[r] = c * z0[r] - s * zf;
       }
-      if (m != l) {
-        if (++iter > 90) return false;
-        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = std::hypot(g, 1.0);
-        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
-        double s = 1.0, c = 1.0, pp = 0.0;
-        int i;
-        bool underflow = false;
-        for (i = m - 1; i >= l; --i) {
-          double f = s * e[i], b = c * e[i];
-          r = std::hypot(f, g);
-          e[i + 1] = r;
-          if (r == 0.0) {
-            d[i + 1] -= pp;
-            e[m] = 0.0;
-            underflow = true;
-            break;
-          }
-          s = f / r;
-          c = g / r;
-          g = d[i + 1] - pp;
-          r = (d[i] - g) * s + 2.0 * c * b;
-          pp = s * r;
-          d[i + 1] = g + pp;
-          g = c * r - b;
-          double* z0 = &z[(size_t)n * i];
-          double* z1 = &z[(size_t)n * (i + 1)];
-          for (int k = 0; k < n; ++k) {
-            const double zf = z1[k];
-            z1[k] = s * z0[k] + c * zf;
-            z0[k] = c * z0[k] - s * zf;
-          }
-        }
-        if (underflow) continue;
-        d[l] -= pp;
-        e[l] = g;
-        e[m] = 0.0;
-      }
-    } while (m != l);
+    }
   }
   // eigenvectors: X = Q D Z, Q = H_0 H_1 ... applied from the last reflector
   std::vector<int> idx(n);

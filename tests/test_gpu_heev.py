@@ -8,7 +8,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [3, 4, 17, 64, 65, 128, 129, 256, 300, 513, 792, 1024]
+SIZES = [3, 4, 17, 64, 65, 128, 129, 144, 164, 192, 193, 256, 300, 513, 792, 1024]
 
 
 def _matrix(kind, n, rng):

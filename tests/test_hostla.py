@@ -8,16 +8,16 @@ import pytest
 
 @pytest.mark.parametrize("method", ["auto", "jacobi", "ql"])
 @pytest.mark.parametrize("kind", ["random", "degenerate", "rank_deficient", "tiny", "one", "tridiagonal", "diagonal",
-                                  "davidson48"])
+                                  "davidson48", "gram96"])
 def test_host_heev(kind, method):
     from dirac_caspt2_amd.api import host_heev
     rng = np.random.default_rng(len(kind))
-    n = {"tiny": 2, "one": 1, "davidson48": 48}.get(kind, 37)
+    n = {"tiny": 2, "one": 1, "davidson48": 48, "gram96": 96}.get(kind, 37)
     Z = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     Q, _ = np.linalg.qr(Z)
     if kind == "degenerate":
         lam = np.repeat(rng.normal(size=(n + 1) // 2), 2)[:n]
-    elif kind == "rank_deficient":
+    elif kind in ("rank_deficient", "gram96"):   # gram96: a metric with two thirds null space
         lam = np.concatenate([rng.uniform(0.1, 1.0, n // 3), np.zeros(n - n // 3)])
     elif kind == "davidson48":   # Rayleigh quotient of a CI matrix: clustered low end near -40
         lam = np.concatenate([-40.0 + np.repeat(rng.uniform(0, 0.01, 3), 2), -39.0 + rng.uniform(0, 5, n - 6)])
