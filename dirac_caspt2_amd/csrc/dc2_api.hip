@@ -1187,8 +1187,8 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1o.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);
   check_finite(c, "the CASCI 1-RDM", c->G1.get(), na2);
 
-  // Subspaces: A and C (the large metrics) on their own lanes, D and F on a third, the rest
-  // on the library stream.  Without RAS limits the A/C lanes start now with the
+  // Subspaces: A and C (the large metrics) on their own lanes, D on a third, the rest on the
+  // library stream.  Without RAS limits the A/C lanes start now with the
   // eigendecompositions of their determinant-space metrics, which need only E_uv|0> of the
   // CASCI state in the original orbitals (the metric is the same operator in every orbital
   // basis of the active space), and carry the eigenvectors to the canonical orbitals once
@@ -1198,14 +1198,15 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   const bool doD = ni > 0 && ns > 0 && first <= 3, doE = ni > 0 && ns > 0 && first <= 4;
   const bool doF = ns > 0 && first <= 5, doG = ni > 0 && ns > 0 && first <= 6, doH = ni > 0 && ns > 0 && first <= 7;
   const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
-  // lanes: 1 all on the library stream; 2: A and C on lane 1; 3: A on 1, C on 2; 4: D and F
-  // (the next largest) on lane 3 as well
+  // lanes: 1 all on the library stream; 2: A and C on lane 1; 3: A on 1, C on 2; 4: D (the
+  // next largest) on lane 3 as well.  F stays with B, E, G and H: on lane 3 after D it made
+  // that lane the longest (28.7 against 26.4 ms per energy at 192 spinors)
   std::vector<std::vector<int>> plan(std::max(1, std::min(c->nlanes, 4)));
   for (int k = 0; k < 8; ++k) {
     if (!do_sub[k]) continue;
     int lane = 0;
     if (plan.size() >= 2 && (k == 0 || k == 2)) lane = k == 0 ? 1 : std::min<int>(2, (int)plan.size() - 1);
-    if (plan.size() >= 4 && (k == 3 || k == 5)) lane = 3;
+    if (plan.size() >= 4 && k == 3) lane = 3;
     plan[lane].push_back(k);
   }
   const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;

@@ -149,7 +149,7 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  C metrics, 3 for A the formula matrices on the tuples t < u (a+_t a+_u is
  *                  antisymmetric, so these span the same space and S's eigenvalues halve);
  *   "lanes":       1..4 concurrent streams (each with a host thread) for the subspace
- *                  solves (default 4: A, C, D+F and the rest; 3: A, C and the rest);
+ *                  solves (default 4: A, C, D and the rest; 3: A, C and the rest);
  *   "schur":       1 (default) E2 of the determinant-space route through the compressed
  *                  resolvent when at most 32 directions are cut, 0 always through the
  *                  eigendecomposition of B~;
