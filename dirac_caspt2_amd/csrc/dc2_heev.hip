@@ -359,6 +359,73 @@ __global__ __launch_bounds__(512) void k_hetrd_back(int n, const double* __restr
   }
 }
 
+// Blocked back-transformation (n >= BT_BLOCKED_MIN): the reflectors k = a .. a+nb-1 of panel
+// p (a = p nb) multiply as H_a ... H_{a+nb-1} = I - V T V^H (compact WY, T upper triangular:
+// T_jj = tau_{a+j}, T[0:j, j] = -tau_{a+j} T[0:j, 0:j] V[:, 0:j]^H v_j), so X = Q D Z is
+// D Z with the panels applied from the last to the first as X <- X - (V T) (V^H X), three
+// matrix products per panel on the MFMA kernel.
+constexpr int BT_NB = 64, BT_BLOCKED_MIN = 256;
+
+// X = D Z (X complex, lda; Z real n x n)
+__global__ void k_dz(int n, const double* __restrict__ Z, const cplx* __restrict__ delta, cplx* __restrict__ X,
+                     int64_t ldx) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  const int i = (int)(t % n), c = (int)(t / n);
+  X[i + ldx * c] = cscale(delta[i], Z[t]);
+}
+
+// T of every panel from its Gram matrix G = V^H V (nb x nb per panel, column-major) and tau;
+// one workgroup of BT_NB threads per panel, thread i owns row i of T.  Reflectors past
+// nrefl (and those with tau = 0) give zero columns.
+__global__ __launch_bounds__(BT_NB) void k_wy_t(int nrefl, const double* __restrict__ tau,
+                                                const cplx* __restrict__ G, cplx* __restrict__ T) {
+  __shared__ cplx tcol[BT_NB];
+  const int p = blockIdx.x, i = threadIdx.x, a = p * BT_NB;
+  const cplx* g = G + (int64_t)BT_NB * BT_NB * p;
+  cplx* t = T + (int64_t)BT_NB * BT_NB * p;
+  for (int j = 0; j < BT_NB; ++j) {
+    const double tj = a + j < nrefl ? tau[a + j] : 0.0;
+    // T[i][j] = -tau_j sum_{l=i}^{j-1} T[i][l] G[l][j] (i < j), tau_j (i = j), 0 (i > j)
+    cplx v = mk(0.0, 0.0);
+    if (i < j && tj != 0.0) {
+      cplx acc = mk(0.0, 0.0);
+      for (int l = i; l < j; ++l) cfma(acc, t[i + BT_NB * l], g[l + BT_NB * j]);
+      v = mk(-tj * acc.x, -tj * acc.y);
+    } else if (i == j) {
+      v = mk(tj, 0.0);
+    }
+    tcol[i] = v;
+    __syncthreads();
+    t[i + BT_NB * j] = tcol[i];
+    __syncthreads();
+  }
+}
+
+// X = Q D Z by panels; work: G and T (nb^2 per panel), VT (n x nb per panel), W (nb x n)
+void back_blocked(hipStream_t s, int n, const double* Z, const cplx* delta, const double* tau, const cplx* Vh,
+                  cplx* X, int64_t ldx, cplx* G, cplx* T, cplx* VT, cplx* W) {
+  const int nrefl = n - 2, np = (int)cdiv(nrefl, BT_NB);
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0), mone = mk(-1.0, 0.0);
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(k_dz, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, Z, delta, X, ldx);
+  // every panel's G = V^H V and V T in batched products (the columns of Vh past the last
+  // reflector are zero)
+  zgemm(s, 'C', 'N', BT_NB, BT_NB, n, one, Vh, n, (int64_t)n * BT_NB, Vh, n, (int64_t)n * BT_NB, zero, G, BT_NB,
+        (int64_t)BT_NB * BT_NB, np);
+  hipLaunchKernelGGL(k_wy_t, dim3(np), dim3(BT_NB), 0, s, nrefl, tau, G, T);
+  zgemm(s, 'N', 'N', n, BT_NB, BT_NB, one, Vh, n, (int64_t)n * BT_NB, T, BT_NB, (int64_t)BT_NB * BT_NB, zero, VT, n,
+        (int64_t)n * BT_NB, np);
+  for (int p = np - 1; p >= 0; --p) {
+    const int a = p * BT_NB, nb = std::min(BT_NB, nrefl - a), rows = n - a - 1;
+    // rows a+1 .. n-1 are the only ones V touches
+    zgemm(s, 'C', 'N', nb, n, rows, one, Vh + (a + 1) + (int64_t)n * a, n, 0, X + (a + 1), ldx, 0, zero, W, nb, 0,
+          1);
+    zgemm(s, 'N', 'N', rows, n, nb, mone, VT + (a + 1) + (int64_t)n * a, n, 0, W, nb, 0, one, X + (a + 1), ldx, 0,
+          1);
+  }
+}
+
 // The workgroups of one k_hetrd wait for each other, so all of them must be resident at
 // once.  Each one fits a compute unit on its own (one workgroup of 8 waves at <= 256 VGPRs
 // and 49 KB LDS), and the reductions in flight in this process hold at most one workgroup
@@ -420,10 +487,21 @@ void hetrd_stats_reset() {
   g_hetrd_time.clear();
 }
 
+// the blocked back-transformation's G, T, VT and W
+int64_t back_blocked_bytes(int n) {
+  const int64_t np = cdiv(n - 2, BT_NB);
+  return 16 * (2 * np * BT_NB * BT_NB + np * BT_NB * (int64_t)n + (int64_t)BT_NB * n) + 4 * 256;
+}
+
+// columns of the reflector array: whole panels of the blocked back-transformation
+int64_t vh_cols(int n) { return n >= BT_BLOCKED_MIN ? std::max<int64_t>(n, cdiv(n - 2, BT_NB) * BT_NB) : n; }
+
 int64_t heev_native_work_bytes(int n) {
   const int64_t nn = (int64_t)n * n;
   const int nwg = (int)cdiv(n, HT_COLS);
-  return nn * 16 + nn * 8 + (int64_t)n * (16 * 6 + 8 * 3) + 16 * nwg + 16 + 10 * 256;   // 10 pieces, 256-aligned
+  int64_t b = (int64_t)n * vh_cols(n) * 16 + nn * 8 + (int64_t)n * (16 * 6 + 8 * 3) + 16 * nwg + 16 + 10 * 256;
+  if (n >= BT_BLOCKED_MIN) b += back_blocked_bytes(n);
+  return b;
 }
 
 bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev) {
@@ -436,7 +514,7 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     p += (bytes + 255) / 256 * 256;
     return q;
   };
-  cplx* Vh = reinterpret_cast<cplx*>(take(nn * 16));
+  cplx* Vh = reinterpret_cast<cplx*>(take((int64_t)n * vh_cols(n) * 16));
   double* Z = reinterpret_cast<double*>(take(nn * 8));
   cplx* sub = reinterpret_cast<cplx*>(take(n * 16));
   cplx* delta = reinterpret_cast<cplx*>(take(n * 16));
@@ -448,6 +526,9 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
   unsigned* flags = reinterpret_cast<unsigned*>(take(16));
   HetrdWork hw{P, NC, S, flags, reinterpret_cast<int*>(flags + 1)};
   DC2_HIP(hipMemsetAsync(flags, 0, 16, s));
+  // reflectors 0 .. n-2 are written; the columns after them take part in the blocked
+  // back-transformation's batched products (whole panels), as zeros
+  DC2_HIP(hipMemsetAsync(Vh + (int64_t)n * (n - 1), 0, (vh_cols(n) - n + 1) * n * sizeof(cplx), s));
   hipLaunchKernelGGL(k_herm_from_upper, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, A, lda);
   DC2_HIP(hipGetLastError());
   int dev = 0, cus = 0;
@@ -486,6 +567,16 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
   DC2_HIP(hipGetLastError());
   const rocblas_status rs = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, e, Z, n, info_dev);
   if (rs != rocblas_status_success) return false;
+  if (n >= BT_BLOCKED_MIN) {
+    const int64_t np = cdiv(n - 2, BT_NB);
+    cplx* G = reinterpret_cast<cplx*>(take(16 * np * BT_NB * BT_NB));
+    cplx* T = reinterpret_cast<cplx*>(take(16 * np * BT_NB * BT_NB));
+    cplx* VT = reinterpret_cast<cplx*>(take(16 * np * BT_NB * (int64_t)n));
+    cplx* W = reinterpret_cast<cplx*>(take(16 * (int64_t)BT_NB * n));
+    back_blocked(s, n, Z, delta, tau, Vh, A, lda, G, T, VT, W);
+    DC2_HIP(hipGetLastError());
+    return true;
+  }
   if (n <= 128) launch_back<2>(s, n, Z, delta, tau, Vh, A, lda);
   else if (n <= 256) launch_back<4>(s, n, Z, delta, tau, Vh, A, lda);
   else if (n <= 512) launch_back<8>(s, n, Z, delta, tau, Vh, A, lda);
