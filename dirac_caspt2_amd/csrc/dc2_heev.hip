@@ -376,30 +376,32 @@ __global__ void k_dz(int n, const double* __restrict__ Z, const cplx* __restrict
 }
 
 // T of every panel from its Gram matrix G = V^H V (nb x nb per panel, column-major) and tau;
-// one workgroup of BT_NB threads per panel, thread i owns row i of T.  Reflectors past
-// nrefl (and those with tau = 0) give zero columns.
+// one workgroup of BT_NB threads per panel with G and T in LDS, thread i owns row i of T.
+// Reflectors past nrefl (and those with tau = 0) give zero columns.
 __global__ __launch_bounds__(BT_NB) void k_wy_t(int nrefl, const double* __restrict__ tau,
                                                 const cplx* __restrict__ G, cplx* __restrict__ T) {
-  __shared__ cplx tcol[BT_NB];
+  __shared__ cplx gs[BT_NB][BT_NB + 1], ts[BT_NB][BT_NB + 1];   // [column][row]
   const int p = blockIdx.x, i = threadIdx.x, a = p * BT_NB;
   const cplx* g = G + (int64_t)BT_NB * BT_NB * p;
-  cplx* t = T + (int64_t)BT_NB * BT_NB * p;
+  for (int j = 0; j < BT_NB; ++j) gs[j][i] = g[i + BT_NB * j];
+  __syncthreads();
   for (int j = 0; j < BT_NB; ++j) {
     const double tj = a + j < nrefl ? tau[a + j] : 0.0;
-    // T[i][j] = -tau_j sum_{l=i}^{j-1} T[i][l] G[l][j] (i < j), tau_j (i = j), 0 (i > j)
+    // T[i][j] = -tau_j sum_{l=i}^{j-1} T[i][l] G[l][j] (i < j), tau_j (i = j), 0 (i > j);
+    // column j reads only columns l < j, written in earlier steps
     cplx v = mk(0.0, 0.0);
     if (i < j && tj != 0.0) {
       cplx acc = mk(0.0, 0.0);
-      for (int l = i; l < j; ++l) cfma(acc, t[i + BT_NB * l], g[l + BT_NB * j]);
+      for (int l = i; l < j; ++l) cfma(acc, ts[l][i], gs[j][l]);
       v = mk(-tj * acc.x, -tj * acc.y);
     } else if (i == j) {
       v = mk(tj, 0.0);
     }
-    tcol[i] = v;
-    __syncthreads();
-    t[i + BT_NB * j] = tcol[i];
+    ts[j][i] = v;
     __syncthreads();
   }
+  cplx* t = T + (int64_t)BT_NB * BT_NB * p;
+  for (int j = 0; j < BT_NB; ++j) t[i + BT_NB * j] = ts[j][i];
 }
 
 // X = Q D Z by panels; work: G and T (nb^2 per panel), VT (n x nb per panel), W (nb x n)
