@@ -538,9 +538,18 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
   DC2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   if (nwg > cus) return false;
   hipError_t st;
-  thread_local hipEvent_t ev[2] = {nullptr, nullptr};
-  for (auto& e : ev)
-    if (!e) DC2_HIP(hipEventCreate(&e));
+  // timing events of this thread, destroyed when it exits (lane threads live for one call)
+  struct ThreadEvents {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~ThreadEvents() {
+      for (auto x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  };
+  thread_local ThreadEvents tev;
+  hipEvent_t* ev = tev.e;
+  for (int q = 0; q < 2; ++q)
+    if (!ev[q]) DC2_HIP(hipEventCreate(&ev[q]));
   {
     CoopSlot slot(nwg, cus);
     DC2_HIP(hipEventRecord(ev[0], s));

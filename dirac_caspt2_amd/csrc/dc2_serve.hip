@@ -4,10 +4,13 @@
 // integral blocks are formed on the device and stay resident in HBM, shared
 // read-only by every request on that molecule (points of a potential-energy scan
 // differ in their one-electron matrix).  Each of nslots slots owns a dc2_context
-// (its streams, rocBLAS handles and grow-only scratch) and a host thread.  A free
-// slot takes the oldest pending request as soon as the device-memory footprint of
-// the running requests plus its own fits the budget, so requests that arrive while
-// others run are admitted as slots and memory free up, and their kernels overlap
+// (its streams, rocBLAS handles and grow-only scratch) and a host thread.  A slot's
+// context keeps its scratch after a request, so admission counts the memory every slot
+// holds (the largest footprint it has run): a free slot takes the oldest pending request as
+// soon as the held memory plus the growth that request needs in this slot fits the budget.
+// When nothing runs and the oldest request still does not fit, idle slots release their
+// memory (each recreates its context).  Requests that arrive while others run are admitted
+// as slots and memory free up, and their kernels overlap
 // on the device.  No counterpart in the reference (its driver runs one problem per
 // process, dcaspt2_run_subprograms.F90:57-62).
 #include <hip/hip_runtime.h>
@@ -204,7 +207,10 @@ int64_t binom(int n, int k) {
 
 // device bytes one request holds while it runs (besides the molecule's resident blocks):
 // transformed blocks + transform scratch, CI matrix and LU copy, E_pq tables and images,
-// density tensors, the A/C metric work and the V matrices
+// density tensors, the A/C metric work and the V matrices, doubled to cover what the sum
+// leaves out (lane scratch, split-precision slices, solver workspaces): measured, one
+// context with four lanes held 2.2 GB at 192 spinors (sum 1.3 GB) and 8.8 GB at 324
+// spinors (sum 7.7 GB)
 int64_t footprint_bytes(int64_t ni, int64_t na, int64_t ns, int64_t ne) {
   const int64_t z = 16;
   const int64_t blocks[] = {na * ni * na * na, na * ni * ni * ni, na * ni * na * ni, ns * na * na * na,
@@ -220,7 +226,7 @@ int64_t footprint_bytes(int64_t ni, int64_t na, int64_t ns, int64_t ne) {
   const int64_t ac = 2 * (std::min(mA, na3) * na3 + 2 * std::min(mA, na3) * std::min(mA, na3)) +
                      na3 * (ni + ns) * 3;
   const int64_t other = 8 * (ns + ni) * (ns + ni) + 4 * ns * ns * ni + 4 * ns * ni * ni * na;
-  return z * (tb + mx + ci + dens + ac + other) + (64ll << 20);
+  return 2 * (z * (tb + mx + ci + dens + ac + other) + (64ll << 20));
 }
 
 }  // namespace
@@ -228,8 +234,9 @@ int64_t footprint_bytes(int64_t ni, int64_t na, int64_t ns, int64_t ne) {
 struct dc2_server {
   int32_t device = 0, nslots = 1, lanes = 1;
   std::unique_ptr<dc2::EigBatcher> batcher;   // cross-request eigensolve batching (null: off)
-  int64_t budget = 0, in_use = 0, peak = 0, completed = 0;
+  int64_t budget = 0, in_use = 0, peak = 0, completed = 0;   // in_use: memory the slots hold
   int running = 0;
+  std::vector<int64_t> held;   // per slot: the largest footprint it has run (its retained scratch)
   bool stopping = false;
   int64_t next_id = 1;
   std::vector<std::unique_ptr<Molecule>> mols;
@@ -282,32 +289,69 @@ void run_request(dc2_server* S, int k, Request* r) {
   if (st) r->err = dc2_last_error();
 }
 
+// a fresh context for slot k (its scratch released); the slot's thread only
+int renew_context(dc2_server* S, int k) {
+  dc2_context_destroy(S->slot_ctx[k]);
+  S->slot_ctx[k] = nullptr;
+  S->slot_mol[k] = nullptr;
+  dc2_context* c = nullptr;
+  int st = dc2_context_create(S->device, &c);
+  if (!st) st = dc2_set_option(c, "lanes", S->lanes);
+  if (st) {
+    if (c) dc2_context_destroy(c);
+    return st;
+  }
+  if (S->batcher) dc2::set_batcher(c, S->batcher.get());
+  S->slot_ctx[k] = c;
+  return 0;
+}
+
 void slot_main(dc2_server* S, int k) {
   (void)hipSetDevice(S->device);
   for (;;) {
     Request* r = nullptr;
+    bool trim = false;
     {
       std::unique_lock<std::mutex> lk(S->mu);
+      auto growth = [&] { return std::max<int64_t>(0, S->pending.front()->footprint - S->held[k]); };
       S->cv_work.wait(lk, [&] {
-        if (!S->pending.empty() && (S->running == 0 || S->in_use + S->pending.front()->footprint <= S->budget))
-          return true;
-        return S->stopping && S->pending.empty();
+        if (!S->pending.empty()) {
+          if (S->in_use + growth() <= S->budget) return true;
+          return S->running == 0 && S->held[k] > 0;   // stuck: release this slot's memory
+        }
+        return S->stopping;
       });
       if (S->pending.empty()) return;   // stopping and drained
-      r = S->pending.front();
-      S->pending.pop_front();
-      S->in_use += r->footprint;
-      S->peak = std::max(S->peak, S->in_use);
-      ++S->running;
-      r->t_start = S->now();
-      r->slot = k;
+      const int64_t g = growth();
+      if (S->in_use + g <= S->budget) {
+        r = S->pending.front();
+        S->pending.pop_front();
+        S->in_use += g;
+        S->held[k] = std::max(S->held[k], r->footprint);
+        S->peak = std::max(S->peak, S->in_use);
+        ++S->running;
+        r->t_start = S->now();
+        r->slot = k;
+      } else {
+        trim = true;
+        S->in_use -= S->held[k];
+        S->held[k] = 0;
+      }
+    }
+    if (trim) {
+      const int st = S->slot_ctx[k] ? renew_context(S, k) : 0;
+      if (st || !S->slot_ctx[k]) {   // no context: this slot stops taking requests
+        S->cv_work.notify_all();
+        return;
+      }
+      S->cv_work.notify_all();
+      continue;
     }
     if (S->batcher) S->batcher->enter();
     run_request(S, k, r);
     if (S->batcher) S->batcher->leave();
     {
       std::lock_guard<std::mutex> lk(S->mu);
-      S->in_use -= r->footprint;
       --S->running;
       ++S->completed;
       r->t_end = S->now();
@@ -362,6 +406,7 @@ int dc2_server_create_batched(int32_t device, int32_t nslots, int64_t mem_budget
     if (S->batcher) dc2::set_batcher(c, S->batcher.get());
     S->slot_ctx.push_back(c);
     S->slot_mol.push_back(nullptr);
+    S->held.push_back(0);
   }
   dc2_server* raw = S.get();
   for (int k = 0; k < nslots; ++k) S->threads.emplace_back(slot_main, raw, k);
@@ -437,7 +482,7 @@ int dc2_server_submit(dc2_server* S, int32_t mol_id, const double* h, double eco
     *req_id = r->id;
     S->pending.push_back(r.release());
   }
-  S->cv_work.notify_one();
+  S->cv_work.notify_all();   // every free slot re-checks (the one that fits may not be the first)
   return 0;
 }
 

@@ -230,10 +230,12 @@ int dc2_get_integral_block_device(dc2_context* ctx, const char* name, const doub
  * as dc2_set_integral_factors; the blocks stay resident in HBM and are shared read-only
  * by all requests on the molecule).  A request is one CASCI + CASPT2 energy for a
  * one-electron matrix h (nmo x nmo complex, copied at submit), root and eshift, with
- * an optional RAS restriction ras[7] (the arguments of dc2_set_ras, or NULL).  A free
- * slot admits the oldest pending request once the summed device footprint of the running
- * requests (dc2_request_footprint) plus its own fits mem_budget bytes; a request larger
- * than the budget is refused at submit.  flags: 0 FP64 integral transforms, 3..8 split-
+ * an optional RAS restriction ras[7] (the arguments of dc2_set_ras, or NULL).  A slot's
+ * context keeps its scratch between requests, so each slot holds the largest footprint
+ * (dc2_request_footprint) it has run; a free slot admits the oldest pending request once
+ * the memory all slots hold plus the growth the request needs in this slot fits mem_budget
+ * bytes, and when nothing runs and it still does not fit, idle slots release their memory.
+ * A request larger than the budget is refused at submit.  flags: 0 FP64 integral transforms, 3..8 split-
  * precision transforms with that many slices (the option "split_slices"). */
 typedef struct dc2_server dc2_server;
 int dc2_request_footprint(int32_t ninact, int32_t nact, int32_t nsec, int32_t nelec, int64_t* bytes);
@@ -256,7 +258,7 @@ int dc2_server_submit(dc2_server* srv, int32_t mol_id, const double* h, double e
  * out[22..24] submit/start/end times (s since server creation), out[25] slot;
  * *req_status is the request's status (its message via dc2_last_error) */
 int dc2_server_wait(dc2_server* srv, double timeout_s, int64_t* req_id, int32_t* req_status, double* out);
-/* out[7] = {pending, running, completed, footprint bytes in use, peak footprint bytes,
+/* out[7] = {pending, running, completed, footprint bytes the slots hold, its peak,
  *           eigensolve batches, eigenproblems solved in them} */
 int dc2_server_stats(dc2_server* srv, int64_t* out);
 

@@ -124,3 +124,42 @@ def test_served_with_batched_eigensolves(mix):
         for X in "ABCDEFGH":
             assert abs(alone[k].e2[X] - got[k].e2[X]) < 1e-9, (k, X)
         assert abs(alone[k].total - got[k].total) < 1e-9, k
+
+
+def test_admission_counts_memory_slots_hold(mix):
+    """Slots keep their scratch between requests, so admission counts what every slot holds:
+    with a budget of one large request plus a little, two slots first run two small
+    requests each (each slot then holds a small footprint), then a large request that fits
+    only after an idle slot releases its memory.  The held memory never exceeds the budget
+    and every energy equals the request served alone (1e-10 Eh)."""
+    from dirac_caspt2_amd.api import Server, request_footprint
+    from gpu_helpers import say
+    mols, reqs = mix
+    alone = SHARED.get("alone")
+    if alone is None:
+        pytest.skip("needs test_served_alone_equals_mixed")
+    fp = [request_footprint(s.ninact, s.nact, s.nsec, s.nelec) for s, _, _, _ in mols]
+    big = int(np.argmax(fp))
+    small = int(np.argmin(fp))
+    assert fp[small] < fp[big]
+    budget = fp[big] + fp[small] // 2        # one large request, not a large and a small one
+    ks = [k for k, r in enumerate(reqs) if r[0] == small][:2]
+    kb = [k for k, r in enumerate(reqs) if r[0] == big][:1]
+    assert ks and kb
+    with Server(0, nslots=2, mem_budget=budget, lanes_per_slot=1) as srv:
+        ids = [srv.add_molecule((s.ninact, s.nact, s.nsec, s.nelec), sp.B) for s, sp, _, _ in mols]
+        got = {}
+        for order in (ks + ks, kb):   # the small requests on both slots, then the large one
+            sub = {_submit(srv, ids, mols, reqs[k]): k for k in order}
+            while sub:
+                res = srv.wait(60.0)
+                assert res is not None, "timed out waiting for a served result"
+                assert res.status == 0, res.error
+                got.setdefault(sub.pop(res.id), res)
+        st = srv.stats()
+    say("serve admission: stats", st, "budget", budget, "footprints", fp)
+    assert st["peak"] <= budget
+    for k, res in got.items():
+        for X in "ABCDEFGH":
+            assert abs(alone[k].e2[X] - res.e2[X]) < 1e-10, (k, X)
+        assert abs(alone[k].total - res.total) < 1e-10, k
