@@ -351,8 +351,14 @@ def main():
     ap.add_argument("--slot-lanes", type=int, default=1, help="subspace lanes per slot in the served run")
     ap.add_argument("--split", type=int, default=7, help="INT8 slices of the split-precision transforms (0: off)")
     ap.add_argument("--eig-batch", type=int, default=64, help="eigensolves batched across served requests (<=1: off)")
-    ap.add_argument("--eig-window", type=float, default=0.003, help="batching window of the served eigensolves (s)")
+    ap.add_argument("--eig-window", type=float, default=None,
+                    help="batching window of the served eigensolves (s); groups are mostly flushed earlier, when "
+                         "every running request waits for one.  Default 0.1 for the runs that submit all requests "
+                         "at once (0.003: 72, 0.03: 79, 0.1-0.25: 83-85 energies/s at 64 slots), 0.003 for the "
+                         "Poisson stream, whose p50/p99 latency at 15 requests/s it keeps at 161/609 ms (0.1: 361/929)")
     args = ap.parse_args()
+    if args.eig_window is None:
+        args.eig_window = 0.003 if args.workload == "serve" else 0.1
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(the sharding design is documented in DESIGN.md)", file=sys.stderr)
