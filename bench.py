@@ -341,6 +341,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serve workload: Poisson arrival rate (1/s)")
     ap.add_argument("--molecules", type=int, default=12, help="serve workload: resident molecules")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the GEMM autotuning of the warm-up")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run 3 device-to-device copies of MIB MiB (a plain copy kernel "
                          "of known size, the reference for the HBM counters)")
@@ -383,6 +384,13 @@ def main():
         ctx.casci(spec.selectroot)
         return ctx.caspt2(spec.selectroot, spec.eshift)
 
+    tuned = None
+    if not args.no_autotune:   # untimed: the GEMM shapes of one step, each tuned on its own
+        from dirac_caspt2_amd import autotune
+        at = autotune.tune_workload(ctx, step)
+        tuned = {k: at[k] for k in ("shapes", "changed", "builtin_us", "tuned_us")}
+        log(f"autotune: {at['shapes']} GEMM shapes, {at['changed']} with a tuned choice, "
+            f"{at['builtin_us']:.0f} -> {at['tuned_us']:.0f} us per set of launches")
     for _ in range(args.warmup):
         out = step()
     ctx.synchronize()
@@ -459,6 +467,7 @@ def main():
         "energies_last_step": {"total": out.total, "e2": out.e2},
         "roofline": roof,
         "eig_fallbacks": ctx.eig_fallbacks(),
+        "gemm_autotune": tuned,
         "casci_ndet_roots_davidson_iterations": ctx.casci_info(),
     }
     if args.split:

@@ -293,6 +293,25 @@ class Caspt2Context:
         check("dc2_get_gemm_op_stats", self.lib.dc2_get_gemm_op_stats(self.h, opA.encode(), opB.encode(), _ptr(out)))
         return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
 
+    def gemm_shapes(self):
+        """Distinct shapes (opA, opB, M, N, K, batch) of the MFMA GEMM launches recorded while
+        profiling was on."""
+        n = ctypes.c_int64()
+        check("dc2_get_gemm_shapes", self.lib.dc2_get_gemm_shapes(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(6 * max(n.value, 1), dtype=np.int64)
+        check("dc2_get_gemm_shapes", self.lib.dc2_get_gemm_shapes(self.h, _ptr(out), n.value, ctypes.byref(n)))
+        return [(chr(r[0]), chr(r[1]), int(r[2]), int(r[3]), int(r[4]), int(r[5]))
+                for r in out[:6 * n.value].reshape(-1, 6)]
+
+    def gemm_tune(self, opA: str, opB: str, M: int, N: int, K: int, batch: int = 1, reps: int = 3):
+        """Time every kernel variant / split-K candidate for this shape and install the fastest
+        (dc2_gemm_tune): {variant, split (0, 0: built-in kept), seconds, builtin_seconds}."""
+        out = np.zeros(4)
+        check("dc2_gemm_tune", self.lib.dc2_gemm_tune(self.h, opA.encode(), opB.encode(), M, N, K, batch, reps,
+                                                      _ptr(out)))
+        return {"variant": int(out[0]), "split": int(out[1]), "seconds": float(out[2]),
+                "builtin_seconds": float(out[3])}
+
     def casci_info(self):
         """(ndet, roots held, Davidson iterations of the last casci; 0 = dense solver)."""
         out = np.zeros(3, dtype=np.int64)
@@ -482,6 +501,19 @@ def host_heev(a: np.ndarray, method: str = "auto"):
     w = np.zeros(n)
     check("dc2_host_heev", _lib.lib().dc2_host_heev(n, _ptr(A), _ptr(w), HOST_HEEV_METHODS[method]))
     return w, A
+
+
+def gemm_set_choice(opA: str, opB: str, M: int, N: int, K: int, batch: int, variant: int, split: int):
+    """Install a GEMM kernel choice for one exact shape (dc2_gemm_set_choice)."""
+    check("dc2_gemm_set_choice", _lib.lib().dc2_gemm_set_choice(opA.encode(), opB.encode(), M, N, K, batch, variant,
+                                                                split))
+
+
+def gemm_clear_choices() -> int:
+    """Empty the GEMM choice table; returns how many choices it held."""
+    n = ctypes.c_int64()
+    check("dc2_gemm_clear_choices", _lib.lib().dc2_gemm_clear_choices(ctypes.byref(n)))
+    return n.value
 
 
 def request_footprint(ninact: int, nact: int, nsec: int, nelec: int) -> int:

@@ -2017,6 +2017,42 @@ int dc2_get_gemm_op_stats(dc2_context* c, char opA, char opB, double* out) {
   });
 }
 
+int dc2_get_gemm_shapes(dc2_context* c, int64_t* out, int64_t cap, int64_t* count) {
+  return guarded([&] {
+    DC2_CHECK(c && count && (cap == 0 || out), 1, "null argument");
+    std::vector<std::array<int64_t, 6>> shapes;
+    for (auto& r : c->gstats.rec) {
+      if (!r.mfma) continue;
+      std::array<int64_t, 6> k{(int64_t)r.opA, (int64_t)r.opB, r.M, r.N, r.K, r.batch};
+      if (std::find(shapes.begin(), shapes.end(), k) == shapes.end()) shapes.push_back(k);
+    }
+    *count = (int64_t)shapes.size();
+    for (int64_t i = 0; i < std::min<int64_t>(cap, *count); ++i)
+      for (int j = 0; j < 6; ++j) out[6 * i + j] = shapes[i][j];
+  });
+}
+
+int dc2_gemm_tune(dc2_context* c, char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int32_t reps,
+                  double* out) {
+  return guarded([&] {
+    DC2_CHECK(c && out, 1, "null argument");
+    DC2_HIP(hipSetDevice(c->device));
+    gemm_tune(c->stream, opA, opB, M, N, K, batch, reps, out);
+  });
+}
+
+int dc2_gemm_set_choice(char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int32_t variant,
+                        int32_t split) {
+  return guarded([&] { gemm_set_choice(opA, opB, M, N, K, batch, variant, split); });
+}
+
+int dc2_gemm_clear_choices(int64_t* count_before) {
+  return guarded([&] {
+    if (count_before) *count_before = gemm_choice_count();
+    gemm_clear_choices();
+  });
+}
+
 int dc2_zgemm_split(dc2_context* c, int32_t slices, char opB, int64_t M, int64_t N, int64_t K, const double* A,
                     int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc) {
   return guarded([&] {

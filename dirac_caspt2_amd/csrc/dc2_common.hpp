@@ -91,6 +91,7 @@ struct GemmRecord {
   int tag;
   char opA, opB;   // the kernel instantiation k_zgemm<opA, opB>
   bool mfma;        // false: one of the kernels for skinny shapes
+  int64_t M, N, K, batch;
 };
 struct GemmStats {
   bool on = false;
@@ -99,6 +100,15 @@ struct GemmStats {
 void set_gemm_stats(GemmStats* st);
 // tag of the GEMM launches issued by the calling host thread
 void set_gemm_tag(int tag);
+// GEMM tuning table (exact shapes): the MFMA kernel variant (1..4: K tile 16/32 x LDS padding
+// 1/16) and split-K factor (1: none) chosen for (opA, opB, M, N, K, batch) by measurement.
+// gemm_tune times every candidate on scratch operands on stream s and installs the fastest;
+// out: {best variant, best split, best seconds, seconds of the built-in choice}.
+void gemm_tune(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int reps,
+               double* out);
+void gemm_set_choice(char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int variant, int split);
+void gemm_clear_choices();
+int64_t gemm_choice_count();
 // free the split-K workspace of stream s (call before destroying the stream)
 void release_stream_workspace(hipStream_t s);
 

@@ -13,9 +13,12 @@
 // plain FP64 kernels instead: one wave per output row over a long K, one thread per
 // output over a short K, one thread per row for at most 8 columns over a long K.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdlib>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "dc2_common.hpp"
 
@@ -417,6 +420,30 @@ int g_variant = [] {
 std::mutex g_stats_mu;
 thread_local int g_tag = 0;
 
+// tuned choices by exact shape (heap, never destroyed; looked up only when not empty)
+struct TuneKey {
+  char oa, ob;
+  int64_t M, N, K, batch;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(oa, ob, M, N, K, batch) < std::tie(o.oa, o.ob, o.M, o.N, o.K, o.batch);
+  }
+};
+struct TuneChoice {
+  int variant, split;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneChoice>* g_tune = new std::map<TuneKey, TuneChoice>;
+std::atomic<bool> g_tune_any{false};
+
+bool tuned_choice(char oa, char ob, int64_t M, int64_t N, int64_t K, int64_t batch, TuneChoice& c) {
+  if (!g_tune_any.load(std::memory_order_relaxed)) return false;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune->find(TuneKey{oa, ob, M, N, K, batch});
+  if (it == g_tune->end()) return false;
+  c = it->second;
+  return true;
+}
+
 // Kernels for the shapes the 64x64 MFMA tiles serve badly (single products only): small
 // outputs with A read along K (k_zgemm_dot), a short K (k_zgemm_smallk), at most 8 columns
 // and a long K (k_zgemm_tall).  Returns false when the MFMA kernel is the one to use.
@@ -451,9 +478,12 @@ bool launch_skinny(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, c
   return false;
 }
 
+// force_variant > 0: that MFMA variant and split-K factor force_split (the tuner's trials; no
+// skinny kernels, no table lookup)
 template <char OPA, char OPB>
 void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda, int64_t sA,
-            const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC, int64_t batch) {
+            const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC, int64_t batch,
+            int force_variant = 0, int force_split = 1) {
   GemmRecord r{};
   const bool acct = g_stats && g_stats->on;
   if (acct) {
@@ -469,10 +499,15 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     r.tag = g_tag;
     r.opA = OPA;
     r.opB = OPB;
+    r.M = M;
+    r.N = N;
+    r.K = K;
+    r.batch = batch;
     std::lock_guard<std::mutex> lk(g_stats_mu);
     g_stats->rec.push_back(r);
   };
-  if (batch == 1 && g_variant != 0 && launch_skinny<OPA, OPB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)) {
+  if (force_variant == 0 && batch == 1 && g_variant != 0 &&
+      launch_skinny<OPA, OPB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)) {
     DC2_HIP(hipGetLastError());
     record(false);
     return;
@@ -487,6 +522,24 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     kchunk = ((K + want - 1) / want + BK - 1) / BK * BK;
     sk = (int)((K + kchunk - 1) / kchunk);
   }
+  int variant = g_variant;
+  TuneChoice tc{0, 0};
+  int want_split = 0;   // 0: the rule above
+  if (force_variant > 0) {
+    variant = force_variant;
+    want_split = force_split;
+  } else if (g_variant != 0 && tuned_choice(OPA, OPB, M, N, K, batch, tc)) {
+    variant = tc.variant;
+    want_split = tc.split;
+  }
+  if (want_split > 0 && batch == 1) {
+    sk = 1;
+    kchunk = K;
+    if (want_split > 1) {
+      kchunk = ((K + want_split - 1) / want_split + BK - 1) / BK * BK;
+      sk = (int)((K + kchunk - 1) / kchunk);
+    }
+  }
   dim3 grid(cdiv(M, BM), cdiv(N, BN), (unsigned)(batch * sk));
   cplx* Cw = C;
   int64_t ldw = ldc, sw = sC;
@@ -498,7 +551,7 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     a1 = mk(1.0, 0.0);
     b1 = mk(0.0, 0.0);
   }
-  switch (g_variant) {
+  switch (variant) {
     case 1:
       hipLaunchKernelGGL((k_zgemm2<OPA, OPB, 16, 1>), grid, dim3(256), 0, s, M, N, K, a1, A, lda, sA, B, ldb, sB, b1,
                          Cw, ldw, sw, sk, kchunk);
@@ -528,13 +581,24 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
 template <char OPA>
 void dispatchB(char opB, hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A, int64_t lda,
                int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc, int64_t sC,
-               int64_t batch) {
+               int64_t batch, int fv = 0, int fs = 1) {
   switch (opB) {
-    case 'N': launch<OPA, 'N'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
-    case 'R': launch<OPA, 'R'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
-    case 'T': launch<OPA, 'T'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
-    case 'C': launch<OPA, 'C'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
+    case 'N': launch<OPA, 'N'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, fv, fs); break;
+    case 'R': launch<OPA, 'R'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, fv, fs); break;
+    case 'T': launch<OPA, 'T'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, fv, fs); break;
+    case 'C': launch<OPA, 'C'>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, fv, fs); break;
     default: throw Error(1, "zgemm: bad opB");
+  }
+}
+
+void dispatch(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, cplx alpha, const cplx* A,
+              int64_t lda, int64_t sA, const cplx* B, int64_t ldb, int64_t sB, cplx beta, cplx* C, int64_t ldc,
+              int64_t sC, int64_t batch, int fv = 0, int fs = 1) {
+  switch (opA) {
+    case 'N': dispatchB<'N'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, fv, fs); break;
+    case 'T': dispatchB<'T'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, fv, fs); break;
+    case 'C': dispatchB<'C'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, fv, fs); break;
+    default: throw Error(1, "zgemm: bad opA");
   }
 }
 }  // namespace
@@ -556,12 +620,90 @@ void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, c
   if (M <= 0 || N <= 0 || batch <= 0) return;
   DC2_CHECK(batch < 65536, 1, "zgemm: batch too large for one grid dimension");
   DC2_CHECK(cdiv(M, BM) < (1u << 31) && cdiv(N, BN) < 65536, 1, "zgemm: matrix too large for the grid");
-  switch (opA) {
-    case 'N': dispatchB<'N'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
-    case 'T': dispatchB<'T'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
-    case 'C': dispatchB<'C'>(opB, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch); break;
-    default: throw Error(1, "zgemm: bad opA");
+  dispatch(s, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+void gemm_set_choice(char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int variant, int split) {
+  DC2_CHECK(variant >= 1 && variant <= 4 && split >= 1 && split <= 64, 1, "gemm choice: variant 1..4, split 1..64");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  (*g_tune)[TuneKey{opA, opB, M, N, K, batch}] = TuneChoice{variant, split};
+  g_tune_any.store(true);
+}
+
+void gemm_clear_choices() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune->clear();
+  g_tune_any.store(false);
+}
+
+int64_t gemm_choice_count() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return (int64_t)g_tune->size();
+}
+
+void gemm_tune(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int reps,
+               double* out) {
+  DC2_CHECK(M > 0 && N > 0 && K > 0 && batch > 0 && batch < 65536 && reps >= 1, 1, "gemm_tune: invalid shape");
+  DC2_CHECK(std::string("NTC").find(opA) != std::string::npos && std::string("NRTC").find(opB) != std::string::npos, 1,
+            "gemm_tune: bad op");
+  // scratch operands (zeros: the timing does not depend on the values)
+  const int64_t lda = opA == 'N' ? M : K, ldb = (opB == 'N' || opB == 'R') ? K : N;
+  const int64_t sA = M * K, sB = K * N, sC = M * N;
+  DevBuf<cplx> A((size_t)(sA * batch)), B((size_t)(sB * batch)), C((size_t)(sC * batch));
+  DC2_HIP(hipMemsetAsync(A.get(), 0, sA * batch * sizeof(cplx), s));
+  DC2_HIP(hipMemsetAsync(B.get(), 0, sB * batch * sizeof(cplx), s));
+  DC2_HIP(hipMemsetAsync(C.get(), 0, sC * batch * sizeof(cplx), s));
+  hipEvent_t e0, e1;
+  DC2_HIP(hipEventCreate(&e0));
+  DC2_HIP(hipEventCreate(&e1));
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  auto time = [&](int fv, int fs) {   // fv = 0: the built-in choice (the table is bypassed below)
+    dispatch(s, opA, opB, M, N, K, one, A.get(), lda, sA, B.get(), ldb, sB, zero, C.get(), M, sC, batch, fv, fs);
+    DC2_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r)
+      dispatch(s, opA, opB, M, N, K, one, A.get(), lda, sA, B.get(), ldb, sB, zero, C.get(), M, sC, batch, fv, fs);
+    DC2_HIP(hipEventRecord(e1, s));
+    DC2_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    DC2_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return 1e-3 * ms / reps;
+  };
+  // the built-in choice: variant g_variant and the split-K rule, without a table entry
+  bool had = false;
+  TuneChoice prev{0, 0};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune->find(TuneKey{opA, opB, M, N, K, batch});
+    if (it != g_tune->end()) { had = true; prev = it->second; g_tune->erase(it); }
+    g_tune_any.store(!g_tune->empty());
   }
+  const double t_builtin = time(0, 1);
+  if (had) gemm_set_choice(opA, opB, M, N, K, batch, prev.variant, prev.split);
+  const int64_t tiles = (int64_t)cdiv(M, BM) * cdiv(N, BN) * batch;
+  double best = INFINITY;
+  int bv = 1, bs = 1;
+  for (int v = 1; v <= 4; ++v)
+    for (int sp : {1, 2, 4, 8, 16, 32}) {
+      if (sp > 1 && (batch != 1 || K < 2 * BK * sp || tiles * sp > 4096)) continue;
+      const double t = time(v, sp);
+      if (t < best) { best = t; bv = v; bs = sp; }
+    }
+  DC2_HIP(hipEventDestroy(e0));
+  DC2_HIP(hipEventDestroy(e1));
+  DC2_HIP(hipStreamSynchronize(s));   // the scratch operands are freed next
+  if (best < t_builtin) {
+    gemm_set_choice(opA, opB, M, N, K, batch, bv, bs);
+  } else {   // the built-in choice stays (it may split K by a factor outside the candidates)
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune->erase(TuneKey{opA, opB, M, N, K, batch});
+    g_tune_any.store(!g_tune->empty());
+    bv = bs = 0;
+    best = t_builtin;
+  }
+  out[0] = bv;
+  out[1] = bs;
+  out[2] = best;
+  out[3] = t_builtin;
 }
 
 }  // namespace dc2

@@ -134,6 +134,25 @@ int dc2_synchronize(dc2_context* ctx);
  * 4 S^-1/2 projections, 5 integral blocks from factors, 0 other */
 int dc2_set_profiling(dc2_context* ctx, int32_t on);
 int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
+/* GEMM autotuning (no counterpart in the reference: module_blas.F90 calls the vendor BLAS).
+ * dc2_get_gemm_shapes: the distinct shapes of the MFMA-kernel launches recorded while
+ *   profiling was on, rows of 6 int64 {opA, opB, M, N, K, batch} (opA/opB as character
+ *   codes), at most cap rows written, *count = all of them.
+ * dc2_gemm_tune: times every candidate for one shape on scratch operands on the context's
+ *   stream (kernel variant 1..4: K tile 16/32 x LDS padding 1/16; split-K 1, 2, 4 .. 32 for
+ *   single products with few tiles), installs the fastest in the process-wide table when it
+ *   beats the built-in choice; out[4] = {variant, split (0, 0: built-in kept), its seconds,
+ *   the built-in choice's seconds}.  reps: timed repetitions per candidate.
+ * dc2_gemm_set_choice: installs a choice (a table saved from an earlier tuning run).
+ * dc2_gemm_clear_choices: empties the table (*count_before may be NULL).
+ * Choices apply to products of exactly that shape; results change only by the order of
+ * the split-K summation (still fixed, so repeatable). */
+int dc2_get_gemm_shapes(dc2_context* ctx, int64_t* out, int64_t cap, int64_t* count);
+int dc2_gemm_tune(dc2_context* ctx, char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int32_t reps,
+                  double* out);
+int dc2_gemm_set_choice(char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int32_t variant,
+                        int32_t split);
+int dc2_gemm_clear_choices(int64_t* count_before);
 /* the same figures for the launches of one MFMA kernel instantiation k_zgemm<opA, opB>
  * (products routed to the kernels for skinny shapes are counted by dc2_get_gemm_stats only) */
 int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);

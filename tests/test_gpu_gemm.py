@@ -68,3 +68,32 @@ def test_zgemm_batched(ctx):
     out = dC.to_host().reshape(nb, N, M)
     for b in range(nb):
         np.testing.assert_allclose(out[b].T, A[b].T @ B[b].T, atol=1e-12)
+
+
+@pytest.mark.parametrize("opA,opB,shape,batch", [("C", "N", (48, 48, 924), 1), ("T", "N", (130, 70, 97), 1),
+                                                  ("T", "R", (300, 164, 164), 1), ("N", "C", (200, 200, 600), 1),
+                                                  ("N", "N", (40, 24, 30), 5)])
+def test_autotuned_choice_still_exact(ctx, opA, opB, shape, batch):
+    """The tuner installs the fastest variant / split-K for the shape (or keeps the built-in
+    choice), and products of that shape still match numpy to the same bound."""
+    from dirac_caspt2_amd.api import DeviceArray, gemm_clear_choices, zgemm
+    M, N, K = shape
+    r = ctx.gemm_tune(opA, opB, M, N, K, batch, 2)
+    assert r["seconds"] > 0 and r["seconds"] <= r["builtin_seconds"] * 1.0000001
+    assert (r["variant"], r["split"]) == (0, 0) or (1 <= r["variant"] <= 4 and r["split"] in (1, 2, 4, 8, 16, 32))
+    rng = np.random.default_rng(M + N + K)
+    a_shape = (M, K) if opA == "N" else (K, M)
+    b_shape = (K, N) if opB in "NR" else (N, K)
+    A = rng.normal(size=(batch,) + a_shape) + 1j * rng.normal(size=(batch,) + a_shape)
+    B = rng.normal(size=(batch,) + b_shape) + 1j * rng.normal(size=(batch,) + b_shape)
+    dev = lambda x: DeviceArray(ctx, np.concatenate([np.asfortranarray(x[b]).ravel(order="F") for b in range(batch)]))
+    dA, dB = dev(A), dev(B)
+    dC = DeviceArray(ctx, np.zeros(batch * M * N, dtype=np.complex128))
+    zgemm(ctx, opA, opB, M, N, K, 1.0 + 0j, dA, a_shape[0], a_shape[0] * a_shape[1], dB, b_shape[0],
+          b_shape[0] * b_shape[1], 0j, dC, M, M * N, batch)
+    ctx.synchronize()
+    out = dC.to_host().reshape(batch, N, M)
+    for b in range(batch):
+        np.testing.assert_allclose(out[b].T, _op(A[b], opA) @ _op(B[b], opB), rtol=0, atol=1e-12 * max(1, K))
+    assert gemm_clear_choices() >= 0
+
