@@ -52,6 +52,7 @@ _SIGS = {
     "dc2_set_option": (c_int, [P, ctypes.c_char_p, c_i64]),
     "dc2_get_gemm_op_stats": (c_int, [P, ctypes.c_char, ctypes.c_char, P]),
     "dc2_get_gemm_shapes": (c_int, [P, P, c_i64, P]),
+    "dc2_get_graph_stats": (c_int, [P, P]),
     "dc2_gemm_tune": (c_int, [P, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, c_i64, c_int, P]),
     "dc2_gemm_set_choice": (c_int, [ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, c_i64, c_int, c_int]),
     "dc2_gemm_clear_choices": (c_int, [P]),

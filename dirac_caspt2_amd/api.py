@@ -293,6 +293,12 @@ class Caspt2Context:
         check("dc2_get_gemm_op_stats", self.lib.dc2_get_gemm_op_stats(self.h, opA.encode(), opB.encode(), _ptr(out)))
         return {"launches": int(out[0]), "seconds": float(out[1]), "flops": float(out[2]), "max_flops": float(out[3])}
 
+    def graph_stats(self):
+        """Captured Davidson graphs: {captures, replays, held, enabled}."""
+        out = np.zeros(4, dtype=np.int64)
+        check("dc2_get_graph_stats", self.lib.dc2_get_graph_stats(self.h, _ptr(out)))
+        return dict(zip(("captures", "replays", "held", "enabled"), (int(x) for x in out)))
+
     def gemm_shapes(self):
         """Distinct shapes (opA, opB, M, N, K, batch) of the MFMA GEMM launches recorded while
         profiling was on."""
@@ -327,11 +333,12 @@ class Caspt2Context:
     OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3, "native": 4},
                "ac_metric": {"auto": 0, "formula": 1, "determinant": 2, "antisymmetric": 3},
                "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {},
-               "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}}
+               "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}, "graphs": {"off": 0, "on": 1}}
 
     def set_option(self, key: str, value):
         """Options: eigensolver auto|qr|dc|dcj|jacobi|native, ac_metric auto|formula|determinant,
-        lanes 1..3 (concurrent subspace streams), schur on|off."""
+        lanes 1..4 (concurrent subspace streams), schur on|off, graphs on|off (captured
+        Davidson iterations)."""
         v = self.OPTIONS[key][value] if isinstance(value, str) else int(value)
         check("dc2_set_option", self.lib.dc2_set_option(self.h, key.encode(), v))
 

@@ -153,6 +153,11 @@ struct dc2_context {
   std::unique_ptr<Lane> lane0;                  // the library stream as a lane
   std::vector<std::unique_ptr<Lane>> lanes;     // worker lanes (created on first use)
   GemmStats gstats;        // GEMM launch accounting (dc2_set_profiling)
+  // block Davidson: the launches between two host round trips of an iteration replayed from
+  // captured graphs, keyed by the sizes and every buffer address they use
+  bool graphs = true;
+  std::map<std::vector<int64_t>, hipGraphExec_t> dav_graphs;
+  int64_t graph_launches = 0, graph_captures = 0;
 
   int nmo() const { return ni + na + ns; }
   int nmoc() const { return ni + na; }
@@ -519,24 +524,69 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     std::memcpy(up_y.data() + (size_t)maxsub * b, theta.data(), b * sizeof(double));
     to_dev(Sy, up_y.data(), (size_t)m * b * sizeof(cplx));
     to_dev(th, up_y.data() + (size_t)maxsub * b, b * sizeof(double));
-    zgemm(s, 'N', 'N', n, b, m, one, V, n, 0, Sy, m, 0, zero, X, n, 0, 1);
-    zgemm(s, 'N', 'N', n, b, m, one, W, n, 0, Sy, m, 0, zero, AX, n, 0, 1);
-    davidson_residual(s, n, b, AX, X, th, R);
-    zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, RG, b, 0, 1);   // residual norms
-    davidson_precond(s, n, b, sel, R, th, d, T);
-    davidson_normalize(s, n, b, T);
     const bool restart = m + b > maxsub;
-    if (restart) {   // thick restart on the b Ritz vectors: V^H H V = diag(theta)
-      DC2_HIP(hipMemcpyAsync(V, X, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-      DC2_HIP(hipMemcpyAsync(W, AX, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
-      m = b;
+    const int m_in = m;
+    // Ritz vectors and residuals, preconditioned corrections, (restart,) two classical
+    // Gram-Schmidt passes against V, then the Gram matrix of the block: launches only
+    auto segment = [&](int mm, bool rs) {
+      zgemm(s, 'N', 'N', n, b, mm, one, V, n, 0, Sy, mm, 0, zero, X, n, 0, 1);
+      zgemm(s, 'N', 'N', n, b, mm, one, W, n, 0, Sy, mm, 0, zero, AX, n, 0, 1);
+      davidson_residual(s, n, b, AX, X, th, R);
+      zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, RG, b, 0, 1);   // residual norms
+      davidson_precond(s, n, b, sel, R, th, d, T);
+      davidson_normalize(s, n, b, T);
+      if (rs) {   // thick restart on the b Ritz vectors: V^H H V = diag(theta)
+        DC2_HIP(hipMemcpyAsync(V, X, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+        DC2_HIP(hipMemcpyAsync(W, AX, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+        mm = b;
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        zgemm(s, 'C', 'N', mm, b, n, one, V, n, 0, T, n, 0, zero, Sm, mm, 0, 1);
+        zgemm(s, 'N', 'N', n, b, mm, mone, V, n, 0, Sm, mm, 0, one, T, n, 0, 1);
+      }
+      zgemm(s, 'C', 'N', b, b, n, one, T, n, 0, T, n, 0, zero, RG + (size_t)b * b, b, 0, 1);
+    };
+    // Replayed from a captured graph when every product above goes to the skinny kernels
+    // (no split-K workspace, no allocation), no restart copy is involved and the launches
+    // are not being timed; graphs are cached by sizes and buffer addresses
+    const bool graph_ok = c->graphs && !c->gstats.on && !restart && n * b <= 65536 && (int64_t)m_in * b <= 2048;
+    if (graph_ok) {
+      const std::vector<int64_t> key = {n, b, m_in, (int64_t)V, (int64_t)W, (int64_t)X, (int64_t)AX, (int64_t)R,
+                                        (int64_t)T, (int64_t)Sy, (int64_t)th, (int64_t)RG, (int64_t)Sm, (int64_t)d,
+                                        (int64_t)sel, (int64_t)s};
+      auto it = c->dav_graphs.find(key);
+      if (it == c->dav_graphs.end()) {   // capture; on any failure the context stops using graphs
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+          try {
+            segment(m_in, false);
+          } catch (...) {
+            ok = false;
+          }
+          ok = hipStreamEndCapture(s, &g) == hipSuccess && ok && g != nullptr;
+          ok = ok && hipGraphInstantiateWithFlags(&ge, g, 0) == hipSuccess;
+          if (g) (void)hipGraphDestroy(g);
+        }
+        if (ok) {
+          it = c->dav_graphs.emplace(key, ge).first;
+          ++c->graph_captures;
+        } else {
+          (void)hipGetLastError();
+          c->graphs = false;
+        }
+      }
+      if (it != c->dav_graphs.end()) {
+        DC2_HIP(hipGraphLaunch(it->second, s));
+        ++c->graph_launches;
+      } else {
+        segment(m_in, false);
+      }
+    } else {
+      segment(m_in, restart);
     }
-    // two classical Gram-Schmidt passes against V, then the Gram matrix of the block
-    for (int pass = 0; pass < 2; ++pass) {
-      zgemm(s, 'C', 'N', m, b, n, one, V, n, 0, T, n, 0, zero, Sm, m, 0, 1);
-      zgemm(s, 'N', 'N', n, b, m, mone, V, n, 0, Sm, m, 0, one, T, n, 0, 1);
-    }
-    zgemm(s, 'C', 'N', b, b, n, one, T, n, 0, T, n, 0, zero, RG + (size_t)b * b, b, 0, 1);
+    if (restart) m = b;
     to_host(rg.data(), RG, rg.size() * sizeof(cplx));
     double worst = 0.0;
     for (int j = 0; j < nroot; ++j) worst = std::max(worst, std::sqrt(std::fabs(rg[j + (size_t)b * j].real())));
@@ -1529,6 +1579,7 @@ int dc2_context_destroy(dc2_context* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->gstats.on) set_gemm_stats(nullptr);
     for (auto& r : ctx->gstats.rec) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto& kv : ctx->dav_graphs) (void)hipGraphExecDestroy(kv.second);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     hipStream_t s = ctx->stream;
     delete ctx;
@@ -1984,6 +2035,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "schur") {
       DC2_CHECK(value == 0 || value == 1, 1, "schur must be 0 or 1");
       c->schur = value != 0;
+    } else if (k == "graphs") {
+      DC2_CHECK(value == 0 || value == 1, 1, "graphs must be 0 or 1");
+      c->graphs = value != 0;
     } else if (k == "casci_solver") {
       DC2_CHECK(value >= 0 && value <= 2, 1, "casci_solver must be 0..2");
       c->casci_solver = (int)value;
@@ -2014,6 +2068,16 @@ int dc2_get_gemm_op_stats(dc2_context* c, char opA, char opB, double* out) {
       mx = std::max(mx, r.flops);
     }
     out[0] = n; out[1] = sec; out[2] = fl; out[3] = mx;
+  });
+}
+
+int dc2_get_graph_stats(dc2_context* c, int64_t* out) {
+  return guarded([&] {
+    DC2_CHECK(c && out, 1, "null argument");
+    out[0] = c->graph_captures;
+    out[1] = c->graph_launches;
+    out[2] = (int64_t)c->dav_graphs.size();
+    out[3] = c->graphs ? 1 : 0;
   });
 }
 

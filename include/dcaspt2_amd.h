@@ -148,6 +148,11 @@ int dc2_get_gemm_stats(dc2_context* ctx, int32_t tag, double* out);
  * Choices apply to products of exactly that shape; results change only by the order of
  * the split-K summation (still fixed, so repeatable). */
 int dc2_get_gemm_shapes(dc2_context* ctx, int64_t* out, int64_t cap, int64_t* count);
+/* Captured graphs of the block Davidson (no counterpart in the reference): the launches
+ * between two host round trips of an iteration are captured once per size and buffer set
+ * and replayed (option "graphs", default 1).  out[4] = {captures, replays, graphs held,
+ * graphs still enabled (a failed capture turns them off for the context)} */
+int dc2_get_graph_stats(dc2_context* ctx, int64_t* out);
 int dc2_gemm_tune(dc2_context* ctx, char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int32_t reps,
                   double* out);
 int dc2_gemm_set_choice(char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int32_t variant,
@@ -169,6 +174,8 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  antisymmetric, so these span the same space and S's eigenvalues halve);
  *   "lanes":       1..4 concurrent streams (each with a host thread) for the subspace
  *                  solves (default 4: A, C, D and the rest; 3: A, C and the rest);
+ *   "graphs":      1 (default) block-Davidson iterations replayed from captured graphs
+ *                  (dc2_get_graph_stats), 0 every launch issued directly;
  *   "schur":       1 (default) E2 of the determinant-space route through the compressed
  *                  resolvent when at most 32 directions are cut, 0 always through the
  *                  eigendecomposition of B~;
