@@ -870,9 +870,7 @@ GramMetric gram_eig(dc2_context* c, Lane& L, char X, int64_t n, const cplx* Y, c
   double* f = L.d(nm("_f").c_str(), m);
   double* lam = L.d(nm("_lam").c_str(), m);
   build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), Y, eps_act, m, strings, Phi, f);
-  set_gemm_tag(4);
-  zgemm(s, 'N', 'C', m, m, n, one, Phi, m, 0, Phi, m, 0, zero, G, m, 0, 1);
-  set_gemm_tag(0);
+  zherk_upper(s, m, n, Phi, m, G, m);   // Hermitian: the upper tiles only
   check_finite(c, s, "Phi Phi^H", G, m * m);
   // a RAS-restricted CI vector leaves directions of the (nelec+-1) space unreached: G is then
   // rank-deficient, where plain divide and conquer can return NaN eigenvectors

@@ -142,12 +142,15 @@ __global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t Kfu
 // while the current one is multiplied out of LDS, so load latency hides behind the MFMAs.
 // KB: K per tile (16 or 32); PAD: LDS row padding in doubles (bank spread of the
 // transposed-operand stores and of the MFMA operand reads).
-template <char OPA, char OPB, int KB, int PAD>
+// UPPER: only the tiles that reach the upper triangle (a Hermitian product; the tiles
+// entirely below the diagonal leave at once)
+template <char OPA, char OPB, int KB, int PAD, bool UPPER = false>
 __global__ __launch_bounds__(256) void k_zgemm2(int64_t M, int64_t N, int64_t Kfull, cplx alpha,
                                                 const cplx* __restrict__ A, int64_t lda, int64_t sA,
                                                 const cplx* __restrict__ B, int64_t ldb, int64_t sB,
                                                 cplx beta, cplx* __restrict__ C, int64_t ldc, int64_t sC, int sk,
                                                 int64_t kchunk) {
+  if (UPPER && (int64_t)blockIdx.x * BM > (int64_t)blockIdx.y * BN + BN - 1) return;   // uniform
   constexpr int LS = BM + PAD;               // BM == BN
   constexpr int NE = BM * KB / 256;          // elements of each operand tile per thread
   __shared__ double Ar[KB][LS], Ai[KB][LS], Br[KB][LS], Bi[KB][LS];
@@ -374,6 +377,14 @@ __global__ __launch_bounds__(256) void k_zgemm_tall(int64_t M, int64_t N, int64_
 #pragma unroll
   for (int j = 0; j < TS_NMAX; ++j)   // static indices: acc stays in registers
     if (j < N) w[M * j] = acc[j];
+}
+
+// the strict lower triangle as the conjugate of the upper one (the diagonal is kept)
+__global__ void k_lower_from_upper(int64_t m, cplx* __restrict__ C, int64_t ldc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * m) return;
+  const int64_t i = t % m, j = t / m;
+  if (i > j) C[i + ldc * j] = cconj(C[j + ldc * i]);
 }
 
 // C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order)
@@ -621,6 +632,19 @@ void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, c
   DC2_CHECK(batch < 65536, 1, "zgemm: batch too large for one grid dimension");
   DC2_CHECK(cdiv(M, BM) < (1u << 31) && cdiv(N, BN) < 65536, 1, "zgemm: matrix too large for the grid");
   dispatch(s, opA, opB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+// C (m x m, ldc) = A A^H for A (m x k, lda): the tiles on and above the diagonal on the MFMA
+// kernel, then the lower triangle as the conjugate of the upper one
+void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc) {
+  if (m <= 0) return;
+  DC2_CHECK(cdiv(m, BM) < 65536, 1, "zherk_upper: matrix too large for the grid");
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  dim3 grid(cdiv(m, BM), cdiv(m, BN), 1);
+  hipLaunchKernelGGL((k_zgemm2<'N', 'C', 16, 1, true>), grid, dim3(256), 0, s, m, m, k, one, A, lda, (int64_t)0, A,
+                     lda, (int64_t)0, zero, C, ldc, (int64_t)0, 1, k);
+  hipLaunchKernelGGL(k_lower_from_upper, dim3(cdiv(m * m, 256)), dim3(256), 0, s, m, C, ldc);
+  DC2_HIP(hipGetLastError());
 }
 
 void gemm_set_choice(char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int variant, int split) {
