@@ -1155,7 +1155,9 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
           V_D(s, na, ni, ns, d, blk("D1"), blk("D2"), eff, V);
         } break;
         case 'E': V_E(s, na, ni, ns, d, blk("E"), c->ipairs.get(), V); break;
-        case 'F': V_F(s, na, ns, d, blk("F"), c->tpairs.get(), c->spairs.get(), V); break;
+        case 'F':
+          V_F(s, na, ns, d, blk("F"), c->tpairs.get(), c->spairs.get(), V, L.z("F_vscr", V_F_scratch(na, ns)));
+          break;
         case 'G': V_G(s, na, ni, ns, d, blk("G"), c->spairs.get(), V); break;
       }
       alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert, alpha);

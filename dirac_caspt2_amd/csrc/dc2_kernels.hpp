@@ -127,7 +127,11 @@ void V_B(hipStream_t s, int na, int ni, const Dens& d, const cplx* Bb, const int
 void V_D(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* D1b, const cplx* D2b, const cplx* effh,
          cplx* V);
 void V_E(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Eb, const int* ipairs, cplx* V);
-void V_F(hipStream_t s, int na, int ns, const Dens& d, const cplx* Fb, const int* tpairs, const int* spairs, cplx* V);
+// scratch of V_F_scratch(na, ns) complex elements (the permuted block, the density matrix
+// and the all-pairs product)
+int64_t V_F_scratch(int na, int ns);
+void V_F(hipStream_t s, int na, int ns, const Dens& d, const cplx* Fb, const int* tpairs, const int* spairs, cplx* V,
+         cplx* scratch);
 void V_G(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Gb, const int* spairs, cplx* V);
 // alpha per perturber of subspace X (solve_X_subspace.F90 'alpha =' lines)
 void alphas(hipStream_t s, char X, int ni, int na, int ns, const double* eps, const double* e0_dev, double eshift,

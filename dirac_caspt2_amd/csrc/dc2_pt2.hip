@@ -257,18 +257,49 @@ __global__ void k_V_E(int na, int ni, int ns, DV d, const cplx* Eb, const int* i
   V[t + Na * pa] = acc;
 }
 
-// V_F(tu; a>b) (solve_F_subspace.F90:505-535), Fb = (s,a|s,a)
-__global__ void k_V_F(int na, int ns, DV d, const cplx* Fb, const int* tp, const int* sp, int64_t ntp, cplx* V) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
-  if (k >= ntp) return;
-  const int t = tp[2 * k], u = tp[2 * k + 1], a = sp[2 * m], b = sp[2 * m + 1];
+// V_F(tu; a>b) (solve_F_subspace.F90:505-535), Fb = (s,a|s,a): sum_pq D2(t,p,u,q) F(a,p,b,q)
+// - sum_p D1(t,p) F(a,u,b,p), as a matrix product: with the pair index k = (t > u),
+// Am[k, (p,q)] = D2(t,p,u,q) - [p = u] D1(t,q) and Fp[(p,q), (a,b)] = F(a,p,b,q),
+// C = Am Fp holds V_F(k; a, b) for every a, b; the a > b columns are gathered into V.
+__global__ void k_vf_amat(int na, DV d, const int* tp, int64_t ntp, cplx* Am) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t na2 = (int64_t)na * na;
+  if (i >= ntp * na2) return;
+  const int64_t k = i % ntp, pq = i / ntp;
+  const int p = (int)(pq % na), q = (int)(pq / na), t = tp[2 * k], u = tp[2 * k + 1];
+  cplx v = d.D2(t, p, u, q);
+  if (p == u) v = csub(v, d.D1(t, q));
+  Am[i] = v;
+}
+
+// Fp[(p + na q) + na^2 (a + ns b)] = Fb[a + ns (p + na (b + ns q))]: for each b a transpose of
+// (a) x (p,q) through a 32 x 32 LDS tile, both sides coalesced
+__global__ __launch_bounds__(256) void k_vf_perm(int na, int ns, const cplx* __restrict__ Fb, cplx* __restrict__ Fp) {
+  __shared__ cplx tile[32][33];
+  const int b = blockIdx.z, a0 = blockIdx.x * 32, pq0 = blockIdx.y * 32;
+  const int na2 = na * na, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
   const int64_t S = ns, Na = na;
-  auto Fv = [&](int aa, int p, int bb, int q) { return Fb[aa + S * (p + Na * (bb + S * q))]; };
-  cplx acc = mk(0.0, 0.0);
-  for (int q = 0; q < na; ++q)
-    for (int p = 0; p < na; ++p) cfma(acc, d.D2(t, p, u, q), Fv(a, p, b, q));
-  for (int p = 0; p < na; ++p) acc = csub(acc, cmul(d.D1(t, p), Fv(a, u, b, p)));
-  V[k + ntp * m] = acc;
+  for (int r = ty; r < 32; r += 8) {   // read: a = a0 + tx contiguous, pq = pq0 + r
+    const int a = a0 + tx, pq = pq0 + r;
+    if (a < ns && pq < na2) {
+      const int p = pq % na, q = pq / na;
+      tile[r][tx] = Fb[a + S * (p + Na * (b + S * q))];
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {   // write: pq = pq0 + tx contiguous, a = a0 + r
+    const int a = a0 + r, pq = pq0 + tx;
+    if (a < ns && pq < na2) Fp[pq + (int64_t)na2 * (a + S * b)] = tile[tx][r];
+  }
+}
+
+__global__ void k_vf_gather(const cplx* __restrict__ C, int ns, const int* sp, int64_t ntp, int64_t nsp,
+                            cplx* __restrict__ V) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntp * nsp) return;
+  const int64_t k = i % ntp, m = i / ntp;
+  const int64_t ab = sp[2 * m] + (int64_t)ns * sp[2 * m + 1];
+  V[i] = C[k + ntp * ab];
 }
 
 // V_G(t; a>b, i) (solve_G_subspace.F90:445-462), Gb = (s,i|s,a)
@@ -538,13 +569,25 @@ void V_E(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Eb, c
   hipLaunchKernelGGL(k_V_E, dim3((unsigned)(nip * ns)), dim3(64), 0, s, na, ni, ns, dv(d), Eb, ipairs, V);
   DC2_HIP(hipGetLastError());
 }
+int64_t V_F_scratch(int na, int ns) {
+  const int64_t ntp = (int64_t)na * (na - 1) / 2, na2 = (int64_t)na * na, ns2 = (int64_t)ns * ns;
+  return na2 * ns2 + ntp * na2 + ntp * ns2;
+}
+
 void V_F(hipStream_t s, int na, int ns, const Dens& d, const cplx* Fb, const int* tpairs, const int* spairs,
-         cplx* V) {
+         cplx* V, cplx* scratch) {
   const int64_t ntp = (int64_t)na * (na - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
   if (ntp == 0 || nsp == 0) return;
-  DC2_CHECK(nsp < 65536 * 64, 1, "V_F: too many secondary pairs");
-  hipLaunchKernelGGL(k_V_F, dim3(cdiv(ntp, 64), (unsigned)nsp), dim3(64), 0, s, na, ns, dv(d), Fb, tpairs, spairs,
-                     ntp, V);
+  const int64_t na2 = (int64_t)na * na, ns2 = (int64_t)ns * ns;
+  DC2_CHECK(ns < 65536 && cdiv(na2, 32) < 65536, 1, "V_F: too many orbitals for the grid");
+  cplx* Fp = scratch;
+  cplx* Am = Fp + na2 * ns2;
+  cplx* C = Am + ntp * na2;
+  hipLaunchKernelGGL(k_vf_amat, dim3(cdiv(ntp * na2, 256)), dim3(256), 0, s, na, dv(d), tpairs, ntp, Am);
+  hipLaunchKernelGGL(k_vf_perm, dim3(cdiv(ns, 32), cdiv(na2, 32), (unsigned)ns), dim3(256), 0, s, na, ns, Fb, Fp);
+  DC2_HIP(hipGetLastError());
+  zgemm(s, 'N', 'N', ntp, ns2, na2, mk(1.0, 0.0), Am, ntp, 0, Fp, na2, 0, mk(0.0, 0.0), C, ntp, 0, 1);
+  hipLaunchKernelGGL(k_vf_gather, dim3(cdiv(ntp * nsp, 256)), dim3(256), 0, s, C, ns, spairs, ntp, nsp, V);
   DC2_HIP(hipGetLastError());
 }
 void V_G(hipStream_t s, int na, int ni, int ns, const Dens& d, const cplx* Gb, const int* spairs, cplx* V) {
