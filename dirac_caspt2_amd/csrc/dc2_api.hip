@@ -1430,15 +1430,20 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     calce0(s, na, eps_act, c->G1.get(), c->e0.get());
     zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1.get(), nf, 0, zero, c->G2.get(), na2, 0, 1);
     zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1F.get(), nf, 0, zero, c->G2F.get(), na2, 0, 1);
+    // the F-weighted 3-RDM enters only the formula S/B matrices of A and C (not their
+    // determinant-space route, not any V)
+    const bool need_g3f = (doA && !ac_route_of(c, 'A').gram) || (doC && !ac_route_of(c, 'C').gram);
     if (doA || doC) {
       const int64_t na4 = na2 * na2;
       c->G3.ensure(na2 * na4);
-      c->G3F.ensure(na2 * na4);
       cplx* Y2 = L0.z("Y2", nf * na4);
       apply_E(s, na, nf, na2, c->exc.get(), c->Y1.get(), Y2);
       zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3.get(), na2, 0, 1);
-      apply_E(s, na, nf, na2, c->exc.get(), c->Y1F.get(), Y2);
-      zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3F.get(), na2, 0, 1);
+      if (need_g3f) {
+        c->G3F.ensure(na2 * na4);
+        apply_E(s, na, nf, na2, c->exc.get(), c->Y1F.get(), Y2);
+        zgemm(s, 'C', 'N', na2, na4, nf, one, c->Y1.get(), nf, 0, Y2, nf, 0, zero, c->G3F.get(), na2, 0, 1);
+      }
     }
     // h' = U^H h U (tramo1)
     c->hp.ensure((int64_t)nmo * nmo);
@@ -1454,10 +1459,8 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     check_finite(c, "G1F", c->G1F.get(), na2);
     check_finite(c, "G2", c->G2.get(), na2 * na2);
     check_finite(c, "G2F", c->G2F.get(), na2 * na2);
-    if (doA || doC) {
-      check_finite(c, "G3", c->G3.get(), na2 * na2 * na2);
-      check_finite(c, "G3F", c->G3F.get(), na2 * na2 * na2);
-    }
+    if (doA || doC) check_finite(c, "G3", c->G3.get(), na2 * na2 * na2);
+    if (need_g3f) check_finite(c, "G3F", c->G3F.get(), na2 * na2 * na2);
     check_finite(c, "h'", c->hp.get(), (int64_t)nmo * nmo);
     upload_ints(s, c->tpairs, pair_list(na));
     upload_ints(s, c->ipairs, pair_list(ni));
