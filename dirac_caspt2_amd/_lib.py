@@ -43,6 +43,7 @@ _SIGS = {
     "dc2_cdiag": (c_int, [P, P, c_int, P, P, c_dbl]),
     "dc2_zgemm": (c_int, [P, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, P, P, c_i64, c_i64, P, c_i64, c_i64,
                           P, P, c_i64, c_i64, c_i64]),
+    "dc2_zherk": (c_int, [P, c_i64, c_i64, P, c_i64, P, c_i64, c_int]),
     "dc2_synchronize": (c_int, [P]),
     "dc2_malloc": (c_int, [P, c_i64, ctypes.POINTER(P)]),
     "dc2_free": (c_int, [P, P]),

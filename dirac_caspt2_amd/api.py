@@ -367,6 +367,12 @@ def zgemm(ctx: Caspt2Context, opA: str, opB: str, M: int, N: int, K: int, alpha:
                                          _ptr(B), ldb, sB, _ptr(be), _ptr(C), ldc, sC, batch))
 
 
+def zherk(ctx: Caspt2Context, m: int, k: int, A, lda: int, C, ldc: int, split: int = 0):
+    """C = A A^H (m x m, the whole Hermitian matrix) on device memory; split 0 lets the
+    library choose the number of k chunks, 1 leaves k unsplit."""
+    check("dc2_zherk", ctx.lib.dc2_zherk(ctx.h, m, k, _ptr(A), lda, _ptr(C), ldc, split))
+
+
 def space_sizes(dims):
     ni, na, ns = dims[:3]
     return {"i": ni, "a": na, "s": ns, "o": ni + na}

@@ -1852,6 +1852,16 @@ int dc2_zgemm(dc2_context* c, char opA, char opB, int64_t M, int64_t N, int64_t 
   });
 }
 
+int dc2_zherk(dc2_context* c, int64_t m, int64_t k, const double* A, int64_t lda, double* C, int64_t ldc,
+              int32_t split) {
+  return guarded([&] {
+    DC2_CHECK(c && A && C, 1, "null argument");
+    DC2_CHECK(m >= 0 && k >= 0 && lda >= std::max<int64_t>(m, 1) && ldc >= std::max<int64_t>(m, 1), 1,
+              "dc2_zherk: bad size or leading dimension");
+    zherk_upper(c->stream, m, k, reinterpret_cast<const cplx*>(A), lda, reinterpret_cast<cplx*>(C), ldc, split);
+  });
+}
+
 int dc2_malloc(dc2_context* c, int64_t bytes, void** ptr) {
   return guarded([&] {
     DC2_CHECK(c && ptr && bytes >= 0, 1, "invalid argument");

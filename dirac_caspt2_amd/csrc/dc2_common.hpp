@@ -100,8 +100,10 @@ struct GemmStats {
 void set_gemm_stats(GemmStats* st);
 // tag of the GEMM launches issued by the calling host thread
 void set_gemm_tag(int tag);
-// C = A A^H (m x m) from the tiles on and above the diagonal, the lower triangle mirrored
-void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc);
+// C = A A^H (m x m) from the tiles on and above the diagonal, the lower triangle mirrored;
+// split: k chunks summed by a reduction (0: chosen by size, 1: unsplit)
+void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc,
+                 int split = 0);
 // GEMM tuning table (exact shapes): the MFMA kernel variant (1..4: K tile 16/32 x LDS padding
 // 1/16) and split-K factor (1: none) chosen for (opA, opB, M, N, K, batch) by measurement.
 // gemm_tune times every candidate on scratch operands on stream s and installs the fastest;

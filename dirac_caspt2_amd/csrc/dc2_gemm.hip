@@ -387,6 +387,27 @@ __global__ void k_lower_from_upper(int64_t m, cplx* __restrict__ C, int64_t ldc)
   if (i > j) C[i + ldc * j] = cconj(C[j + ldc * i]);
 }
 
+// C = sum_kz W[kz] on and above the diagonal (the split-K partials of a Hermitian product,
+// summed in a fixed order), the strict lower triangle as the conjugate of the upper one
+__global__ void k_herk_reduce(int64_t m, int sk, const cplx* __restrict__ W, cplx* __restrict__ C, int64_t ldc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * m) return;
+  const int64_t i = t % m, j = t / m;
+  if (i > j) return;
+  cplx acc = mk(0.0, 0.0);
+  int z = 0;
+  for (; z + 4 <= sk; z += 4) {
+    cplx v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = W[t + m * m * (z + u)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc = cadd(acc, v[u]);
+  }
+  for (; z < sk; ++z) acc = cadd(acc, W[t + m * m * z]);
+  C[i + ldc * j] = acc;
+  if (i < j) C[j + ldc * i] = cconj(acc);
+}
+
 // C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order)
 __global__ void k_splitk_reduce(int64_t M, int64_t N, int sk, const cplx* __restrict__ W, cplx alpha, cplx beta,
                                 cplx* __restrict__ C, int64_t ldc) {
@@ -635,12 +656,35 @@ void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, c
 }
 
 // C (m x m, ldc) = A A^H for A (m x k, lda): the tiles on and above the diagonal on the MFMA
-// kernel, then the lower triangle as the conjugate of the upper one
-void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc) {
+// kernel, then the lower triangle as the conjugate of the upper one.  Few tiles over a long
+// k (the determinant-space Gram matrices: 91 upper tiles over k = 1728) are split over k
+// into about 1024 workgroups, at most 12 chunks of at least 128, summed in a fixed order by
+// k_herk_reduce (m = 792, k = 1728: 421 us unsplit, 207 / 162 / 164 / 162 / 145 us in 3 / 4
+// / 8 / 12 / 16 chunks).  split: 0 this rule, >= 1 that many k chunks (1: unsplit).
+void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc, int split) {
   if (m <= 0) return;
   DC2_CHECK(cdiv(m, BM) < 65536, 1, "zherk_upper: matrix too large for the grid");
+  DC2_CHECK(split >= 0 && split <= 64, 1, "zherk_upper: split 0..64");
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  dim3 grid(cdiv(m, BM), cdiv(m, BN), 1);
+  const int64_t T = cdiv(m, BM), tiles = T * (T + 1) / 2;
+  int64_t want = split;
+  if (want == 0 && tiles < 256 && k >= 256) want = std::min<int64_t>(std::min<int64_t>(k / 128, cdiv(1024, tiles)), 12);
+  int64_t kchunk = k;
+  int sk = 1;
+  if (want > 1 && k > 0) {
+    kchunk = (cdiv(k, want) + BK - 1) / BK * BK;
+    sk = (int)cdiv(k, kchunk);
+  }
+  if (sk > 1) {
+    cplx* W = splitk_workspace(s, (size_t)sk * m * m);
+    dim3 grid(T, T, (unsigned)sk);
+    hipLaunchKernelGGL((k_zgemm2<'N', 'C', 16, 1, true>), grid, dim3(256), 0, s, m, m, k, one, A, lda, (int64_t)0, A,
+                       lda, (int64_t)0, zero, W, m, m * m, sk, kchunk);
+    hipLaunchKernelGGL(k_herk_reduce, dim3(cdiv(m * m, 256)), dim3(256), 0, s, m, sk, W, C, ldc);
+    DC2_HIP(hipGetLastError());
+    return;
+  }
+  dim3 grid(T, T, 1);
   hipLaunchKernelGGL((k_zgemm2<'N', 'C', 16, 1, true>), grid, dim3(256), 0, s, m, m, k, one, A, lda, (int64_t)0, A,
                      lda, (int64_t)0, zero, C, ldc, (int64_t)0, 1, k);
   hipLaunchKernelGGL(k_lower_from_upper, dim3(cdiv(m * m, 256)), dim3(256), 0, s, m, C, ldc);

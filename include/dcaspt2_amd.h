@@ -125,6 +125,12 @@ int dc2_cdiag(dc2_context* ctx, double* c, int32_t n, int32_t* dimm, double* w, 
 int dc2_zgemm(dc2_context* ctx, char opA, char opB, int64_t M, int64_t N, int64_t K, const double* alpha,
               const double* A, int64_t lda, int64_t strideA, const double* B, int64_t ldb, int64_t strideB,
               const double* beta, double* C, int64_t ldc, int64_t strideC, int64_t batch);
+/* C (m x m, ldc) = A A^H for A (m x k, lda) on device memory, the whole Hermitian matrix
+ * (no counterpart in the reference, which forms the A/C metrics element by element: the
+ * Gram matrix of the determinant-space metric route).  split: 0 chosen by size, 1 unsplit,
+ * 2..64 that many chunks of k summed in a fixed order. */
+int dc2_zherk(dc2_context* ctx, int64_t m, int64_t k, const double* A, int64_t lda, double* C, int64_t ldc,
+              int32_t split);
 /* wait for all work queued on the library stream */
 int dc2_synchronize(dc2_context* ctx);
 /* GEMM launch accounting with HIP events on the library stream (resets on each call):

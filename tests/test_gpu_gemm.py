@@ -97,3 +97,26 @@ def test_autotuned_choice_still_exact(ctx, opA, opB, shape, batch):
         np.testing.assert_allclose(out[b].T, _op(A[b], opA) @ _op(B[b], opB), rtol=0, atol=1e-12 * max(1, K))
     assert gemm_clear_choices() >= 0
 
+
+
+@pytest.mark.parametrize("m,k", [(1, 1), (5, 3), (64, 64), (65, 300), (130, 97), (792, 1728), (200, 4096),
+                                 (300, 0)])
+@pytest.mark.parametrize("split", [0, 1, 3, 16])
+def test_zherk_whole_hermitian(ctx, m, k, split):
+    """A A^H from the upper tiles (split over k or not) and mirrored: the whole matrix
+    matches numpy, the mirrored triangle is the exact conjugate transpose."""
+    from dirac_caspt2_amd.api import DeviceArray, zherk
+    from gpu_helpers import say
+    say(f"zherk m={m} k={k} split={split}")
+    rng = np.random.default_rng(m * 11 + k + split)
+    A = rng.normal(size=(m, k)) + 1j * rng.normal(size=(m, k))
+    ldc = m + 3
+    dA = DeviceArray(ctx, np.asfortranarray(A).ravel(order="F") if k else np.zeros(1, dtype=np.complex128))
+    dC = DeviceArray(ctx, np.full(ldc * m, 7.0 + 7.0j))
+    zherk(ctx, m, k, dA, max(m, 1), dC, ldc, split)
+    ctx.synchronize()
+    out = dC.to_host().reshape((m, ldc)).T
+    np.testing.assert_allclose(out[:m], A @ A.conj().T, rtol=0, atol=1e-12 * max(1, k))
+    iu = np.triu_indices(m, 1)
+    assert np.array_equal(out[:m][(iu[1], iu[0])], np.conj(out[:m][iu]))
+    assert np.all(out[m:] == 7.0 + 7.0j)   # rows past m untouched
