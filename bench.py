@@ -434,7 +434,8 @@ def main():
     if ht_calls > 0 and ht_sec > dom["seconds"]:
         fl = 16.0 / 3.0 * n_g ** 3
         ach = fl * ht_calls / ht_sec / 1e12
-        roof = {"kernel": f"k_hetrd<16> (Householder tridiagonalisation, n = {n_g}, one grid barrier per column)",
+        rows = next(r for lim, r in ((128, 2), (256, 4), (512, 8), (768, 12), (832, 13), (1024, 16)) if n_g <= lim)
+        roof = {"kernel": f"k_hetrd<{rows}> (Householder tridiagonalisation, n = {n_g}, one grid barrier per column)",
                 "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.workload, "k_hetrd", n_g),
                 "launches_per_step": ht_calls / args.steps, "avg_launch_us": 1e6 * ht_sec / ht_calls,

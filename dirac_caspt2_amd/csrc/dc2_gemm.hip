@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int64_t M, int64_t N, int64_t Kfu
   const int64_t bz = blockIdx.z / sk, kz = blockIdx.z % sk;
   A += bz * sA;
   B += bz * sB;
-  C += (sk > 1 ? kz : bz) * sC;
+  C += (sk > 1 ? bz * sk + kz : bz) * sC;   // split K: partial kz of product bz
   const int64_t kbeg = kz * kchunk;
   const int64_t K = (kbeg + kchunk < Kfull) ? kbeg + kchunk : Kfull;   // loop end
   const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_zgemm2(int64_t M, int64_t N, int64_t Kf
   const int64_t bz = blockIdx.z / sk, kz = blockIdx.z % sk;
   A += bz * sA;
   B += bz * sB;
-  C += (sk > 1 ? kz : bz) * sC;
+  C += (sk > 1 ? bz * sk + kz : bz) * sC;   // split K: partial kz of product bz
   const int64_t kbeg = kz * kchunk;
   const int64_t kend = (kbeg + kchunk < Kfull) ? kbeg + kchunk : Kfull;
   const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
@@ -408,11 +408,14 @@ __global__ void k_herk_reduce(int64_t m, int sk, const cplx* __restrict__ W, cpl
   if (i < j) C[j + ldc * i] = cconj(acc);
 }
 
-// C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order)
+// C = alpha * sum_kz W[kz] + beta * C (the split-K partial products, fixed summation order);
+// blockIdx.y: the product of a batch (partials W + (bz sk + kz) M N, output C + bz sC)
 __global__ void k_splitk_reduce(int64_t M, int64_t N, int sk, const cplx* __restrict__ W, cplx alpha, cplx beta,
-                                cplx* __restrict__ C, int64_t ldc) {
+                                cplx* __restrict__ C, int64_t ldc, int64_t sC) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M * N) return;
+  W += (int64_t)blockIdx.y * sk * M * N;
+  C += (int64_t)blockIdx.y * sC;
   const int64_t m = i % M, n = i / M;
   cplx acc = mk(0.0, 0.0);
   int z = 0;
@@ -504,7 +507,7 @@ bool launch_skinny(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, c
     hipLaunchKernelGGL((k_zgemm_tall<OPB>), dim3((unsigned)rb, (unsigned)sk), dim3(256), 0, s, M, N, K, A, lda, B,
                        ldb, W, kchunk);
     hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, s, M, N, (int)sk, W, alpha, beta, C,
-                       ldc);
+                       ldc, (int64_t)0);
     return true;
   }
   return false;
@@ -548,7 +551,7 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
   const int64_t tiles = (int64_t)cdiv(M, BM) * cdiv(N, BN) * batch;
   int sk = 1;
   int64_t kchunk = K;
-  if (batch == 1 && tiles < 128 && K >= 256) {
+  if (tiles < 128 && K >= 256) {
     int64_t want = std::min<int64_t>(K / 128, (256 + tiles - 1) / tiles);
     want = std::max<int64_t>(2, std::min<int64_t>(want, 64));
     kchunk = ((K + want - 1) / want + BK - 1) / BK * BK;
@@ -564,7 +567,7 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
     variant = tc.variant;
     want_split = tc.split;
   }
-  if (want_split > 0 && batch == 1) {
+  if (want_split > 0) {
     sk = 1;
     kchunk = K;
     if (want_split > 1) {
@@ -572,12 +575,16 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
       sk = (int)((K + kchunk - 1) / kchunk);
     }
   }
+  if (batch * sk >= 65536) {   // one grid dimension holds batch x split
+    sk = 1;
+    kchunk = K;
+  }
   dim3 grid(cdiv(M, BM), cdiv(N, BN), (unsigned)(batch * sk));
   cplx* Cw = C;
   int64_t ldw = ldc, sw = sC;
   cplx a1 = alpha, b1 = beta;
   if (sk > 1) {
-    Cw = splitk_workspace(s, (size_t)sk * M * N);
+    Cw = splitk_workspace(s, (size_t)sk * M * N * batch);
     ldw = M;
     sw = M * N;
     a1 = mk(1.0, 0.0);
@@ -605,7 +612,8 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
                          ldw, sw, sk, kchunk);
   }
   if (sk > 1)
-    hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, s, M, N, sk, Cw, alpha, beta, C, ldc);
+    hipLaunchKernelGGL(k_splitk_reduce, dim3(cdiv(M * N, 256), (unsigned)batch), dim3(256), 0, s, M, N, sk, Cw, alpha,
+                       beta, C, ldc, sC);
   DC2_HIP(hipGetLastError());
   record(true);
 }
@@ -752,7 +760,7 @@ void gemm_tune(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t 
   int bv = 1, bs = 1;
   for (int v = 1; v <= 4; ++v)
     for (int sp : {1, 2, 4, 8, 16, 32}) {
-      if (sp > 1 && (batch != 1 || K < 2 * BK * sp || tiles * sp > 4096)) continue;
+      if (sp > 1 && (K < 2 * BK * sp || tiles * sp > 4096 || batch * sp >= 65536)) continue;
       const double t = time(v, sp);
       if (t < best) { best = t; bv = v; bs = sp; }
     }

@@ -40,7 +40,7 @@ namespace {
 
 constexpr int HT_WAVES = 8;
 constexpr int HT_THREADS = 64 * HT_WAVES;
-constexpr int HT_CPW = 2;                       // columns per wave
+constexpr int HT_CPW = 2;                       // columns per wave (4 spills registers at n > 768)
 constexpr int HT_COLS = HT_WAVES * HT_CPW;      // columns per workgroup
 constexpr int HT_NMAX = 1024;
 
@@ -553,10 +553,12 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
   {
     CoopSlot slot(nwg, cus);
     DC2_HIP(hipEventRecord(ev[0], s));
+    // S = rows per lane (n <= 64 S)
     if (n <= 128) st = launch_hetrd<2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 256) st = launch_hetrd<4>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 512) st = launch_hetrd<8>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 768) st = launch_hetrd<12>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 832) st = launch_hetrd<13>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     if (st != hipSuccess) {
       (void)hipGetLastError();

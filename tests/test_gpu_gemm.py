@@ -32,7 +32,10 @@ def _op(x, op):
                                    # at most 8 columns over a long K, and their edges
                                    (48, 6, 924), (6, 6, 924), (924, 1, 924), (45, 7, 63), (2048, 1, 3),
                                    (924, 6, 48), (924, 6, 64), (255, 8, 65), (924, 6, 924), (3000, 5, 4097),
-                                   (33, 9, 700)])
+                                   (33, 9, 700),
+                                   # K <= 16 with one output dimension <= 16 (index rotations
+                                   # of the transforms: the MFMA kernel)
+                                   (4100, 16, 16), (4100, 5, 12), (12, 4100, 12), (7, 3000, 16), (16, 300, 9)])
 def test_zgemm_ops(ctx, opA, opB, shape):
     from dirac_caspt2_amd.api import DeviceArray, zgemm
     from gpu_helpers import say
@@ -120,3 +123,59 @@ def test_zherk_whole_hermitian(ctx, m, k, split):
     iu = np.triu_indices(m, 1)
     assert np.array_equal(out[:m][(iu[1], iu[0])], np.conj(out[:m][iu]))
     assert np.all(out[m:] == 7.0 + 7.0j)   # rows past m untouched
+
+
+@pytest.mark.parametrize("opA,opB,M,N,K,nb", [("C", "N", 64, 64, 792, 13), ("N", "N", 40, 24, 700, 5),
+                                               ("T", "R", 17, 9, 1000, 3), ("N", "C", 130, 70, 520, 2)])
+def test_zgemm_batched_split_k(ctx, opA, opB, M, N, K, nb):
+    """Batched products with few output tiles over a long K are split over K (partials of
+    every product summed in a fixed order into its own output): padded strides, alpha and
+    beta, each product against numpy."""
+    from dirac_caspt2_amd.api import DeviceArray, zgemm
+    from gpu_helpers import say
+    say(f"batched split-K {opA}{opB} {M}x{N}x{K} batch {nb}")
+    rng = np.random.default_rng(M + N + K + nb)
+    a_shape = (M, K) if opA == "N" else (K, M)
+    b_shape = (K, N) if opB in "NR" else (N, K)
+    sA, sB, sC = a_shape[0] * a_shape[1] + 5, b_shape[0] * b_shape[1] + 3, M * N + 7
+    A = [rng.normal(size=a_shape) + 1j * rng.normal(size=a_shape) for _ in range(nb)]
+    B = [rng.normal(size=b_shape) + 1j * rng.normal(size=b_shape) for _ in range(nb)]
+    C = [rng.normal(size=(M, N)) + 1j * rng.normal(size=(M, N)) for _ in range(nb)]
+
+    def pack(xs, stride):
+        buf = np.full(stride * len(xs), 3.0 - 1.0j)
+        for b, x in enumerate(xs):
+            buf[b * stride:b * stride + x.size] = np.asfortranarray(x).ravel(order="F")
+        return buf
+    dA, dB, dC = DeviceArray(ctx, pack(A, sA)), DeviceArray(ctx, pack(B, sB)), DeviceArray(ctx, pack(C, sC))
+    alpha, beta = 0.4 + 0.9j, -0.6 + 0.1j
+    zgemm(ctx, opA, opB, M, N, K, alpha, dA, a_shape[0], sA, dB, b_shape[0], sB, beta, dC, M, sC, nb)
+    ctx.synchronize()
+    out = dC.to_host()
+    for b in range(nb):
+        ref = alpha * _op(A[b], opA) @ _op(B[b], opB) + beta * C[b]
+        got = out[b * sC:b * sC + M * N].reshape((M, N), order="F")
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * K)
+        assert np.all(out[b * sC + M * N:(b + 1) * sC] == 3.0 - 1.0j)   # padding untouched
+
+
+@pytest.mark.parametrize("opA,opB", [("T", "N"), ("C", "R"), ("C", "N")])
+@pytest.mark.parametrize("M,N,K", [(5000, 16, 16), (12, 5000, 12), (300, 11, 7)])
+def test_zgemm_short_k_padded(ctx, opA, opB, M, N, K):
+    """Short-K index rotations with leading dimensions larger than the operands: rows of C
+    past M stay untouched."""
+    from dirac_caspt2_amd.api import DeviceArray, zgemm
+    rng = np.random.default_rng(M + 3 * N + K)
+    lda, ldb, ldc = K + 3, K + 1, M + 2
+    A = rng.normal(size=(lda, M)) + 1j * rng.normal(size=(lda, M))
+    B = rng.normal(size=(ldb, N)) + 1j * rng.normal(size=(ldb, N))
+    C = rng.normal(size=(ldc, N)) + 1j * rng.normal(size=(ldc, N))
+    alpha, beta = 1.1 - 0.2j, 0.3 + 0.3j
+    dev = lambda x: DeviceArray(ctx, np.asfortranarray(x).ravel(order="F"))
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    zgemm(ctx, opA, opB, M, N, K, alpha, dA, lda, 0, dB, ldb, 0, beta, dC, ldc, 0, 1)
+    ctx.synchronize()
+    out = dC.to_host().reshape((ldc, N), order="F")
+    ref = alpha * _op(A[:K], opA) @ _op(B[:K], opB) + beta * C[:M]
+    np.testing.assert_allclose(out[:M], ref, rtol=0, atol=1e-12 * K)
+    assert np.array_equal(out[M:], C[M:])
