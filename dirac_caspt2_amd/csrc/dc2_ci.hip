@@ -516,6 +516,16 @@ __global__ void k_fill_zero(cplx* p, int64_t n) {
   if (i < n) p[i] = mk(0.0, 0.0);
 }
 
+__global__ void k_flag_clear(int* flag) { *flag = 0; }
+
+// flag = 1 when any of the n doubles is not finite (each workgroup that finds one stores 1)
+__global__ __launch_bounds__(256) void k_nonfinite(const double* __restrict__ p, int64_t n, int* __restrict__ flag) {
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    bad = bad || !isfinite(p[i]);
+  if (__syncthreads_or(bad) && threadIdx.x == 0) *flag = 1;
+}
+
 __global__ void k_scale_cols_tail(int64_t n, int64_t dimm, const cplx* vs, const double* ws, int inverse,
                                   cplx* uc) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
@@ -755,6 +765,18 @@ void fill_zero(hipStream_t s, cplx* p, int64_t n) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_zero, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n);
   DC2_HIP(hipGetLastError());
+}
+
+bool all_finite_dev(hipStream_t s, const double* p, int64_t n, int* flag_dev) {
+  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, flag_dev);
+  if (n > 0)
+    hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)std::min<int64_t>(cdiv(n, 256 * 8), 1024)), dim3(256), 0, s, p, n,
+                       flag_dev);
+  DC2_HIP(hipGetLastError());
+  int flag = 0;
+  DC2_HIP(hipMemcpyAsync(&flag, flag_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipStreamSynchronize(s));
+  return flag == 0;
 }
 
 void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc,

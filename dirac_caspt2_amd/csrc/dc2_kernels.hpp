@@ -54,6 +54,9 @@ void permute6(hipStream_t s, const int* dims_out, const int* src_stride_of_out_a
 // copy a 4-index sub-block of a column-major tensor
 void extract4(hipStream_t s, const cplx* in, const int64_t* ld_in, const int* off, const int* n, cplx* out);
 void fill_zero(hipStream_t s, cplx* p, int64_t n);
+// true when none of the n doubles at p is NaN or infinite (a device reduction into flag_dev,
+// then a 4-byte copy); synchronises the stream
+bool all_finite_dev(hipStream_t s, const double* p, int64_t n, int* flag_dev);
 // uc[:, c] = vs[:, n-dimm+c] * ws[n-dimm+c]^(-1/2)  (or ^-1 with inverse)
 void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc,
                      bool inverse = false);
