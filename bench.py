@@ -397,7 +397,6 @@ def main():
         out = step()
     ctx.synchronize()
     hetrd_stats(0)
-    ctx.set_profiling(True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -406,6 +405,16 @@ def main():
         lat.append(time.perf_counter() - t)
     ctx.synchronize()
     total = time.perf_counter() - t0
+    # the tridiagonalisation is timed with HIP events around every launch, timed steps included
+    n_g = max(comb(spec.nact, spec.nelec + 1), comb(spec.nact, spec.nelec - 1))
+    ht_calls, ht_sec = hetrd_stats(n_g)
+    # per-launch GEMM accounting (two HIP events per launch, Davidson graphs off) over extra
+    # untimed steps, so that it does not slow the timed ones
+    prof_steps = max(1, min(args.steps, 5))
+    ctx.set_profiling(True)
+    for _ in range(prof_steps):
+        step()
+    ctx.synchronize()
     g_all = ctx.gemm_stats("all")
     g_tr = ctx.gemm_stats("transform")
     ops = {f"{oa}{ob}": ctx.gemm_op_stats(oa, ob) for oa in "NTC" for ob in "NRTC"}
@@ -414,24 +423,22 @@ def main():
     stage = out.timings
     lat_ms = np.array(lat) * 1e3
 
-    # Roofline of the dominant hand-written kernel: the one with the largest summed HIP-event
-    # time over the timed steps among the k_zgemm<opA,opB> instantiations and the
-    # tridiagonalisation of the largest metric (the A/C determinant-space G, n = C(nact,
+    # Roofline of the dominant hand-written kernel: the one with the largest HIP-event time per
+    # step among the k_zgemm<opA,opB> instantiations (over the accounting steps) and the
+    # tridiagonalisation of the largest metric (over the timed steps) (the A/C determinant-space G, n = C(nact,
     # nelec+-1)).  Algorithmic flops per launch: 8*M*N*K per complex GEMM, 16/3 n^3 for the
     # Householder tridiagonalisation of an n x n Hermitian matrix (LAPACK's zhetrd count).
-    n_g = max(comb(spec.nact, spec.nelec + 1), comb(spec.nact, spec.nelec - 1))
-    ht_calls, ht_sec = hetrd_stats(n_g)
     achieved_g = dom["flops"] / dom["seconds"] / 1e12 if dom["seconds"] > 0 else 0.0
     ach_all = g_all["flops"] / g_all["seconds"] / 1e12 if g_all["seconds"] > 0 else 0.0
     ach_tr = g_tr["flops"] / g_tr["seconds"] / 1e12 if g_tr["seconds"] > 0 else 0.0
     gemm_roof = {"kernel": f"k_zgemm2<'{dom_op[0]}','{dom_op[1]}',16,1> (MFMA FP64 complex GEMM)",
                  "bound": "mfma", "achieved": achieved_g, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                  "frac": achieved_g / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.workload, "k_zgemm", dom_op),
-                 "launches_per_step": dom["launches"] / args.steps,
+                 "launches_per_step": dom["launches"] / prof_steps,
                  "avg_launch_us": 1e6 * dom["seconds"] / max(dom["launches"], 1),
                  "algorithmic_flops_per_launch": dom["flops"] / max(dom["launches"], 1),
-                 "seconds_per_step": dom["seconds"] / args.steps}
-    if ht_calls > 0 and ht_sec > dom["seconds"]:
+                 "seconds_per_step": dom["seconds"] / prof_steps}
+    if ht_calls > 0 and ht_sec / args.steps > dom["seconds"] / prof_steps:
         fl = 16.0 / 3.0 * n_g ** 3
         ach = fl * ht_calls / ht_sec / 1e12
         rows = next(r for lim, r in ((128, 2), (256, 4), (512, 8), (768, 12), (832, 13), (1024, 16)) if n_g <= lim)
@@ -443,7 +450,7 @@ def main():
                 "dominant_gemm": gemm_roof}
     else:
         roof = gemm_roof
-    roof["all_gemm_launches_per_step"] = g_all["launches"] / args.steps
+    roof["all_gemm_launches_per_step"] = g_all["launches"] / prof_steps
     roof["all_gemm_achieved_tflops"] = ach_all
     roof["transform_launches_achieved_tflops"] = ach_tr
     result = {
