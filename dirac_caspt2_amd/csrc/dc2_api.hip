@@ -81,6 +81,23 @@ struct Lane {
     b.ensure((size_t)std::max<int64_t>(n, 1));
     return b.get();
   }
+  // grow-only page-locked host memory for the lane's per-iteration transfers (pageable
+  // copies go through the runtime's staging buffer, which concurrent threads share)
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  void* host(size_t bytes) {
+    if (bytes > pinned_bytes) {
+      if (pinned) {
+        DC2_HIP(hipStreamSynchronize(stream));   // no copy still reads or writes it
+        DC2_HIP(hipHostFree(pinned));
+        pinned = nullptr;
+        pinned_bytes = 0;
+      }
+      DC2_HIP(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
+      pinned_bytes = bytes;
+    }
+    return pinned;
+  }
   Lane() = default;
   Lane(const Lane&) = delete;
   Lane& operator=(const Lane&) = delete;
@@ -91,6 +108,7 @@ struct Lane {
     lb.clear();
     ib.clear();
     info.release();
+    if (pinned) (void)hipHostFree(pinned);
     if (owned) {
       if (blas) rocblas_destroy_handle(blas);
       if (stream) {
@@ -463,9 +481,15 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   cplx* Sc = L.z("dv_sc", (int64_t)b * b);
   double* d = L.d("dv_d", n);
   int* idx = L.i("dv_idx", b);
-  // host sources of the asynchronous uploads stay alive until the next synchronising download
-  std::vector<zc> up_y((size_t)maxsub * b + b), up_sc((size_t)b * b);
-  std::vector<int> up_idx(b);
+  // page-locked host side of the per-iteration transfers: downloads Ch (V^H W_new) and rg
+  // (R^H R, T^H T); uploads up_y (Ritz coefficients, theta), up_sc, up_idx, whose contents
+  // stay untouched until the next synchronising download
+  const size_t nCh = (size_t)maxsub * b, nrg = 2 * (size_t)b * b, ny = (size_t)maxsub * b + b, nsc = (size_t)b * b;
+  zc* const Ch = reinterpret_cast<zc*>(L.host((nCh + nrg + ny + nsc) * sizeof(zc) + b * sizeof(int)));
+  zc* const rg = Ch + nCh;
+  zc* const up_y = rg + nrg;
+  zc* const up_sc = up_y + ny;
+  int* const up_idx = reinterpret_cast<int*>(up_sc + nsc);
   auto to_host = [&](void* dst, const void* src, size_t bytes) {
     DC2_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     DC2_HIP(hipStreamSynchronize(s));
@@ -480,8 +504,8 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::partial_sort(order.begin(), order.begin() + b, order.end(), [&](int x, int y) { return dh[x] < dh[y]; });
-  std::copy(order.begin(), order.begin() + b, up_idx.begin());
-  to_dev(idx, up_idx.data(), b * sizeof(int));
+  std::copy(order.begin(), order.begin() + b, up_idx);
+  to_dev(idx, up_idx, b * sizeof(int));
   davidson_unit(s, n, b, idx, V);
   zgemm(s, 'N', 'N', n, b, n, one, H, n, 0, V, n, 0, zero, W, n, 0, 1);
   std::vector<int> seq(b);
@@ -490,14 +514,14 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   to_dev(sel, seq.data(), b * sizeof(int));
   int m = b, nk = b;   // subspace size; the last nk columns are new
   const double tol = 1e-11;
-  std::vector<zc> Th((size_t)maxsub * maxsub), Tm, Yh, G, Q, Ch((size_t)maxsub * b), rg(2 * (size_t)b * b);
+  std::vector<zc> Th((size_t)maxsub * maxsub), Tm, Yh, G, Q;
   std::vector<double> theta, gw;
   double* th = reinterpret_cast<double*>(Sy + (size_t)maxsub * b);
   for (int it = 1; it <= 200; ++it) {
     c->davidson_iters = it;
     // Rayleigh matrix: the new columns V^H W[:, m-nk:m]
     zgemm(s, 'C', 'N', m, nk, n, one, V, n, 0, W + n * (m - nk), n, 0, zero, Cn, m, 0, 1);
-    to_host(Ch.data(), Cn, (size_t)m * nk * sizeof(cplx));
+    to_host(Ch, Cn, (size_t)m * nk * sizeof(cplx));
     for (int j = 0; j < nk; ++j)
       for (int i = 0; i < m; ++i) {
         const zc x = Ch[i + (size_t)m * j];
@@ -517,10 +541,10 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     theta.resize(m);
     Yh.resize((size_t)m * m);
     herm_eig(m, Tm.data(), theta.data(), Yh.data());
-    std::copy(Yh.begin(), Yh.begin() + (size_t)m * b, up_y.begin());     // first b Ritz vectors
-    std::memcpy(up_y.data() + (size_t)maxsub * b, theta.data(), b * sizeof(double));
-    to_dev(Sy, up_y.data(), (size_t)m * b * sizeof(cplx));
-    to_dev(th, up_y.data() + (size_t)maxsub * b, b * sizeof(double));
+    std::copy(Yh.begin(), Yh.begin() + (size_t)m * b, up_y);     // first b Ritz vectors
+    std::memcpy(up_y + (size_t)maxsub * b, theta.data(), b * sizeof(double));
+    to_dev(Sy, up_y, (size_t)m * b * sizeof(cplx));
+    to_dev(th, up_y + (size_t)maxsub * b, b * sizeof(double));
     const bool restart = m + b > maxsub;
     const int m_in = m;
     // Ritz vectors and residuals, preconditioned corrections, (restart,) two classical
@@ -584,7 +608,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       segment(m_in, restart);
     }
     if (restart) m = b;
-    to_host(rg.data(), RG, rg.size() * sizeof(cplx));
+    to_host(rg, RG, nrg * sizeof(cplx));
     double worst = 0.0;
     for (int j = 0; j < nroot; ++j) worst = std::max(worst, std::sqrt(std::fabs(rg[j + (size_t)b * j].real())));
     if (worst < tol) {
@@ -599,7 +623,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       for (int j = 0; j < b; ++j)
         for (int i = 0; i < b; ++i) Th[i + (size_t)maxsub * j] = (i == j) ? zc(theta[j], 0.0) : zc(0.0, 0.0);
     }
-    G.assign(rg.begin() + (size_t)b * b, rg.end());
+    G.assign(rg + (size_t)b * b, rg + nrg);
     Q.resize(G.size());
     gw.resize(b);
     herm_eig(b, G.data(), gw.data(), Q.data());
@@ -610,7 +634,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       ++nk;
     }
     if (nk == 0) return false;
-    to_dev(Sc, up_sc.data(), (size_t)b * nk * sizeof(cplx));
+    to_dev(Sc, up_sc, (size_t)b * nk * sizeof(cplx));
     zgemm(s, 'N', 'N', n, nk, b, one, T, n, 0, Sc, b, 0, zero, V + n * m, n, 0, 1);
     zgemm(s, 'N', 'N', n, nk, n, one, H, n, 0, V + n * m, n, 0, zero, W + n * m, n, 0, 1);
     m += nk;
