@@ -81,22 +81,31 @@ struct Lane {
     b.ensure((size_t)std::max<int64_t>(n, 1));
     return b.get();
   }
-  // grow-only page-locked host memory for the lane's per-iteration transfers (pageable
-  // copies go through the runtime's staging buffer, which concurrent threads share)
-  void* pinned = nullptr;
-  size_t pinned_bytes = 0;
-  void* host(size_t bytes) {
-    if (bytes > pinned_bytes) {
-      if (pinned) {
+  // grow-only page-locked host memory for the lane's transfers (pageable copies go through
+  // the runtime's staging buffer, which concurrent threads share): slot 0 the Davidson's
+  // per-iteration arrays, slot 1 the downloads of solver results (download())
+  void* pinned[2] = {nullptr, nullptr};
+  size_t pinned_bytes[2] = {0, 0};
+  void* host(size_t bytes, int slot = 0) {
+    if (bytes > pinned_bytes[slot]) {
+      if (pinned[slot]) {
         DC2_HIP(hipStreamSynchronize(stream));   // no copy still reads or writes it
-        DC2_HIP(hipHostFree(pinned));
-        pinned = nullptr;
-        pinned_bytes = 0;
+        DC2_HIP(hipHostFree(pinned[slot]));
+        pinned[slot] = nullptr;
+        pinned_bytes[slot] = 0;
       }
-      DC2_HIP(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
-      pinned_bytes = bytes;
+      DC2_HIP(hipHostMalloc(&pinned[slot], bytes, hipHostMallocDefault));
+      pinned_bytes[slot] = bytes;
     }
-    return pinned;
+    return pinned[slot];
+  }
+  // bytes from device memory to dst on the host through slot 1; synchronises the stream
+  void download(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    void* h = host(bytes, 1);
+    DC2_HIP(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, stream));
+    DC2_HIP(hipStreamSynchronize(stream));
+    std::memcpy(dst, h, bytes);
   }
   Lane() = default;
   Lane(const Lane&) = delete;
@@ -108,7 +117,8 @@ struct Lane {
     lb.clear();
     ib.clear();
     info.release();
-    if (pinned) (void)hipHostFree(pinned);
+    for (void* h : pinned)
+      if (h) (void)hipHostFree(h);
     if (owned) {
       if (blas) rocblas_destroy_handle(blas);
       if (stream) {
@@ -299,7 +309,8 @@ void check_finite(dc2_context* c, const char* what, const T* p, int64_t count) {
 // every element of the eigensolver output finite: checked on the device (a 792 x 792
 // result copied to pageable host memory for the scan took ~3 ms)
 bool all_finite(Lane& L, const cplx* A, int64_t count) {
-  return all_finite_dev(L.stream, reinterpret_cast<const double*>(A), 2 * count, L.i("finite_flag", 1));
+  return all_finite_dev(L.stream, reinterpret_cast<const double*>(A), 2 * count, L.i("finite_flag", 1),
+                        reinterpret_cast<int*>(L.host(sizeof(int), 1)));
 }
 
 int run_eig(Lane& L, int method, int n, cplx* A, double* w) {
@@ -324,8 +335,7 @@ int run_eig(Lane& L, int method, int n, cplx* A, double* w) {
                                     L.info.get()));
   }
   int info = 0;
-  DC2_HIP(hipMemcpyAsync(&info, L.info.get(), sizeof(int), hipMemcpyDeviceToHost, L.stream));
-  DC2_HIP(hipStreamSynchronize(L.stream));
+  L.download(&info, L.info.get(), sizeof(int));
   return info;
 }
 
@@ -339,18 +349,21 @@ enum class EigKind { general, rank_deficient };
 // Eigenpairs of the Hermitian matrix whose upper triangle A (n x n, device) holds, as the
 // device solvers read it, on the host (herm_eig): eigenvectors overwrite A, ascending
 // eigenvalues go to w.  Synchronises the stream.
-void host_heev_upper(hipStream_t s, int n, cplx* A, double* w) {
-  std::vector<zc> hA((size_t)n * n), hV((size_t)n * n);
-  std::vector<double> hw(n);
-  DC2_HIP(hipMemcpyAsync(hA.data(), A, hA.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
-  DC2_HIP(hipStreamSynchronize(s));
+void host_heev_upper(Lane& L, int n, cplx* A, double* w) {
+  const hipStream_t s = L.stream;
+  const size_t nn = (size_t)n * n;
+  std::vector<zc> hA(nn);
+  L.download(hA.data(), A, nn * sizeof(cplx));
   for (int j = 0; j < n; ++j) {
     hA[j + (size_t)n * j] = hA[j + (size_t)n * j].real();
     for (int i = 0; i < j; ++i) hA[j + (size_t)n * i] = std::conj(hA[i + (size_t)n * j]);
   }
-  herm_eig(n, hA.data(), hw.data(), hV.data());
-  DC2_HIP(hipMemcpyAsync(A, hV.data(), hV.size() * sizeof(cplx), hipMemcpyHostToDevice, s));
-  DC2_HIP(hipMemcpyAsync(w, hw.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  // eigenvectors and eigenvalues written straight into page-locked memory, uploaded from it
+  zc* hV = reinterpret_cast<zc*>(L.host(nn * sizeof(zc) + n * sizeof(double), 1));
+  double* hw = reinterpret_cast<double*>(hV + nn);
+  herm_eig(n, hA.data(), hw, hV);
+  DC2_HIP(hipMemcpyAsync(A, hV, nn * sizeof(cplx), hipMemcpyHostToDevice, s));
+  DC2_HIP(hipMemcpyAsync(w, hw, n * sizeof(double), hipMemcpyHostToDevice, s));
   DC2_HIP(hipStreamSynchronize(s));   // the host arrays are the sources of the copies
 }
 
@@ -363,7 +376,7 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
             EigKind kind = EigKind::general) {
   if (n <= 0) return;
   if (c->eig_method < 0 && n <= kHostEigMax) {
-    host_heev_upper(L.stream, n, A, w);
+    host_heev_upper(L, n, A, w);
     check_finite(c, L.stream, what, A, (int64_t)n * n);
     return;
   }
@@ -380,10 +393,10 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
     // or its result is not usable
     void* work = L.d("eig_native", heev_native_work_bytes(n) / 8 + 1);
     L.info.ensure(2);
-    bool done = heev_native(L.stream, L.blas, n, A, n, w, work, L.info.get());
+    bool done = heev_native(L.stream, L.blas, n, A, n, w, work, L.info.get(),
+                            reinterpret_cast<int*>(L.host(2 * sizeof(int), 1)));
     if (done) {
-      DC2_HIP(hipMemcpyAsync(&info, L.info.get(), sizeof(int), hipMemcpyDeviceToHost, L.stream));
-      DC2_HIP(hipStreamSynchronize(L.stream));
+      L.download(&info, L.info.get(), sizeof(int));
       done = info == 0 && all_finite(L, A, nn);
     }
     if (!done) {
@@ -756,8 +769,7 @@ struct SolveOut {
 // e2 and sumc2 from the two-double result of a reduction on lane L
 void read_out2(Lane& L, const double* out2, SolveOut& r) {
   double o[2];
-  DC2_HIP(hipMemcpyAsync(o, out2, sizeof(o), hipMemcpyDeviceToHost, L.stream));
-  DC2_HIP(hipStreamSynchronize(L.stream));
+  L.download(o, out2, sizeof(o));
   r.e2 = o[0];
   r.sumc2 = o[1];
 }
@@ -787,8 +799,7 @@ void finish_block(dc2_context* c, Lane& L, int64_t dimm, int64_t npert, cplx* bc
 // of the reference's S matrix (2 for the antisymmetric reduction of A)
 int64_t count_retained(Lane& L, int64_t n, const double* w_dev, double& margin, double scale = 1.0) {
   std::vector<double> w(n);
-  DC2_HIP(hipMemcpyAsync(w.data(), w_dev, n * sizeof(double), hipMemcpyDeviceToHost, L.stream));
-  DC2_HIP(hipStreamSynchronize(L.stream));
+  L.download(w.data(), w_dev, n * sizeof(double));
   const double thr = 1.0e-8;
   int64_t dimm = 0;
   for (int64_t k = 0; k < n; ++k) {
@@ -1369,7 +1380,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       cplx* scr = L0.z("fd_scr", (int64_t)n * n);
       double* w = L0.d("fd_w", n);
       gather_block(s, n, dind, c->fock.get(), nmo, blk);
-      if (n <= kHostFockEig) host_heev_upper(s, n, blk, w);
+      if (n <= kHostFockEig) host_heev_upper(L0, n, blk, w);
       else zheevd(c, n, blk, w, "a Fock block");
       canonical_gauge(s, n, w, blk, scr, L0.i("fd_starts", n + 2));
       scatter_block(s, n, dind, blk, w, c->U.get(), nmo, c->eps.get());

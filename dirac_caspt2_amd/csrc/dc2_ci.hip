@@ -767,16 +767,15 @@ void fill_zero(hipStream_t s, cplx* p, int64_t n) {
   DC2_HIP(hipGetLastError());
 }
 
-bool all_finite_dev(hipStream_t s, const double* p, int64_t n, int* flag_dev) {
+bool all_finite_dev(hipStream_t s, const double* p, int64_t n, int* flag_dev, int* flag_host) {
   hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, flag_dev);
   if (n > 0)
     hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)std::min<int64_t>(cdiv(n, 256 * 8), 1024)), dim3(256), 0, s, p, n,
                        flag_dev);
   DC2_HIP(hipGetLastError());
-  int flag = 0;
-  DC2_HIP(hipMemcpyAsync(&flag, flag_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+  DC2_HIP(hipMemcpyAsync(flag_host, flag_dev, sizeof(int), hipMemcpyDeviceToHost, s));
   DC2_HIP(hipStreamSynchronize(s));
-  return flag == 0;
+  return *flag_host == 0;
 }
 
 void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc,

@@ -506,7 +506,8 @@ int64_t heev_native_work_bytes(int n) {
   return b;
 }
 
-bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev) {
+bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev,
+                 int* flags_host) {
   DC2_CHECK(n >= 3 && n <= HT_NMAX, 1, "heev_native: n must be in 3..1024");
   const int nwg = (int)cdiv(n, HT_COLS);
   const int64_t nn = (int64_t)n * n;
@@ -565,10 +566,9 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
       return false;   // e.g. the workgroup does not fit a compute unit: the caller uses rocSOLVER
     }
     DC2_HIP(hipEventRecord(ev[1], s));
-    int flag[2] = {0, 0};
-    DC2_HIP(hipMemcpyAsync(flag, flags, sizeof(flag), hipMemcpyDeviceToHost, s));
+    DC2_HIP(hipMemcpyAsync(flags_host, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     DC2_HIP(hipStreamSynchronize(s));
-    if (flag[1] != 0) return false;
+    if (flags_host[1] != 0) return false;
     float ms = 0.0f;
     DC2_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     std::lock_guard<std::mutex> lk(g_hetrd_mu);

@@ -55,8 +55,8 @@ void permute6(hipStream_t s, const int* dims_out, const int* src_stride_of_out_a
 void extract4(hipStream_t s, const cplx* in, const int64_t* ld_in, const int* off, const int* n, cplx* out);
 void fill_zero(hipStream_t s, cplx* p, int64_t n);
 // true when none of the n doubles at p is NaN or infinite (a device reduction into flag_dev,
-// then a 4-byte copy); synchronises the stream
-bool all_finite_dev(hipStream_t s, const double* p, int64_t n, int* flag_dev);
+// then a 4-byte copy to flag_host, page-locked host memory); synchronises the stream
+bool all_finite_dev(hipStream_t s, const double* p, int64_t n, int* flag_dev, int* flag_host);
 // uc[:, c] = vs[:, n-dimm+c] * ws[n-dimm+c]^(-1/2)  (or ^-1 with inverse)
 void scale_cols_tail(hipStream_t s, int64_t n, int64_t dimm, const cplx* vs, const double* ws, cplx* uc,
                      bool inverse = false);
@@ -97,9 +97,11 @@ void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const in
 // Hermitian eigendecomposition of A (n x n, 3 <= n <= 1024, the matrix its upper triangle
 // defines): eigenvectors over A, ascending eigenvalues in w; dstedc's info in info_dev.  false:
 // nothing usable was computed (the caller falls back to rocSOLVER).  work: at least
-// heev_native_work_bytes(n) bytes of device memory.
+// heev_native_work_bytes(n) bytes of device memory; flags_host: two ints of page-locked host
+// memory for the reduction's status.
 int64_t heev_native_work_bytes(int n);
-bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev);
+bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev,
+                 int* flags_host);
 // {calls, summed device seconds} of the tridiagonalisation kernel for size n in this process
 void hetrd_stats(int n, double* out);
 void hetrd_stats_reset();
