@@ -1420,8 +1420,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
         DC2_ROCSOLVER(rocsolver_zgetrf(c->blas, (int)nd, (int)nd, reinterpret_cast<rocblas_double_complex*>(ds),
                                        (int)nd, ipiv, c->info.get()));
         int info = 0;
-        DC2_HIP(hipMemcpyAsync(&info, c->info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        DC2_HIP(hipStreamSynchronize(s));
+        L0.download(&info, c->info.get(), sizeof(int));
         if (info != 0) throw Error(100 + info, "zgetrf (tracic) failed, info = " + std::to_string(info));
         DC2_ROCSOLVER(rocsolver_zgetrs(c->blas, rocblas_operation_none, (int)nd, 1,
                                        reinterpret_cast<rocblas_double_complex*>(ds), (int)nd, ipiv,
@@ -1543,8 +1542,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   for (auto& e : errs)
     if (e) std::rethrow_exception(e);
   double e0h = 0.0;
-  DC2_HIP(hipMemcpyAsync(&e0h, c->e0.get(), sizeof(double), hipMemcpyDeviceToHost, s));
-  DC2_HIP(hipStreamSynchronize(s));
+  L0.download(&e0h, c->e0.get(), sizeof(double));
   double e2all = 0.0, s2 = 0.0;
   for (int k = 0; k < 8; ++k) {
     const double fe = e2_file ? e2_file[k] : 0.0, fs = sumc2_file ? sumc2_file[k] : 0.0;
