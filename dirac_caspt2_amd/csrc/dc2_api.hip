@@ -1681,7 +1681,11 @@ int dc2_set_one_electron(dc2_context* c, const double* h, double ecore) {
     need_spaces(c);
     const int64_t n = (int64_t)c->nmo() * c->nmo();
     c->h.ensure(n);
-    DC2_HIP(hipMemcpyAsync(c->h.get(), h, n * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+    // through the lane's page-locked memory: a pageable upload waits on the runtime's staging
+    // buffer, which the slots of a request server share
+    void* hp = c->lane0->host(n * sizeof(cplx), 1);
+    std::memcpy(hp, h, n * sizeof(cplx));
+    DC2_HIP(hipMemcpyAsync(c->h.get(), hp, n * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
     DC2_HIP(hipStreamSynchronize(c->stream));
     c->ecore = ecore;
   });
