@@ -7,10 +7,13 @@ integral transforms, subspaces A-H) for the 192-spinor workload
 complex128, seeded synthetic Kramers-paired integrals), with the integrals
 already resident in HBM.  Only one GPU is used.
 
-    python bench.py [--steps K] [--warmup W] [--workload fp64|small|split]
+    python bench.py [--steps K] [--warmup W] [--workload fp64|small|split|batch|serve]
+                    [--served N] [--slots S] [--split SLICES] [--no-autotune]
                     [--no-cpu-baseline] [--copy-calibration MIB]
 
-Prints ONE JSON line.
+Prints ONE JSON line.  With the default workload the line also carries the split-precision
+rate of the same steps, the served run (N scan points through the request server), the
+roofline of the dominant kernel and the CPU baseline.
 """
 from __future__ import annotations
 
