@@ -146,6 +146,70 @@ module dcaspt2_amd
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function
+        ! CASCI result readers (CIDATA contents: determinants and CI vectors)
+        integer(c_int) function dc2_get_dets(ctx, dets) bind(C, name="dc2_get_dets")
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), intent(out) :: dets(*)
+        end function
+        integer(c_int) function dc2_get_ci_vector(ctx, root, c) bind(C, name="dc2_get_ci_vector")
+            import :: c_int, c_int32_t, c_double_complex, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: root
+            complex(c_double_complex), intent(out) :: c(*)
+        end function
+        ! request server: many energies on one GPU (no counterpart in the reference)
+        integer(c_int) function dc2_request_footprint(ninact, nact, nsec, nelec, bytes) &
+                bind(C, name="dc2_request_footprint")
+            import :: c_int, c_int32_t, c_int64_t
+            integer(c_int32_t), value :: ninact, nact, nsec, nelec
+            integer(c_int64_t), intent(out) :: bytes
+        end function
+        integer(c_int) function dc2_server_create_batched(device, nslots, mem_budget, lanes_per_slot, eig_batch, &
+                                                          eig_window_s, srv) bind(C, name="dc2_server_create_batched")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            integer(c_int32_t), value :: device, nslots, lanes_per_slot, eig_batch
+            integer(c_int64_t), value :: mem_budget
+            real(c_double), value :: eig_window_s
+            type(c_ptr), intent(out) :: srv
+        end function
+        integer(c_int) function dc2_server_add_molecule(srv, ninact, nact, nsec, nelec, b, naux, mol_id) &
+                bind(C, name="dc2_server_add_molecule")
+            import :: c_int, c_int32_t, c_int64_t, c_double_complex, c_ptr
+            type(c_ptr), value :: srv
+            integer(c_int32_t), value :: ninact, nact, nsec, nelec
+            complex(c_double_complex), intent(in) :: b(*)
+            integer(c_int64_t), value :: naux
+            integer(c_int32_t), intent(out) :: mol_id
+        end function
+        integer(c_int) function dc2_server_submit(srv, mol_id, h, ecore, selectroot, eshift, ras, flags, req_id) &
+                bind(C, name="dc2_server_submit")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr
+            type(c_ptr), value :: srv
+            integer(c_int32_t), value :: mol_id, selectroot, flags
+            complex(c_double_complex), intent(in) :: h(*)
+            real(c_double), value :: ecore, eshift
+            type(c_ptr), value :: ras     ! c_loc of integer(c_int32_t) ras(7), or c_null_ptr
+            integer(c_int64_t), intent(out) :: req_id
+        end function
+        integer(c_int) function dc2_server_wait(srv, timeout_s, req_id, req_status, out) &
+                bind(C, name="dc2_server_wait")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: srv
+            real(c_double), value :: timeout_s
+            integer(c_int64_t), intent(out) :: req_id
+            integer(c_int32_t), intent(out) :: req_status
+            real(c_double), intent(out) :: out(26)
+        end function
+        integer(c_int) function dc2_server_stats(srv, out) bind(C, name="dc2_server_stats")
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: srv
+            integer(c_int64_t), intent(out) :: out(7)
+        end function
+        integer(c_int) function dc2_server_destroy(srv) bind(C, name="dc2_server_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: srv
+        end function
     end interface
 
 contains
