@@ -80,3 +80,18 @@ def test_fortran_host_caspt2(tmp_path):
         assert abs(got["e2"][X] - ref.e2[X]) < 1e-9, X
         assert abs(got["sumc2"][X] - ref.sumc2[X]) < 1e-9, X
     assert abs(got["total"] - ref.total) < 1e-9
+
+
+def test_every_fortran_interface_names_a_declared_entry_point():
+    """Each bind(C) name of the Fortran module is a function of include/dcaspt2_amd.h, and the
+    module covers the driver path, the CI readers and the request server."""
+    import re
+    f90 = open(os.path.join(ROOT, "dirac_caspt2_amd", "fortran", "dcaspt2_amd_interface.F90")).read()
+    header = open(os.path.join(ROOT, "include", "dcaspt2_amd.h")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="([a-z0-9_]+)"\)', f90))
+    declared = set(re.findall(r"\b(dc2_[a-z0-9_]+)\s*\(", header))
+    assert bound and bound <= declared, sorted(bound - declared)
+    for name in ("dc2_casci", "dc2_caspt2", "dc2_caspt2_restart", "dc2_set_integral_records", "dc2_get_dets",
+                 "dc2_get_ci_vector", "dc2_server_create_batched", "dc2_server_submit", "dc2_server_wait",
+                 "dc2_server_destroy", "dc2_request_footprint"):
+        assert name in bound, name
