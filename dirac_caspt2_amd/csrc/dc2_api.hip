@@ -580,9 +580,9 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       }
       zgemm(s, 'C', 'N', b, b, n, one, T, n, 0, T, n, 0, zero, RG + (size_t)b * b, b, 0, 1);
     };
-    // Replayed from a captured graph when every product above goes to the skinny kernels
-    // (no split-K workspace, no allocation), no restart copy is involved and the launches
-    // are not being timed; graphs are cached by sizes and buffer addresses
+    // Replayed from a captured graph when no restart copy is involved and the launches are
+    // not being timed; the captured products run without split-K (their workspace can be
+    // reallocated later), and graphs are cached by sizes and buffer addresses
     const bool graph_ok = c->graphs && !c->gstats.on && !restart && n * b <= 65536 && (int64_t)m_in * b <= 2048;
     if (graph_ok) {
       const std::vector<int64_t> key = {n, b, m_in, (int64_t)V, (int64_t)W, (int64_t)X, (int64_t)AX, (int64_t)R,
@@ -594,11 +594,13 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
         hipGraphExec_t ge = nullptr;
         bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
+          set_gemm_no_workspace(true);   // the graph must not hold the grow-only split-K buffer
           try {
             segment(m_in, false);
           } catch (...) {
             ok = false;
           }
+          set_gemm_no_workspace(false);
           ok = hipStreamEndCapture(s, &g) == hipSuccess && ok && g != nullptr;
           ok = ok && hipGraphInstantiateWithFlags(&ge, g, 0) == hipSuccess;
           if (g) (void)hipGraphDestroy(g);
@@ -709,7 +711,10 @@ void run_casci(dc2_context* c, int nroot) {
          c->cimat.get());
   const bool dav = c->casci_solver == 2 || (c->casci_solver == 0 && nd >= 300 && nroot <= 16);
   c->davidson_iters = 0;
-  if (!dav || !davidson(c, nd, c->cimat.get(), std::max(nroot, 1))) {
+  // odd electron counts: one more root, so that the Kramers partner of the last requested
+  // root is held (root_vector needs the whole degenerate group)
+  const int nwant = (int)std::min<int64_t>(std::max(nroot, 1) + (c->ne % 2), nd);
+  if (!dav || !davidson(c, nd, c->cimat.get(), nwant)) {
     c->civec.ensure(nd * nd);
     DC2_HIP(hipMemcpyAsync(c->civec.get(), c->cimat.get(), nd * nd * sizeof(cplx), hipMemcpyDeviceToDevice, s));
     double* w = L0.d("ci_w", nd);
@@ -1235,6 +1240,45 @@ Lane& worker_lane(dc2_context* c, int i) {
 // values read from that file (zeros where it lists nothing, may be null).  As in
 // r4dcaspt2_tra.F90:168 and solve_X_subspace, the totals start from the sums of the file's
 // values and every subspace computed adds its own.
+// CI vector of root selectroot (1-based).  A root whose energy lies within 1e-8 Eh of other
+// held roots (odd electron counts: the Kramers pair) is represented by the normalised
+// projection P e_m of one determinant onto the group's eigenspace: m has the largest weight
+// P_mm = sum_k |c_mk|^2, the lowest index among weights within a relative 1e-6 of it.  The
+// eigensolver's basis of a degenerate eigenspace is arbitrary (ZHEEV's choice in the
+// reference, diag.F90:109) and E2 depends on the vector; this one does not depend on that
+// basis.  A group that may continue beyond the held roots is an error.
+const cplx* root_vector(dc2_context* c, int selectroot) {
+  const int64_t nd = c->ndet;
+  const int k = selectroot - 1;
+  std::vector<int> grp;
+  for (int j = 0; j < c->nroots_avail; ++j)
+    if (std::fabs(c->ecas[j] - c->ecas[k]) < 1.0e-8) grp.push_back(j);
+  if (grp.size() == 1) return c->civec.get() + (int64_t)k * nd;
+  DC2_CHECK(grp.back() < c->nroots_avail - 1 || c->nroots_avail == nd, 1,
+            "the selected root is degenerate with the last root dc2_casci computed: request more roots");
+  const int g = (int)grp.size();
+  std::vector<zc> V((size_t)nd * g), out((size_t)nd, zc(0.0, 0.0));
+  for (int q = 0; q < g; ++q)
+    DC2_HIP(hipMemcpy(V.data() + (size_t)nd * q, c->civec.get() + (int64_t)grp[q] * nd, nd * sizeof(cplx),
+                      hipMemcpyDeviceToHost));
+  std::vector<double> wt((size_t)nd, 0.0);
+  double mx = 0.0;
+  for (int64_t i = 0; i < nd; ++i) {
+    for (int q = 0; q < g; ++q) wt[i] += std::norm(V[i + (size_t)nd * q]);
+    mx = std::max(mx, wt[i]);
+  }
+  int64_t m = 0;
+  while (wt[m] < mx * (1.0 - 1.0e-6)) ++m;
+  const double inv = 1.0 / std::sqrt(wt[m]);
+  for (int q = 0; q < g; ++q) {
+    const zc a = std::conj(V[m + (size_t)nd * q]) * inv;
+    for (int64_t i = 0; i < nd; ++i) out[i] += V[i + (size_t)nd * q] * a;
+  }
+  cplx* dst = c->lane0->z("root_vector", nd);
+  DC2_HIP(hipMemcpy(dst, out.data(), nd * sizeof(cplx), hipMemcpyHostToDevice));
+  return dst;
+}
+
 void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, double* sumc2, double* scal,
                 int first = 0, const double* e2_file = nullptr, const double* sumc2_file = nullptr) {
   DC2_CHECK(c->have_casci, 1, "dc2_casci must run before dc2_caspt2");
@@ -1263,7 +1307,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   c->G1.ensure(na2);
   double* zeros_act = L0.d("zeros_act", std::max(na, 1));
   DC2_HIP(hipMemsetAsync(zeros_act, 0, std::max(na, 1) * sizeof(double), s));
-  const cplx* c0 = c->civec.get() + (int64_t)(selectroot - 1) * nd;
+  const cplx* c0 = root_vector(c, selectroot);
   embed_ci(s, na, nf, nd, c->cas2full.get(), c0, c->full_dets.get(), zeros_act, c->c_full.get(), c->cF.get());
   apply_E(s, na, nf, 1, c->exc.get(), c->c_full.get(), c->Y1o.get());
   zgemm(s, 'T', 'R', na2, 1, nf, one, c->Y1o.get(), nf, 0, c->c_full.get(), nf, 0, zero, c->G1.get(), na2, 0, 1);

@@ -100,6 +100,9 @@ struct GemmStats {
 void set_gemm_stats(GemmStats* st);
 // tag of the GEMM launches issued by the calling host thread
 void set_gemm_tag(int tag);
+// while on, the GEMM launches of the calling host thread use no split-K workspace (no
+// grow-only buffer that a later reallocation could move): set around stream capture
+void set_gemm_no_workspace(bool on);
 // C = A A^H (m x m) from the tiles on and above the diagonal, the lower triangle mirrored;
 // split: k chunks summed by a reduction (0: chosen by size, 1: unsplit)
 void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc,

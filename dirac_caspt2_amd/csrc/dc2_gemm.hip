@@ -454,6 +454,7 @@ int g_variant = [] {
 }();
 std::mutex g_stats_mu;
 thread_local int g_tag = 0;
+thread_local bool g_no_ws = false;
 
 // tuned choices by exact shape (heap, never destroyed; looked up only when not empty)
 struct TuneKey {
@@ -498,7 +499,7 @@ bool launch_skinny(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, c
                          lda, B, ldb, beta, C, ldc);
       return true;
     }
-    if (N > TS_NMAX || K <= 64 || M > 65536) return false;
+    if (N > TS_NMAX || K <= 64 || M > 65536 || g_no_ws) return false;
     const int64_t rb = cdiv(M, 256);
     const int64_t want = std::max<int64_t>(cdiv(K, TS_KMAX), std::min<int64_t>({cdiv(256, rb), 32, cdiv(K, 16)}));
     const int64_t kchunk = cdiv(K, want), sk = cdiv(K, kchunk);
@@ -575,7 +576,7 @@ void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, const cp
       sk = (int)((K + kchunk - 1) / kchunk);
     }
   }
-  if (batch * sk >= 65536) {   // one grid dimension holds batch x split
+  if (batch * sk >= 65536 || g_no_ws) {   // one grid dimension holds batch x split
     sk = 1;
     kchunk = K;
   }
@@ -648,6 +649,7 @@ void set_gemm_stats(GemmStats* st) {
   g_stats = st;
 }
 void set_gemm_tag(int tag) { g_tag = tag; }
+void set_gemm_no_workspace(bool on) { g_no_ws = on; }
 
 void release_stream_workspace(hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_ws_mu);

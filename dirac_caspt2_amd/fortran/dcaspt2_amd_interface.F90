@@ -21,6 +21,8 @@ module dcaspt2_amd
               dc2_set_one_electron, dc2_set_integral_block, dc2_set_integral_records, dc2_set_integral_factors, &
               dc2_casci, dc2_caspt2, dc2_set_casci_result, dc2_caspt2_restart, &
               dc2_get_subspace_info, dc2_cdiag, dc2_casmat, dc2_synchronize
+    public :: dc2_get_dets, dc2_get_ci_vector, dc2_request_footprint, dc2_server_create_batched, &
+              dc2_server_add_molecule, dc2_server_submit, dc2_server_wait, dc2_server_stats, dc2_server_destroy
     public :: dcaspt2_amd_check, dcaspt2_amd_last_error, dcaspt2_amd_open, dcaspt2_amd_close, dcaspt2_amd_casci, &
               dcaspt2_amd_casmat, dcaspt2_amd_cdiag, dcaspt2_amd_r4dcaspt2_tra
 

@@ -344,6 +344,33 @@ def cdiag(c: np.ndarray, threshold: float = 0.0):
     return w, v, dimm
 
 
+# roots closer than this (Eh) form one degenerate group (a Kramers pair for odd electron counts)
+DEGENERACY_TOL = 1.0e-8
+
+
+def root_vector(w: np.ndarray, v: np.ndarray, selectroot: int, tol: float = DEGENERACY_TOL) -> np.ndarray:
+    """CI vector of root `selectroot` (1-based), with a convention for degenerate roots.
+
+    A non-degenerate root is its eigenvector.  For a root inside a group of eigenvalues
+    within `tol` (odd electron counts: the Kramers pair), the eigensolver's vector is an
+    arbitrary unit vector of the group's eigenspace (ZHEEV's choice in the reference,
+    diag.F90:109, casci.F90:54), and E2 depends on it.  The convention here takes the
+    normalised projection P e_m of one determinant onto the eigenspace, P the projector of
+    the group: m has the largest weight P_mm = sum_k |v_mk|^2, the lowest index among
+    weights within a relative 1e-6 of that largest one (the two determinants of a Kramers
+    image pair carry equal weights).  The vector does not depend on the basis the
+    eigensolver returned for the group; its component m is real and positive.
+    """
+    k = selectroot - 1
+    grp = [j for j in range(len(w)) if abs(w[j] - w[k]) < tol]
+    if len(grp) == 1:
+        return v[:, k]
+    V = v[:, grp]
+    wt = np.sum(np.abs(V) ** 2, axis=1)
+    m = int(np.nonzero(wt >= wt.max() * (1.0 - 1.0e-6))[0][0])
+    return V @ (np.conj(V[m, :]) / np.sqrt(wt[m]))
+
+
 def casci(p: Problem):
     """casci.F90:31-72: determinant space, CI matrix, eigenpairs, total energies."""
     dets = search_cas_configuration(p.nact, p.nelec, p.ninact, p.ras, p.sym, p.totsym)
@@ -1027,7 +1054,7 @@ def run_caspt2(p: Problem, casci_out=None, subspaces: Optional[Sequence[str]] = 
     dets, w, v, eig = casci_out
     subs = active_subspaces(p) if subspaces is None else [s for s in subspaces if s in active_subspaces(p)]
     tab = ExcitationTables(p.nact, p.nelec)
-    st = rotate(p, v[:, p.selectroot - 1], dets, tab, need=subs)
+    st = rotate(p, root_vector(w, v, p.selectroot), dets, tab, need=subs)
     e2: Dict[str, float] = {X: 0.0 for X in SUBSPACES}
     s2: Dict[str, float] = {X: 0.0 for X in SUBSPACES}
     blk: Dict[str, list] = {X: [] for X in SUBSPACES}

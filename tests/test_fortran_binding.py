@@ -95,3 +95,40 @@ def test_every_fortran_interface_names_a_declared_entry_point():
                  "dc2_get_ci_vector", "dc2_server_create_batched", "dc2_server_submit", "dc2_server_wait",
                  "dc2_server_destroy", "dc2_request_footprint"):
         assert name in bound, name
+
+
+def test_module_exports_readers_and_server(tmp_path):
+    """A Fortran program that `use`s the module can call a CI reader and the server entry
+    points (compiled and linked against the library; nothing is run)."""
+    flang = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    moddir = os.path.join(ROOT, "dirac_caspt2_amd", "_build")
+    if not os.path.exists(flang) or not os.path.exists(os.path.join(moddir, "dcaspt2_amd.mod")):
+        pytest.skip("no Fortran compiler or module not built")
+    src = tmp_path / "uses_module.F90"
+    src.write_text("""program uses_module
+    use, intrinsic :: iso_c_binding
+    use dcaspt2_amd
+    implicit none
+    type(c_ptr) :: ctx, srv
+    integer(c_int64_t) :: bytes, dets(70), rid, st(7)
+    integer(c_int32_t) :: mol, rstat
+    complex(c_double_complex) :: c(70), h(4), b(8)
+    real(c_double) :: out(26)
+    integer(c_int) :: s
+    ctx = c_null_ptr
+    s = dc2_request_footprint(4_c_int32_t, 8_c_int32_t, 40_c_int32_t, 4_c_int32_t, bytes)
+    s = dc2_get_dets(ctx, dets)
+    s = dc2_get_ci_vector(ctx, 1_c_int32_t, c)
+    s = dc2_server_create_batched(0_c_int32_t, 4_c_int32_t, 1_c_int64_t, 1_c_int32_t, 8_c_int32_t, 0.1d0, srv)
+    s = dc2_server_add_molecule(srv, 0_c_int32_t, 2_c_int32_t, 0_c_int32_t, 1_c_int32_t, b, 2_c_int64_t, mol)
+    s = dc2_server_submit(srv, mol, h, 0d0, 1_c_int32_t, 0d0, c_null_ptr, 0_c_int32_t, rid)
+    s = dc2_server_wait(srv, 1d0, rid, rstat, out)
+    s = dc2_server_stats(srv, st)
+    s = dc2_server_destroy(srv)
+end program
+""")
+    exe = tmp_path / "uses_module"
+    lib = os.path.join(ROOT, "dirac_caspt2_amd")
+    run = subprocess.run([flang, "-O0", "-I", moddir, str(src), os.path.join(moddir, "dcaspt2_amd_interface.o"),
+                          "-L", lib, "-ldcaspt2_amd", "-o", str(exe)], capture_output=True, text=True)
+    assert run.returncode == 0, run.stdout + run.stderr

@@ -76,7 +76,9 @@ def test_split_energies_vs_fp64(workload):
         assert abs(out.total - ref.total) < 1e-8
 
 
-def test_split_324_spinors_vs_oracle():
+@pytest.mark.parametrize("slices", [6, 7])
+def test_split_324_spinors_vs_oracle(slices):
+    """6 slices is what bench.py and the served runs use."""
     from dirac_caspt2_amd import synth
     from gpu_helpers import say
     if not os.path.exists(GOLDEN):
@@ -84,9 +86,9 @@ def test_split_324_spinors_vs_oracle():
     g = json.load(open(GOLDEN))
     sp = synth.generate(synth.CONFIG_SPLIT)
     fp64, _ = _energies(sp, 0)
-    split, (fs, fd) = _energies(sp, 7)
+    split, (fs, fd) = _energies(sp, slices)
     d64 = {X: abs(fp64.e2[X] - g["e2"][X]) for X in "ABCDEFGH"}
     dsp = {X: abs(split.e2[X] - g["e2"][X]) for X in "ABCDEFGH"}
-    say("324 spinors: fp64 vs oracle", d64, "split vs oracle", dsp, "split share %.3f" % (fs / (fs + fd)))
+    say("324 spinors, %d slices: fp64 vs oracle" % slices, d64, "split vs oracle", dsp, "split share %.3f" % (fs / (fs + fd)))
     assert max(d64.values()) < 1e-9 and abs(fp64.total - g["total"]) < 1e-9
     assert max(dsp.values()) < 1e-8 and abs(split.total - g["total"]) < 1e-8
