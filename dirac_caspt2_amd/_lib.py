@@ -63,6 +63,7 @@ _SIGS = {
     "dc2_get_transform_flops": (c_int, [P, P]),
     "dc2_get_casci_info": (c_int, [P, P]),
     "dc2_host_heev": (c_int, [c_int, P, P, c_int]),
+    "dc2_fockivo": (c_int, [P, c_int, P, c_int, P, P, P, P]),
     "dc2_get_hetrd_stats": (c_int, [c_int, P]),
     "dc2_set_casci_result": (c_int, [P, c_i64, c_int, P, P, P]),
     "dc2_caspt2_restart": (c_int, [P, c_int, ctypes.c_double, c_int, P, P, P, P, P]),

@@ -342,6 +342,19 @@ class Caspt2Context:
         v = self.OPTIONS[key][value] if isinstance(value, str) else int(value)
         check("dc2_set_option", self.lib.dc2_set_option(self.h, key.encode(), v))
 
+    def fockivo(self, numh: int, mo_energy: np.ndarray, irreps: Sequence[int]):
+        """dc2_fockivo: (IVO Fock matrix of the secondary spinors, per-irrep eigenvalues at the
+        spinors' positions, block-diagonal eigenvector matrix U)."""
+        ns = self.dims[2]
+        eps = np.ascontiguousarray(mo_energy, dtype=np.float64)
+        irr = np.ascontiguousarray(irreps, dtype=np.int32)
+        fock = np.zeros((ns, ns), dtype=np.complex128, order="F")
+        w = np.zeros(ns)
+        U = np.zeros((ns, ns), dtype=np.complex128, order="F")
+        check("dc2_fockivo", self.lib.dc2_fockivo(self.h, int(numh), _ptr(eps), len(irr), _ptr(irr), _ptr(fock),
+                                                  _ptr(w), _ptr(U)))
+        return fock, w, U
+
     def eig_fallbacks(self) -> int:
         n = ctypes.c_int64()
         check("dc2_get_eig_fallbacks", self.lib.dc2_get_eig_fallbacks(self.h, ctypes.byref(n)))
@@ -503,7 +516,7 @@ def hetrd_stats(n: int):
     return int(out[0]), float(out[1])
 
 
-HOST_HEEV_METHODS = {"auto": 0, "jacobi": 1, "ql": 2}
+HOST_HEEV_METHODS = {"auto": 0, "jacobi": 1, "ql": 2, "zheev": 3}
 
 
 def host_heev(a: np.ndarray, method: str = "auto"):

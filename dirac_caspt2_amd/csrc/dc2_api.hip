@@ -25,6 +25,8 @@
 #include "dc2_hostla.hpp"
 #include "dc2_kernels.hpp"
 
+#include <functional>
+
 using namespace dc2;
 
 namespace {
@@ -381,7 +383,7 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
     return;
   }
   const int64_t nn = (int64_t)n * n;
-  const bool native_ok = n >= 3 && n <= 1024;
+  const bool native_ok = n >= 3 && n <= 2048;
   int method = c->eig_method >= 0 ? c->eig_method
                                   : (kind == EigKind::rank_deficient ? 2 : (native_ok && !c->batcher ? 4 : 1));
   if (method == 4 && !native_ok) method = 1;
@@ -2233,12 +2235,63 @@ int dc2_get_transform_flops(dc2_context* c, double* out) {
 
 int dc2_host_heev(int32_t n, double* a, double* w, int32_t method) {
   return guarded([&] {
-    DC2_CHECK(a && w && n >= 1 && n <= 4096 && method >= 0 && method <= 2, 1, "invalid argument");
+    DC2_CHECK(a && w && n >= 1 && n <= 4096 && method >= 0 && method <= 3, 1, "invalid argument");
     std::vector<zc> A(reinterpret_cast<zc*>(a), reinterpret_cast<zc*>(a) + (size_t)n * n), V((size_t)n * n);
-    if (method == 0) herm_eig(n, A.data(), w, V.data());
+    if (method == 3) DC2_CHECK(herm_eig_zheev(n, A.data(), w, V.data()), 3, "QL/QR iteration did not converge");
+    else if (method == 0) herm_eig(n, A.data(), w, V.data());
     else if (method == 1) herm_jacobi(n, A.data(), w, V.data());
     else DC2_CHECK(herm_tql(n, A.data(), w, V.data()), 3, "QL iteration did not converge");
     std::copy(V.begin(), V.end(), reinterpret_cast<zc*>(a));
+  });
+}
+
+int dc2_fockivo(dc2_context* c, int32_t numh, const double* mo_energy, int32_t nirr, const int32_t* irreps,
+                double* fock, double* w, double* U) {
+  return guarded([&] {
+    DC2_CHECK(c && mo_energy && w && U && (nirr == 0 || irreps), 1, "null argument");
+    need_spaces(c);
+    DC2_HIP(hipSetDevice(c->device));
+    const int ni = c->ni, na = c->na, ns = c->ns, nmoc = ni + na;
+    DC2_CHECK(ns > 0 && nmoc > 0, 1, "dc2_fockivo needs secondary and occupied spinors");
+    if (numh <= 0) {   // fockivo.F90:40-42: the spinors within 0.1 Eh of the HOMO energy
+      DC2_CHECK(c->ne >= 1 && ni + c->ne <= nmoc, 1, "dc2_fockivo: no HOMO (nelec)");
+      const double eh = mo_energy[ni + c->ne - 1];
+      numh = 0;
+      for (int i = 0; i < nmoc; ++i) numh += std::fabs(mo_energy[i] - eh) < 1.0e-1;
+    }
+    DC2_CHECK(numh >= 1 && numh <= nmoc, 1, "dc2_fockivo: numh out of range");
+    const cplx* F1 = get_block(c, "F1");
+    const cplx* F2 = get_block(c, "F2");
+    Lane& L0 = *c->lane0;
+    hipStream_t s = c->stream;
+    double* eps_d = L0.d("ivo_eps", ns);
+    cplx* fock_d = L0.z("ivo_fock", (int64_t)ns * ns);
+    DC2_HIP(hipMemcpyAsync(eps_d, mo_energy + nmoc, ns * sizeof(double), hipMemcpyHostToDevice, s));
+    fockivo_build(s, nmoc, ns, numh, c->ne % 2 == 1, eps_d, F1, F2, fock_d);
+    std::vector<zc> F((size_t)ns * ns);
+    DC2_HIP(hipMemcpyAsync(F.data(), fock_d, F.size() * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    DC2_HIP(hipStreamSynchronize(s));
+    if (fock) std::copy(F.begin(), F.end(), reinterpret_cast<zc*>(fock));
+    zc* Uo = reinterpret_cast<zc*>(U);
+    std::fill(Uo, Uo + (size_t)ns * ns, zc(0.0, 0.0));
+    std::fill(w, w + ns, 0.0);
+    // fockivo.F90:181-230: one block per irreducible representation, ZHEEV's eigenvectors
+    for (int q = 0; q < nirr; ++q) {
+      std::vector<int> mos;
+      for (int a = 0; a < ns; ++a)
+        if ((c->irpamo.empty() ? 1 : c->irpamo[nmoc + a]) == irreps[q]) mos.push_back(a);
+      const int nv = (int)mos.size();
+      if (nv == 0) continue;
+      std::vector<zc> B((size_t)nv * nv), V((size_t)nv * nv);
+      std::vector<double> wv(nv);
+      for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < nv; ++i) B[i + (size_t)nv * j] = F[mos[i] + (size_t)ns * mos[j]];
+      DC2_CHECK(herm_eig_zheev(nv, B.data(), wv.data(), V.data()), 3, "dc2_fockivo: eigensolver did not converge");
+      for (int j = 0; j < nv; ++j) {
+        w[mos[j]] = wv[j];
+        for (int i = 0; i < nv; ++i) Uo[mos[i] + (size_t)ns * mos[j]] = V[i + (size_t)nv * j];
+      }
+    }
   });
 }
 

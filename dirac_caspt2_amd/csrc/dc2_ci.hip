@@ -278,6 +278,31 @@ __global__ void k_fock(int ni, int na, int ns, const cplx* __restrict__ h, int l
   if (i != j) fock[j + (int64_t)nmo * i] = cconj(f);
 }
 
+// fockivo.F90:46-78: the secondary-space Fock matrix of the IVO with the numh highest occupied
+// spinors k = nmoc-numh .. nmoc-1 removed; thread per (a, b), a <= b, both triangles written
+// (F_ba = conj(F_ab); the diagonal is conjugated as well, as the reference's hermitisation does)
+__global__ void k_fockivo(int nmoc, int ns, int numh, int odd, const double* __restrict__ eps_sec,
+                          const cplx* __restrict__ F1, const cplx* __restrict__ F2, cplx* __restrict__ fock) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (a >= ns || a > b) return;
+  const int64_t S = ns, m = nmoc;
+  cplx f = a == b ? mk(eps_sec[a], 0.0) : mk(0.0, 0.0);
+  const double inv = 1.0 / (double)numh;
+  for (int k = nmoc - numh; k < nmoc; ++k) {
+    const cplx f1 = F1[a + S * (b + S * (k + m * k))];      // (ab|kk)
+    const cplx f2 = F2[a + S * (k + m * (k + m * b))];      // (ak|kb)
+    if (odd && k >= nmoc - 2) {
+      f = csub(f, cscale(f1, 0.5 * inv));
+      f = cadd(f, cscale(f2, 0.5 * inv));
+    } else {
+      f = cadd(f, cscale(f2, inv));
+      f = csub(f, cscale(f1, inv));
+    }
+  }
+  fock[b + S * a] = cconj(f);
+  if (a != b) fock[a + S * b] = f;
+}
+
 __global__ void k_gather_block(int n, const int* ind, const cplx* fock, int ldf, cplx* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= n) return;
@@ -692,6 +717,14 @@ void fock_build(hipStream_t s, int ni, int na, int ns, const cplx* h, int ldh, c
   const int nmo = ni + na + ns;
   hipLaunchKernelGGL(k_fill_zero, dim3(cdiv((int64_t)nmo * nmo, 256)), dim3(256), 0, s, fock, (int64_t)nmo * nmo);
   hipLaunchKernelGGL(k_fock, dim3(cdiv(nmo, 128), nmo), dim3(128), 0, s, ni, na, ns, h, ldh, O, F1, F2, D1, fock);
+  DC2_HIP(hipGetLastError());
+}
+
+void fockivo_build(hipStream_t s, int nmoc, int ns, int numh, bool odd_nelec, const double* eps_sec, const cplx* F1,
+                   const cplx* F2, cplx* fock) {
+  if (ns <= 0) return;
+  hipLaunchKernelGGL(k_fockivo, dim3(cdiv(ns, 128), ns), dim3(128), 0, s, nmoc, ns, numh, odd_nelec ? 1 : 0, eps_sec,
+                     F1, F2, fock);
   DC2_HIP(hipGetLastError());
 }
 

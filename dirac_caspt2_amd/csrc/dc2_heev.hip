@@ -1,4 +1,4 @@
-// Hermitian eigensolver for 3 <= n <= 1024 (the matrix its upper triangle defines, as
+// Hermitian eigensolver for 3 <= n <= 2048 (the matrix its upper triangle defines, as
 // rocSOLVER's fill_upper): Householder tridiagonalisation in one persistent
 // kernel, divide and conquer on the real tridiagonal (rocSOLVER dstedc), back-transformation
 // with one wave per eigenvector.
@@ -13,7 +13,7 @@
 // T = D^H T' D real symmetric with subdiagonal |alpha_k|, so the eigenvectors are
 // X = Q D Z for the eigenvectors Z of T.
 //
-// Layout: workgroup w of nwg owns the columns j = w + nwg * l (l = wave + 8 c, c < 2), held
+// Layout: workgroup w of nwg owns the columns j = w + nwg * l (l = wave + 8 c, c < CPW), held
 // in registers for the whole reduction; lane L of a wave holds rows L + 64 s.  Per step:
 //   (a) every workgroup forms p_j for its columns (a wave reduction per column) and a partial
 //       of Re(v^H p); the owner of column k+1 publishes that column as it stands;
@@ -40,9 +40,11 @@ namespace {
 
 constexpr int HT_WAVES = 8;
 constexpr int HT_THREADS = 64 * HT_WAVES;
-constexpr int HT_CPW = 2;                       // columns per wave (4 spills registers at n > 768)
-constexpr int HT_COLS = HT_WAVES * HT_CPW;      // columns per workgroup
-constexpr int HT_NMAX = 1024;
+constexpr int HT_NMAX = 2048;
+// columns per wave: 2 up to n = 1024 (4 spills registers at n > 768); 1 above, where the
+// 2048 / 8 = 256 workgroups of the largest matrix take every compute unit
+inline int hetrd_cpw(int n) { return n <= 1024 ? 2 : 1; }
+inline int hetrd_nwg(int n) { return (n + HT_WAVES * hetrd_cpw(n) - 1) / (HT_WAVES * hetrd_cpw(n)); }
 
 struct HetrdWork {
   cplx* P;          // [2][n] p of the current step
@@ -170,21 +172,21 @@ __device__ void reflector(int n, int k, const cplx* sc, cplx* sv, double* red, d
   __syncthreads();
 }
 
-template <int S>
+template <int S, int CPW>
 __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restrict__ A, int64_t lda, HetrdWork w,
                                                       double* __restrict__ d, cplx* __restrict__ sub,
                                                       double* __restrict__ tau, cplx* __restrict__ delta,
                                                       cplx* __restrict__ Vh) {
-  __shared__ cplx sv[HT_NMAX], sq[HT_NMAX], sc[HT_NMAX];
+  __shared__ cplx sv[64 * S], sq[64 * S], sc[64 * S];
   __shared__ double red[HT_WAVES];
   __shared__ double s_tau;
   __shared__ cplx s_delta;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwg = gridDim.x, wg = blockIdx.x;
-  int col[HT_CPW];
-  cplx a[HT_CPW][S];
+  int col[CPW];
+  cplx a[CPW][S];
 #pragma unroll
-  for (int c = 0; c < HT_CPW; ++c) {
+  for (int c = 0; c < CPW; ++c) {
     col[c] = wg + nwg * (wave + HT_WAVES * c);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
     double part = 0.0;
     if (t != 0.0) {
 #pragma unroll
-      for (int c = 0; c < HT_CPW; ++c) {
+      for (int c = 0; c < CPW; ++c) {
         const int j = col[c];
         if (j >= n || j <= k) continue;   // uniform over the wave
         cplx acc = mk(0.0, 0.0);
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
     }
     // the owner publishes column k+1
 #pragma unroll
-    for (int c = 0; c < HT_CPW; ++c)
+    for (int c = 0; c < CPW; ++c)
       if (col[c] == k + 1) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
         return mk(p.x - K * v.x, p.y - K * v.y);
       };
 #pragma unroll
-      for (int c = 0; c < HT_CPW; ++c) {
+      for (int c = 0; c < CPW; ++c) {
         const int j = col[c];
         if (j >= n || j <= k) continue;
         const cplx cqj = cconj(qat(j)), cvj = cconj(sv[j]);
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
   }
   // the last diagonal element, as updated by the last step
 #pragma unroll
-  for (int c = 0; c < HT_CPW; ++c)
+  for (int c = 0; c < CPW; ++c)
     if (col[c] == n - 1) {
 #pragma unroll
       for (int s = 0; s < S; ++s)
@@ -312,7 +314,7 @@ template <int S>
 __global__ __launch_bounds__(512) void k_hetrd_back(int n, const double* __restrict__ Z, const cplx* __restrict__ delta,
                                                    const double* __restrict__ tau, const cplx* __restrict__ Vh,
                                                    cplx* __restrict__ X, int64_t ldx) {
-  __shared__ cplx vs[BT_CH][HT_NMAX];
+  __shared__ cplx vs[BT_CH][64 * S];
   __shared__ double ts[BT_CH];
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 8 + (threadIdx.x >> 6);
@@ -454,14 +456,14 @@ struct CoopSlot {
   }
 };
 
-template <int S>
+template <int S, int CPW = 2>
 hipError_t launch_hetrd(hipStream_t s, int nwg, int n, const cplx* A, int64_t lda, HetrdWork w, double* d, cplx* sub,
                         double* tau, cplx* delta, cplx* Vh) {
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_hetrd<S>),
-                                                              HT_THREADS, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(&k_hetrd<S, CPW>), HT_THREADS, 0);
   if (e != hipSuccess || per_cu < 1) return e != hipSuccess ? e : hipErrorLaunchOutOfResources;
-  hipLaunchKernelGGL((k_hetrd<S>), dim3(nwg), dim3(HT_THREADS), 0, s, n, A, lda, w, d, sub, tau, delta, Vh);
+  hipLaunchKernelGGL((k_hetrd<S, CPW>), dim3(nwg), dim3(HT_THREADS), 0, s, n, A, lda, w, d, sub, tau, delta, Vh);
   return hipGetLastError();
 }
 
@@ -500,7 +502,7 @@ int64_t vh_cols(int n) { return n >= BT_BLOCKED_MIN ? std::max<int64_t>(n, cdiv(
 
 int64_t heev_native_work_bytes(int n) {
   const int64_t nn = (int64_t)n * n;
-  const int nwg = (int)cdiv(n, HT_COLS);
+  const int nwg = hetrd_nwg(n);
   int64_t b = (int64_t)n * vh_cols(n) * 16 + nn * 8 + (int64_t)n * (16 * 6 + 8 * 3) + 16 * nwg + 16 + 10 * 256;
   if (n >= BT_BLOCKED_MIN) b += back_blocked_bytes(n);
   return b;
@@ -508,8 +510,8 @@ int64_t heev_native_work_bytes(int n) {
 
 bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev,
                  int* flags_host) {
-  DC2_CHECK(n >= 3 && n <= HT_NMAX, 1, "heev_native: n must be in 3..1024");
-  const int nwg = (int)cdiv(n, HT_COLS);
+  DC2_CHECK(n >= 3 && n <= HT_NMAX, 1, "heev_native: n must be in 3..2048");
+  const int nwg = hetrd_nwg(n);
   const int64_t nn = (int64_t)n * n;
   char* p = reinterpret_cast<char*>(work);
   auto take = [&](int64_t bytes) {
@@ -560,7 +562,11 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     else if (n <= 512) st = launch_hetrd<8>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 768) st = launch_hetrd<12>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 832) st = launch_hetrd<13>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
-    else st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 1024) st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 1280) st = launch_hetrd<20, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 1536) st = launch_hetrd<24, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 1792) st = launch_hetrd<28, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else st = launch_hetrd<32, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     if (st != hipSuccess) {
       (void)hipGetLastError();
       return false;   // e.g. the workgroup does not fit a compute unit: the caller uses rocSOLVER

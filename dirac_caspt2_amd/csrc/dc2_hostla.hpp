@@ -5,6 +5,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -275,6 +276,264 @@ inline bool herm_tql(int n, zc* A, double* w, zc* V) {
 inline void herm_eig(int n, zc* A, double* w, zc* V) {
   std::vector<zc> keep(A, A + (size_t)n * n);
   if (!herm_tql(n, A, w, V)) herm_jacobi(n, keep.data(), w, V);
+}
+
+
+// Eigendecomposition of a Hermitian n x n matrix by the sequence of LAPACK's ZHEEV('V', 'U')
+// (the reference's cdiag, diag.F90:66-128): Householder reduction of the upper triangle from
+// the last column (zhetd2), the unitary Q from the reflectors (zungtr / zung2l), implicit QL or
+// QR iterations with Wilkinson shifts on the real tridiagonal (zsteqr: the end with the smaller
+// diagonal element starts the sweeps, rotations from dlartg in the form that keeps r > 0 unless
+// |f| > |g| with f < 0, 2 x 2 blocks by dlaev2), then a selection sort of the eigenpairs.  The
+// eigenvectors come out with ZHEEV's signs and phases, which the IVO coefficients depend on.
+// A (column-major, only the upper triangle read) is not modified; w ascending, V column k for w[k].
+// Returns false when the iteration does not converge in 30 n sweeps.
+inline bool herm_eig_zheev(int n, const zc* A, double* w, zc* V) {
+  auto at = [n](std::vector<zc>& M, int i, int j) -> zc& { return M[(size_t)i + (size_t)n * j]; };
+  if (n <= 0) return true;
+  if (n == 1) {
+    w[0] = A[0].real();
+    V[0] = zc(1.0, 0.0);
+    return true;
+  }
+  std::vector<zc> T(A, A + (size_t)n * n), tau(n - 1), x(n), v(n), wv(n);
+  std::vector<double> d(n), e(n - 1);
+  // zhetd2, UPLO = 'U' (0-based: reflector of column i+1 annihilates rows 0 .. i-2)
+  at(T, n - 1, n - 1) = at(T, n - 1, n - 1).real();
+  for (int i = n - 1; i >= 1; --i) {
+    zc alpha = at(T, i - 1, i);
+    double xnorm = 0.0;
+    for (int r = 0; r < i - 1; ++r) xnorm = std::hypot(xnorm, std::abs(at(T, r, i)));
+    zc taui(0.0, 0.0);
+    if (!(xnorm == 0.0 && alpha.imag() == 0.0)) {   // zlarfg
+      const double beta = -std::copysign(std::sqrt(std::norm(alpha) + xnorm * xnorm), alpha.real());
+      taui = zc((beta - alpha.real()) / beta, -alpha.imag() / beta);
+      const zc sc = 1.0 / (alpha - beta);
+      for (int r = 0; r < i - 1; ++r) at(T, r, i) *= sc;
+      alpha = zc(beta, 0.0);
+    }
+    e[i - 1] = alpha.real();
+    if (taui != zc(0.0, 0.0)) {
+      at(T, i - 1, i) = 1.0;
+      for (int r = 0; r < i; ++r) v[r] = at(T, r, i);
+      // x = taui * H v with H the Hermitian matrix of the upper triangle of T[0:i, 0:i]
+      for (int r = 0; r < i; ++r) {
+        zc acc(0.0, 0.0);
+        for (int c = 0; c < i; ++c) {
+          const zc h = r < c ? at(T, r, c) : (r == c ? zc(at(T, r, r).real(), 0.0) : std::conj(at(T, c, r)));
+          acc += h * v[c];
+        }
+        x[r] = taui * acc;
+      }
+      zc dot(0.0, 0.0);
+      for (int r = 0; r < i; ++r) dot += std::conj(x[r]) * v[r];
+      const zc al = -0.5 * taui * dot;
+      for (int r = 0; r < i; ++r) wv[r] = x[r] + al * v[r];
+      for (int c = 0; c < i; ++c)
+        for (int r = 0; r <= c; ++r) at(T, r, c) -= v[r] * std::conj(wv[c]) + wv[r] * std::conj(v[c]);
+      for (int r = 0; r < i; ++r) at(T, r, r) = at(T, r, r).real();
+    } else {
+      at(T, i - 1, i - 1) = at(T, i - 1, i - 1).real();
+    }
+    at(T, i - 1, i) = e[i - 1];
+    d[i] = at(T, i, i).real();
+    tau[i - 1] = taui;
+  }
+  d[0] = at(T, 0, 0).real();
+  // zungtr (UPLO = 'U'): the reflectors shifted one column left, then zung2l on the
+  // leading (n-1) x (n-1) block; the last row and column are those of the unit matrix
+  std::vector<zc> Q((size_t)n * n, zc(0.0, 0.0));
+  for (int j = 0; j < n - 1; ++j)
+    for (int r = 0; r < j; ++r) at(Q, r, j) = at(T, r, j + 1);
+  at(Q, n - 1, n - 1) = 1.0;
+  const int m = n - 1;
+  for (int i = 1; i <= m; ++i) {   // H(i) applied to Q[0:i, 0:i-1] from the left
+    const int ii = i - 1, rows = i;
+    at(Q, rows - 1, ii) = 1.0;
+    for (int c = 0; c < ii; ++c) {
+      zc acc(0.0, 0.0);
+      for (int r = 0; r < rows; ++r) acc += std::conj(at(Q, r, c)) * at(Q, r, ii);
+      const zc f = tau[i - 1] * std::conj(acc);
+      for (int r = 0; r < rows; ++r) at(Q, r, c) -= at(Q, r, ii) * f;
+    }
+    for (int r = 0; r < rows - 1; ++r) at(Q, r, ii) *= -tau[i - 1];
+    at(Q, rows - 1, ii) = 1.0 - tau[i - 1];
+    for (int r = rows; r < m; ++r) at(Q, r, ii) = 0.0;
+  }
+  // zsteqr (1-based indices as in LAPACK; d1/e1 address them)
+  const double eps = 0.5 * std::numeric_limits<double>::epsilon(), eps2 = eps * eps;
+  const double safmin = std::numeric_limits<double>::min();
+  auto d1 = [&](int i) -> double& { return d[i - 1]; };
+  auto e1 = [&](int i) -> double& { return e[i - 1]; };
+  auto lartg = [](double f, double g, double& c, double& s, double& r) {
+    if (g == 0.0) {
+      c = 1.0; s = 0.0; r = f;
+    } else if (f == 0.0) {
+      c = 0.0; s = 1.0; r = g;
+    } else {
+      r = std::sqrt(f * f + g * g);
+      c = f / r; s = g / r;
+      if (std::fabs(f) > std::fabs(g) && c < 0.0) { c = -c; s = -s; r = -r; }
+    }
+  };
+  auto laev2 = [](double a, double b, double c, double& rt1, double& rt2, double& cs1, double& sn1) {
+    const double sm = a + c, df = a - c, adf = std::fabs(df), tb = b + b, ab = std::fabs(tb);
+    const double acmx = std::fabs(a) > std::fabs(c) ? a : c, acmn = std::fabs(a) > std::fabs(c) ? c : a;
+    double rt;
+    if (adf > ab) rt = adf * std::sqrt(1.0 + (ab / adf) * (ab / adf));
+    else if (adf < ab) rt = ab * std::sqrt(1.0 + (adf / ab) * (adf / ab));
+    else rt = ab * std::sqrt(2.0);
+    int sgn1, sgn2;
+    if (sm < 0.0) { rt1 = 0.5 * (sm - rt); sgn1 = -1; rt2 = (acmx / rt1) * acmn - (b / rt1) * b; }
+    else if (sm > 0.0) { rt1 = 0.5 * (sm + rt); sgn1 = 1; rt2 = (acmx / rt1) * acmn - (b / rt1) * b; }
+    else { rt1 = 0.5 * rt; rt2 = -0.5 * rt; sgn1 = 1; }
+    double cs;
+    if (df >= 0.0) { cs = df + rt; sgn2 = 1; } else { cs = df - rt; sgn2 = -1; }
+    if (std::fabs(cs) > ab) {
+      const double ct = -tb / cs;
+      sn1 = 1.0 / std::sqrt(1.0 + ct * ct); cs1 = ct * sn1;
+    } else if (ab == 0.0) {
+      cs1 = 1.0; sn1 = 0.0;
+    } else {
+      const double tn = -cs / tb;
+      cs1 = 1.0 / std::sqrt(1.0 + tn * tn); sn1 = tn * cs1;
+    }
+    if (sgn1 == sgn2) { const double tn = cs1; cs1 = -sn1; sn1 = tn; }
+  };
+  // columns j0 .. j0+k-1 (1-based) of Q rotated by (c_j, s_j), j = 1 .. k-1, backward or forward
+  auto lasr = [&](bool backward, int j0, int k, const double* cw, const double* sw) {
+    for (int t = 0; t < k - 1; ++t) {
+      const int j = backward ? k - 2 - t : t;
+      const double ct = cw[j], st = sw[j];
+      zc* a = &at(Q, 0, j0 - 1 + j);
+      zc* b = &at(Q, 0, j0 + j);
+      for (int r = 0; r < n; ++r) {
+        const zc tmp = b[r];
+        b[r] = ct * tmp - st * a[r];
+        a[r] = st * tmp + ct * a[r];
+      }
+    }
+  };
+  std::vector<double> cw(n), sw(n);
+  const int nmaxit = 30 * n;
+  int jtot = 0, l1 = 1;
+  while (l1 <= n) {
+    if (l1 > 1) e1(l1 - 1) = 0.0;
+    int mm = n;
+    for (int q = l1; q <= n - 1; ++q) {
+      const double tst = std::fabs(e1(q));
+      if (tst == 0.0) { mm = q; break; }
+      if (tst <= (std::sqrt(std::fabs(d1(q))) * std::sqrt(std::fabs(d1(q + 1)))) * eps) { e1(q) = 0.0; mm = q; break; }
+    }
+    int l = l1, lsv = l, lend = mm, lendsv = lend;
+    l1 = mm + 1;
+    if (lend == l) continue;
+    if (std::fabs(d1(lend)) < std::fabs(d1(l))) { lend = lsv; l = lendsv; }
+    if (lend > l) {   // QL iteration
+      for (;;) {
+        int m2 = lend;
+        if (l != lend)
+          for (int q = l; q <= lend - 1; ++q) {
+            const double tst = e1(q) * e1(q);
+            if (tst <= (eps2 * std::fabs(d1(q))) * std::fabs(d1(q + 1)) + safmin) { m2 = q; break; }
+          }
+        if (m2 < lend) e1(m2) = 0.0;
+        double p = d1(l);
+        if (m2 == l) {
+          d1(l) = p; ++l;
+          if (l <= lend) continue;
+          break;
+        }
+        if (m2 == l + 1) {
+          double rt1, rt2, c, s;
+          laev2(d1(l), e1(l), d1(l + 1), rt1, rt2, c, s);
+          lasr(true, l, 2, &c, &s);
+          d1(l) = rt1; d1(l + 1) = rt2; e1(l) = 0.0; l += 2;
+          if (l <= lend) continue;
+          break;
+        }
+        if (jtot == nmaxit) break;
+        ++jtot;
+        double g = (d1(l + 1) - p) / (2.0 * e1(l));
+        double r = std::hypot(g, 1.0);
+        g = d1(m2) - p + (e1(l) / (g + std::copysign(r, g)));
+        double s = 1.0, c = 1.0;
+        p = 0.0;
+        for (int i = m2 - 1; i >= l; --i) {
+          const double f = s * e1(i), b = c * e1(i);
+          lartg(g, f, c, s, r);
+          if (i != m2 - 1) e1(i + 1) = r;
+          g = d1(i + 1) - p;
+          r = (d1(i) - g) * s + 2.0 * c * b;
+          p = s * r;
+          d1(i + 1) = g + p;
+          g = c * r - b;
+          cw[i - 1] = c; sw[i - 1] = -s;
+        }
+        lasr(true, l, m2 - l + 1, cw.data() + l - 1, sw.data() + l - 1);
+        d1(l) -= p; e1(l) = g;
+      }
+    } else {          // QR iteration
+      for (;;) {
+        int m2 = lend;
+        if (l != lend)
+          for (int q = l; q >= lend + 1; --q) {
+            const double tst = e1(q - 1) * e1(q - 1);
+            if (tst <= (eps2 * std::fabs(d1(q))) * std::fabs(d1(q - 1)) + safmin) { m2 = q; break; }
+          }
+        if (m2 > lend) e1(m2 - 1) = 0.0;
+        double p = d1(l);
+        if (m2 == l) {
+          d1(l) = p; --l;
+          if (l >= lend) continue;
+          break;
+        }
+        if (m2 == l - 1) {
+          double rt1, rt2, c, s;
+          laev2(d1(l - 1), e1(l - 1), d1(l), rt1, rt2, c, s);
+          lasr(false, l - 1, 2, &c, &s);
+          d1(l - 1) = rt1; d1(l) = rt2; e1(l - 1) = 0.0; l -= 2;
+          if (l >= lend) continue;
+          break;
+        }
+        if (jtot == nmaxit) break;
+        ++jtot;
+        double g = (d1(l - 1) - p) / (2.0 * e1(l - 1));
+        double r = std::hypot(g, 1.0);
+        g = d1(m2) - p + (e1(l - 1) / (g + std::copysign(r, g)));
+        double s = 1.0, c = 1.0;
+        p = 0.0;
+        for (int i = m2; i <= l - 1; ++i) {
+          const double f = s * e1(i), b = c * e1(i);
+          lartg(g, f, c, s, r);
+          if (i != m2) e1(i - 1) = r;
+          g = d1(i) - p;
+          r = (d1(i + 1) - g) * s + 2.0 * c * b;
+          p = s * r;
+          d1(i) = g + p;
+          g = c * r - b;
+          cw[i - 1] = c; sw[i - 1] = s;
+        }
+        lasr(false, m2, l - m2 + 1, cw.data() + m2 - 1, sw.data() + m2 - 1);
+        d1(l) -= p; e1(l - 1) = g;
+      }
+    }
+    if (jtot >= nmaxit) return false;
+  }
+  for (int ii = 2; ii <= n; ++ii) {   // selection sort, ascending
+    const int i = ii - 1;
+    int k = i;
+    double p = d1(i);
+    for (int j = ii; j <= n; ++j)
+      if (d1(j) < p) { k = j; p = d1(j); }
+    if (k != i) {
+      d1(k) = d1(i); d1(i) = p;
+      for (int r = 0; r < n; ++r) std::swap(at(Q, r, i - 1), at(Q, r, k - 1));
+    }
+  }
+  std::copy(d.begin(), d.end(), w);
+  std::copy(Q.begin(), Q.end(), V);
+  return true;
 }
 
 }  // namespace dc2

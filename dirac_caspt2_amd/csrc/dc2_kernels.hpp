@@ -35,6 +35,10 @@ void casmat(hipStream_t s, int ni, int na, int ne, int64_t ndet, const int64_t* 
             const cplx* O, int nmoc, const cplx* h, int ldh, cplx* mat);
 // Fock matrix, occupied and secondary diagonal blocks (fockcasci.F90:4-90); upper
 // triangles written as computed, lower triangles as conjugates
+// fockivo.F90:46-78: IVO secondary-space Fock matrix (ns x ns, both triangles) from F1, F2 and
+// the secondary orbital energies; the numh highest occupied spinors of the nmoc are removed
+void fockivo_build(hipStream_t s, int nmoc, int ns, int numh, bool odd_nelec, const double* eps_sec, const cplx* F1,
+                   const cplx* F2, cplx* fock);
 void fock_build(hipStream_t s, int ni, int na, int ns, const cplx* h, int ldh, const cplx* O, const cplx* F1,
                 const cplx* F2, const cplx* D1, cplx* fock);
 // gather a (space, irrep) block: out(i,j) = fock(ind_i, ind_j) (upper values)

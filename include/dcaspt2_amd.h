@@ -211,8 +211,25 @@ int dc2_get_casci_info(dc2_context* ctx, int64_t* out);
  * complex, column-major) is overwritten by the eigenvectors, w by the ascending eigenvalues.
  * method 0: Householder tridiagonalisation + implicit QL, Jacobi if QL does not converge
  * (what the Davidson solver uses); 1: cyclic Jacobi; 2: QL only (status 3 if it does not
- * converge).  No GPU needed. */
+ * converge); 3: the sequence of LAPACK's ZHEEV('V','U') (zhetd2, zungtr, zsteqr), whose
+ * eigenvector signs the reference's IVO coefficients carry (diag.F90:66-128).  Only the upper
+ * triangle is read.  No GPU needed. */
 int dc2_host_heev(int32_t n, double* a, double* w, int32_t method);
+/* fockivo (fockivo.F90:40-230, driven by r4divo.F90:60-80): the IVO Fock matrix of the
+ * secondary spinors, F_ab = delta_ab eps_a + sum_k [(ak|kb) - (ab|kk)] / numh over the numh
+ * highest occupied spinors k of the inactive+active space (a factor 1/2 on the last two for odd
+ * nelec), built on the device from the resident F1/F2 blocks, then diagonalised per irreducible
+ * representation irreps[0..nirr-1] (values of irpamo; every spinor is irrep 1 without
+ * dc2_set_symmetry) over the secondary spinors of that irrep, by the sequence of LAPACK's ZHEEV
+ * (the reference's cdiag) on the host: these blocks are small and the IVO coefficients carry
+ * ZHEEV's eigenvector signs.  numh <= 0: the number of spinors within 0.1 Eh of the HOMO energy
+ * mo_energy[ninact + nelec - 1].  mo_energy: nmo orbital energies (CAS order, caspt2_mo_energy).
+ * Outputs: fock (nsec x nsec complex, NULL to skip), w (nsec: the eigenvalue of eigenvector j of
+ * an irrep's block at the position of that block's j-th spinor), U (nsec x nsec complex, block
+ * diagonal: U[mos[i] + nsec * mos[j]] = component i of eigenvector j, mos the irrep's spinors
+ * in order; zero outside the blocks of the irreps listed). */
+int dc2_fockivo(dc2_context* ctx, int32_t numh, const double* mo_energy, int32_t nirr, const int32_t* irreps,
+                double* fock, double* w, double* U);
 /* number of eigensolves that fell back to the tridiagonal QR algorithm */
 int dc2_get_eig_fallbacks(dc2_context* ctx, int64_t* n);
 

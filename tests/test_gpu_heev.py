@@ -70,3 +70,29 @@ def test_upper_triangle_only(ctx):
     np.testing.assert_allclose(w, np.linalg.eigvalsh(A), atol=1e-12 * np.abs(w).max(), rtol=0)
     assert np.abs(A @ V - V * w).max() < 1e-11 * np.abs(w).max()
     ctx.set_option("eigensolver", "auto")
+
+
+BIG = [(1025, "random"), (1025, "pairs"), (1274, "rank_deficient"), (1281, "pairs"), (1536, "random"),
+       (1792, "rank_deficient"), (2002, "pairs"), (2002, "rank_deficient"), (2048, "random")]
+
+
+@pytest.mark.parametrize("n,kind", BIG)
+def test_cdiag_one_column_per_wave(ctx, n, kind):
+    """Sizes above 1024, where every wave of the tridiagonalisation holds one column (up to
+    2048 / 8 = 256 workgroups); 1274 and 2002 are the A and C metrics of the 324-spinor
+    workload."""
+    from dirac_caspt2_amd.api import cdiag
+    from gpu_helpers import say
+    say("cdiag native", kind, n)
+    rng = np.random.default_rng(7 * n + len(kind))
+    A = _matrix(kind, n, rng)
+    ctx.set_option("eigensolver", "native")
+    before = ctx.eig_fallbacks()
+    w, V, dimm = cdiag(ctx, A)
+    assert ctx.eig_fallbacks() == before, "the native solver fell back to rocSOLVER"
+    ref = np.linalg.eigvalsh(A)
+    scale = max(1.0, np.abs(ref).max())
+    np.testing.assert_allclose(w, ref, atol=1e-12 * scale, rtol=0)
+    assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-12
+    assert np.abs(A @ V - V * w).max() < 1e-11 * scale
+    ctx.set_option("eigensolver", "auto")
