@@ -64,7 +64,7 @@ def cpu_baseline(sp, steps=1):
                            f"({dt:.2f} s per energy, BLAS threads = {cores})"}
 
 
-def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_window=0.002, split=0):
+def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_window=0.002, split=0, native_min=0):
     """Scan points of the same molecule through the request server, all submitted at once:
     energies/s over the whole batch, per-request latency, and the largest |dE2| against the
     plain FP64 path on the points in plain_points ({index: Caspt2Energies})."""
@@ -73,6 +73,8 @@ def served_run(spec, sp, nreq, slots, lanes, plain_points, eig_batch=0, eig_wind
     direction = synth.scan_displacements(spec, spec.seed + 1)
     hs = [synth.scan_point(sp, lam, direction) for lam in np.linspace(0.0, 1.0, nreq)]
     with Server(0, nslots=slots, lanes_per_slot=lanes, eig_batch=eig_batch, eig_window=eig_window) as srv:
+        if native_min:
+            srv.set_option("native_min", native_min)
         mid = srv.add_molecule((spec.ninact, spec.nact, spec.nsec, spec.nelec), sp.B)
         for k in range(slots):                                   # warm every slot
             srv.submit(mid, hs[k % nreq], sp.ecore, spec.selectroot, spec.eshift, split_slices=split)
@@ -491,7 +493,8 @@ def run_workload(name, args, steps, warmup, served, slots, autotune_on, accounti
         ctx.set_option("split_slices", 0)
         pts = plain_points(ctx, spec, sp, served, idx)
         ctx.close()
-        srv = served_run(spec, sp, served, slots, args.slot_lanes, pts, args.eig_batch, args.eig_window, args.split)
+        srv = served_run(spec, sp, served, slots, args.slot_lanes, pts, args.eig_batch, args.eig_window, args.split,
+                         args.native_min)
         srv["speedup_vs_plain_fp64"] = srv["energies_per_s"] / fp64["energies_per_s"]
         res["served"] = srv
         log(f"{name} served: {srv['energies_per_s']:.2f} energies/s")
@@ -531,6 +534,9 @@ def main():
                     help="INT8 slices of the split-precision transforms; 6: |dE2| <= 3e-11 Eh against the "
                          "oracle at 324 spinors (5: 3e-9, too close to the 1e-8 bound for the totals; 7: 3e-13)")
     ap.add_argument("--eig-batch", type=int, default=64, help="eigensolves batched across served requests (<=1: off)")
+    ap.add_argument("--native-min", type=int, default=0,
+                    help="served runs: eigenproblems of at least this size use this library's eigensolver instead "
+                         "of the cross-request batched rocSOLVER one (0: never)")
     ap.add_argument("--eig-window", type=float, default=None,
                     help="batching window of the served eigensolves (s); groups are mostly flushed earlier, when "
                          "every running request waits for one.  Default 0.1 for the runs that submit all requests "

@@ -75,6 +75,12 @@ _SIGS = {
     "dc2_server_submit": (c_int, [P, c_int, P, ctypes.c_double, c_int, ctypes.c_double, P, c_int, P]),
     "dc2_server_wait": (c_int, [P, ctypes.c_double, P, P, P]),
     "dc2_server_stats": (c_int, [P, P]),
+    "dc2_server_set_checkpoint_dir": (c_int, [P, ctypes.c_char_p]),
+    "dc2_server_set_option": (c_int, [P, ctypes.c_char_p, c_i64]),
+    "dc2_server_submit_restart": (c_int, [P, c_int, P, ctypes.c_double, c_int, ctypes.c_double, P, c_int, c_int, P, P,
+                                          P]),
+    "dc2_set_subspace_callback": (c_int, [P, P, P]),
+    "dc2_write_restart": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, P, P]),
     "dc2_get_gemm_stats": (c_int, [P, c_int, P]),
     "dc2_get_eig_fallbacks": (c_int, [P, P]),
 }

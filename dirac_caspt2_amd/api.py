@@ -460,14 +460,31 @@ class Server:
         self.dims[mid.value] = tuple(dims)
         return mid.value
 
+    def set_option(self, key: str, value: int):
+        """dc2_set_option on every slot's context (dc2_server_set_option)."""
+        check("dc2_server_set_option", self.lib.dc2_server_set_option(self.h, key.encode(), int(value)))
+
+    def set_checkpoint_dir(self, path: Optional[str]):
+        """Per-subspace checkpoints <path>/<request id>/caspt2_restart_1_<root> (None: off)."""
+        check("dc2_server_set_checkpoint_dir",
+              self.lib.dc2_server_set_checkpoint_dir(self.h, (path or "").encode()))
+
     def submit(self, mol: int, h: np.ndarray, ecore: float, selectroot: int = 1, eshift: float = 0.0,
-               ras=None, split_slices: int = 0) -> int:
+               ras=None, split_slices: int = 0, restart=None) -> int:
+        """restart: (first, e2_file, sumc2_file) to resume from a caspt2_restart file's values
+        (subspaces before first, 0 = A .. 8 = none, taken from the file)."""
         hf = _fortran_complex(h)
         r = None if ras is None else np.ascontiguousarray(ras, dtype=np.int32)
         rid = ctypes.c_int64()
-        check("dc2_server_submit", self.lib.dc2_server_submit(self.h, mol, _ptr(hf), float(ecore), selectroot,
-                                                              float(eshift), None if r is None else _ptr(r),
-                                                              int(split_slices), ctypes.byref(rid)))
+        first, e2f, s2f = 0, None, None
+        if restart is not None:
+            first = int(restart[0])
+            e2f = np.ascontiguousarray([restart[1].get(X, 0.0) for X in SUBSPACES], dtype=np.float64)
+            s2f = np.ascontiguousarray([restart[2].get(X, 0.0) for X in SUBSPACES], dtype=np.float64)
+        check("dc2_server_submit", self.lib.dc2_server_submit_restart(
+            self.h, mol, _ptr(hf), float(ecore), selectroot, float(eshift), None if r is None else _ptr(r),
+            int(split_slices), first, None if e2f is None else _ptr(e2f), None if s2f is None else _ptr(s2f),
+            ctypes.byref(rid)))
         return rid.value
 
     def wait(self, timeout: float = 1.0) -> Optional[ServedResult]:
@@ -514,6 +531,15 @@ def hetrd_stats(n: int):
     out = np.zeros(2)
     check("dc2_get_hetrd_stats", _lib.lib().dc2_get_hetrd_stats(n, _ptr(out)))
     return int(out[0]), float(out[1])
+
+
+def write_restart(path: str, totsym: int, selectroot: int, next_subspace: int, e2: Dict[str, float],
+                  sumc2: Dict[str, float]):
+    """dc2_write_restart: the caspt2_restart file for the subspaces before next_subspace (0..8)."""
+    a = np.ascontiguousarray([e2.get(X, 0.0) for X in SUBSPACES], dtype=np.float64)
+    b = np.ascontiguousarray([sumc2.get(X, 0.0) for X in SUBSPACES], dtype=np.float64)
+    check("dc2_write_restart", _lib.lib().dc2_write_restart(path.encode(), totsym, selectroot, next_subspace, _ptr(a),
+                                                            _ptr(b)))
 
 
 HOST_HEEV_METHODS = {"auto": 0, "jacobi": 1, "ql": 2, "zheev": 3}

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <atomic>
 #include <exception>
 #include <future>
@@ -186,8 +187,15 @@ struct dc2_context {
   // block Davidson: the launches between two host round trips of an iteration replayed from
   // captured graphs, keyed by the sizes and every buffer address they use
   bool graphs = true;
+  // eigenproblems with n >= native_min (and n within this library's eigensolver) take it even
+  // when the eigensolves are batched across requests (0: never)
+  int native_min = 0;
   std::map<std::vector<int64_t>, hipGraphExec_t> dav_graphs;
   int64_t graph_launches = 0, graph_captures = 0;
+
+  // called from the lane that finished subspace k (0 = A .. 7 = H) with its e2 and sumc2
+  void (*on_subspace)(void* user, int32_t k, double e2, double sumc2) = nullptr;
+  void* on_subspace_user = nullptr;
 
   int nmo() const { return ni + na + ns; }
   int nmoc() const { return ni + na; }
@@ -384,8 +392,9 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
   }
   const int64_t nn = (int64_t)n * n;
   const bool native_ok = n >= 3 && n <= 2048;
+  const bool batched = c->batcher && !(c->native_min > 0 && n >= c->native_min);
   int method = c->eig_method >= 0 ? c->eig_method
-                                  : (kind == EigKind::rank_deficient ? 2 : (native_ok && !c->batcher ? 4 : 1));
+                                  : (kind == EigKind::rank_deficient ? 2 : (native_ok && !batched ? 4 : 1));
   if (method == 4 && !native_ok) method = 1;
   cplx* keep = L.z("eig_keep", nn);
   DC2_HIP(hipMemcpyAsync(keep, A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
@@ -1216,6 +1225,7 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
   }
   e2[k] = r.e2;
   sumc2[k] = r.sumc2;
+  if (c->on_subspace) c->on_subspace(c->on_subspace_user, k, r.e2, r.sumc2);
   c->sub_info[4 * k + 0] = n;
   c->sub_info[4 * k + 1] = r.dimm;
   c->sub_info[4 * k + 2] = npert;
@@ -2140,6 +2150,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "split_min_k") {
       DC2_CHECK(value >= 1 && value <= 4096, 1, "split_min_k must be in 1..4096");
       c->split_min_k = (int)value;
+    } else if (k == "native_min") {
+      DC2_CHECK(value >= 0, 1, "native_min must be >= 0");
+      c->native_min = (int)value;
     } else {
       throw Error(1, "unknown option '" + k + "'");
     }
@@ -2242,6 +2255,30 @@ int dc2_host_heev(int32_t n, double* a, double* w, int32_t method) {
     else if (method == 1) herm_jacobi(n, A.data(), w, V.data());
     else DC2_CHECK(herm_tql(n, A.data(), w, V.data()), 3, "QL iteration did not converge");
     std::copy(V.begin(), V.end(), reinterpret_cast<zc*>(a));
+  });
+}
+
+int dc2_set_subspace_callback(dc2_context* c, void (*cb)(void* user, int32_t subspace, double e2, double sumc2),
+                              void* user) {
+  return guarded([&] {
+    DC2_CHECK(c, 1, "null context");
+    c->on_subspace = cb;
+    c->on_subspace_user = user;
+  });
+}
+
+int dc2_write_restart(const char* path, int32_t totsym, int32_t selectroot, int32_t next, const double* e2,
+                      const double* sumc2) {
+  return guarded([&] {
+    DC2_CHECK(path && e2 && sumc2 && next >= 0 && next <= 8, 1, "invalid argument");
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    DC2_CHECK(f != nullptr, 1, std::string("cannot write ") + tmp);
+    std::fprintf(f, "totsym: %d\nselectroot: %d\nNext subspace: %c\n", totsym, selectroot, (char)('A' + next));
+    for (int k = 0; k < next; ++k)
+      std::fprintf(f, "Subspace: %c Sumc2: %.17g Energy: %.17g\n", (char)('A' + k), sumc2[k], e2[k]);
+    const bool ok = std::fclose(f) == 0;
+    DC2_CHECK(ok && std::rename(tmp.c_str(), path) == 0, 1, std::string("cannot write ") + path);
   });
 }
 

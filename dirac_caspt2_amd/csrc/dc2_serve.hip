@@ -30,6 +30,10 @@
 #include "../../include/dcaspt2_amd.h"
 #include "dc2_batch.hpp"
 
+#include <sys/stat.h>
+
+#include <cerrno>
+
 namespace dc2 {
 void set_last_error(const std::string& msg);   // dc2_api.hip: the message dc2_last_error() returns
 void set_batcher(dc2_context* c, EigBatcher* b);   // dc2_api.hip: route the context's eigensolves
@@ -193,6 +197,15 @@ struct Request {
   int status = 0;
   std::string err;
   double e2[8] = {0}, sumc2[8] = {0}, scal[6] = {0};
+  // resumed from a restart: subspaces before `first` taken from e2_file / sumc2_file
+  int32_t first = 0;
+  double e2_file[8] = {0}, sumc2_file[8] = {0};
+  // checkpoint state (filled by the subspace callback of the slot's context)
+  std::string ckpt_path;
+  std::mutex ckpt_mu;
+  bool sub_done[8] = {false};
+  double ck_e2[8] = {0}, ck_sumc2[8] = {0};
+  int ck_next = 0;
 };
 
 const char* kBlockNames[] = {"OOOO", "F1", "F2", "A1", "A2", "B", "C1", "C2", "C3",
@@ -246,6 +259,8 @@ struct dc2_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::vector<std::thread> threads;
+  std::string ckpt_dir;   // per-request checkpoint files (empty: off)
+  std::vector<std::pair<std::string, int64_t>> options;   // applied to every slot context
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -270,9 +285,38 @@ int bind_molecule(dc2_context* c, const Molecule& m) {
   return st;
 }
 
+// subspace k of request r finished: extend the completed prefix and rewrite the checkpoint
+void on_subspace_done(void* user, int32_t k, double e2, double sumc2) {
+  Request* r = static_cast<Request*>(user);
+  std::lock_guard<std::mutex> lk(r->ckpt_mu);
+  r->sub_done[k] = true;
+  r->ck_e2[k] = e2;
+  r->ck_sumc2[k] = sumc2;
+  const int before = r->ck_next;
+  while (r->ck_next < 8 && r->sub_done[r->ck_next]) ++r->ck_next;
+  if (r->ck_next != before) (void)dc2_write_restart(r->ckpt_path.c_str(), 1, r->root, r->ck_next, r->ck_e2, r->ck_sumc2);
+}
+
 void run_request(dc2_server* S, int k, Request* r) {
   dc2_context* c = S->slot_ctx[k];
   int st = 0;
+  if (!r->ckpt_path.empty()) {
+    for (int q = 0; q < 8; ++q)
+      if (q < r->first) {   // the resumed prefix is done already
+        r->sub_done[q] = true;
+        r->ck_e2[q] = r->e2_file[q];
+        r->ck_sumc2[q] = r->sumc2_file[q];
+      }
+    // subspaces the request skips (no inactive or secondary spinors) count as done
+    const Molecule& m = *r->mol;
+    const bool skip[8] = {m.ni == 0, m.ni == 0, m.ns == 0, m.ni == 0 || m.ns == 0, m.ni == 0 || m.ns == 0,
+                          m.ns == 0, m.ni == 0 || m.ns == 0, m.ni == 0 || m.ns == 0};
+    for (int q = 0; q < 8; ++q) r->sub_done[q] = r->sub_done[q] || skip[q];
+    r->ck_next = 0;
+    while (r->ck_next < 8 && r->sub_done[r->ck_next]) ++r->ck_next;
+    st = dc2_write_restart(r->ckpt_path.c_str(), 1, r->root, r->ck_next, r->ck_e2, r->ck_sumc2);
+    if (!st) st = dc2_set_subspace_callback(c, on_subspace_done, r);
+  }
   if (S->slot_mol[k] != r->mol) {
     S->slot_mol[k] = nullptr;
     st = bind_molecule(c, *r->mol);
@@ -284,9 +328,14 @@ void run_request(dc2_server* S, int k, Request* r) {
   std::vector<double> eig(r->root);
   int64_t ndet = 0;
   if (!st) st = dc2_casci(c, r->root, eig.data(), &ndet);
-  if (!st) st = dc2_caspt2(c, r->root, r->eshift, r->e2, r->sumc2, r->scal);
+  if (!st) {
+    st = r->first > 0 ? dc2_caspt2_restart(c, r->root, r->eshift, r->first, r->e2_file, r->sumc2_file, r->e2,
+                                           r->sumc2, r->scal)
+                      : dc2_caspt2(c, r->root, r->eshift, r->e2, r->sumc2, r->scal);
+  }
   r->status = st;
   if (st) r->err = dc2_last_error();
+  if (!r->ckpt_path.empty()) (void)dc2_set_subspace_callback(c, nullptr, nullptr);
 }
 
 // a fresh context for slot k (its scratch released); the slot's thread only
@@ -302,6 +351,17 @@ int renew_context(dc2_server* S, int k) {
     return st;
   }
   if (S->batcher) dc2::set_batcher(c, S->batcher.get());
+  std::vector<std::pair<std::string, int64_t>> opts;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    opts = S->options;
+  }
+  for (const auto& o : opts)
+    if (!st) st = dc2_set_option(c, o.first.c_str(), o.second);
+  if (st) {
+    dc2_context_destroy(c);
+    return st;
+  }
   S->slot_ctx[k] = c;
   return 0;
 }
@@ -454,9 +514,35 @@ int dc2_server_add_molecule(dc2_server* S, int32_t ninact, int32_t nact, int32_t
   return 0;
 }
 
+int dc2_server_set_option(dc2_server* S, const char* key, int64_t value) {
+  if (!S || !key) return fail(1, "dc2_server_set_option: invalid argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (dc2_context* c : S->slot_ctx) {
+    if (!c) continue;
+    const int st = dc2_set_option(c, key, value);
+    if (st) return st;
+  }
+  S->options.emplace_back(key, value);
+  return 0;
+}
+
+int dc2_server_set_checkpoint_dir(dc2_server* S, const char* dir) {
+  if (!S) return fail(1, "dc2_server_set_checkpoint_dir: null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->ckpt_dir = dir ? dir : "";
+  return 0;
+}
+
 int dc2_server_submit(dc2_server* S, int32_t mol_id, const double* h, double ecore, int32_t selectroot, double eshift,
                       const int32_t* ras, int32_t flags, int64_t* req_id) {
-  if (!S || !h || !req_id || selectroot < 1 || !(flags == 0 || (flags >= 3 && flags <= 8)))
+  return dc2_server_submit_restart(S, mol_id, h, ecore, selectroot, eshift, ras, flags, 0, nullptr, nullptr, req_id);
+}
+
+int dc2_server_submit_restart(dc2_server* S, int32_t mol_id, const double* h, double ecore, int32_t selectroot,
+                              double eshift, const int32_t* ras, int32_t flags, int32_t first, const double* e2_file,
+                              const double* sumc2_file, int64_t* req_id) {
+  if (!S || !h || !req_id || selectroot < 1 || !(flags == 0 || (flags >= 3 && flags <= 8)) || first < 0 ||
+      first > 8 || (first > 0 && (!e2_file || !sumc2_file)))
     return fail(1, "dc2_server_submit: invalid argument");
   auto r = std::make_unique<Request>();
   {
@@ -473,11 +559,21 @@ int dc2_server_submit(dc2_server* S, int32_t mol_id, const double* h, double eco
   r->eshift = eshift;
   r->flags = flags;
   if (ras) std::memcpy(r->ras, ras, sizeof(r->ras));
+  r->first = first;
+  if (first > 0) {
+    std::memcpy(r->e2_file, e2_file, sizeof(r->e2_file));
+    std::memcpy(r->sumc2_file, sumc2_file, sizeof(r->sumc2_file));
+  }
   r->footprint = footprint_bytes(m.ni, m.na, m.ns, m.ne);
   if (r->footprint > S->budget) return fail(1, "dc2_server_submit: request exceeds the memory budget");
   {
     std::lock_guard<std::mutex> lk(S->mu);
     r->id = S->next_id++;
+    if (!S->ckpt_dir.empty()) {
+      const std::string d = S->ckpt_dir + "/" + std::to_string(r->id);
+      if (::mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) return fail(1, "dc2_server_submit: cannot create " + d);
+      r->ckpt_path = d + "/caspt2_restart_1_" + std::to_string(selectroot);
+    }
     r->t_submit = S->now();
     *req_id = r->id;
     S->pending.push_back(r.release());

@@ -112,6 +112,20 @@ int dc2_get_transformed_block(dc2_context* ctx, const char* name, double* out, i
  * t[4..11] subspaces A..H (assembly + solves, transforms excluded) */
 int dc2_get_timings(dc2_context* ctx, double* t, int32_t n);
 
+/* Progress and checkpoints (no counterpart in the reference, whose restart files
+ * module_restart_file.F90:55-172 reads are written by hand): cb(user, k, e2, sumc2) is called
+ * from the stream thread that finished subspace k (0 = A .. 7 = H) of a dc2_caspt2 on this
+ * context, as soon as its energy is known (subspaces run concurrently, so in any order; the
+ * callback must be thread-safe and must not call into this context).  NULL removes it. */
+int dc2_set_subspace_callback(dc2_context* ctx, void (*cb)(void* user, int32_t subspace, double e2, double sumc2),
+                              void* user);
+/* Writes the caspt2_restart file read_and_validate_restart_file reads (module_restart_file.F90:
+ * 55-172): "totsym: N", "selectroot: N", "Next subspace: X" (X = 'A' + next; 'I' when next = 8)
+ * and one line "Subspace: Y Sumc2: v Energy: v" for each subspace before X, values printed to
+ * 17 significant digits; written to path.tmp, then renamed.  No GPU needed. */
+int dc2_write_restart(const char* path, int32_t totsym, int32_t selectroot, int32_t next, const double* e2,
+                      const double* sumc2);
+
 /* ---- stand-alone pieces --------------------------------------------------------- */
 /* CI matrix <i|H|j> (ndet x ndet complex, host) after dc2_casci (casmat.F90:3-222) */
 int dc2_casmat(dc2_context* ctx, double* mat);
@@ -188,6 +202,9 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *   "split_slices": 0 (default) FP64 integral transforms, 3..8 split-precision passes
  *                  (dc2_zgemm_split) with this many slices;
  *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32);
+ *   "native_min":  n >= this (and n within this library's eigensolver, 3..2048) uses this
+ *                  library's eigensolver even where the eigensolves are batched across served
+ *                  requests (default 0: batched rocSOLVER for every size there);
  *   "casci_solver": 0 automatic (default: block Davidson for ndet >= 300 and nroot <= 16,
  *                  the dense eigensolver otherwise), 1 dense (all roots, as cdiag), 2 Davidson
  *                  (the nroot lowest roots; falls back to dense when it does not converge). */
@@ -307,6 +324,19 @@ int dc2_server_submit(dc2_server* srv, int32_t mol_id, const double* h, double e
  * out[22..24] submit/start/end times (s since server creation), out[25] slot;
  * *req_status is the request's status (its message via dc2_last_error) */
 int dc2_server_wait(dc2_server* srv, double timeout_s, int64_t* req_id, int32_t* req_status, double* out);
+/* Per-subspace checkpoints of the served requests: with a directory set, every request writes
+ * <dir>/<request id>/caspt2_restart_1_<selectroot> (dc2_write_restart) whenever the subspaces
+ * finished so far extend the completed prefix A, B, ... (the reference resumes from a prefix), so
+ * an interrupted request can be resubmitted with dc2_server_submit_restart.  NULL or "": off. */
+int dc2_server_set_checkpoint_dir(dc2_server* srv, const char* dir);
+/* dc2_server_submit resumed from a restart: subspaces before `first` (0 = A .. 8 = none) are taken
+ * from e2_file[8] / sumc2_file[8], as dc2_caspt2_restart does. */
+int dc2_server_submit_restart(dc2_server* srv, int32_t mol_id, const double* h, double ecore, int32_t selectroot,
+                              double eshift, const int32_t* ras, int32_t flags, int32_t first, const double* e2_file,
+                              const double* sumc2_file, int64_t* req_id);
+/* dc2_set_option on every slot's context, now and for contexts the slots create later; call it
+ * before the first dc2_server_submit */
+int dc2_server_set_option(dc2_server* srv, const char* key, int64_t value);
 /* out[7] = {pending, running, completed, footprint bytes the slots hold, its peak,
  *           eigensolve batches, eigenproblems solved in them} */
 int dc2_server_stats(dc2_server* srv, int64_t* out);

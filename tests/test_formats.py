@@ -91,3 +91,21 @@ def test_cidata_checks(tmp_path):
     open(p, "wb").write(raw[:-1])                        # a cut record marker
     with pytest.raises(ValueError):
         F.read_cidata(p)
+
+
+def test_library_restart_writer_reads_back(tmp_path):
+    """dc2_write_restart (what the request server writes per subspace) gives the file the
+    reference's reader layout describes: read back by formats.read_restart, every value exact,
+    only the subspaces before the next one listed."""
+    from dirac_caspt2_amd.api import write_restart
+    from dirac_caspt2_amd.formats import read_restart
+    e2 = {X: -0.1 * (k + 1) / 3.0 for k, X in enumerate("ABCDEFGH")}
+    s2 = {X: 1e-3 * (k + 1) / 7.0 for k, X in enumerate("ABCDEFGH")}
+    for nxt in (0, 3, 8):
+        path = str(tmp_path / f"caspt2_restart_1_{nxt}")
+        write_restart(path, 1, 2, nxt, e2, s2)
+        rs = read_restart(path, 1, 2)
+        assert rs.next_subspace == "ABCDEFGHI"[nxt]
+        assert sorted(rs.e2) == list("ABCDEFGH"[:nxt])
+        for X in rs.e2:
+            assert rs.e2[X] == e2[X] and rs.sumc2[X] == s2[X]

@@ -163,3 +163,37 @@ def test_admission_counts_memory_slots_hold(mix):
         for X in "ABCDEFGH":
             assert abs(alone[k].e2[X] - res.e2[X]) < 1e-10, (k, X)
         assert abs(alone[k].total - res.total) < 1e-10, k
+
+
+def test_served_checkpoints_and_resume(mix, tmp_path):
+    """With a checkpoint directory each request writes caspt2_restart_1_<root> as its subspaces
+    finish (the completed prefix); resubmitting a request from such a file (its first four
+    subspaces taken from it) gives the uninterrupted energies within 1e-10 Eh."""
+    from dirac_caspt2_amd.api import Server
+    from dirac_caspt2_amd.formats import read_restart
+    from gpu_helpers import say
+    mols, reqs = mix
+    spec, sp, direction, ras = mols[0]
+    from dirac_caspt2_amd import synth
+    h = synth.scan_point(sp, 0.3, direction)
+    with Server(0, nslots=2, lanes_per_slot=4) as srv:
+        mid = srv.add_molecule((spec.ninact, spec.nact, spec.nsec, spec.nelec), sp.B)
+        srv.set_checkpoint_dir(str(tmp_path))
+        rid = srv.submit(mid, h, sp.ecore, 1, 0.0)
+        full = srv.wait(120.0)
+        assert full is not None and full.id == rid and full.status == 0, full
+        rs = read_restart(str(tmp_path / str(rid) / "caspt2_restart_1_1"), 1, 1)
+        say("checkpoint after the request:", rs.next_subspace, sorted(rs.e2))
+        assert rs.next_subspace == "I" and sorted(rs.e2) == list("ABCDEFGH")
+        for X in "ABCDEFGH":
+            assert rs.e2[X] == full.e2[X] and rs.sumc2[X] == full.sumc2[X]
+        part = ({X: rs.e2[X] for X in "ABCD"}, {X: rs.sumc2[X] for X in "ABCD"})
+        rid2 = srv.submit(mid, h, sp.ecore, 1, 0.0, restart=(4, part[0], part[1]))
+        res = srv.wait(120.0)
+        assert res is not None and res.id == rid2 and res.status == 0, res
+        rs2 = read_restart(str(tmp_path / str(rid2) / "caspt2_restart_1_1"), 1, 1)
+        assert rs2.next_subspace == "I"
+    for X in "ABCDEFGH":
+        assert abs(res.e2[X] - full.e2[X]) < 1e-10, X
+        assert abs(res.sumc2[X] - full.sumc2[X]) < 1e-10, X
+    assert abs(res.total - full.total) < 1e-10
