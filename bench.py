@@ -480,7 +480,10 @@ def run_workload(name, args, steps, warmup, served, slots, autotune_on, accounti
         res["vs_oracle"] = {"fp64_gpu": go[0], "split": go[1]}
     if accounting:
         # every eigenproblem size of the path: S (dimn) and B~ (dimm) of each subspace
-        ht = sorted({int(v) for X in "ABCDEFGH" for v in split_out.info.get(X, ())[:2] if int(v) >= 3})
+        # and the determinant-space and tuple bases of the A/C metrics
+        ht = sorted({int(v) for X in "ABCDEFGH" for v in split_out.info.get(X, ())[:2] if int(v) >= 3} |
+                    {comb(spec.nact, spec.nelec + 1), comb(spec.nact, spec.nelec - 1),
+                     spec.nact * comb(spec.nact, 2)})
         hr = hetrd_roofline(ctx, ht, steps, name)
         gr = gemm_accounting(ctx, step, name, max(1, min(steps, 5)))
         if hr is not None and hr["seconds_per_step"] > gr["seconds_per_step"]:

@@ -190,6 +190,7 @@ struct dc2_context {
   // eigenproblems with n >= native_min (and n within this library's eigensolver) take it even
   // when the eigensolves are batched across requests (0: never)
   int native_min = 0;
+  int dav_maxsub = 0;   // block-Davidson subspace before a thick restart (0: max(6 b, 24))
   std::map<std::vector<int64_t>, hipGraphExec_t> dav_graphs;
   int64_t graph_launches = 0, graph_captures = 0;
 
@@ -490,7 +491,8 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   // b = 4, maxsub = 24 for one root: measured fastest on the 924-determinant CASCI (20
   // iterations, against 19 with 6 / 48, whose larger host eigenproblems cost more)
   const int b = (int)std::min<int64_t>(n, std::max(nroot + 3, 4));
-  const int maxsub = (int)std::min<int64_t>(n, std::max(6 * b, 24));
+  const int maxsub = (int)std::min<int64_t>(n, c->dav_maxsub > 0 ? std::max(c->dav_maxsub, 2 * b)
+                                                                  : std::max(6 * b, 24));
   if (maxsub < 2 * b) return false;
   cplx* V = L.z("dv_V", n * maxsub);
   cplx* W = L.z("dv_W", n * maxsub);
@@ -2150,6 +2152,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "split_min_k") {
       DC2_CHECK(value >= 1 && value <= 4096, 1, "split_min_k must be in 1..4096");
       c->split_min_k = (int)value;
+    } else if (k == "davidson_maxsub") {
+      DC2_CHECK(value >= 0 && value <= 512, 1, "davidson_maxsub must be in 0..512");
+      c->dav_maxsub = (int)value;
     } else if (k == "native_min") {
       DC2_CHECK(value >= 0, 1, "native_min must be >= 0");
       c->native_min = (int)value;

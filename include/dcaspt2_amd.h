@@ -202,6 +202,8 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *   "split_slices": 0 (default) FP64 integral transforms, 3..8 split-precision passes
  *                  (dc2_zgemm_split) with this many slices;
  *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32);
+ *   "davidson_maxsub": vectors of the block Davidson's subspace before a thick restart
+ *                  (0: automatic, max(6 b, 24) for a block of b vectors);
  *   "native_min":  n >= this (and n within this library's eigensolver, 3..2048) uses this
  *                  library's eigensolver even where the eigensolves are batched across served
  *                  requests (default 0: batched rocSOLVER for every size there);
