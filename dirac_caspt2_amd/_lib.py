@@ -60,6 +60,7 @@ _SIGS = {
     "dc2_get_integral_block_device": (c_int, [P, ctypes.c_char_p, P, P]),
     "dc2_set_integral_records": (c_int, [P, c_i64, P, P, c_int]),
     "dc2_zgemm_split": (c_int, [P, c_int, ctypes.c_char, c_i64, c_i64, c_i64, P, c_i64, P, c_i64, P, c_i64]),
+    "dc2_zgemm_split_noalloc": (c_int, [P, c_int, ctypes.c_char, c_i64, c_i64, c_i64, P, c_i64, P, c_i64, P, c_i64]),
     "dc2_get_transform_flops": (c_int, [P, P]),
     "dc2_get_casci_info": (c_int, [P, P]),
     "dc2_host_heev": (c_int, [c_int, P, P, c_int]),

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <array>
 #include <cstdio>
+#include <functional>
 #include <atomic>
 #include <exception>
 #include <future>
@@ -1628,6 +1629,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
 namespace dc2 {
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 void set_batcher(dc2_context* c, EigBatcher* b) { c->batcher = b; }
+int run_guarded(const std::function<void()>& f) { return guarded(f); }
+hipStream_t context_stream(dc2_context* c) { return c->stream; }
+int context_device(dc2_context* c) { return c->device; }
 }  // namespace dc2
 
 // ===================================================================== C ABI

@@ -218,6 +218,16 @@ int dc2_set_option(dc2_context* ctx, const char* key, int64_t value);
  * use it for passes with K >= "split_min_k" when the option "split_slices" is non-zero. */
 int dc2_zgemm_split(dc2_context* ctx, int32_t slices, char opB, int64_t M, int64_t N, int64_t K, const double* A,
                     int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc);
+/* The same product as dc2_zgemm_split, allocating nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  The INT8 slices and exponents live in a fixed 64 MiB device array of the library
+ * (held on every device it runs on until the process ends); a product needing more,
+ * slices * (M + 2N) * 2 * ceil32(K) + 4 (M + N) bytes rounded up to 256, is refused with
+ * status 1.  Asynchronous on the context's stream (dc2_synchronize before reading C), no
+ * hipMalloc/hipFree and no host synchronisation.  Every call on a device shares the array:
+ * calls on one context are ordered by its stream; calls that overlap on one device race. */
+int dc2_zgemm_split_noalloc(dc2_context* ctx, int32_t slices, char opB, int64_t M, int64_t N, int64_t K,
+                            const double* A, int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc);
 /* transform flops of the last dc2_caspt2: out[0] in split precision, out[1] in FP64 */
 int dc2_get_transform_flops(dc2_context* ctx, double* out);
 /* Measurement, no counterpart in the reference: out[2] = {calls, summed device seconds} of
