@@ -191,7 +191,8 @@ struct dc2_context {
   // eigenproblems with n >= native_min (and n within this library's eigensolver) take it even
   // when the eigensolves are batched across requests (0: never)
   int native_min = 0;
-  int dav_maxsub = 0;   // block-Davidson subspace before a thick restart (0: max(6 b, 24))
+  int dav_maxsub = 0;
+  int native_max = 2048;   // largest eigenproblem this library's eigensolver takes   // block-Davidson subspace before a thick restart (0: max(6 b, 24))
   std::map<std::vector<int64_t>, hipGraphExec_t> dav_graphs;
   int64_t graph_launches = 0, graph_captures = 0;
 
@@ -393,7 +394,7 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
     return;
   }
   const int64_t nn = (int64_t)n * n;
-  const bool native_ok = n >= 3 && n <= 2048;
+  const bool native_ok = n >= 3 && n <= std::min(2048, c->native_max);
   const bool batched = c->batcher && !(c->native_min > 0 && n >= c->native_min);
   int method = c->eig_method >= 0 ? c->eig_method
                                   : (kind == EigKind::rank_deficient ? 2 : (native_ok && !batched ? 4 : 1));
@@ -2159,6 +2160,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "davidson_maxsub") {
       DC2_CHECK(value >= 0 && value <= 512, 1, "davidson_maxsub must be in 0..512");
       c->dav_maxsub = (int)value;
+    } else if (k == "native_max") {
+      DC2_CHECK(value >= 0 && value <= 2048, 1, "native_max must be in 0..2048");
+      c->native_max = (int)value;
     } else if (k == "native_min") {
       DC2_CHECK(value >= 0, 1, "native_min must be >= 0");
       c->native_min = (int)value;

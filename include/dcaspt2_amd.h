@@ -204,6 +204,8 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32);
  *   "davidson_maxsub": vectors of the block Davidson's subspace before a thick restart
  *                  (0: automatic, max(6 b, 24) for a block of b vectors);
+ *   "native_max":  largest n this library's eigensolver takes (default 2048, at most 2048);
+ *                  rocSOLVER's divide and conquer above;
  *   "native_min":  n >= this (and n within this library's eigensolver, 3..2048) uses this
  *                  library's eigensolver even where the eigensolves are batched across served
  *                  requests (default 0: batched rocSOLVER for every size there);
