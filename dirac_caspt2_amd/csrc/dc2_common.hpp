@@ -30,6 +30,19 @@ __host__ __device__ inline void cfma(cplx& a, cplx b, cplx c) {
   a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
 }
 
+// XCD-aware tile order of a 2D grid of Mt x Nt output tiles (blockIdx.x over Mt, .y over Nt):
+// workgroups are dealt to the 8 XCDs in turn by their linear id, so the i-th workgroup an XCD
+// receives takes tile i of that XCD's contiguous share of the tiles in n-fastest order.  The
+// tiles of one row block (one stretch of A) then run on one XCD one after another and reuse
+// that XCD's L2 copy of it, instead of being spread over all eight.  A bijection on the tiles.
+__device__ inline void xcd_tile(int64_t Mt, int64_t Nt, int64_t bx, int64_t by, int64_t& mt, int64_t& nt) {
+  const int64_t T = Mt * Nt, lin = by * Mt + bx;
+  const int64_t q = T / 8, r = T % 8, x = lin % 8, i = lin / 8;
+  const int64_t g = x * q + (x < r ? x : r) + i;
+  mt = g / Nt;
+  nt = g - mt * Nt;
+}
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}

@@ -150,7 +150,9 @@ __global__ __launch_bounds__(256) void k_zgemm2(int64_t M, int64_t N, int64_t Kf
                                                 const cplx* __restrict__ B, int64_t ldb, int64_t sB,
                                                 cplx beta, cplx* __restrict__ C, int64_t ldc, int64_t sC, int sk,
                                                 int64_t kchunk) {
-  if (UPPER && (int64_t)blockIdx.x * BM > (int64_t)blockIdx.y * BN + BN - 1) return;   // uniform
+  int64_t mt, nt;
+  xcd_tile(gridDim.x, gridDim.y, blockIdx.x, blockIdx.y, mt, nt);
+  if (UPPER && mt * BM > nt * BN + BN - 1) return;   // uniform
   constexpr int LS = BM + PAD;               // BM == BN
   constexpr int NE = BM * KB / 256;          // elements of each operand tile per thread
   __shared__ double Ar[KB][LS], Ai[KB][LS], Br[KB][LS], Bi[KB][LS];
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(256) void k_zgemm2(int64_t M, int64_t N, int64_t Kf
   C += (sk > 1 ? bz * sk + kz : bz) * sC;   // split K: partial kz of product bz
   const int64_t kbeg = kz * kchunk;
   const int64_t kend = (kbeg + kchunk < Kfull) ? kbeg + kchunk : Kfull;
-  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+  const int64_t m0 = mt * BM, n0 = nt * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   d4 accr[2][2], acci[2][2];

@@ -43,7 +43,7 @@ constexpr int HT_THREADS = 64 * HT_WAVES;
 constexpr int HT_NMAX = 2048;
 // columns per wave: 2 up to n = 1024 (4 spills registers at n > 768); 1 above, where the
 // 2048 / 8 = 256 workgroups of the largest matrix take every compute unit
-inline int hetrd_cpw(int n) { return n <= 1024 ? 2 : 1; }
+inline int hetrd_cpw(int n) { return n <= 1280 ? 2 : 1; }
 inline int hetrd_nwg(int n) { return (n + HT_WAVES * hetrd_cpw(n) - 1) / (HT_WAVES * hetrd_cpw(n)); }
 
 struct HetrdWork {
@@ -563,7 +563,7 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     else if (n <= 768) st = launch_hetrd<12>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 832) st = launch_hetrd<13>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1024) st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
-    else if (n <= 1280) st = launch_hetrd<20, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 1280) st = launch_hetrd<20, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1536) st = launch_hetrd<24, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1792) st = launch_hetrd<28, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else st = launch_hetrd<32, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);

@@ -77,8 +77,10 @@ __global__ __launch_bounds__(256) void k_oz_gemm(int64_t M, int64_t N, int64_t K
                                                  const int8_t* __restrict__ Bim, const int* __restrict__ eb,
                                                  cplx* __restrict__ C, int64_t ldc) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t m0 = (int64_t)blockIdx.x * OZ_BM + (wave & 1) * 32;
-  const int64_t n0 = (int64_t)blockIdx.y * OZ_BN + (wave >> 1) * 16;
+  int64_t mt, nt;
+  xcd_tile(gridDim.x, gridDim.y, blockIdx.x, blockIdx.y, mt, nt);
+  const int64_t m0 = mt * OZ_BM + (wave & 1) * 32;
+  const int64_t n0 = nt * OZ_BN + (wave >> 1) * 16;
   const int r16 = lane & 15, g = lane >> 4;
   const int64_t sliceA = M * K2, sliceB = N * K2;
   v4i accr[S][2], acci[S][2];

@@ -203,7 +203,8 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  (dc2_zgemm_split) with this many slices;
  *   "split_min_k": smallest contracted dimension of a split-precision pass (default 32);
  *   "davidson_maxsub": vectors of the block Davidson's subspace before a thick restart
- *                  (0: automatic, max(6 b, 24) for a block of b vectors);
+ *                  (0: automatic, max(6 b, 24) for a block of b vectors, 12 b from 1500
+ *                  determinants);
  *   "native_max":  largest n this library's eigensolver takes (default 2048, at most 2048);
  *                  rocSOLVER's divide and conquer above;
  *   "native_min":  n >= this (and n within this library's eigensolver, 3..2048) uses this
