@@ -172,6 +172,23 @@ def measured_traffic(workload, kernel, key):
     return None
 
 
+def stage_hbm(workload):
+    """Achieved HBM bandwidth of the bandwidth-bound stage kernels from the committed rocprofv3
+    summary of this workload (profiles/stage_hbm_<workload>.txt: counter bytes per launch and
+    the traced mean launch duration), or None."""
+    path = os.path.join(HERE, "profiles", f"stage_hbm_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        w = line.split()
+        if len(w) == 4 and w[0] == "stage_hbm":
+            b, t = float(w[2]), float(w[3])
+            out[w[1]] = {"bytes_per_launch": b, "mean_launch_us": 1e6 * t, "achieved_GBs": b / t / 1e9,
+                         "frac_of_hbm_peak": b / t / 1e9 / PEAK_HBM_GBS}
+    return out or None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -491,6 +508,7 @@ def run_workload(name, args, steps, warmup, served, slots, autotune_on, accounti
             res["roofline"] = hr
         else:
             res["roofline"] = gr
+        res["roofline"]["stage_hbm"] = stage_hbm(name)
     if served:
         idx = sorted({0, served // 3, (2 * served) // 3, served - 1})
         ctx.set_option("split_slices", 0)

@@ -43,7 +43,7 @@ constexpr int HT_THREADS = 64 * HT_WAVES;
 constexpr int HT_NMAX = 2048;
 // columns per wave: 2 up to n = 1024 (4 spills registers at n > 768); 1 above, where the
 // 2048 / 8 = 256 workgroups of the largest matrix take every compute unit
-inline int hetrd_cpw(int n) { return n <= 1280 ? 2 : 1; }
+inline int hetrd_cpw(int n) { return n <= 1024 ? 2 : 1; }
 inline int hetrd_nwg(int n) { return (n + HT_WAVES * hetrd_cpw(n) - 1) / (HT_WAVES * hetrd_cpw(n)); }
 
 struct HetrdWork {
@@ -239,13 +239,36 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
         }
       }
     if (!grid_sync(w.bar, (unsigned)(k + 1) * (unsigned)nwg, w.err, t != 0.0 ? Sp + wg : nullptr, red)) return;
-    // (b) one round of coherent loads: column k+1, p and the partials (nwg <= 64, one per
-    // thread, summed in the same order in every workgroup); q = p - K v is formed where used
+    // (b) one round of coherent loads: column k+1, p and the partials (nwg <= 256, one per
+    // thread, summed in the same order in every workgroup); q = p - K v is formed where used.
+    // Every load of the round is issued before the first is used (NL per thread), so their
+    // latencies overlap instead of adding up over the rows
+    // (in batches of two per thread for the largest tiles, whose columns fill the registers)
+    constexpr int NL = (64 * S + HT_THREADS - 1) / HT_THREADS;
+    constexpr int NB = S >= 30 ? 2 : NL;
     double kp = 0.0;
     if (t != 0.0 && threadIdx.x < nwg) kp = ld_coherent(Sp + threadIdx.x);
-    for (int i = k + 1 + threadIdx.x; i < n; i += HT_THREADS) {
-      sc[i] = ld_coherent(NC + i);
-      if (t != 0.0) sq[i] = ld_coherent(P + i);
+#pragma unroll
+    for (int u0 = 0; u0 < NL; u0 += NB) {
+      cplx lc[NB], lq[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int i = k + 1 + threadIdx.x + (u0 + u) * HT_THREADS;
+        lc[u] = mk(0.0, 0.0);
+        lq[u] = mk(0.0, 0.0);
+        if (u0 + u < NL && i < n) {
+          lc[u] = ld_coherent(NC + i);
+          if (t != 0.0) lq[u] = ld_coherent(P + i);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int i = k + 1 + threadIdx.x + (u0 + u) * HT_THREADS;
+        if (u0 + u < NL && i < n) {
+          sc[i] = lc[u];
+          if (t != 0.0) sq[i] = lq[u];
+        }
+      }
     }
     if (t != 0.0) {
       const double K = 0.5 * t * block_sum(kp, red);   // its barriers also publish sc and sq
@@ -563,7 +586,7 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     else if (n <= 768) st = launch_hetrd<12>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 832) st = launch_hetrd<13>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1024) st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
-    else if (n <= 1280) st = launch_hetrd<20, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 1280) st = launch_hetrd<20, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1536) st = launch_hetrd<24, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1792) st = launch_hetrd<28, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else st = launch_hetrd<32, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
