@@ -333,12 +333,14 @@ class Caspt2Context:
     OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3, "native": 4},
                "ac_metric": {"auto": 0, "formula": 1, "determinant": 2, "antisymmetric": 3},
                "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {},
-               "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}, "graphs": {"off": 0, "on": 1}}
+               "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}, "graphs": {"off": 0, "on": 1},
+               "tridiag_solver": {"own": 0, "rocsolver": 1}}
 
     def set_option(self, key: str, value):
         """Options: eigensolver auto|qr|dc|dcj|jacobi|native, ac_metric auto|formula|determinant,
         lanes 1..4 (concurrent subspace streams), schur on|off, graphs on|off (captured
-        Davidson iterations)."""
+        Davidson iterations), tridiag_solver own|rocsolver (the tridiagonal stage of the native
+        eigensolver)."""
         v = self.OPTIONS[key][value] if isinstance(value, str) else int(value)
         check("dc2_set_option", self.lib.dc2_set_option(self.h, key.encode(), v))
 

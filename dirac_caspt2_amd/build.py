@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdcaspt2_amd.so")
-SOURCES = ["dc2_gemm.hip", "dc2_ozaki.hip", "dc2_ci.hip", "dc2_pt2.hip", "dc2_ingest.hip", "dc2_heev.hip", "dc2_api.hip",
+SOURCES = ["dc2_gemm.hip", "dc2_ozaki.hip", "dc2_ci.hip", "dc2_pt2.hip", "dc2_ingest.hip", "dc2_heev.hip", "dc2_tridc.hip",
+           "dc2_api.hip",
            "dc2_serve.hip", "dc2_noalloc.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")

@@ -191,8 +191,11 @@ struct dc2_context {
   // eigenproblems with n >= native_min (and n within this library's eigensolver) take it even
   // when the eigensolves are batched across requests (0: never)
   int native_min = 0;
-  int dav_maxsub = 0;
-  int native_max = 2048;   // largest eigenproblem this library's eigensolver takes   // block-Davidson subspace before a thick restart (0: max(6 b, 24))
+  int dav_maxsub = 0;   // block-Davidson subspace before a thick restart (0: max(6 b, 24))
+  int native_max = 2048;   // largest eigenproblem this library's eigensolver takes
+  // the tridiagonal stage of this library's eigensolver: its own divide and conquer (true) or
+  // rocSOLVER dstedc
+  bool tri_own = true;
   std::map<std::vector<int64_t>, hipGraphExec_t> dav_graphs;
   int64_t graph_launches = 0, graph_captures = 0;
 
@@ -282,10 +285,19 @@ const cplx* get_block(dc2_context* c, const std::string& name) {
   throw Error(1, "integral block " + name + " has not been set");
 }
 
+// copy ordered on the context's stream, then waited for.  Never the legacy stream: another
+// context's thread may be capturing a graph on a blocking stream, and a legacy-stream copy
+// would have to wait for it, which the runtime refuses.
+void copy_sync(dc2_context* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (bytes == 0) return;
+  DC2_HIP(hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+  DC2_HIP(hipStreamSynchronize(c->stream));
+}
+
 // diagnostics of a matrix handed to an eigensolver that failed
 std::string matrix_report(dc2_context* c, int n, const cplx* A) {
   std::vector<cplx> h((size_t)n * n);
-  (void)hipMemcpy(h.data(), A, h.size() * sizeof(cplx), hipMemcpyDeviceToHost);
+  copy_sync(c, h.data(), A, h.size() * sizeof(cplx), hipMemcpyDeviceToHost);
   int64_t nonfinite = 0;
   double herm = 0.0, amax = 0.0, dimag = 0.0;
   for (int j = 0; j < n; ++j)
@@ -403,16 +415,21 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
   DC2_HIP(hipMemcpyAsync(keep, A, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
   int info = 0;
   if (method == 4) {
-    // tridiagonalisation of this library + rocSOLVER dstedc; rocSOLVER D&C if it cannot run
-    // or its result is not usable
+    // tridiagonalisation and tridiagonal divide and conquer of this library (or rocSOLVER
+    // dstedc); rocSOLVER's D&C if it cannot run or its result is not usable
     void* work = L.d("eig_native", heev_native_work_bytes(n) / 8 + 1);
     L.info.ensure(2);
     bool done = heev_native(L.stream, L.blas, n, A, n, w, work, L.info.get(),
-                            reinterpret_cast<int*>(L.host(2 * sizeof(int), 1)));
+                            reinterpret_cast<int*>(L.host(2 * sizeof(int), 1)), c->tri_own, c->debug);
+    bool ran = done, finite = false;
     if (done) {
       L.download(&info, L.info.get(), sizeof(int));
-      done = info == 0 && all_finite(L, A, nn);
+      finite = all_finite(L, A, nn);
+      done = info == 0 && finite;
     }
+    if (!done && c->debug)
+      std::fprintf(stderr, "dc2: native eigensolver rejected for %s (n=%d): %s, info %d, finite %d\n", what, n,
+                   ran ? "ran" : "did not run", info, (int)finite);
     if (!done) {
       ++c->eig_fallbacks;
       DC2_HIP(hipMemcpyAsync(A, keep, nn * sizeof(cplx), hipMemcpyDeviceToDevice, L.stream));
@@ -1277,8 +1294,8 @@ const cplx* root_vector(dc2_context* c, int selectroot) {
   const int g = (int)grp.size();
   std::vector<zc> V((size_t)nd * g), out((size_t)nd, zc(0.0, 0.0));
   for (int q = 0; q < g; ++q)
-    DC2_HIP(hipMemcpy(V.data() + (size_t)nd * q, c->civec.get() + (int64_t)grp[q] * nd, nd * sizeof(cplx),
-                      hipMemcpyDeviceToHost));
+    copy_sync(c, V.data() + (size_t)nd * q, c->civec.get() + (int64_t)grp[q] * nd, nd * sizeof(cplx),
+              hipMemcpyDeviceToHost);
   std::vector<double> wt((size_t)nd, 0.0);
   double mx = 0.0;
   for (int64_t i = 0; i < nd; ++i) {
@@ -1293,7 +1310,7 @@ const cplx* root_vector(dc2_context* c, int selectroot) {
     for (int64_t i = 0; i < nd; ++i) out[i] += V[i + (size_t)nd * q] * a;
   }
   cplx* dst = c->lane0->z("root_vector", nd);
-  DC2_HIP(hipMemcpy(dst, out.data(), nd * sizeof(cplx), hipMemcpyHostToDevice));
+  copy_sync(c, dst, out.data(), nd * sizeof(cplx), hipMemcpyHostToDevice);
   return dst;
 }
 
@@ -1801,7 +1818,7 @@ int dc2_casci(dc2_context* c, int32_t nroot, double* eigen, int64_t* ndet_out) {
 int dc2_get_dets(dc2_context* c, int64_t* dets) {
   return guarded([&] {
     DC2_CHECK(c && c->have_casci, 1, "no CASCI result");
-    DC2_HIP(hipMemcpy(dets, c->cas_dets.get(), c->ndet * sizeof(int64_t), hipMemcpyDeviceToHost));
+    copy_sync(c, dets, c->cas_dets.get(), c->ndet * sizeof(int64_t), hipMemcpyDeviceToHost);
   });
 }
 
@@ -1809,8 +1826,8 @@ int dc2_get_ci_vector(dc2_context* c, int32_t root, double* v) {
   return guarded([&] {
     DC2_CHECK(c && c->have_casci, 1, "no CASCI result");
     DC2_CHECK(root >= 1 && root <= c->nroots_avail, 1, "root out of range (not computed by dc2_casci)");
-    DC2_HIP(hipMemcpy(v, c->civec.get() + (int64_t)(root - 1) * c->ndet, c->ndet * sizeof(cplx),
-                      hipMemcpyDeviceToHost));
+    copy_sync(c, v, c->civec.get() + (int64_t)(root - 1) * c->ndet, c->ndet * sizeof(cplx),
+              hipMemcpyDeviceToHost);
   });
 }
 
@@ -1869,15 +1886,15 @@ int dc2_get_fock_rotation(dc2_context* c, double* eps, double* U) {
   return guarded([&] {
     DC2_CHECK(c && c->have_pt2, 1, "no CASPT2 result");
     const int64_t n = c->nmo();
-    if (eps) DC2_HIP(hipMemcpy(eps, c->eps.get(), n * sizeof(double), hipMemcpyDeviceToHost));
-    if (U) DC2_HIP(hipMemcpy(U, c->U.get(), n * n * sizeof(cplx), hipMemcpyDeviceToHost));
+    if (eps) copy_sync(c, eps, c->eps.get(), n * sizeof(double), hipMemcpyDeviceToHost);
+    if (U) copy_sync(c, U, c->U.get(), n * n * sizeof(cplx), hipMemcpyDeviceToHost);
   });
 }
 
 int dc2_get_rotated_ci(dc2_context* c, double* v) {
   return guarded([&] {
     DC2_CHECK(c && c->have_pt2, 1, "no CASPT2 result");
-    DC2_HIP(hipMemcpy(v, c->cnew.get(), c->ndet * sizeof(cplx), hipMemcpyDeviceToHost));
+    copy_sync(c, v, c->cnew.get(), c->ndet * sizeof(cplx), hipMemcpyDeviceToHost);
   });
 }
 
@@ -1895,7 +1912,7 @@ int dc2_get_density(dc2_context* c, int32_t which, double* out, int64_t count) {
       default: throw Error(1, "density index out of range");
     }
     DC2_CHECK(count == n, 1, "density: wrong element count");
-    DC2_HIP(hipMemcpy(out, src, n * sizeof(cplx), hipMemcpyDeviceToHost));
+    copy_sync(c, out, src, n * sizeof(cplx), hipMemcpyDeviceToHost);
   });
 }
 
@@ -1906,7 +1923,7 @@ int dc2_get_transformed_block(dc2_context* c, const char* name, double* out, int
     DC2_CHECK(it != c->tblk.end(), 1, std::string("block ") + name + " was not transformed");
     const int64_t n = block_count(c, shape_of(name));
     DC2_CHECK(count == n, 1, "transformed block: wrong element count");
-    DC2_HIP(hipMemcpy(out, it->second.get(), n * sizeof(cplx), hipMemcpyDeviceToHost));
+    copy_sync(c, out, it->second.get(), n * sizeof(cplx), hipMemcpyDeviceToHost);
   });
 }
 
@@ -2168,6 +2185,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "native_min") {
       DC2_CHECK(value >= 0, 1, "native_min must be >= 0");
       c->native_min = (int)value;
+    } else if (k == "tridiag_solver") {
+      DC2_CHECK(value == 0 || value == 1, 1, "tridiag_solver must be 0 or 1");
+      c->tri_own = value == 0;
     } else {
       throw Error(1, "unknown option '" + k + "'");
     }

@@ -1,7 +1,7 @@
 // Hermitian eigensolver for 3 <= n <= 2048 (the matrix its upper triangle defines, as
 // rocSOLVER's fill_upper): Householder tridiagonalisation in one persistent
-// kernel, divide and conquer on the real tridiagonal (rocSOLVER dstedc), back-transformation
-// with one wave per eigenvector.
+// kernel, divide and conquer on the real tridiagonal (this library's, dc2_tridc.hip, or
+// rocSOLVER dstedc), back-transformation with one wave per eigenvector.
 //
 // Tridiagonalisation (k_hetrd).  Step k (k = 0 .. n-3) applies the reflector
 // H_k = I - tau_k v_k v_k^H, v_k supported on rows k+1 .. n-1, to the trailing block
@@ -323,11 +323,11 @@ __global__ void k_herm_from_upper(int n, cplx* __restrict__ A, int64_t lda) {
   else if (i == j) A[i + lda * i].y = 0.0;
 }
 
-// e_k = |alpha_k| (the real subdiagonal); Z <- identity (dstedc reads it)
+// e_k = |alpha_k| (the real subdiagonal); Z <- identity if Z is given (dstedc reads it)
 __global__ void k_tri_prep(int n, const cplx* __restrict__ sub, double* __restrict__ e, double* __restrict__ Z) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n - 1) e[t] = sqrt(cabs2(sub[t]));
-  if (t < (int64_t)n * n) Z[t] = (t % (n + 1) == 0) ? 1.0 : 0.0;
+  if (Z && t < (int64_t)n * n) Z[t] = (t % (n + 1) == 0) ? 1.0 : 0.0;
 }
 
 // X[:, c] = H_0 H_1 ... H_{n-3} (delta .* Z[:, c]); one wave per column, 8 columns per
@@ -527,12 +527,13 @@ int64_t heev_native_work_bytes(int n) {
   const int64_t nn = (int64_t)n * n;
   const int nwg = hetrd_nwg(n);
   int64_t b = (int64_t)n * vh_cols(n) * 16 + nn * 8 + (int64_t)n * (16 * 6 + 8 * 3) + 16 * nwg + 16 + 10 * 256;
-  if (n >= BT_BLOCKED_MIN) b += back_blocked_bytes(n);
+  // the tridiagonal divide and conquer's buffers, later the blocked back-transformation's
+  b += std::max<int64_t>(tri_dc_work_bytes(n), n >= BT_BLOCKED_MIN ? back_blocked_bytes(n) : 0);
   return b;
 }
 
 bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev,
-                 int* flags_host) {
+                 int* flags_host, bool own_tridiag, bool report) {
   DC2_CHECK(n >= 3 && n <= HT_NMAX, 1, "heev_native: n must be in 3..2048");
   const int nwg = hetrd_nwg(n);
   const int64_t nn = (int64_t)n * n;
@@ -605,10 +606,16 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     e.first += 1;
     e.second += 1e-3 * ms;
   }
-  hipLaunchKernelGGL(k_tri_prep, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, sub, e, Z);
-  DC2_HIP(hipGetLastError());
-  const rocblas_status rs = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, e, Z, n, info_dev);
-  if (rs != rocblas_status_success) return false;
+  if (own_tridiag) {
+    hipLaunchKernelGGL(k_tri_prep, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sub, e, nullptr);
+    DC2_HIP(hipGetLastError());
+    tri_dc(s, n, w, e, Z, p, info_dev, report);   // its buffers end where the back-transformation's begin
+  } else {
+    hipLaunchKernelGGL(k_tri_prep, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, sub, e, Z);
+    DC2_HIP(hipGetLastError());
+    const rocblas_status rs = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, e, Z, n, info_dev);
+    if (rs != rocblas_status_success) return false;
+  }
   if (n >= BT_BLOCKED_MIN) {
     const int64_t np = cdiv(n - 2, BT_NB);
     cplx* G = reinterpret_cast<cplx*>(take(16 * np * BT_NB * BT_NB));

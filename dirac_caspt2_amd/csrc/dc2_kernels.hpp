@@ -98,14 +98,25 @@ void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const in
                     const int* ijkl, const double* val);
 
 // ---------------------------------------------------------------- dc2_heev.hip
-// Hermitian eigendecomposition of A (n x n, 3 <= n <= 1024, the matrix its upper triangle
-// defines): eigenvectors over A, ascending eigenvalues in w; dstedc's info in info_dev.  false:
+// Hermitian eigendecomposition of A (n x n, 3 <= n <= 2048, the matrix its upper triangle
+// defines): eigenvectors over A, ascending eigenvalues in w; the tridiagonal solver's info in
+// info_dev (own_tridiag: this library's divide and conquer, else rocSOLVER dstedc).  false:
 // nothing usable was computed (the caller falls back to rocSOLVER).  work: at least
 // heev_native_work_bytes(n) bytes of device memory; flags_host: two ints of page-locked host
 // memory for the reduction's status.
 int64_t heev_native_work_bytes(int n);
 bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev,
-                 int* flags_host);
+                 int* flags_host, bool own_tridiag = true, bool report = false);
+
+// ---------------------------------------------------------------- dc2_tridc.hip
+// Eigenpairs of the real symmetric tridiagonal (d in w, off-diagonal e[0 .. n-2]), 1 <= n <= 2048,
+// by divide and conquer on the device: ascending eigenvalues over w, eigenvectors in Z
+// (n x n, leading dimension n), info_dev[0] = 0, or 1 / 2 if a leaf's QL iteration / a secular
+// root did not converge (info_dev[1]: roots that did not; report: print the first few).
+// Asynchronous on s; work: tri_dc_work_bytes(n) bytes of device memory.
+int64_t tri_dc_work_bytes(int n);
+void tri_dc(hipStream_t s, int n, double* w, const double* e, double* Z, void* work, int* info_dev,
+            bool report = false);
 // {calls, summed device seconds} of the tridiagonalisation kernel for size n in this process
 void hetrd_stats(int n, double* out);
 void hetrd_stats_reset();

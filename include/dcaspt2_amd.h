@@ -210,6 +210,8 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *   "native_min":  n >= this (and n within this library's eigensolver, 3..2048) uses this
  *                  library's eigensolver even where the eigensolves are batched across served
  *                  requests (default 0: batched rocSOLVER for every size there);
+ *   "tridiag_solver": the tridiagonal stage of this library's eigensolver: 0 (default) its
+ *                  own divide and conquer on the device, 1 rocSOLVER dstedc;
  *   "casci_solver": 0 automatic (default: block Davidson for ndet >= 300 and nroot <= 16,
  *                  the dense eigensolver otherwise), 1 dense (all roots, as cdiag), 2 Davidson
  *                  (the nroot lowest roots; falls back to dense when it does not converge). */
