@@ -59,6 +59,7 @@ _SIGS = {
     "dc2_gemm_clear_choices": (c_int, [P]),
     "dc2_get_integral_block_device": (c_int, [P, ctypes.c_char_p, P, P]),
     "dc2_set_integral_records": (c_int, [P, c_i64, P, P, c_int]),
+    "dc2_mdcint_records": (c_int, [P, c_i64, P, P, P, P, P, P, P, c_i64, P, c_i64, P, c_int, c_i64, P, P, P, c_int]),
     "dc2_zgemm_split": (c_int, [P, c_int, ctypes.c_char, c_i64, c_i64, c_i64, P, c_i64, P, c_i64, P, c_i64]),
     "dc2_zgemm_split_noalloc": (c_int, [P, c_int, ctypes.c_char, c_i64, c_i64, c_i64, P, c_i64, P, c_i64, P, c_i64]),
     "dc2_get_transform_flops": (c_int, [P, P]),

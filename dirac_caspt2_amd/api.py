@@ -163,6 +163,33 @@ class Caspt2Context:
         check("dc2_set_integral_records", self.lib.dc2_set_integral_records(self.h, ijkl.shape[0], _ptr(ijkl),
                                                                             _ptr(v), 0))
 
+    def mdcint_records(self, ikr, jkr, nz, indk, indl, rklr, rkli, nkr: int, kr, d2c, scheme: int,
+                       ingest: bool = False, want_records: bool = True):
+        """dc2_mdcint_records: DIRAC's MDCINT records (per record ikr, jkr, nz; entries indk, indl,
+        rklr, rkli or None for real integrals; kr[nkr + m], d2c[k]) -> MDCINTNEW-equivalent
+        records (idx (n, 4) 1-based CAS labels, val (n,) complex) converted on the GPU, in the
+        reference's order; ingest: they also fill every integral block."""
+        i64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+        f64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        ikr, jkr, nz, indk, indl, kr, d2c = map(i64, (ikr, jkr, nz, indk, indl, kr, d2c))
+        rklr = f64(rklr)
+        rkli = None if rkli is None else f64(rkli)
+        nrec = len(ikr)
+        nout = ctypes.c_int64()
+        args = [self.h, nrec, _ptr(ikr), _ptr(jkr), _ptr(nz), _ptr(indk), _ptr(indl), _ptr(rklr),
+                None if rkli is None else _ptr(rkli), int(nkr), _ptr(kr), len(d2c), _ptr(d2c), int(scheme)]
+        cap = int(np.sum(nz)) * 4
+        if want_records:
+            ijkl = np.zeros((max(cap, 1), 4), dtype=np.int32)
+            val = np.zeros(max(cap, 1), dtype=np.complex128)
+            check("dc2_mdcint_records", self.lib.dc2_mdcint_records(*args, cap, ctypes.byref(nout), _ptr(ijkl),
+                                                                    _ptr(val), int(ingest)))
+            n = nout.value
+            return ijkl[:n].copy(), val[:n].copy()
+        check("dc2_mdcint_records", self.lib.dc2_mdcint_records(*args, 0, ctypes.byref(nout), None, None,
+                                                                int(ingest)))
+        return nout.value
+
     def integral_block(self, name: str) -> np.ndarray:
         """Copy of a resident original-basis integral block (column-major, its four extents)."""
         p = ctypes.c_void_p()

@@ -2084,56 +2084,134 @@ int dc2_get_gemm_stats(dc2_context* c, int32_t tag, double* out) {
   });
 }
 
+namespace dc2 {
+// every integral block from n records already on the device (the 16 images of each)
+void ingest_device(dc2_context* c, int64_t n, const int* dijkl, const double* dval) {
+  hipStream_t s = c->stream;
+  std::vector<std::string> names;
+  std::vector<std::array<int, 4>> spaces, dims;
+  std::vector<int64_t> counts;
+  for (const auto& sh : kShapes) {
+    const int64_t cnt = block_count(c, sh.sp);
+    if (cnt <= 0) continue;
+    std::array<int, 4> sp, dm;
+    for (int k = 0; k < 4; ++k) {
+      const char ch = sh.sp[k];
+      sp[k] = ch == 'i' ? 0 : ch == 'a' ? 1 : ch == 's' ? 2 : 3;
+      dm[k] = space_dim(c, ch);
+    }
+    names.push_back(sh.name);
+    spaces.push_back(sp);
+    dims.push_back(dm);
+    counts.push_back(cnt);
+  }
+  std::vector<DevBuf<unsigned long long>> owner(names.size());
+  std::vector<cplx*> data(names.size());
+  std::vector<unsigned long long*> own(names.size());
+  std::vector<const int*> psp(names.size()), pdm(names.size());
+  for (size_t b = 0; b < names.size(); ++b) {
+    Block& blk = c->blocks[names[b]];
+    blk.derived = false;
+    blk.count = counts[b];
+    blk.own.alloc(counts[b]);
+    blk.ptr = blk.own.get();
+    owner[b].alloc(counts[b]);
+    data[b] = blk.own.get();
+    own[b] = owner[b].get();
+    psp[b] = spaces[b].data();
+    pdm[b] = dims[b].data();
+  }
+  ingest_records(s, c->ni, c->na, c->ns, (int)names.size(), psp.data(), pdm.data(), data.data(), own.data(),
+                 counts.data(), n, dijkl, dval);
+  DC2_HIP(hipStreamSynchronize(s));
+  c->have_casci = c->have_pt2 = false;
+}
+}  // namespace dc2
+
 int dc2_set_integral_records(dc2_context* c, int64_t n, const int32_t* ijkl, const double* values, int32_t where) {
   return guarded([&] {
     DC2_CHECK(c && (n == 0 || (ijkl && values)), 1, "null argument");
     DC2_CHECK(where == 0 || where == 1, 1, "where must be 0 (host) or 1 (device)");
     need_spaces(c);
-    hipStream_t s = c->stream;
-    std::vector<std::string> names;
-    std::vector<std::array<int, 4>> spaces, dims;
-    std::vector<int64_t> counts;
-    for (const auto& sh : kShapes) {
-      const int64_t cnt = block_count(c, sh.sp);
-      if (cnt <= 0) continue;
-      std::array<int, 4> sp, dm;
-      for (int k = 0; k < 4; ++k) {
-        const char ch = sh.sp[k];
-        sp[k] = ch == 'i' ? 0 : ch == 'a' ? 1 : ch == 's' ? 2 : 3;
-        dm[k] = space_dim(c, ch);
-      }
-      names.push_back(sh.name);
-      spaces.push_back(sp);
-      dims.push_back(dm);
-      counts.push_back(cnt);
-    }
-    std::vector<DevBuf<unsigned long long>> owner(names.size());
-    std::vector<cplx*> data(names.size());
-    std::vector<unsigned long long*> own(names.size());
-    std::vector<const int*> psp(names.size()), pdm(names.size());
-    for (size_t b = 0; b < names.size(); ++b) {
-      Block& blk = c->blocks[names[b]];
-      blk.derived = false;
-      blk.count = counts[b];
-      blk.own.alloc(counts[b]);
-      blk.ptr = blk.own.get();
-      owner[b].alloc(counts[b]);
-      data[b] = blk.own.get();
-      own[b] = owner[b].get();
-      psp[b] = spaces[b].data();
-      pdm[b] = dims[b].data();
-    }
     DevBuf<int> dijkl(std::max<int64_t>(4 * n, 1));
     DevBuf<double> dval(std::max<int64_t>(2 * n, 1));
     const hipMemcpyKind kind = where == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     if (n > 0) {
-      DC2_HIP(hipMemcpyAsync(dijkl.get(), ijkl, 4 * n * sizeof(int), kind, s));
-      DC2_HIP(hipMemcpyAsync(dval.get(), values, 2 * n * sizeof(double), kind, s));
+      DC2_HIP(hipMemcpyAsync(dijkl.get(), ijkl, 4 * n * sizeof(int), kind, c->stream));
+      DC2_HIP(hipMemcpyAsync(dval.get(), values, 2 * n * sizeof(double), kind, c->stream));
     }
-    ingest_records(s, c->ni, c->na, c->ns, (int)names.size(), psp.data(), pdm.data(), data.data(), own.data(),
-                   counts.data(), n, dijkl.get(), dval.get());
-    DC2_HIP(hipStreamSynchronize(s));
-    c->have_casci = c->have_pt2 = false;
+    ingest_device(c, n, dijkl.get(), dval.get());
+  });
+}
+
+int dc2_mdcint_records(dc2_context* c, int64_t nrec, const int64_t* ikr, const int64_t* jkr, const int64_t* nz,
+                       const int64_t* indk, const int64_t* indl, const double* rklr, const double* rkli, int64_t nkr,
+                       const int64_t* kr, int64_t nd2c, const int64_t* d2c, int32_t scheme, int64_t cap,
+                       int64_t* nout, int32_t* ijkl, double* values, int32_t ingest) {
+  return guarded([&] {
+    DC2_CHECK(c && nout && nrec >= 0 && nkr >= 0 && nd2c >= 1 && kr && d2c, 1, "invalid argument");
+    DC2_CHECK(nrec == 0 || (ikr && jkr && nz), 1, "null record arrays");
+    DC2_CHECK((ijkl == nullptr) == (values == nullptr), 1, "ijkl and values are given together or not at all");
+    if (ingest) need_spaces(c);
+    // the labels are checked here, so that every device access stays inside kr and d2c
+    for (int64_t x = 0; x < 2 * nkr + 1; ++x)
+      DC2_CHECK(kr[x] >= 0 && kr[x] < nd2c, 1, "kr holds a spinor outside d2c");
+    for (int64_t k = 0; k < nd2c; ++k)
+      DC2_CHECK(d2c[k] >= 0 && d2c[k] < (1 << 30), 1, "d2c holds an index out of range");
+    std::vector<int64_t> eoff(nrec + 1), coff(nrec + 1);
+    int64_t nent = 0, ncand = 0;
+    for (int64_t r = 0; r < nrec; ++r) {
+      DC2_CHECK(nz[r] >= 0 && std::llabs(ikr[r]) <= nkr && std::llabs(jkr[r]) <= nkr, 1,
+                "record " + std::to_string(r) + ": label or count out of range");
+      eoff[r] = nent;
+      coff[r] = ncand;
+      nent += nz[r];
+      ncand += nz[r] * ((scheme != 6 && std::llabs(ikr[r]) != std::llabs(jkr[r])) ? 4 : 2);
+    }
+    eoff[nrec] = nent;
+    coff[nrec] = ncand;
+    DC2_CHECK(nent == 0 || (indk && indl && rklr), 1, "null entry arrays");
+    std::vector<int> kl((size_t)std::max<int64_t>(2 * nent, 1));
+    for (int64_t e = 0; e < nent; ++e) {
+      DC2_CHECK(std::llabs(indk[e]) <= nkr && std::llabs(indl[e]) <= nkr, 1,
+                "entry " + std::to_string(e) + ": label out of range");
+      kl[e] = (int)indk[e];
+      kl[nent + e] = (int)indl[e];
+    }
+    hipStream_t s = c->stream;
+    auto up = [&](auto* dst, const auto* src, int64_t count) {
+      if (count > 0) DC2_HIP(hipMemcpyAsync(dst, src, count * sizeof(*src), hipMemcpyHostToDevice, s));
+    };
+    DevBuf<int64_t> d_ikr(std::max<int64_t>(nrec, 1)), d_jkr(std::max<int64_t>(nrec, 1)),
+        d_nz(std::max<int64_t>(nrec, 1)), d_eoff(nrec + 1), d_coff(nrec + 1), d_kr(2 * nkr + 1), d_d2c(nd2c);
+    DevBuf<int> d_kl((size_t)std::max<int64_t>(2 * nent, 1));
+    DevBuf<double> d_rr(std::max<int64_t>(nent, 1)), d_ri(rkli ? std::max<int64_t>(nent, 1) : 1);
+    up(d_ikr.get(), ikr, nrec);
+    up(d_jkr.get(), jkr, nrec);
+    up(d_nz.get(), nz, nrec);
+    up(d_eoff.get(), eoff.data(), nrec + 1);
+    up(d_coff.get(), coff.data(), nrec + 1);
+    up(d_kr.get(), kr, 2 * nkr + 1);
+    up(d_d2c.get(), d2c, nd2c);
+    up(d_kl.get(), kl.data(), 2 * nent);
+    up(d_rr.get(), rklr, nent);
+    if (rkli) up(d_ri.get(), rkli, nent);
+    DevBuf<int> d_ijkl(std::max<int64_t>(4 * ncand, 1));
+    DevBuf<double> d_val(std::max<int64_t>(2 * ncand, 1));
+    const int64_t nsc = mdcint_convert_scratch(ncand) / 8;
+    DevBuf<int64_t> d_sc(nsc);
+    mdcint_convert(s, nrec, d_ikr.get(), d_jkr.get(), d_nz.get(), d_eoff.get(), d_coff.get(), d_kl.get(),
+                   d_kl.get() + nent, d_rr.get(), rkli ? d_ri.get() : nullptr, nkr, d_kr.get(), nd2c, d_d2c.get(),
+                   scheme, ncand, d_sc.get(), d_ijkl.get(), d_val.get());
+    int64_t count = 0;
+    copy_sync(c, &count, d_sc.get() + nsc - 1, sizeof(int64_t), hipMemcpyDeviceToHost);
+    *nout = count;
+    if (ijkl) {
+      DC2_CHECK(cap >= count, 1, "dc2_mdcint_records: room for " + std::to_string(count) + " records is needed");
+      copy_sync(c, ijkl, d_ijkl.get(), 4 * count * sizeof(int), hipMemcpyDeviceToHost);
+      copy_sync(c, values, d_val.get(), 2 * count * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    if (ingest) ingest_device(c, count, d_ijkl.get(), d_val.get());
   });
 }
 

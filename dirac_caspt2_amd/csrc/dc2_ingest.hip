@@ -13,6 +13,10 @@
 // the later record wins (then the later image), as in a sequential fill: a first pass
 // takes the maximum key (record * 16 + image) per element with an atomic max, a second
 // pass lets only the winner write.
+//
+// Before it, mdcint_convert turns DIRAC's unique integral set (MDCINT records) into those
+// records on the device, as create_newmdcint.F90 does on the host: one thread per candidate
+// entry, a count / scan / write pass keeping the reference's write order.
 #include "dc2_kernels.hpp"
 
 namespace dc2 {
@@ -108,7 +112,175 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+// ------------------------------------------------------------------ MDCINT -> records
+// Candidate c of the conversion is entry (record r, image n, loop index i) in the order
+// create_newmdcint.F90:86-177 visits them: records in file order; n = 0 the record as read,
+// n = 1 its Kramers image (every label negated, re -> -re s, im -> im s with s the sign of
+// ikr jkr indk indl); i over the nz entries and, when the SCHEME amplifies (scheme != 6 and
+// |ikr| != |jkr|), over the nz particle-exchanged entries after them ((ij|kl) -> (ji|lk),
+// every label negated for the types with one barred label on each side, else im -> -im).
+// coff[r] = first candidate of record r (coff[nrec] = all).
+struct MdcintDev {
+  int64_t nrec, nkr, nd2c;
+  int scheme;
+  const int64_t *ikr, *jkr, *nz, *eoff, *coff;
+  const int *indk, *indl;
+  const double *rr, *ri;   // ri null: real integrals
+  const int64_t *kr, *d2c;
+};
+
+struct MdOut {
+  int q[4];
+  double re, im;
+  bool keep;
+};
+
+__device__ inline int64_t kramers_label(int64_t x) {   // 1 -1 2 -2 ... from CAS index 1 2 3 4 ...
+  return (x & 1) ? x / 2 + 1 : -(x / 2);
+}
+
+__device__ MdOut mdcint_candidate(const MdcintDev& m, int64_t c) {
+  int64_t lo = 0, hi = m.nrec;   // last record with coff <= c
+  while (hi - lo > 1) {
+    const int64_t md = (lo + hi) >> 1;
+    if (m.coff[md] <= c) lo = md;
+    else hi = md;
+  }
+  const int64_t r = lo, nz = m.nz[r];
+  int64_t ci = m.ikr[r], cj = m.jkr[r];
+  const bool amplify = m.scheme != 6 && llabs(ci) != llabs(cj);
+  const int64_t half = (amplify ? 2 : 1) * nz;
+  const int64_t loc = c - m.coff[r];
+  const int64_t n = loc / half, i = loc % half;
+  const int64_t e = m.eoff[r] + i % nz;
+  int64_t ck = m.indk[e], cl = m.indl[e];
+  double re = m.rr[e], im = m.ri ? m.ri[e] : 0.0;
+  if (n == 1) {
+    const int neg = (ci < 0) + (cj < 0) + (ck < 0) + (cl < 0);
+    const double sg = (neg & 1) ? -1.0 : 1.0;
+    ci = -ci; cj = -cj; ck = -ck; cl = -cl;
+    re = -re * sg;
+    im = im * sg;
+  }
+  const double mag_re = fabs(re), mag_im = fabs(im);   // the cutoff reads the entry as stored
+  if (i >= nz) {
+    int64_t t = ci; ci = cj; cj = t;
+    t = ck; ck = cl; cl = t;
+    if ((ci < 0) + (cj < 0) == 1 && (ck < 0) + (cl < 0) == 1) {
+      ci = -ci; cj = -cj; ck = -ck; cl = -cl;
+    } else {
+      im = -im;
+    }
+  }
+  MdOut o;
+  const int64_t lab[4] = {ci, cj, ck, cl};
+  int64_t kl[4], ab[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t x = m.d2c[m.kr[m.nkr + lab[a]]];   // CAS index
+    kl[a] = kramers_label(x);
+    ab[a] = llabs(kl[a]);
+    o.q[a] = (int)((x & 1) ? x + 1 : x - 1);        // the Kramers partner, as written to MDCINTNEW
+  }
+  const double cutoff = 0.25e-12;
+  bool ok = m.ri ? !(mag_re <= cutoff && mag_im <= cutoff) : !(mag_re <= cutoff);
+  if (ok) {
+    const int64_t a = ab[0], b = ab[1], cc = ab[2], d = ab[3];
+    if (kl[0] > 0 && kl[1] > 0 && kl[2] > 0 && kl[3] > 0)
+      ok = (a <= b && cc <= d && (a < cc || (a == cc && b <= d))) || (a <= b && d <= cc && (a < d || (a == d && b <= cc)));
+    else if (kl[0] > 0 && kl[1] < 0 && kl[2] > 0 && kl[3] < 0)
+      ok = a <= b && cc <= d && (a < cc || (a == cc && b <= d));
+    else if (kl[0] > 0 && kl[1] < 0 && kl[2] < 0 && kl[3] > 0)
+      ok = a <= b && cc <= d && (a < cc || (a == cc && b <= d));
+    else if (kl[0] > 0 && kl[1] < 0 && kl[2] < 0 && kl[3] < 0)
+      ok = a <= b;
+    else
+      ok = false;
+  }
+  o.re = re;
+  o.im = -im;
+  o.keep = ok;
+  return o;
+}
+
+constexpr int MD_BLOCK = 256;
+
+__global__ __launch_bounds__(MD_BLOCK) void k_md_count(MdcintDev m, int64_t ncand, int64_t* __restrict__ bcount) {
+  const int64_t c = (int64_t)blockIdx.x * MD_BLOCK + threadIdx.x;
+  const bool keep = c < ncand && mdcint_candidate(m, c).keep;
+  const int k = __syncthreads_count(keep);
+  if (threadIdx.x == 0) bcount[blockIdx.x] = k;
+}
+
+// exclusive scan of nb block counts in one workgroup of 1024 threads; boff[nb] = the total
+__global__ __launch_bounds__(1024) void k_md_scan(int64_t nb, const int64_t* __restrict__ bcount,
+                                                  int64_t* __restrict__ boff) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (nb + 1023) / 1024, b0 = t * per, b1 = min(nb, b0 + per);
+  int64_t sum = 0;
+  for (int64_t b = b0; b < b1; ++b) sum += bcount[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {   // inclusive Hillis-Steele scan
+    const int64_t v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = part[t] - sum;
+  for (int64_t b = b0; b < b1; ++b) {
+    boff[b] = run;
+    run += bcount[b];
+  }
+  if (t == 1023) boff[nb] = part[1023];
+}
+
+__global__ __launch_bounds__(MD_BLOCK) void k_md_write(MdcintDev m, int64_t ncand, const int64_t* __restrict__ boff,
+                                                       int* __restrict__ ijkl, double* __restrict__ val) {
+  __shared__ int wsum[MD_BLOCK / 64];
+  const int64_t c = (int64_t)blockIdx.x * MD_BLOCK + threadIdx.x;
+  MdOut o;
+  o.keep = false;
+  if (c < ncand) o = mdcint_candidate(m, c);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long mask = __ballot(o.keep);
+  const int below = __popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[wave] = __popcll(mask);
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  if (o.keep) {
+    const int64_t p = boff[blockIdx.x] + before + below;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) ijkl[4 * p + a] = o.q[a];
+    val[2 * p] = o.re;
+    val[2 * p + 1] = o.im;
+  }
+}
+
 }  // namespace
+
+int64_t mdcint_convert_scratch(int64_t ncand) { return 8 * (2 * cdiv(ncand, MD_BLOCK) + 1); }
+
+void mdcint_convert(hipStream_t s, int64_t nrec, const int64_t* ikr, const int64_t* jkr, const int64_t* nz,
+                    const int64_t* eoff, const int64_t* coff, const int* indk, const int* indl,
+                    const double* rr, const double* ri, int64_t nkr, const int64_t* kr, int64_t nd2c,
+                    const int64_t* d2c, int scheme, int64_t ncand, int64_t* scratch, int* ijkl, double* val) {
+  const int64_t nb = cdiv(ncand, MD_BLOCK);
+  DC2_CHECK(nb < (1ll << 31), 1, "mdcint: too many entries for one launch");
+  MdcintDev m{nrec, nkr, nd2c, scheme, ikr, jkr, nz, eoff, coff, indk, indl, rr, ri, kr, d2c};
+  int64_t* bcount = scratch;
+  int64_t* boff = scratch + nb;
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_md_count, dim3(nb), dim3(MD_BLOCK), 0, s, m, ncand, bcount);
+    hipLaunchKernelGGL(k_md_scan, dim3(1), dim3(1024), 0, s, nb, bcount, boff);
+    hipLaunchKernelGGL(k_md_write, dim3(nb), dim3(MD_BLOCK), 0, s, m, ncand, boff, ijkl, val);
+  } else {
+    hipLaunchKernelGGL(k_md_scan, dim3(1), dim3(1024), 0, s, nb, bcount, boff);
+  }
+  DC2_HIP(hipGetLastError());
+}
 
 void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const int* const* spaces, const int* const* dims,
                     cplx* const* data, unsigned long long* const* owner, const int64_t* counts, int64_t nrec,

@@ -18,7 +18,8 @@ module dcaspt2_amd
     private
     public :: c_ptr, c_null_ptr
     public :: dc2_context_create, dc2_context_destroy, dc2_version, dc2_set_spaces, dc2_set_symmetry, dc2_set_ras, &
-              dc2_set_one_electron, dc2_set_integral_block, dc2_set_integral_records, dc2_set_integral_factors, &
+              dc2_set_one_electron, dc2_set_integral_block, dc2_set_integral_records, dc2_mdcint_records, &
+              dc2_set_integral_factors, &
               dc2_casci, dc2_caspt2, dc2_set_casci_result, dc2_caspt2_restart, &
               dc2_get_subspace_info, dc2_cdiag, dc2_casmat, dc2_synchronize
     public :: dc2_get_dets, dc2_get_ci_vector, dc2_request_footprint, dc2_server_create_batched, &
@@ -83,6 +84,21 @@ module dcaspt2_amd
             integer(c_int32_t), intent(in) :: ijkl(*)
             complex(c_double_complex), intent(in) :: values(*)
             integer(c_int32_t), value :: where
+        end function
+        ! create_newmdcint.F90:86-177 on the GPU: the MDCINT records as read_one_line_of_mdcint
+        ! reads them (entries of all records concatenated), kr(-nkr:nkr), indmo_dirac_to_cas
+        ! (0:nmo); rkli, ijkl and values may be c_null_ptr (real integrals / no copy back)
+        integer(c_int) function dc2_mdcint_records(ctx, nrec, ikr, jkr, nz, indk, indl, rklr, rkli, nkr, kr, &
+                                                   nd2c, d2c, scheme, cap, nout, ijkl, values, ingest) &
+            bind(C, name="dc2_mdcint_records")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: nrec, nkr, nd2c, cap
+            integer(c_int64_t), intent(in) :: ikr(*), jkr(*), nz(*), indk(*), indl(*), kr(*), d2c(*)
+            real(c_double), intent(in) :: rklr(*)
+            type(c_ptr), value :: rkli, ijkl, values
+            integer(c_int32_t), value :: scheme, ingest
+            integer(c_int64_t), intent(out) :: nout
         end function
         integer(c_int) function dc2_set_casci_result(ctx, ndet, nroot, dets, coeffs, ecas) &
             bind(C, name="dc2_set_casci_result")

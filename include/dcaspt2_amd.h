@@ -302,6 +302,27 @@ int dc2_caspt2_restart(dc2_context* ctx, int32_t selectroot, double eshift, int3
  * wins.  This replaces the dense fill and the subspace files of
  * divide_2_elec_integral_into_subspaces.F90.  where: 0 host arrays, 1 device arrays. */
 int dc2_set_integral_records(dc2_context* ctx, int64_t n, const int32_t* ijkl, const double* values, int32_t where);
+/* DIRAC's unique integral set -> MDCINTNEW-equivalent records on the GPU (create_newmdcint.F90:
+ * 86-177, the step before dc2_set_integral_records; the reference writes the records to the
+ * MDCINTNEW file, here they stay on the device).  Input as create_newmdcint reads it
+ * (read_mdcint_header, read_one_line_of_mdcint, create_newmdcint.F90:209-280): the nrec
+ * records (ikr, jkr, nz) of MDCINT, MDCINXXXX1, ... in file order (the ikr = 0 record that
+ * ends a file excluded), their entries concatenated: indk[], indl[], rklr[] and rkli[] (NULL:
+ * real integrals, realonly) of sum(nz) elements; kr[2 nkr + 1] the header's kr(-nkr:nkr)
+ * (kr[nkr + m] = DIRAC spinor of Kramers label m); d2c[nd2c] = indmo_dirac_to_cas (d2c[k] the
+ * 1-based CAS index of DIRAC spinor k, read_mrconee.F90:500-541); scheme the MDCINT SCHEME
+ * of active.inp (6: no amplification).  Every record and its Kramers image are expanded
+ * (and particle-exchanged when the SCHEME amplifies), relabelled, and kept under the
+ * reference's cutoff (|v| > 0.25e-12) and canonical-order conditions, in the order the
+ * reference writes them.  *nout = number of records; when ijkl/values are not NULL (both or
+ * neither) and cap >= *nout, the records (1-based CAS labels, (re, im) as written to
+ * MDCINTNEW) are copied there (cap too small: status 1, *nout set); ingest != 0: the records
+ * also replace every integral block as dc2_set_integral_records does.  Labels outside kr /
+ * d2c: status 1. */
+int dc2_mdcint_records(dc2_context* ctx, int64_t nrec, const int64_t* ikr, const int64_t* jkr, const int64_t* nz,
+                       const int64_t* indk, const int64_t* indl, const double* rklr, const double* rkli, int64_t nkr,
+                       const int64_t* kr, int64_t nd2c, const int64_t* d2c, int32_t scheme, int64_t cap,
+                       int64_t* nout, int32_t* ijkl, double* values, int32_t ingest);
 /* device pointer and element count of a resident integral block, e.g. to lend it to
  * another context with dc2_set_integral_block(..., where = 2) */
 int dc2_get_integral_block_device(dc2_context* ctx, const char* name, const double** ptr, int64_t* count);

@@ -235,80 +235,124 @@ def _type2_or_3(i, j, k, l) -> bool:
     return (int(i < 0) + int(j < 0)) == 1 and (int(k < 0) + int(l < 0)) == 1
 
 
-def mdcintnew_records(dirpath: str, d2c: np.ndarray, scheme: int):
-    """Yield MDCINTNEW-equivalent records (i, j, k, l, re, im) (create_newmdcint.F90:86-177).
+@dataclass
+class MdcintSet:
+    """DIRAC's unique integral set as create_newmdcint.F90 reads it: per record (ikr, jkr, nz)
+    and its nz entries (indk, indl, rklr[, rkli]) concatenated over the records of all
+    MDCINT files in file order; kr[nkr + m] = DIRAC spinor of Kramers label m (the header's
+    kr(-nkr:nkr)); ``rkli`` is None for real (realonly) integrals."""
+    ikr: np.ndarray
+    jkr: np.ndarray
+    nz: np.ndarray
+    indk: np.ndarray
+    indl: np.ndarray
+    rklr: np.ndarray
+    rkli: Optional[np.ndarray]
+    nkr: int
+    kr: np.ndarray
 
-    Indices are 1-based CAS-order spinors.  ``im`` is already the value written
-    to MDCINTNEW (i.e. the negated DIRAC imaginary part).
-    """
+
+def read_mdcint_set(dirpath: str, max_records: Optional[int] = None) -> MdcintSet:
+    """The records of MDCINT, MDCINXXXX1, ... (read_mdcint_header and read_one_line_of_mdcint,
+    create_newmdcint.F90:209-280; a record with ikr = 0 ends a file), at most max_records."""
     files = mdcint_files(dirpath)
     hdr = fortran_records(files[0])[0]
     (nkr,) = struct.unpack_from("<q", hdr, 18)
-    kr = {}
     vals = struct.unpack_from("<" + "q" * (2 * nkr), hdr, 26)
+    kr = np.zeros(2 * nkr + 1, np.int64)
     for i0 in range(1, nkr + 1):
-        kr[i0] = vals[2 * (i0 - 1)]
-        kr[-i0] = vals[2 * (i0 - 1) + 1]
-    cutoff = 0.25e-12
+        kr[nkr + i0] = vals[2 * (i0 - 1)]
+        kr[nkr - i0] = vals[2 * (i0 - 1) + 1]
+    ikr, jkr, nzs, ks, ls, rr, ri = [], [], [], [], [], [], []
     is_complex = None
     for f in files:
         recs = fortran_records(f)
         for rec in recs[1:]:
-            ikr, jkr, nz = struct.unpack_from("<qqq", rec, 0)
-            if ikr == 0:
+            a, b, nz = struct.unpack_from("<qqq", rec, 0)
+            if a == 0:
+                break
+            if max_records is not None and len(ikr) >= max_records:
                 break
             if is_complex is None:
                 is_complex = _mdcint_is_complex(rec)
             idx = np.frombuffer(rec, dtype="<i8", count=2 * nz, offset=24).reshape(nz, 2)
+            ikr.append(a); jkr.append(b); nzs.append(nz)
+            ks.append(idx[:, 0].copy()); ls.append(idx[:, 1].copy())
             if is_complex:
                 v = np.frombuffer(rec, dtype="<f8", count=2 * nz, offset=24 + 16 * nz).reshape(nz, 2)
-                rr, ri = v[:, 0].copy(), v[:, 1].copy()
+                rr.append(v[:, 0].copy()); ri.append(v[:, 1].copy())
             else:
-                rr = np.frombuffer(rec, dtype="<f8", count=nz, offset=24 + 16 * nz).copy()
-                ri = np.zeros(nz)
-            indk = idx[:, 0].copy(); indl = idx[:, 1].copy()
-            for n in (1, 2):
-                if n == 2:
-                    ikr, jkr = -ikr, -jkr
-                    indk, indl = -indk, -indl
-                    sg = np.sign(ikr * jkr * indk * indl)
-                    sg[sg == 0] = 1
-                    rr = -rr * sg
-                    ri = ri * sg
-                amplify = (scheme != 6 and abs(ikr) != abs(jkr))
-                nloop = 2 * nz if amplify else nz
-                for ii in range(1, nloop + 1):
-                    inz = nz if ii % nz == 0 else ii % nz
-                    ci, cj, ck, cl = ikr, jkr, int(indk[inz - 1]), int(indl[inz - 1])
-                    re, im = float(rr[inz - 1]), float(ri[inz - 1])
-                    if ii > nz:
-                        ci, cj = cj, ci
-                        ck, cl = cl, ck
-                        if _type2_or_3(ci, cj, ck, cl):
-                            ci, cj, ck, cl = -ci, -cj, -ck, -cl
-                        else:
-                            im = -im
-                    iii, jjj, kkk, lll = (int(d2c[kr[x]]) for x in (ci, cj, ck, cl))
-                    iikr, jjkr, kkkr, llkr = ((1 if x % 2 == 1 else -1) * (x // 2 + x % 2) for x in (iii, jjj, kkk, lll))
-                    iiit, jjjt, kkkt, lllt = (x + 1 if x % 2 == 1 else x - 1 for x in (iii, jjj, kkk, lll))
-                    a, b, c, d = abs(iikr), abs(jjkr), abs(kkkr), abs(llkr)
-                    if is_complex:
-                        if abs(re) <= cutoff and abs(im) <= cutoff:
-                            continue
-                    elif abs(re) <= cutoff:
+                rr.append(np.frombuffer(rec, dtype="<f8", count=nz, offset=24 + 16 * nz).copy())
+    cat = (lambda x, dt: np.concatenate(x).astype(dt) if x else np.zeros(0, dt))
+    return MdcintSet(np.array(ikr, np.int64), np.array(jkr, np.int64), np.array(nzs, np.int64),
+                     cat(ks, np.int64), cat(ls, np.int64), cat(rr, np.float64),
+                     cat(ri, np.float64) if is_complex else None, int(nkr), kr)
+
+
+def mdcintnew_from_set(ms: MdcintSet, d2c: np.ndarray, scheme: int):
+    """Yield MDCINTNEW-equivalent records (i, j, k, l, re, im) (create_newmdcint.F90:86-177) in
+    the order the reference writes them: record by record, the record as read (n = 1) then
+    its sign-flipped Kramers image (n = 2), each over its nz entries and, when the SCHEME
+    amplifies (scheme != 6 and |ikr| != |jkr|), over the nz particle-exchanged entries after
+    them.  Indices are 1-based CAS-order spinors (d2c[k] for DIRAC spinor k); ``im`` is the
+    value written to MDCINTNEW (the negated DIRAC imaginary part)."""
+    is_complex = ms.rkli is not None
+    cutoff = 0.25e-12
+    off = 0
+    for r in range(len(ms.ikr)):
+        ikr, jkr, nz = int(ms.ikr[r]), int(ms.jkr[r]), int(ms.nz[r])
+        indk = ms.indk[off:off + nz].copy(); indl = ms.indl[off:off + nz].copy()
+        rr = ms.rklr[off:off + nz].copy()
+        ri = ms.rkli[off:off + nz].copy() if is_complex else np.zeros(nz)
+        off += nz
+        for n in (1, 2):
+            if n == 2:
+                ikr, jkr = -ikr, -jkr
+                indk, indl = -indk, -indl
+                sg = np.sign(ikr * jkr * indk * indl)
+                sg[sg == 0] = 1
+                rr = -rr * sg
+                ri = ri * sg
+            amplify = (scheme != 6 and abs(ikr) != abs(jkr))
+            nloop = 2 * nz if amplify else nz
+            for ii in range(1, nloop + 1):
+                inz = nz if ii % nz == 0 else ii % nz
+                ci, cj, ck, cl = ikr, jkr, int(indk[inz - 1]), int(indl[inz - 1])
+                re, im = float(rr[inz - 1]), float(ri[inz - 1])
+                if ii > nz:
+                    ci, cj = cj, ci
+                    ck, cl = cl, ck
+                    if _type2_or_3(ci, cj, ck, cl):
+                        ci, cj, ck, cl = -ci, -cj, -ck, -cl
+                    else:
+                        im = -im
+                iii, jjj, kkk, lll = (int(d2c[ms.kr[ms.nkr + x]]) for x in (ci, cj, ck, cl))
+                iikr, jjkr, kkkr, llkr = ((1 if x % 2 == 1 else -1) * (x // 2 + x % 2) for x in (iii, jjj, kkk, lll))
+                iiit, jjjt, kkkt, lllt = (x + 1 if x % 2 == 1 else x - 1 for x in (iii, jjj, kkk, lll))
+                a, b, c, d = abs(iikr), abs(jjkr), abs(kkkr), abs(llkr)
+                if is_complex:
+                    if abs(re) <= cutoff and abs(im) <= cutoff:
                         continue
-                    ok = False
-                    if iikr > 0 and jjkr > 0 and kkkr > 0 and llkr > 0:
-                        ok = ((a <= b and c <= d and (a < c or (a == c and b <= d))) or
-                              (a <= b and d <= c and (a < d or (a == d and b <= c))))
-                    elif iikr > 0 and jjkr < 0 and kkkr > 0 and llkr < 0:
-                        ok = a <= b and c <= d and (a < c or (a == c and b <= d))
-                    elif iikr > 0 and jjkr < 0 and kkkr < 0 and llkr > 0:
-                        ok = a <= b and c <= d and (a < c or (a == c and b <= d))
-                    elif iikr > 0 and jjkr < 0 and kkkr < 0 and llkr < 0:
-                        ok = a <= b
-                    if ok:
-                        yield iiit, jjjt, kkkt, lllt, re, -im
+                elif abs(re) <= cutoff:
+                    continue
+                ok = False
+                if iikr > 0 and jjkr > 0 and kkkr > 0 and llkr > 0:
+                    ok = ((a <= b and c <= d and (a < c or (a == c and b <= d))) or
+                          (a <= b and d <= c and (a < d or (a == d and b <= c))))
+                elif iikr > 0 and jjkr < 0 and kkkr > 0 and llkr < 0:
+                    ok = a <= b and c <= d and (a < c or (a == c and b <= d))
+                elif iikr > 0 and jjkr < 0 and kkkr < 0 and llkr > 0:
+                    ok = a <= b and c <= d and (a < c or (a == c and b <= d))
+                elif iikr > 0 and jjkr < 0 and kkkr < 0 and llkr < 0:
+                    ok = a <= b
+                if ok:
+                    yield iiit, jjjt, kkkt, lllt, re, -im
+
+
+def mdcintnew_records(dirpath: str, d2c: np.ndarray, scheme: int):
+    """Yield MDCINTNEW-equivalent records (i, j, k, l, re, im) of the MDCINT files in dirpath
+    (create_newmdcint.F90:86-177): read_mdcint_set then mdcintnew_from_set."""
+    yield from mdcintnew_from_set(read_mdcint_set(dirpath), d2c, scheme)
 
 
 def dense_from_records(nmo: int, records, nmom: int) -> Tuple[np.ndarray, int]:
