@@ -72,13 +72,17 @@ __device__ inline void st_coherent(cplx* p, cplx v) {
   st_coherent(q + 1, v.y);
 }
 
-// Grid barrier over all workgroups (all resident); false if the wait was abandoned.  The
-// workgroup barrier orders this step's stores of every wave before thread 0's release
-// increment; thread 0 first stores the workgroup's partial (the sum of red[0..7]) to
-// part_dst when that is not null.  The waiting loop polls with relaxed loads and one
-// acquire fence follows.
+// Grid barrier over all workgroups (all resident); false if the wait was abandoned.
+// Everything exchanged between workgroups during the reduction is written and read with
+// device-coherent accesses (write-through stores, loads that bypass the per-XCD L2), so the
+// barrier orders completions only and needs no L2 writeback or invalidation: every wave
+// waits for its own stores to complete (s_waitcnt 0) before the workgroup barrier, thread 0
+// then stores the workgroup's partial (the sum of red[0..7]) to part_dst when that is not
+// null, waits for it, and increments the counter; the waiting loop polls with relaxed
+// loads, after which the exchanged data are read with coherent loads.
 __device__ bool grid_sync(unsigned* bar, unsigned target, int* err, double* part_dst, const double* red) {
   __shared__ int ok;
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     ok = 1;
@@ -87,8 +91,9 @@ __device__ bool grid_sync(unsigned* bar, unsigned target, int* err, double* part
 #pragma unroll
       for (int ww = 0; ww < HT_WAVES; ++ww) ps += red[ww];
       st_coherent(part_dst, ps);
+      __builtin_amdgcn_s_waitcnt(0);
     }
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     long spins = 0;
     while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
@@ -98,7 +103,6 @@ __device__ bool grid_sync(unsigned* bar, unsigned target, int* err, double* part
         break;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
   return ok != 0;
@@ -199,6 +203,9 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
     if (wg == 0) delta[0] = s_delta;
   }
   for (int i = threadIdx.x; i < n; i += HT_THREADS) sc[i] = A[i];   // column 0
+  // the loads of the owned columns complete here, once, rather than being waited for again
+  // before every store that might use them
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   reflector(n, 0, sc, sv, red, &s_tau, &s_delta, d, sub, tau, delta, Vh);
   for (int k = 0; k <= n - 3; ++k) {
