@@ -914,6 +914,7 @@ SolveOut solve_finish(dc2_context* c, Lane& L, char X, const BlockPrep& p, int64
 
 struct GramMetric {
   bool valid = false;
+  bool btilde = false;    // B~ and its eigendecomposition already in X_bc1 / X_wbt
   int64_t m = 0, dimm = 0;
   double margin = INFINITY;
   double seconds = 0.0;   // time spent on the lane before the subspace itself ran
@@ -1001,6 +1002,30 @@ GramMetric gram_metric(dc2_context* c, Lane& L, char X, int64_t n) {
   return g;
 }
 
+// B~ = G_k^H F G_k and its eigendecomposition (lane buffers X_bc1, X_wbt), which need the
+// rotated metric and F only, not V: done as soon as those exist, so that the eigensolve
+// overlaps the integral transforms.  Not needed when E2 will come from the compressed
+// resolvent (at most e2_schur_max_dropped() directions dropped).
+void gram_btilde(dc2_context* c, Lane& L, char X, GramMetric& g) {
+  const int64_t m = g.m, dimm = g.dimm, nd = m - dimm;
+  if (m == 0 || dimm == 0 || (c->schur && nd <= e2_schur_max_dropped())) return;
+  hipStream_t s = L.stream;
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const cplx* Gk = L.z(nm("_G").c_str(), m * m) + m * nd;
+  const double* f = L.d(nm("_f").c_str(), m);
+  cplx* FG = L.z(nm("_FG").c_str(), m * dimm);
+  cplx* bc1 = L.z(nm("_bc1").c_str(), dimm * dimm);
+  double* wb = L.d(nm("_wbt").c_str(), dimm);
+  scale_rows(s, m, dimm, f, Gk, FG);
+  set_gemm_tag(4);
+  zgemm(s, 'C', 'N', dimm, dimm, m, one, Gk, m, 0, FG, m, 0, zero, bc1, dimm, 0, 1);
+  set_gemm_tag(0);
+  check_finite(c, s, "B~", bc1, dimm * dimm);
+  zheevd(c, L, (int)dimm, bc1, wb, "a B~ matrix");
+  g.btilde = true;
+}
+
 // A and C through their determinant-space factor.  With Phi the (m x n) matrix of the
 // vectors a+_t E_uv|0> (A) or a_t E_uv|0> (C) over the full (nelec+-1)-electron string
 // space, the formula matrices of solve_A/C_subspace.F90 are S = Phi^H Phi and
@@ -1046,6 +1071,20 @@ SolveOut solve_block_gram(dc2_context* c, Lane& L, char X, const GramMetric& g, 
     read_out2(L, out2, r);
     if (std::isfinite(r.e2) && std::isfinite(r.sumc2)) return r;
     // a singular compressed system: fall through to the eigendecomposition of B~
+  }
+  if (g.btilde) {   // B~ diagonalised before V was known
+    const cplx* bc1 = L.z(nm("_bc1").c_str(), dimm * dimm);
+    const double* wb = L.d(nm("_wbt").c_str(), dimm);
+    cplx* Xm = L.z(nm("_Xg").c_str(), dimm * npert);
+    set_gemm_tag(4);
+    zgemm(s, 'C', 'N', dimm, npert, dimm, one, bc1, dimm, 0, Y, dimm, 0, zero, Xm, dimm, 0, 1);
+    set_gemm_tag(0);
+    const int64_t np = e2_partials_needed(dimm * npert);
+    double* partial = L.d("partial", 2 * np);
+    double* out2 = L.d("out2", 2);
+    e2_reduce(s, dimm, npert, Xm, wb, alpha, partial, np, out2);
+    read_out2(L, out2, r);
+    return r;
   }
   cplx* FG = L.z(nm("_FG").c_str(), m * dimm);
   cplx* bc1 = L.z(nm("_bc1").c_str(), dimm * dimm);
@@ -1418,6 +1457,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
           } else {
             prep[k].g = gram_metric(c, Lw, kSub[k], n3);
           }
+          gram_btilde(c, Lw, kSub[k], prep[k].g);
           prep[k].seconds += tm.stop();
           prep[k].valid = true;
         }
