@@ -122,11 +122,13 @@ __device__ inline double wave_sum(double x) {
   return x;
 }
 
-// sum over the workgroup of one double per thread (same order in every workgroup)
+// sum over the workgroup of one double per thread (same order in every workgroup), with one
+// workgroup barrier, which also publishes the LDS stores made before the call.  red must not
+// be read by another call between this one and the next workgroup barrier: the two sums of
+// a step use red slots of their own.
 __device__ double block_sum(double x, double* red) {
   x = wave_sum(x);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
   if (lane == 0) red[wave] = x;
   __syncthreads();
   double s = 0.0;
@@ -135,13 +137,12 @@ __device__ double block_sum(double x, double* red) {
   return s;
 }
 
-// Reflector of column k from sc (the column as updated, rows k .. n-1): d_k, alpha_k, tau_k,
-// v_k into sv (zero at rows <= k).  Workgroup 0 writes d, alpha, tau, delta and v_k.
-__device__ void reflector(int n, int k, const cplx* sc, cplx* sv, double* red, double* s_tau, cplx* s_delta,
-                          double* d, cplx* sub, double* tau, cplx* delta, cplx* Vh) {
-  double x2 = 0.0;
-  for (int i = k + 1 + threadIdx.x; i < n; i += HT_THREADS) x2 += cabs2(sc[i]);
-  const double xn2 = block_sum(x2, red);
+// Reflector of column k from sc (the column as updated, rows k .. n-1) and x2, this thread's
+// part of |sc[k+1 ..]|^2 over rows it wrote itself: d_k, alpha_k, tau_k, v_k into sv (zero at
+// rows <= k).  Workgroup 0 writes d, alpha, tau, delta and v_k.
+__device__ void reflector(int n, int k, double x2, const cplx* sc, cplx* sv, double* red, double* s_tau,
+                          cplx* s_delta, double* d, cplx* sub, double* tau, cplx* delta, cplx* Vh) {
+  const double xn2 = block_sum(x2, red);   // its barrier publishes sc
   const cplx x0 = sc[k + 1];
   const double ax0 = sqrt(cabs2(x0));
   const double tail = xn2 - ax0 * ax0;
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
                                                       double* __restrict__ tau, cplx* __restrict__ delta,
                                                       cplx* __restrict__ Vh) {
   __shared__ cplx sv[64 * S], sq[64 * S], sc[64 * S];
-  __shared__ double red[HT_WAVES];
+  __shared__ double red[HT_WAVES], redn[HT_WAVES];
   __shared__ double s_tau;
   __shared__ cplx s_delta;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -202,15 +203,20 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
     s_delta = mk(1.0, 0.0);
     if (wg == 0) delta[0] = s_delta;
   }
-  for (int i = threadIdx.x; i < n; i += HT_THREADS) sc[i] = A[i];   // column 0
+  double x2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += HT_THREADS) {   // column 0
+    const cplx x = A[i];
+    sc[i] = x;
+    if (i > 0) x2 += cabs2(x);
+  }
   // the loads of the owned columns complete here, once, rather than being waited for again
   // before every store that might use them
   __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  reflector(n, 0, sc, sv, red, &s_tau, &s_delta, d, sub, tau, delta, Vh);
+  reflector(n, 0, x2, sc, sv, redn, &s_tau, &s_delta, d, sub, tau, delta, Vh);
   for (int k = 0; k <= n - 3; ++k) {
     const int par = k & 1;
     const double t = s_tau;
+    x2 = 0.0;
     cplx* P = w.P + (int64_t)par * n;
     cplx* NC = w.NC + (int64_t)par * n;
     double* Sp = w.S + (int64_t)par * nwg;
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
       }
     }
     if (t != 0.0) {
-      const double K = 0.5 * t * block_sum(kp, red);   // its barriers also publish sc and sq
+      const double K = 0.5 * t * block_sum(kp, red);   // its barrier also publishes sc and sq
       auto qat = [&](int i) {
         const cplx p = sq[i], v = sv[i];
         return mk(p.x - K * v.x, p.y - K * v.y);
@@ -299,16 +305,22 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
           }
         }
       }
+      // column k+1 updated by the rows this thread loaded, and its part of the next norm
       const cplx cq1 = cconj(qat(k + 1)), cv1 = cconj(sv[k + 1]);
       for (int i = k + 1 + threadIdx.x; i < n; i += HT_THREADS) {
         cplx x = sc[i];
         x = csub(x, cmul(sv[i], cq1));
         x = csub(x, cmul(qat(i), cv1));
         sc[i] = x;
+        if (i > k + 1) x2 += cabs2(x);
       }
+    } else {
+      for (int i = k + 1 + threadIdx.x; i < n; i += HT_THREADS)
+        if (i > k + 1) x2 += cabs2(sc[i]);
     }
-    __syncthreads();
-    reflector(n, k + 1, sc, sv, red, &s_tau, &s_delta, d, sub, tau, delta, Vh);
+    // every thread has finished reading sv and sq of step k before the barrier in reflector,
+    // after which v_{k+1} replaces v_k in sv
+    reflector(n, k + 1, x2, sc, sv, redn, &s_tau, &s_delta, d, sub, tau, delta, Vh);
   }
   // the last diagonal element, as updated by the last step
 #pragma unroll
