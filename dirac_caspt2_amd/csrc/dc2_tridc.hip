@@ -38,7 +38,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int DC_LEAF = 32;
+constexpr int DC_LEAF = 16;
 constexpr int DC_NMAX = 2048;
 constexpr int DC_ITMAX = 200;   // secular Newton/bisection steps per root
 constexpr double DC_EPS = 1.1102230246251565e-16;   // 2^-53
@@ -277,12 +277,21 @@ __global__ __launch_bounds__(256) void k_dc_deflate(int n, int depth, const doub
   }
   __syncthreads();
   if (tid != 0) return;
-  // the serial scan
+  // the serial scan; the next entry's loads are issued one iteration ahead, and the test for
+  // a close pair, |(d_j - d_p) c s| <= tol with c s = -z_j z_p / (z_j^2 + z_p^2), is made
+  // without a square root or division (those are needed only for a rotation)
   int K = 0, nf = 0, nr = 0, pj = -1;
   double pd = 0.0, pz = 0.0;
+  int jn = srt[0];
+  double dn = sd[0], zn = sz[0];
   for (int k = 0; k < N; ++k) {
-    const int jj = srt[k];
-    const double dj = sd[k], zj = sz[k];
+    const int jj = jn;
+    const double dj = dn, zj = zn;
+    if (k + 1 < N) {
+      jn = srt[k + 1];
+      dn = sd[k + 1];
+      zn = sz[k + 1];
+    }
     if (rho * fabs(zj) <= tol) {
       w.dfd[a + nf] = dj;
       w.dfc[a + nf] = jj;
@@ -295,9 +304,18 @@ __global__ __launch_bounds__(256) void k_dc_deflate(int n, int depth, const doub
       pz = zj;
       continue;
     }
-    const double t = dc_hyp(zj, pz);
-    const double cs = zj / t, sn = -pz / t;
-    if (fabs((dj - pd) * cs * sn) <= tol) {
+    const double z2 = zj * zj + pz * pz;
+    const bool scaled_ok = z2 > 1e-280 && z2 < 1e280;   // else the exact form below
+    bool close;
+    if (scaled_ok) {
+      close = fabs((dj - pd) * zj * pz) <= tol * z2;
+    } else {
+      const double t0 = dc_hyp(zj, pz);
+      close = fabs((dj - pd) * (zj / t0) * (pz / t0)) <= tol;
+    }
+    if (close) {
+      const double t = dc_hyp(zj, pz);
+      const double cs = zj / t, sn = -pz / t;
       w.rp[a + nr] = pj;
       w.rq[a + nr] = jj;
       w.rc[a + nr] = cs;
