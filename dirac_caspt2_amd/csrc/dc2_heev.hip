@@ -46,11 +46,15 @@ constexpr int HT_NMAX = 2048;
 inline int hetrd_cpw(int n) { return n <= 1024 ? 2 : 1; }
 inline int hetrd_nwg(int n) { return (n + HT_WAVES * hetrd_cpw(n) - 1) / (HT_WAVES * hetrd_cpw(n)); }
 
+// barrier counters: the top one at bar[0], group g's at bar[32 (g + 1)] (lines of their own)
+constexpr int HT_GROUPS = 16;
+constexpr int HT_BAR_WORDS = 32 * (HT_GROUPS + 1);
+
 struct HetrdWork {
   cplx* P;          // [2][n] p of the current step
   cplx* NC;         // [2][n] column k+1 as it stands before step k's update
   double* S;        // [2][nwg] partial sums of Re(v^H p)
-  unsigned* bar;    // barrier counter, zero on entry
+  unsigned* bar;    // HT_BAR_WORDS barrier counters, zero on entry
   int* err;         // 1: a barrier wait hit its spin limit
 };
 
@@ -72,16 +76,23 @@ __device__ inline void st_coherent(cplx* p, cplx v) {
   st_coherent(q + 1, v.y);
 }
 
-// Grid barrier over all workgroups (all resident); false if the wait was abandoned.
-// Everything exchanged between workgroups during the reduction is written and read with
-// device-coherent accesses (write-through stores, loads that bypass the per-XCD L2), so the
-// barrier orders completions only and needs no L2 writeback or invalidation: every wave
+// Grid barrier of step k over all workgroups (all resident); false if the wait was
+// abandoned.  Everything exchanged between workgroups during the reduction is written and read
+// with device-coherent accesses (write-through stores, loads that bypass the per-XCD L2), so
+// the barrier orders completions only and needs no L2 writeback or invalidation: every wave
 // waits for its own stores to complete (s_waitcnt 0) before the workgroup barrier, thread 0
 // then stores the workgroup's partial (the sum of red[0..7]) to part_dst when that is not
-// null, waits for it, and increments the counter; the waiting loop polls with relaxed
-// loads, after which the exchanged data are read with coherent loads.
-__device__ bool grid_sync(unsigned* bar, unsigned target, int* err, double* part_dst, const double* red) {
+// null, waits for it, and arrives: above 64 workgroups on the counter of its group
+// (workgroups wg = g mod HT_GROUPS), the group's last arrival on the top counter, so that no
+// counter takes more than max(16, nwg / 16) increments per step; up to 64 on the top counter
+// directly (one round trip fewer).  The waiting loop polls the top counter with
+// relaxed loads, after which the exchanged data are read with coherent loads.  While thread
+// 0 waits, workgroup 0 stores reflector k (held in sv) to Vh, off the path to its arrival.
+__device__ bool grid_sync(unsigned* bar, int k, int* err, double* part_dst, const double* red, int n,
+                          const cplx* sv, cplx* Vh) {
   __shared__ int ok;
+  const int nwg = gridDim.x, g = blockIdx.x % HT_GROUPS;
+  const unsigned step = (unsigned)(k + 1);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -93,7 +104,19 @@ __device__ bool grid_sync(unsigned* bar, unsigned target, int* err, double* part
       st_coherent(part_dst, ps);
       __builtin_amdgcn_s_waitcnt(0);
     }
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nwg > 64) {
+      const unsigned gsize = (unsigned)((nwg - g + HT_GROUPS - 1) / HT_GROUPS);
+      const unsigned old =
+          __hip_atomic_fetch_add(bar + 32 * (g + 1), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old + 1 == step * gsize) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < n; i += HT_THREADS) Vh[i + (int64_t)n * k] = sv[i];
+  if (threadIdx.x == 0) {
+    const unsigned target = step * (unsigned)(nwg > 64 ? HT_GROUPS : nwg);
     long spins = 0;
     while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
@@ -139,7 +162,7 @@ __device__ double block_sum(double x, double* red) {
 
 // Reflector of column k from sc (the column as updated, rows k .. n-1) and x2, this thread's
 // part of |sc[k+1 ..]|^2 over rows it wrote itself: d_k, alpha_k, tau_k, v_k into sv (zero at
-// rows <= k).  Workgroup 0 writes d, alpha, tau, delta and v_k.
+// rows <= k).  Workgroup 0 writes d, alpha, tau and delta here, v_k at the next barrier.
 __device__ void reflector(int n, int k, double x2, const cplx* sc, cplx* sv, double* red, double* s_tau,
                           cplx* s_delta, double* d, cplx* sub, double* tau, cplx* delta, cplx* Vh) {
   const double xn2 = block_sum(x2, red);   // its barrier publishes sc
@@ -158,8 +181,7 @@ __device__ void reflector(int n, int k, double x2, const cplx* sc, cplx* sv, dou
   for (int i = threadIdx.x; i < n; i += HT_THREADS) {
     cplx v = mk(0.0, 0.0);
     if (t != 0.0 && i > k) v = (i == k + 1) ? csub(sc[i], alpha) : sc[i];
-    sv[i] = v;
-    if (blockIdx.x == 0) Vh[i + (int64_t)n * k] = v;
+    sv[i] = v;   // workgroup 0 stores it to Vh at the next grid barrier (after the loop for the last)
   }
   if (threadIdx.x == 0) {
     *s_tau = t;
@@ -251,7 +273,7 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
           if (i > k && i < n) st_coherent(NC + i, a[c][s]);
         }
       }
-    if (!grid_sync(w.bar, (unsigned)(k + 1) * (unsigned)nwg, w.err, t != 0.0 ? Sp + wg : nullptr, red)) return;
+    if (!grid_sync(w.bar, k, w.err, t != 0.0 ? Sp + wg : nullptr, red, n, sv, Vh)) return;
     // (b) one round of coherent loads: column k+1, p and the partials (nwg <= 256, one per
     // thread, summed in the same order in every workgroup); q = p - K v is formed where used.
     // Every load of the round is issued before the first is used (NL per thread), so their
@@ -322,6 +344,8 @@ __global__ __launch_bounds__(HT_THREADS) void k_hetrd(int n, const cplx* __restr
     // after which v_{k+1} replaces v_k in sv
     reflector(n, k + 1, x2, sc, sv, redn, &s_tau, &s_delta, d, sub, tau, delta, Vh);
   }
+  if (wg == 0)   // the last reflector, which no barrier follows
+    for (int i = threadIdx.x; i < n; i += HT_THREADS) Vh[i + (int64_t)n * (n - 2)] = sv[i];
   // the last diagonal element, as updated by the last step
 #pragma unroll
   for (int c = 0; c < CPW; ++c)
@@ -545,7 +569,8 @@ int64_t vh_cols(int n) { return n >= BT_BLOCKED_MIN ? std::max<int64_t>(n, cdiv(
 int64_t heev_native_work_bytes(int n) {
   const int64_t nn = (int64_t)n * n;
   const int nwg = hetrd_nwg(n);
-  int64_t b = (int64_t)n * vh_cols(n) * 16 + nn * 8 + (int64_t)n * (16 * 6 + 8 * 3) + 16 * nwg + 16 + 10 * 256;
+  int64_t b = (int64_t)n * vh_cols(n) * 16 + nn * 8 + (int64_t)n * (16 * 6 + 8 * 3) + 16 * nwg +
+              4 * (HT_BAR_WORDS + 32) + 10 * 256;
   // the tridiagonal divide and conquer's buffers, later the blocked back-transformation's
   b += std::max<int64_t>(tri_dc_work_bytes(n), n >= BT_BLOCKED_MIN ? back_blocked_bytes(n) : 0);
   return b;
@@ -571,9 +596,9 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
   double* tau = reinterpret_cast<double*>(take(n * 8));
   double* e = reinterpret_cast<double*>(take(n * 8));
   double* S = reinterpret_cast<double*>(take(2 * nwg * 8));
-  unsigned* flags = reinterpret_cast<unsigned*>(take(16));
-  HetrdWork hw{P, NC, S, flags, reinterpret_cast<int*>(flags + 1)};
-  DC2_HIP(hipMemsetAsync(flags, 0, 16, s));
+  unsigned* flags = reinterpret_cast<unsigned*>(take(4 * (HT_BAR_WORDS + 32)));
+  HetrdWork hw{P, NC, S, flags, reinterpret_cast<int*>(flags + 1)};   // err beside the top counter
+  DC2_HIP(hipMemsetAsync(flags, 0, 4 * (HT_BAR_WORDS + 32), s));
   // reflectors 0 .. n-2 are written; the columns after them take part in the blocked
   // back-transformation's batched products (whole panels), as zeros
   DC2_HIP(hipMemsetAsync(Vh + (int64_t)n * (n - 1), 0, (vh_cols(n) - n + 1) * n * sizeof(cplx), s));
