@@ -2222,8 +2222,11 @@ int dc2_mdcint_records(dc2_context* c, int64_t nrec, const int64_t* ikr, const i
     auto up = [&](auto* dst, const auto* src, int64_t count) {
       if (count > 0) DC2_HIP(hipMemcpyAsync(dst, src, count * sizeof(*src), hipMemcpyHostToDevice, s));
     };
+    std::vector<int> cas(2 * nkr + 1);   // Kramers label -> CAS index, kr and d2c composed
+    for (int64_t x = 0; x < 2 * nkr + 1; ++x) cas[x] = (int)d2c[kr[x]];
     DevBuf<int64_t> d_ikr(std::max<int64_t>(nrec, 1)), d_jkr(std::max<int64_t>(nrec, 1)),
-        d_nz(std::max<int64_t>(nrec, 1)), d_eoff(nrec + 1), d_coff(nrec + 1), d_kr(2 * nkr + 1), d_d2c(nd2c);
+        d_nz(std::max<int64_t>(nrec, 1)), d_eoff(nrec + 1), d_coff(nrec + 1);
+    DevBuf<int> d_cas(2 * nkr + 1);
     DevBuf<int> d_kl((size_t)std::max<int64_t>(2 * nent, 1));
     DevBuf<double> d_rr(std::max<int64_t>(nent, 1)), d_ri(rkli ? std::max<int64_t>(nent, 1) : 1);
     up(d_ikr.get(), ikr, nrec);
@@ -2231,8 +2234,7 @@ int dc2_mdcint_records(dc2_context* c, int64_t nrec, const int64_t* ikr, const i
     up(d_nz.get(), nz, nrec);
     up(d_eoff.get(), eoff.data(), nrec + 1);
     up(d_coff.get(), coff.data(), nrec + 1);
-    up(d_kr.get(), kr, 2 * nkr + 1);
-    up(d_d2c.get(), d2c, nd2c);
+    up(d_cas.get(), cas.data(), 2 * nkr + 1);
     up(d_kl.get(), kl.data(), 2 * nent);
     up(d_rr.get(), rklr, nent);
     if (rkli) up(d_ri.get(), rkli, nent);
@@ -2241,8 +2243,8 @@ int dc2_mdcint_records(dc2_context* c, int64_t nrec, const int64_t* ikr, const i
     const int64_t nsc = mdcint_convert_scratch(ncand) / 8;
     DevBuf<int64_t> d_sc(nsc);
     mdcint_convert(s, nrec, d_ikr.get(), d_jkr.get(), d_nz.get(), d_eoff.get(), d_coff.get(), d_kl.get(),
-                   d_kl.get() + nent, d_rr.get(), rkli ? d_ri.get() : nullptr, nkr, d_kr.get(), nd2c, d_d2c.get(),
-                   scheme, ncand, d_sc.get(), d_ijkl.get(), d_val.get());
+                   d_kl.get() + nent, d_rr.get(), rkli ? d_ri.get() : nullptr, nkr, d_cas.get(), scheme, ncand,
+                   d_sc.get(), d_ijkl.get(), d_val.get());
     int64_t count = 0;
     copy_sync(c, &count, d_sc.get() + nsc - 1, sizeof(int64_t), hipMemcpyDeviceToHost);
     *nout = count;

@@ -121,12 +121,12 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
 // every label negated for the types with one barred label on each side, else im -> -im).
 // coff[r] = first candidate of record r (coff[nrec] = all).
 struct MdcintDev {
-  int64_t nrec, nkr, nd2c;
+  int64_t nrec, nkr;
   int scheme;
   const int64_t *ikr, *jkr, *nz, *eoff, *coff;
   const int *indk, *indl;
   const double *rr, *ri;   // ri null: real integrals
-  const int64_t *kr, *d2c;
+  const int* cas;           // cas[nkr + m]: CAS index of Kramers label m (d2c after kr)
 };
 
 struct MdOut {
@@ -139,14 +139,27 @@ __device__ inline int64_t kramers_label(int64_t x) {   // 1 -1 2 -2 ... from CAS
   return (x & 1) ? x / 2 + 1 : -(x / 2);
 }
 
-__device__ MdOut mdcint_candidate(const MdcintDev& m, int64_t c) {
-  int64_t lo = 0, hi = m.nrec;   // last record with coff <= c
-  while (hi - lo > 1) {
-    const int64_t md = (lo + hi) >> 1;
-    if (m.coff[md] <= c) lo = md;
-    else hi = md;
+// the record of candidate c: a binary search (lane 0 of the wave, for the wave's first
+// candidate) then a forward walk over the records that end inside the wave (usually none:
+// a record has 2 nz or 4 nz candidates)
+__device__ inline int64_t mdcint_record(const MdcintDev& m, int64_t c, int64_t c0) {
+  int64_t r0 = 0;
+  if ((threadIdx.x & 63) == 0) {
+    int64_t lo = 0, hi = m.nrec;   // last record with coff <= c0
+    while (hi - lo > 1) {
+      const int64_t md = (lo + hi) >> 1;
+      if (m.coff[md] <= c0) lo = md;
+      else hi = md;
+    }
+    r0 = lo;
   }
-  const int64_t r = lo, nz = m.nz[r];
+  int64_t r = __shfl(r0, 0, 64);
+  while (r + 1 < m.nrec && m.coff[r + 1] <= c) ++r;
+  return r;
+}
+
+__device__ MdOut mdcint_candidate(const MdcintDev& m, int64_t c, int64_t c0) {
+  const int64_t r = mdcint_record(m, c, c0), nz = m.nz[r];
   int64_t ci = m.ikr[r], cj = m.jkr[r];
   const bool amplify = m.scheme != 6 && llabs(ci) != llabs(cj);
   const int64_t half = (amplify ? 2 : 1) * nz;
@@ -177,7 +190,7 @@ __device__ MdOut mdcint_candidate(const MdcintDev& m, int64_t c) {
   int64_t kl[4], ab[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int64_t x = m.d2c[m.kr[m.nkr + lab[a]]];   // CAS index
+    const int64_t x = m.cas[m.nkr + lab[a]];   // CAS index
     kl[a] = kramers_label(x);
     ab[a] = llabs(kl[a]);
     o.q[a] = (int)((x & 1) ? x + 1 : x - 1);        // the Kramers partner, as written to MDCINTNEW
@@ -207,7 +220,12 @@ constexpr int MD_BLOCK = 256;
 
 __global__ __launch_bounds__(MD_BLOCK) void k_md_count(MdcintDev m, int64_t ncand, int64_t* __restrict__ bcount) {
   const int64_t c = (int64_t)blockIdx.x * MD_BLOCK + threadIdx.x;
-  const bool keep = c < ncand && mdcint_candidate(m, c).keep;
+  const int64_t c0 = c - (threadIdx.x & 63);   // the wave's first candidate (< ncand)
+  bool keep = false;
+  if (c0 < ncand) {   // uniform over the wave: every lane takes part in the search's shuffle
+    const MdOut o = mdcint_candidate(m, c < ncand ? c : c0, c0);
+    keep = c < ncand && o.keep;
+  }
   const int k = __syncthreads_count(keep);
   if (threadIdx.x == 0) bcount[blockIdx.x] = k;
 }
@@ -240,9 +258,13 @@ __global__ __launch_bounds__(MD_BLOCK) void k_md_write(MdcintDev m, int64_t ncan
                                                        int* __restrict__ ijkl, double* __restrict__ val) {
   __shared__ int wsum[MD_BLOCK / 64];
   const int64_t c = (int64_t)blockIdx.x * MD_BLOCK + threadIdx.x;
+  const int64_t c0 = c - (threadIdx.x & 63);
   MdOut o;
   o.keep = false;
-  if (c < ncand) o = mdcint_candidate(m, c);
+  if (c0 < ncand) {
+    o = mdcint_candidate(m, c < ncand ? c : c0, c0);
+    if (c >= ncand) o.keep = false;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long mask = __ballot(o.keep);
   const int below = __popcll(mask & ((1ull << lane) - 1ull));
@@ -265,11 +287,11 @@ int64_t mdcint_convert_scratch(int64_t ncand) { return 8 * (2 * cdiv(ncand, MD_B
 
 void mdcint_convert(hipStream_t s, int64_t nrec, const int64_t* ikr, const int64_t* jkr, const int64_t* nz,
                     const int64_t* eoff, const int64_t* coff, const int* indk, const int* indl,
-                    const double* rr, const double* ri, int64_t nkr, const int64_t* kr, int64_t nd2c,
-                    const int64_t* d2c, int scheme, int64_t ncand, int64_t* scratch, int* ijkl, double* val) {
+                    const double* rr, const double* ri, int64_t nkr, const int* cas, int scheme, int64_t ncand,
+                    int64_t* scratch, int* ijkl, double* val) {
   const int64_t nb = cdiv(ncand, MD_BLOCK);
   DC2_CHECK(nb < (1ll << 31), 1, "mdcint: too many entries for one launch");
-  MdcintDev m{nrec, nkr, nd2c, scheme, ikr, jkr, nz, eoff, coff, indk, indl, rr, ri, kr, d2c};
+  MdcintDev m{nrec, nkr, scheme, ikr, jkr, nz, eoff, coff, indk, indl, rr, ri, cas};
   int64_t* bcount = scratch;
   int64_t* boff = scratch + nb;
   if (nb > 0) {

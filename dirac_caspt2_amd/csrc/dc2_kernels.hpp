@@ -96,15 +96,15 @@ void zgemm_ozaki(hipStream_t s, int slices, char opB, int64_t M, int64_t N, int6
 // MDCINT records -> MDCINTNEW-equivalent records (create_newmdcint.F90:86-177) on the device:
 // nrec records (ikr, jkr, nz) with entries at eoff[r] .. eoff[r] + nz - 1 of indk/indl/rr/ri
 // (ri null: real), coff[r] the record's first candidate (coff[nrec] = ncand, candidates per
-// record 2 nz, 4 nz when the SCHEME amplifies), kr[nkr + m] the DIRAC spinor of Kramers label
-// m, d2c[k] the CAS index of DIRAC spinor k.  Writes the kept records to ijkl[4 .] / val[2 .]
+// record 2 nz, 4 nz when the SCHEME amplifies), cas[nkr + m] the CAS index of Kramers label m
+// (indmo_dirac_to_cas of the header's kr(m)).  Writes the kept records to ijkl[4 .] / val[2 .]
 // (room for ncand) in the reference's order and their count to scratch[cdiv(ncand, 256) * 2];
-// scratch: mdcint_convert_scratch(ncand) bytes.  Labels must be within kr and d2c.
+// scratch: mdcint_convert_scratch(ncand) bytes.  Labels must be within cas.
 int64_t mdcint_convert_scratch(int64_t ncand);
 void mdcint_convert(hipStream_t s, int64_t nrec, const int64_t* ikr, const int64_t* jkr, const int64_t* nz,
                     const int64_t* eoff, const int64_t* coff, const int* indk, const int* indl,
-                    const double* rr, const double* ri, int64_t nkr, const int64_t* kr, int64_t nd2c,
-                    const int64_t* d2c, int scheme, int64_t ncand, int64_t* scratch, int* ijkl, double* val);
+                    const double* rr, const double* ri, int64_t nkr, const int* cas, int scheme, int64_t ncand,
+                    int64_t* scratch, int* ijkl, double* val);
 void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const int* const* spaces, const int* const* dims,
                     cplx* const* data, unsigned long long* const* owner, const int64_t* counts, int64_t nrec,
                     const int* ijkl, const double* val);
