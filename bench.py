@@ -6,7 +6,7 @@ subspaces A-H) with the integrals already resident in HBM.  The default workload
 324-spinor one (30 inactive / 14 active with 6 electrons (3003 determinants) / 280
 secondary, complex128, seeded synthetic Kramers-paired integrals).
 
-    python bench.py [--steps K] [--warmup W] [--workload split|fp64|small|batch|serve]
+    python bench.py [--steps K] [--warmup W] [--workload split|fp64|small|batch|serve|eig]
                     [--served N] [--slots S] [--split SLICES] [--secondary fp64|''] [--stream 0|1]
                     [--requests N] [--rate R] [--no-autotune] [--no-cpu-baseline]
 
@@ -450,6 +450,40 @@ def gemm_accounting(ctx, step, workload, nsteps):
             else 0.0}
 
 
+def eig_workload(args):
+    """The library's Hermitian eigensolver (dc2_cdiag: tridiagonalisation, tridiagonal divide
+    and conquer, back-transformation) on seeded rank-deficient Gram matrices G = X X^H / r, the
+    shape of the A/C metrics (diag.F90:66-128 is the routine it replaces).  Wall time per solve
+    includes the host-to-device and device-to-host copies of the matrix."""
+    from dirac_caspt2_amd.api import Caspt2Context, cdiag, hetrd_stats
+    rng = np.random.default_rng(5)
+    sizes = ((792, 500), (1274, 900), (2002, 1274))
+    out = {}
+    with Caspt2Context(0) as ctx:
+        ctx.set_option("eigensolver", "native")
+        for n, rank in sizes:
+            X = rng.normal(size=(n, rank)) + 1j * rng.normal(size=(n, rank))
+            A = X @ X.conj().T / rank
+            cdiag(ctx, A)
+            hetrd_stats(0)
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                w, V, _ = cdiag(ctx, A)
+            dt = (time.perf_counter() - t) / args.steps
+            calls, sec = hetrd_stats(n)
+            top = V[:, -8:]
+            out[str(n)] = {"rank": rank, "ms_per_solve": 1e3 * dt, "hetrd_ms": 1e3 * sec / max(calls, 1),
+                           "max_residual_top8": float(np.abs(A @ top - top * w[-8:]).max())}
+            log(f"n {n}: {1e3 * dt:.2f} ms per solve")
+    big = out[str(sizes[-1][0])]
+    return {"metric": "Hermitian eigensolves/s of a rank-deficient Gram matrix (n 2002, rank 1274)",
+            "value": 1e3 / big["ms_per_solve"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "higher_is_better": True, "dtype": "f64 (complex128)", "data": "synthetic (seeded X, G = X X^H / rank)",
+            "config": {"workload": "eig: n/rank 792/500, 1274/900, 2002/1274 through dc2_cdiag, host copies "
+                                   "included"},
+            "sizes": out}
+
+
 AMPLITUDE_NOTE = ("two-electron factors at amplitude 0.25 x CN(0,1) (amplitude 1 puts a secondary Fock eigenvalue "
                   "below the inactive ones: an intruder state)")
 
@@ -508,6 +542,8 @@ def run_workload(name, args, steps, warmup, served, slots, autotune_on, accounti
             res["roofline"] = hr
         else:
             res["roofline"] = gr
+            if hr is not None:
+                res["roofline"]["tridiagonalisation"] = hr
         res["roofline"]["stage_hbm"] = stage_hbm(name)
     if served:
         idx = sorted({0, served // 3, (2 * served) // 3, served - 1})
@@ -529,9 +565,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="split", choices=["small", "fp64", "split", "batch", "serve"],
+    ap.add_argument("--workload", default="split", choices=["small", "fp64", "split", "batch", "serve", "eig"],
                     help="split: 324 spinors (30/14/280, 6 electrons, the default line); fp64: 192 spinors "
-                         "(16/12/164); small: 52 spinors; batch: 64 independent instances; serve: Poisson stream")
+                         "(16/12/164); small: 52 spinors; batch: 64 independent instances; serve: Poisson stream; "
+                         "eig: the eigensolver alone on Gram matrices of the A/C metric shape")
     ap.add_argument("--secondary", default="fp64", help="workload of the secondary object of the line ('' : none)")
     ap.add_argument("--secondary-steps", type=int, default=10)
     ap.add_argument("--instances", type=int, default=64, help="batch workload: independent instances")
@@ -568,6 +605,9 @@ def main():
               "(the sharding design is documented in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     window = args.eig_window
+    if args.workload == "eig":
+        print(json.dumps(eig_workload(args)))
+        return
     if args.workload == "batch":
         args.eig_window = 0.1 if window is None else window
         args.slots = args.slots or 64

@@ -490,7 +490,9 @@ bool launch_skinny(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, c
                    const cplx* B, int64_t ldb, cplx beta, cplx* C, int64_t ldc) {
   constexpr bool b_along_k = OPB == 'N' || OPB == 'R';
   if constexpr ((OPA == 'C' || OPA == 'T') && b_along_k) {
-    if (M * N > 2048) return false;
+    // a few outputs, or a matrix-vector product over a long K (one wave per output row
+    // reads its column of A once; on 64 x 64 MFMA tiles N = 1 would use 1/64 of each tile)
+    if (M * N > 2048 && !(N <= DOT_NB && K >= 256)) return false;
     const int64_t waves = M * ((N + DOT_NB - 1) / DOT_NB);
     hipLaunchKernelGGL((k_zgemm_dot<OPA, OPB>), dim3((unsigned)cdiv(waves, 4)), dim3(256), 0, s, M, N, K, alpha, A,
                        lda, B, ldb, beta, C, ldc);

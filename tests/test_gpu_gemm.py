@@ -33,6 +33,8 @@ def _op(x, op):
                                    (48, 6, 924), (6, 6, 924), (924, 1, 924), (45, 7, 63), (2048, 1, 3),
                                    (924, 6, 48), (924, 6, 64), (255, 8, 65), (924, 6, 924), (3000, 5, 4097),
                                    (33, 9, 700),
+                                   # matrix-vector products over a long K (one wave per row)
+                                   (3003, 1, 3003), (2600, 4, 256),
                                    # K <= 16 with one output dimension <= 16 (index rotations
                                    # of the transforms: the MFMA kernel)
                                    (4100, 16, 16), (4100, 5, 12), (12, 4100, 12), (7, 3000, 16), (16, 300, 9)])
