@@ -358,13 +358,15 @@ class Caspt2Context:
         return float(out[0]), float(out[1])
 
     OPTIONS = {"eigensolver": {"auto": -1, "qr": 0, "dc": 1, "dcj": 2, "jacobi": 3, "native": 4},
-               "ac_metric": {"auto": 0, "formula": 1, "determinant": 2, "antisymmetric": 3},
+               "ac_metric": {"auto": 0, "formula": 1, "determinant": 2, "antisymmetric": 3,
+                             "antisymmetric_det": 4},
                "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {},
                "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}, "graphs": {"off": 0, "on": 1},
                "tridiag_solver": {"own": 0, "rocsolver": 1}}
 
     def set_option(self, key: str, value):
-        """Options: eigensolver auto|qr|dc|dcj|jacobi|native, ac_metric auto|formula|determinant,
+        """Options: eigensolver auto|qr|dc|dcj|jacobi|native, ac_metric auto|formula|determinant|
+        antisymmetric|antisymmetric_det,
         lanes 1..4 (concurrent subspace streams), schur on|off, graphs on|off (captured
         Davidson iterations), tridiag_solver own|rocsolver (the tridiagonal stage of the native
         eigensolver)."""

@@ -1098,10 +1098,13 @@ SolveOut solve_block_gram(dc2_context* c, Lane& L, char X, const GramMetric& g, 
 
 struct AcRoute {
   bool gram = false, reduced = false;
+  bool phi = false;   // reduced: S and B as products of the determinant-space factor
 };
 
 // A and C: determinant-space metric when the (nelec+-1)-electron space is the smallest
-// basis; A: the antisymmetric reduction (tuples t < u) when that is smaller still
+// basis; A: the antisymmetric reduction (tuples t < u) when that is smaller still, its S
+// and B formed from the determinant-space factor (no F-weighted 3-RDM, and no wait for the
+// densities)
 AcRoute ac_route_of(const dc2_context* c, char X) {
   AcRoute r;
   if (X != 'A' && X != 'C') return r;
@@ -1111,8 +1114,11 @@ AcRoute ac_route_of(const dc2_context* c, char X) {
   const int64_t nred = (int64_t)na * na * (na - 1) / 2;
   if (c->ac_route == 2) r.gram = true;
   else if (c->ac_route == 3) r.reduced = X == 'A';
-  else if (c->ac_route == 0) {
-    if (X == 'A' && nred < mdet && nred < n) r.reduced = true;
+  else if (c->ac_route == 4) {
+    if (X == 'A') r.reduced = r.phi = true;
+    else r.gram = true;
+  } else if (c->ac_route == 0) {
+    if (X == 'A' && nred < mdet && nred < n) r.reduced = r.phi = true;
     else r.gram = mdet < n;
   }
   return r;
@@ -1178,13 +1184,29 @@ SubPrep subspace_prep(dc2_context* c, Lane& L, int k) {
       DC2_HIP(hipMemcpyAsync(amap, hm.data(), nred * sizeof(int), hipMemcpyHostToDevice, s));
       DC2_HIP(hipStreamSynchronize(s));
     }
-    build_SB(s, X, d, c->eps.get() + c->ni, c->e0.get(), nb, pairs, S, B, amap);
+    if (route.phi) {
+      // the reduced rows and columns of S = Phi^H Phi and B = Phi^H F Phi (solve_block_gram),
+      // from the rows t < u of Phi^H over the (nelec+1)-electron strings
+      int64_t* strings = L.l(nm("_strings").c_str(), mdet);
+      cplx* Psi = L.z(nm("_Psi").c_str(), nred * mdet);
+      cplx* Psif = L.z(nm("_Psif").c_str(), nred * mdet);
+      build_phi_adj(s, na, c->ne, c->nfull, c->rank_full.get(), c->Y1.get(), c->eps.get() + c->ni, mdet, strings,
+                    nred, amap, Psi, Psif);
+      set_gemm_tag(4);
+      zherk_upper(s, nred, mdet, Psi, nred, S, nred);
+      zgemm_herm(s, nred, mdet, Psif, nred, Psi, nred, B, nred);
+      set_gemm_tag(0);
+    } else {
+      build_SB(s, X, d, c->eps.get() + c->ni, c->e0.get(), nb, pairs, S, B, amap);
+    }
     check_finite(c, s, (std::string("S") + X).c_str(), S, nb * nb);
     check_finite(c, s, (std::string("B") + X).c_str(), B, nb * nb);
     if (route.reduced) {
       // S_full = K^H S_red K with K K^H = 2 I (a+_u a+_t = -a+_t a+_u, a+_t a+_t = 0), so the
       // nonzero eigenvalues of the reference's S are twice those of S_red: cutoff at 0.5e-8
-      p.bp = solve_prep(c, L, X, nred, S, B, nred > mdet ? EigKind::rank_deficient : EigKind::general, 2.0);
+      const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
+      p.bp = solve_prep(c, L, X, nred, S, B, nred > mdet || (route.phi && ras) ? EigKind::rank_deficient
+                                                                                 : EigKind::general, 2.0);
     } else {
       const EigKind kind = (X == 'A' || X == 'C') ? EigKind::rank_deficient : EigKind::general;
       p.bp = solve_prep(c, L, X, n, S, B, kind);
@@ -1192,6 +1214,31 @@ SubPrep subspace_prep(dc2_context* c, Lane& L, int k) {
   }
   p.seconds = tx.stop();
   return p;
+}
+
+// The 3-RDM part of V_A and V_C (solve_A_subspace.F90:571-593, solve_C_subspace.F90:547-564)
+// from the E_qr|0> vectors Y1 of the rotated state, without the 3-RDM: for A,
+// V(tuv, i) = -sum_pqr <E_vu E_pt E_qr> (p i|q r) with the block A1[p + na (i + ni (q + na r))];
+// for C, V(tuv, a) = sum_jkl <E_vu E_tj E_kl> (a j|k l) with C1[a + ns (j + na (k + na l))].
+// Zt = M Y1^T sums over the last index pair (M: the block as an (nx na) x na^2 matrix),
+// v3_esum applies the excitation over the summed index through the E_pq table, and
+// Y1^H Wt^T contracts with <0|E_vu: 2 * 8 nf na^3 nx flops against 8 nf na^6 for the 3-RDM.
+void v3_part(dc2_context* c, Lane& L, char X, const cplx* blk, int nx, cplx* V) {
+  hipStream_t s = L.stream;
+  const int na = c->na;
+  const int64_t nf = c->nfull, na2 = (int64_t)na * na, nr = (int64_t)nx * na;
+  if (nx <= 0 || nf <= 0) return;
+  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  cplx* Zt = L.z("v3_Zt", nr * nf);
+  cplx* Wt = L.z("v3_Wt", nr * nf);
+  cplx* Out = L.z("v3_Out", na2 * nr);
+  set_gemm_tag(3);
+  zgemm(s, 'N', 'T', nr, nf, na2, one, blk, nr, 0, c->Y1.get(), nf, 0, zero, Zt, nr, 0, 1);
+  if (X == 'C') v3_esum(s, true, na, nf, nx, 1, nx, nr, c->exc.get(), Zt, Wt);   // Zt[(a + ns j) + nr J]
+  else v3_esum(s, false, na, nf, nx, na, 1, nr, c->exc.get(), Zt, Wt);          // Zt[(p + na i) + nr J]
+  zgemm(s, 'C', 'T', na2, nr, nf, one, c->Y1.get(), nf, 0, Wt, nr, 0, zero, Out, na2, 0, 1);
+  set_gemm_tag(0);
+  v3_permute(s, na, nx, X == 'A' ? -1.0 : 1.0, Out, V);
 }
 
 // one subspace k (0..7 = A..H) on lane L: V and alpha, then the solve with the prepared
@@ -1234,29 +1281,19 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
     if (n > 0 && npert > 0) {
       switch (X) {
         case 'A': {
-          cplx* Dm = L.z("A_Dm", na3 * na3);
-          cplx* Am = L.z("A_Am", na3 * ni);
           cplx* eff = L.z("A_eff", na * ni);
           const cplx* A1 = blk("A1");
           effh_A(s, ni, na, c->hp.get(), nmo, blk("A2"), eff);
-          d3_perm_A(s, na, c->G3.get(), Dm);
-          a1_perm(s, na, ni, A1, Am);
-          set_gemm_tag(3);
-          zgemm(s, 'N', 'N', na3, ni, na3, mk(-1.0, 0.0), Dm, na3, 0, Am, na3, 0, zero, V, na3, 0, 1);
-          set_gemm_tag(0);
+          v3_part(c, L, 'A', A1, ni, V);
           V_A_rest(s, na, ni, d, A1, eff, V);
         } break;
         case 'B': V_B(s, na, ni, d, blk("B"), c->tpairs.get(), c->ipairs.get(), V); break;
         case 'C': {
-          cplx* Dm = L.z("C_Dm", na3 * na3);
           cplx* eff = L.z("C_eff", (int64_t)ns * na);
           const cplx* C1 = blk("C1");
           if (ni) effh_C(s, ni, na, ns, c->hp.get(), nmo, C1, blk("C2"), blk("C3"), eff);
           else effh_C(s, 0, na, ns, c->hp.get(), nmo, C1, C1, C1, eff);
-          d3_perm_C(s, na, c->G3.get(), Dm);
-          set_gemm_tag(3);
-          zgemm(s, 'N', 'T', na3, ns, na3, one, Dm, na3, 0, C1, ns, 0, zero, V, na3, 0, 1);
-          set_gemm_tag(0);
+          v3_part(c, L, 'C', C1, ns, V);
           V_C_rest(s, na, ns, d, eff, V);
         } break;
         case 'D': {
@@ -1461,6 +1498,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
           prep[k].seconds += tm.stop();
           prep[k].valid = true;
         }
+        // A's reduced metric from the determinant-space factor needs the rotated state only
+        for (int k : plan[w])
+          if (k == 0 && ac_route_of(c, 'A').phi) prep[k] = subspace_prep(c, Lw, k);
         if (!go_dens_f.get()) return;
         DC2_HIP(hipStreamWaitEvent(Lw.stream, dens_ready, 0));
         for (int k : plan[w])
@@ -1582,8 +1622,12 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     zgemm(s, 'C', 'N', na2, na2, nf, one, c->Y1.get(), nf, 0, c->Y1F.get(), nf, 0, zero, c->G2F.get(), na2, 0, 1);
     // the F-weighted 3-RDM enters only the formula S/B matrices of A and C (not their
     // determinant-space route, not any V)
-    const bool need_g3f = (doA && !ac_route_of(c, 'A').gram) || (doC && !ac_route_of(c, 'C').gram);
-    if (doA || doC) {
+    // the 3-RDM and its F-weighted form enter only the formula S/B matrices of A and C (V
+    // comes from the E_qr|0> vectors, the determinant-space routes from Phi)
+    const bool need_g3f = (doA && !ac_route_of(c, 'A').gram && !ac_route_of(c, 'A').phi) ||
+                          (doC && !ac_route_of(c, 'C').gram);
+    const bool need_g3 = need_g3f;
+    if (need_g3) {
       const int64_t na4 = na2 * na2;
       c->G3.ensure(na2 * na4);
       cplx* Y2 = L0.z("Y2", nf * na4);
@@ -1609,7 +1653,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     check_finite(c, "G1F", c->G1F.get(), na2);
     check_finite(c, "G2", c->G2.get(), na2 * na2);
     check_finite(c, "G2F", c->G2F.get(), na2 * na2);
-    if (doA || doC) check_finite(c, "G3", c->G3.get(), na2 * na2 * na2);
+    if (need_g3) check_finite(c, "G3", c->G3.get(), na2 * na2 * na2);
     if (need_g3f) check_finite(c, "G3F", c->G3F.get(), na2 * na2 * na2);
     check_finite(c, "h'", c->hp.get(), (int64_t)nmo * nmo);
     upload_ints(s, c->tpairs, pair_list(na));
@@ -2276,7 +2320,7 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
       DC2_CHECK(value >= -1 && value <= 4, 1, "eigensolver must be -1..4");
       c->eig_method = (int)value;
     } else if (k == "ac_metric") {
-      DC2_CHECK(value >= 0 && value <= 3, 1, "ac_metric must be 0..3");
+      DC2_CHECK(value >= 0 && value <= 4, 1, "ac_metric must be 0..4");
       c->ac_route = (int)value;
     } else if (k == "lanes") {
       DC2_CHECK(value >= 1 && value <= 4, 1, "lanes must be 1..4");

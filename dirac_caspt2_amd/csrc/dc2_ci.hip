@@ -602,6 +602,36 @@ __global__ void k_phi(int na, int create, int64_t m, const int64_t* __restrict__
   Phi[r + m * ((int64_t)t * na * na + uv)] = val;
 }
 
+// The antisymmetric reduction of the A factor, stored adjoint: row k of Psi is the tuple
+// tup[k] = t*na*na + u*na + v (t < u) of Phi, Psi[k + nred*r] = conj(<r| a+_t E_uv |0>) over
+// the (ne+1)-electron strings r, and Psif[k + nred*r] = f[r] Psi[k + nred*r] with f[r] =
+// sum_w eps_w n_w(r).  Threads run along k, so the stores of a row r are contiguous; r
+// strides over the grid's y dimension.
+__global__ void k_phi_adj(int na, int64_t m, const int64_t* __restrict__ strings, int64_t nf,
+                          const int* __restrict__ rank_full, const cplx* __restrict__ Y1,
+                          const double* __restrict__ eps_act, int64_t nred, const int* __restrict__ tup,
+                          cplx* __restrict__ Psi, cplx* __restrict__ Psif) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nred) return;
+  const int idx = tup[k];
+  const int t = idx / (na * na), u = (idx / na) % na, v = idx % na;
+  const uint64_t bit = 1ull << t;
+  for (int64_t r = blockIdx.y; r < m; r += gridDim.y) {
+    const uint64_t d = (uint64_t)strings[r];
+    double f = 0.0;
+    for (int w = 0; w < na; ++w)
+      if ((d >> w) & 1ull) f += eps_act[w];
+    cplx val = mk(0.0, 0.0);
+    if (d & bit) {
+      val = Y1[(int64_t)rank_full[d ^ bit] + nf * (u + na * v)];
+      if (__popcll(d >> (t + 1)) & 1) val = mk(-val.x, -val.y);
+    }
+    val = cconj(val);
+    Psi[k + nred * r] = val;
+    Psif[k + nred * r] = mk(f * val.x, f * val.y);
+  }
+}
+
 // f[r] = sum_w eps_w n_w(strings[r])
 __global__ void k_fdet(int na, int64_t m, const int64_t* strings, const double* eps_act, double* f) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -836,6 +866,19 @@ void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int
   hipLaunchKernelGGL(k_phi, dim3(cdiv(m, 256), na, na * na), dim3(256), 0, s, na, create ? 1 : 0, m, strings, nf,
                      rank_full, Y1, Phi);
   hipLaunchKernelGGL(k_fdet, dim3(cdiv(m, 256)), dim3(256), 0, s, na, m, strings, eps_act, fdiag);
+  DC2_HIP(hipGetLastError());
+}
+
+void build_phi_adj(hipStream_t s, int na, int ne, int64_t nf, const int* rank_full, const cplx* Y1,
+                   const double* eps_act, int64_t m, int64_t* strings, int64_t nred, const int* tup, cplx* Psi,
+                   cplx* Psif) {
+  DC2_CHECK(m == binom_host(na, ne + 1), 1, "build_phi_adj: wrong string count");
+  if (m == 0 || nred == 0) return;
+  DC2_CHECK(na >= 2 && na <= 24, 1, "build_phi_adj: nact out of range");
+  DC2_CHECK(nred == (int64_t)na * na * (na - 1) / 2, 1, "build_phi_adj: wrong tuple count");
+  hipLaunchKernelGGL(k_unrank, dim3(cdiv(m, 256)), dim3(256), 0, s, na, ne + 1, m, strings);
+  hipLaunchKernelGGL(k_phi_adj, dim3(cdiv(nred, 256), (unsigned)std::min<int64_t>(m, 65535)), dim3(256), 0, s, na, m,
+                     strings, nf, rank_full, Y1, eps_act, nred, tup, Psi, Psif);
   DC2_HIP(hipGetLastError());
 }
 

@@ -120,6 +120,10 @@ void set_gemm_no_workspace(bool on);
 // split: k chunks summed by a reduction (0: chosen by size, 1: unsplit)
 void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc,
                  int split = 0);
+// C = A B^H (m x m) for a product known to be Hermitian (B = D A, D real diagonal over k):
+// the same upper tiles, reduction and mirror
+void zgemm_herm(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, const cplx* B, int64_t ldb, cplx* C,
+                int64_t ldc, int split = 0);
 // GEMM tuning table (exact shapes): the MFMA kernel variant (1..4: K tile 16/32 x LDS padding
 // 1/16) and split-K factor (1: none) chosen for (opA, opB, M, N, K, batch) by measurement.
 // gemm_tune times every candidate on scratch operands on stream s and installs the fastest;

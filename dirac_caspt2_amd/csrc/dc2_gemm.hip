@@ -676,9 +676,16 @@ void zgemm(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, c
 // k_herk_reduce (m = 792, k = 1728: 421 us unsplit, 207 / 162 / 164 / 162 / 145 us in 3 / 4
 // / 8 / 12 / 16 chunks).  split: 0 this rule, >= 1 that many k chunks (1: unsplit).
 void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, cplx* C, int64_t ldc, int split) {
+  zgemm_herm(s, m, k, A, lda, A, lda, C, ldc, split);
+}
+
+// C (m x m, ldc) = A B^H for A, B (m x k) whose product is Hermitian (B = D A with D real
+// diagonal over k, or B = A): the same upper-tile product and mirror as zherk_upper
+void zgemm_herm(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda, const cplx* B, int64_t ldb, cplx* C,
+                int64_t ldc, int split) {
   if (m <= 0) return;
-  DC2_CHECK(cdiv(m, BM) < 65536, 1, "zherk_upper: matrix too large for the grid");
-  DC2_CHECK(split >= 0 && split <= 64, 1, "zherk_upper: split 0..64");
+  DC2_CHECK(cdiv(m, BM) < 65536, 1, "zgemm_herm: matrix too large for the grid");
+  DC2_CHECK(split >= 0 && split <= 64, 1, "zgemm_herm: split 0..64");
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   const int64_t T = cdiv(m, BM), tiles = T * (T + 1) / 2;
   int64_t want = split;
@@ -692,15 +699,15 @@ void zherk_upper(hipStream_t s, int64_t m, int64_t k, const cplx* A, int64_t lda
   if (sk > 1) {
     cplx* W = splitk_workspace(s, (size_t)sk * m * m);
     dim3 grid(T, T, (unsigned)sk);
-    hipLaunchKernelGGL((k_zgemm2<'N', 'C', 16, 1, true>), grid, dim3(256), 0, s, m, m, k, one, A, lda, (int64_t)0, A,
-                       lda, (int64_t)0, zero, W, m, m * m, sk, kchunk);
+    hipLaunchKernelGGL((k_zgemm2<'N', 'C', 16, 1, true>), grid, dim3(256), 0, s, m, m, k, one, A, lda, (int64_t)0, B,
+                       ldb, (int64_t)0, zero, W, m, m * m, sk, kchunk);
     hipLaunchKernelGGL(k_herk_reduce, dim3(cdiv(m * m, 256)), dim3(256), 0, s, m, sk, W, C, ldc);
     DC2_HIP(hipGetLastError());
     return;
   }
   dim3 grid(T, T, 1);
-  hipLaunchKernelGGL((k_zgemm2<'N', 'C', 16, 1, true>), grid, dim3(256), 0, s, m, m, k, one, A, lda, (int64_t)0, A,
-                     lda, (int64_t)0, zero, C, ldc, (int64_t)0, 1, k);
+  hipLaunchKernelGGL((k_zgemm2<'N', 'C', 16, 1, true>), grid, dim3(256), 0, s, m, m, k, one, A, lda, (int64_t)0, B,
+                     ldb, (int64_t)0, zero, C, ldc, (int64_t)0, 1, k);
   hipLaunchKernelGGL(k_lower_from_upper, dim3(cdiv(m * m, 256)), dim3(256), 0, s, m, C, ldc);
   DC2_HIP(hipGetLastError());
 }

@@ -70,6 +70,13 @@ int64_t binom_host(int n, int k);
 // S = Phi^H Phi and B = Phi^H diag(fdiag) Phi with fdiag[r] = sum_w eps_w n_w(r)
 void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int* rank_full, const cplx* Y1,
                const double* eps_act, int64_t m, int64_t* strings, cplx* Phi, double* fdiag);
+// The antisymmetric reduction of the A factor in adjoint layout: the (ne+1)-electron strings,
+// Psi (nred x m) with Psi[k + nred*r] = conj(<r|a+_t E_uv|0>) for the tuple tup[k] = t*na*na +
+// u*na + v (t < u, nred = na*na*(na-1)/2 of them) and Psif = Psi diag(f), so that the reduced
+// S = Psi Psi^H and B = Psif Psi^H
+void build_phi_adj(hipStream_t s, int na, int ne, int64_t nf, const int* rank_full, const cplx* Y1,
+                   const double* eps_act, int64_t m, int64_t* strings, int64_t nred, const int* tup, cplx* Psi,
+                   cplx* Psif);
 void scale_rows(hipStream_t s, int64_t m, int64_t ncol, const double* f, const cplx* in, cplx* out);
 // block Davidson helpers: diagonal of H, unit start vectors e_rows[j], Ritz residuals
 // R = AX - X diag(theta), diagonal preconditioner T[:, c] = R[:, sel_c] / (theta_sel_c - d)
@@ -149,6 +156,12 @@ void effh_C(hipStream_t s, int ni, int na, int ns, const cplx* hp, int ldh, cons
             const cplx* C3, cplx* effh);
 void effh_D(hipStream_t s, int ni, int na, int ns, const cplx* hp, int ldh, const cplx* D3b, cplx* effh);
 // D3 reshapes used by the A and C V-matrix GEMMs
+// the 3-RDM part of V_A / V_C from the E_qr|0> vectors (no 3-RDM): Wt[(x + nx t) + nx na I] =
+// sum_s <I|E_ts|Z_{x,s}> (sum_annihilation, V_C) or sum_s <I|E_st|Z_{x,s}> (V_A), Z_{x,s} at
+// Zt[x xs + s ss + ld J]; V[(v + na u + na2 t) + na3 x] = scale Out[(u + na v) + na2 (x + nx t)]
+void v3_esum(hipStream_t s, bool sum_annihilation, int na, int64_t nf, int nx, int64_t xs, int64_t ss, int64_t ld,
+             const int* tab, const cplx* Zt, cplx* Wt);
+void v3_permute(hipStream_t s, int na, int nx, double scale, const cplx* Out, cplx* V);
 void d3_perm_A(hipStream_t s, int na, const cplx* G3, cplx* Dm);
 void d3_perm_C(hipStream_t s, int na, const cplx* G3, cplx* Dm);
 void a1_perm(hipStream_t s, int na, int ni, const cplx* A1, cplx* Am);

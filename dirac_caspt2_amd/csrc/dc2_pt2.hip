@@ -170,6 +170,42 @@ __global__ void k_d3_perm_C(DV d, cplx* Dm) {
   const int j = (int)(col % na), k = (int)((col / na) % na), l = (int)(col / (na * na));
   Dm[row + na3 * col] = d.D3(v, u, t, j, k, l);
 }
+// The 3-RDM part of V_A and V_C without the 3-RDM.  With Y1[:, q + na r] = E_qr|0> over the
+// full string space, <E_vu E_ts E_qr> = Y1[:, u + na v]^H E_ts Y1[:, q + na r], so the sum
+// over (q, r) against an integral block is one product Zt = M Y1^T, the sum over the
+// remaining index s goes through the excitation table (this kernel), and the contraction
+// with <0|E_vu one more product (v3_permute_add).
+//   SUMANN (V_C, sum_j <I|E_tj|Z_j>): the annihilation index s is summed, the image of I
+//     under E_st is read;
+//   !SUMANN (V_A, sum_p <I|E_pt|Z_p>): the creation index s is summed, E_ts's image.
+// Wt[(x + nx t) + nx na I] = sum_s sign * Zt[x xs + s ss + ld J], J = that image of I.
+template <bool SUMANN>
+__global__ void k_v3_esum(int na, int64_t nf, int nx, int64_t xs, int64_t ss, int64_t ld, const int* __restrict__ tab,
+                          const cplx* __restrict__ Zt, cplx* __restrict__ Wt) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)nx * nf) return;
+  const int x = (int)(g % nx), t = blockIdx.y;
+  const int64_t I = g / nx;
+  cplx acc = mk(0.0, 0.0);
+  for (int sidx = 0; sidx < na; ++sidx) {
+    const int e = tab[(int64_t)(SUMANN ? sidx + na * t : t + na * sidx) * nf + I];
+    if (e < 0) continue;
+    const cplx v = Zt[(int64_t)x * xs + (int64_t)sidx * ss + ld * (int64_t)(e >> 1)];
+    acc = (e & 1) ? csub(acc, v) : cadd(acc, v);
+  }
+  Wt[(int64_t)x + (int64_t)nx * t + (int64_t)nx * na * I] = acc;
+}
+
+// V[(v + na u + na2 t) + na3 x] = scale * Out[(u + na v) + na2 (x + nx t)]
+__global__ void k_v3_permute(int na, int nx, double scale, const cplx* __restrict__ Out, cplx* __restrict__ V) {
+  const int64_t na3 = (int64_t)na * na * na;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
+  if (row >= na3) return;
+  const int v = (int)(row % na), u = (int)((row / na) % na), t = (int)(row / (na * na));
+  const cplx o = Out[(int64_t)(u + na * v) + (int64_t)na * na * (x + (int64_t)nx * t)];
+  V[row + na3 * x] = mk(scale * o.x, scale * o.y);
+}
+
 // Am[(p + na q + na2 r) + na3 i] = A1[p + na (i + ni (q + na r))],  A1 = (a,i|a,a)
 __global__ void k_a1_perm(int na, int ni, const cplx* A1, cplx* Am) {
   const int64_t na3 = (int64_t)na * na * na;
@@ -530,6 +566,23 @@ void d3_perm_C(hipStream_t s, int na, const cplx* G3, cplx* Dm) {
   Dens d{na, nullptr, nullptr, nullptr, nullptr, G3, nullptr};
   const int64_t na3 = (int64_t)na * na * na;
   hipLaunchKernelGGL(k_d3_perm_C, dim3(cdiv(na3, 256), (unsigned)na3), dim3(256), 0, s, dv(d), Dm);
+  DC2_HIP(hipGetLastError());
+}
+void v3_esum(hipStream_t s, bool sum_annihilation, int na, int64_t nf, int nx, int64_t xs, int64_t ss, int64_t ld,
+             const int* tab, const cplx* Zt, cplx* Wt) {
+  if (nx <= 0 || nf <= 0) return;
+  const dim3 grid((unsigned)cdiv((int64_t)nx * nf, 256), (unsigned)na);
+  if (sum_annihilation)
+    hipLaunchKernelGGL(k_v3_esum<true>, grid, dim3(256), 0, s, na, nf, nx, xs, ss, ld, tab, Zt, Wt);
+  else
+    hipLaunchKernelGGL(k_v3_esum<false>, grid, dim3(256), 0, s, na, nf, nx, xs, ss, ld, tab, Zt, Wt);
+  DC2_HIP(hipGetLastError());
+}
+void v3_permute(hipStream_t s, int na, int nx, double scale, const cplx* Out, cplx* V) {
+  const int64_t na3 = (int64_t)na * na * na;
+  if (nx <= 0) return;
+  DC2_CHECK(nx < 65536, 1, "v3_permute: too many columns");
+  hipLaunchKernelGGL(k_v3_permute, dim3(cdiv(na3, 256), (unsigned)nx), dim3(256), 0, s, na, nx, scale, Out, V);
   DC2_HIP(hipGetLastError());
 }
 void a1_perm(hipStream_t s, int na, int ni, const cplx* A1, cplx* Am) {

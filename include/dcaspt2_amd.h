@@ -182,16 +182,22 @@ int dc2_gemm_clear_choices(int64_t* count_before);
  * (products routed to the kernels for skinny shapes are counted by dc2_get_gemm_stats only) */
 int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
 /* Numerical options (no counterpart in the reference):
- *   "eigensolver": -1 automatic (default: 4 for 3 <= n <= 1024 outside the batched server,
- *                  rocSOLVER divide and conquer otherwise, with Jacobi leaves for the
- *                  rank-deficient nact^3 S matrices), 0 tridiagonal QR (the ZHEEV
- *                  algorithm), 1 divide and conquer, 2 D&C with Jacobi leaves, 3 Jacobi,
- *                  4 this library's Householder tridiagonalisation (one persistent kernel)
- *                  + rocSOLVER dstedc + its back-transformation (n <= 1024; D&C otherwise);
- *   "ac_metric":   0 automatic (default: the smallest exact basis), 1 the RDM-formula S/B
- *                  matrices of size nact^3, 2 the determinant-space factorisation of the A and
- *                  C metrics, 3 for A the formula matrices on the tuples t < u (a+_t a+_u is
- *                  antisymmetric, so these span the same space and S's eigenvalues halve);
+ *   "eigensolver": -1 automatic (default: matrices up to 32 on the host, 4 for 33 <= n <=
+ *                  native_max outside the batched server, rocSOLVER divide and conquer
+ *                  otherwise, with Jacobi leaves for rank-deficient S matrices), 0 tridiagonal
+ *                  QR (the ZHEEV algorithm), 1 divide and conquer, 2 D&C with Jacobi leaves,
+ *                  3 Jacobi, 4 this library's Householder tridiagonalisation (one persistent
+ *                  kernel) + its tridiagonal divide and conquer (or rocSOLVER dstedc, option
+ *                  tridiag_solver) + its back-transformation (n <= 2048; D&C otherwise);
+ *   "ac_metric":   0 automatic (default: the smallest exact basis; for A the tuples t < u
+ *                  through the determinant-space factor (4) when they are fewer than the
+ *                  (nelec+1)-electron strings), 1 the RDM-formula S/B matrices of size
+ *                  nact^3, 2 the determinant-space factorisation of the A and C metrics, 3 for
+ *                  A the formula matrices on the tuples t < u (a+_t a+_u is antisymmetric, so
+ *                  these span the same space and S's eigenvalues halve), 4 for A the same
+ *                  reduced matrices as products of the determinant-space factor (S = Phi^H
+ *                  Phi, B = Phi^H F Phi on the columns t < u; no F-weighted 3-RDM), for C the
+ *                  determinant-space factorisation;
  *   "lanes":       1..4 concurrent streams (each with a host thread) for the subspace
  *                  solves (default 4: A, C, D and the rest; 3: A, C and the rest);
  *   "graphs":      1 (default) block-Davidson iterations replayed from captured graphs

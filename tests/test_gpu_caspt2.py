@@ -36,7 +36,7 @@ def test_small_casci_and_dets(small):
         np.testing.assert_allclose(M, R.casmat(p, cas[0]), atol=1e-12, rtol=0)
 
 
-METRICS = ["formula", "determinant", "antisymmetric"]
+METRICS = ["formula", "determinant", "antisymmetric", "antisymmetric_det"]
 
 
 @pytest.mark.parametrize("metric", METRICS)
@@ -99,7 +99,8 @@ def medium():
 
 @pytest.mark.parametrize("metric,schur,lanes", [("formula", "on", 3), ("determinant", "on", 3),
                                                  ("determinant", "off", 1), ("determinant", "on", 1),
-                                                 ("antisymmetric", "on", 3), ("determinant", "on", 4),
+                                                 ("antisymmetric", "on", 3), ("antisymmetric_det", "on", 3),
+                                                 ("antisymmetric_det", "on", 1), ("determinant", "on", 4),
                                                  ("determinant", "on", 2)])
 def test_medium_caspt2(medium, metric, schur, lanes):
     """192 spinors (16/12/164, 6 electrons, 924 determinants); the retained dimensions of
