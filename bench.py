@@ -32,7 +32,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 # hardware queues per process for the concurrent slots of the served run (HIP reads this
-# when the runtime starts, i.e. at the first library call)
+# when the runtime starts, i.e. at the first library call), where the environment does not
+# set it already (4, HIP's own default, where it does: 4 to 32 gave the same served rate)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X dense FP64 (matrix and vector)
@@ -415,7 +416,8 @@ def hetrd_roofline(ctx, ht, steps, workload):
     ach = fl * calls / sec / 1e12
     rows = next(r for lim, r in ((128, 2), (256, 4), (512, 8), (768, 12), (832, 13), (1024, 16), (1280, 20),
                                  (1536, 24), (1792, 28), (2048, 32)) if n <= lim)
-    return {"kernel": f"k_hetrd<{rows},{2 if n <= 1024 else 1}> (Householder tridiagonalisation, n = {n}, one grid "
+    # two columns per wave up to n = 1024, and up to 1280 on the subspace lanes of an energy
+    return {"kernel": f"k_hetrd<{rows},{2 if n <= 1280 else 1}> (Householder tridiagonalisation, n = {n}, one grid "
                       f"barrier per column)",
             "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(workload, "k_hetrd", n),

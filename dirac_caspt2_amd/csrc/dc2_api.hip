@@ -174,6 +174,8 @@ struct dc2_context {
   // eigensolver: -1 automatic (divide and conquer; D&C with Jacobi leaves for the
   // rank-deficient nact^3 S matrices), else 0 QR (zheev), 1 D&C, 2 D&C + Jacobi, 3 Jacobi
   int eig_method = -1;
+  int lane_plan = -1;        // 4 lanes: 0 D alone on lane 3, 1 D with B and F (their S/B solves
+                             // run while the transforms do), -1 1 from 256 secondary spinors
   int ac_route = 0;          // A/C metric: 0 automatic, 1 RDM formulas (nact^3), 2 determinant space
   int nlanes = 4;            // concurrent subspace lanes (1 = sequential on the library stream)
   bool schur = true;         // E2 of the determinant-space route through the compressed resolvent
@@ -419,8 +421,10 @@ void zheevd(dc2_context* c, Lane& L, int n, cplx* A, double* w, const char* what
     // dstedc); rocSOLVER's D&C if it cannot run or its result is not usable
     void* work = L.d("eig_native", heev_native_work_bytes(n) / 8 + 1);
     L.info.ensure(2);
+    // on a worker lane other lanes' kernels run beside the reduction: it takes fewer units
+    const bool narrow = &L != c->lane0.get();
     bool done = heev_native(L.stream, L.blas, n, A, n, w, work, L.info.get(),
-                            reinterpret_cast<int*>(L.host(2 * sizeof(int), 1)), c->tri_own, c->debug);
+                            reinterpret_cast<int*>(L.host(2 * sizeof(int), 1)), c->tri_own, c->debug, narrow);
     bool ran = done, finite = false;
     if (done) {
       L.download(&info, L.info.get(), sizeof(int));
@@ -1436,14 +1440,17 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   const bool doF = ns > 0 && first <= 5, doG = ni > 0 && ns > 0 && first <= 6, doH = ni > 0 && ns > 0 && first <= 7;
   const bool do_sub[8] = {doA, doB, doC, doD, doE, doF, doG, doH};
   // lanes: 1 all on the library stream; 2: A and C on lane 1; 3: A on 1, C on 2; 4: D (the
-  // next largest) on lane 3 as well.  F stays with B, E, G and H: on lane 3 after D it made
-  // that lane the longest (28.7 against 26.4 ms per energy at 192 spinors)
+  // next largest) on lane 3 as well, with B and F when the secondary space is large: their
+  // S/B eigensolves then run beside the integral transforms instead of after them on the
+  // library stream (324 spinors, 280 secondary: 90.1 against 92.7 ms per energy; 192
+  // spinors, 164 secondary: 16.2 against 16.1 ms, where F on lane 3 made that lane the longest)
+  const bool bf_on_d = c->lane_plan == 1 || (c->lane_plan < 0 && ns >= 256);
   std::vector<std::vector<int>> plan(std::max(1, std::min(c->nlanes, 4)));
   for (int k = 0; k < 8; ++k) {
     if (!do_sub[k]) continue;
     int lane = 0;
     if (plan.size() >= 2 && (k == 0 || k == 2)) lane = k == 0 ? 1 : std::min<int>(2, (int)plan.size() - 1);
-    if (plan.size() >= 4 && k == 3) lane = 3;
+    if (plan.size() >= 4 && (k == 3 || (bf_on_d && (k == 1 || k == 5)))) lane = 3;
     plan[lane].push_back(k);
   }
   const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
@@ -2322,6 +2329,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "ac_metric") {
       DC2_CHECK(value >= 0 && value <= 4, 1, "ac_metric must be 0..4");
       c->ac_route = (int)value;
+    } else if (k == "lane_plan") {
+      DC2_CHECK(value >= -1 && value <= 1, 1, "lane_plan must be -1, 0 or 1");
+      c->lane_plan = (int)value;
     } else if (k == "lanes") {
       DC2_CHECK(value >= 1 && value <= 4, 1, "lanes must be 1..4");
       c->nlanes = (int)value;

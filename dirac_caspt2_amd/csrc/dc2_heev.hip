@@ -42,9 +42,14 @@ constexpr int HT_WAVES = 8;
 constexpr int HT_THREADS = 64 * HT_WAVES;
 constexpr int HT_NMAX = 2048;
 // columns per wave: 2 up to n = 1024 (4 spills registers at n > 768); 1 above, where the
-// 2048 / 8 = 256 workgroups of the largest matrix take every compute unit
-inline int hetrd_cpw(int n) { return n <= 1024 ? 2 : 1; }
-inline int hetrd_nwg(int n) { return (n + HT_WAVES * hetrd_cpw(n) - 1) / (HT_WAVES * hetrd_cpw(n)); }
+// 2048 / 8 = 256 workgroups of the largest matrix take every compute unit.  narrow: 2 up to
+// n = 1280 as well, half the workgroups (80 instead of 160 at n = 1274) at 19% more time
+// per reduction, for reductions that run beside other work (the subspace lanes of a CASPT2
+// energy: 88.0 against 90.4 ms per 324-spinor energy, whose transforms get the freed units)
+inline int hetrd_cpw(int n, bool narrow = false) { return (n <= 1024 || (narrow && n <= 1280)) ? 2 : 1; }
+inline int hetrd_nwg(int n, bool narrow = false) {
+  return (n + HT_WAVES * hetrd_cpw(n, narrow) - 1) / (HT_WAVES * hetrd_cpw(n, narrow));
+}
 
 // barrier counters: the top one at bar[0], group g's at bar[32 (g + 1)] (lines of their own)
 constexpr int HT_GROUPS = 16;
@@ -577,9 +582,9 @@ int64_t heev_native_work_bytes(int n) {
 }
 
 bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev,
-                 int* flags_host, bool own_tridiag, bool report) {
+                 int* flags_host, bool own_tridiag, bool report, bool narrow) {
   DC2_CHECK(n >= 3 && n <= HT_NMAX, 1, "heev_native: n must be in 3..2048");
-  const int nwg = hetrd_nwg(n);
+  const int nwg = hetrd_nwg(n, narrow);   // at most hetrd_nwg(n), which sizes the work area
   const int64_t nn = (int64_t)n * n;
   char* p = reinterpret_cast<char*>(work);
   auto take = [&](int64_t bytes) {
@@ -631,7 +636,9 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     else if (n <= 768) st = launch_hetrd<12>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 832) st = launch_hetrd<13>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1024) st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
-    else if (n <= 1280) st = launch_hetrd<20, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 1280)
+      st = hetrd_cpw(n, narrow) == 2 ? launch_hetrd<20, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh)
+                                     : launch_hetrd<20, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1536) st = launch_hetrd<24, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1792) st = launch_hetrd<28, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else st = launch_hetrd<32, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);

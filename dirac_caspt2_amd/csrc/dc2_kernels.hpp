@@ -122,10 +122,11 @@ void ingest_records(hipStream_t s, int ni, int na, int ns, int nblocks, const in
 // info_dev (own_tridiag: this library's divide and conquer, else rocSOLVER dstedc).  false:
 // nothing usable was computed (the caller falls back to rocSOLVER).  work: at least
 // heev_native_work_bytes(n) bytes of device memory; flags_host: two ints of page-locked host
-// memory for the reduction's status.
+// memory for the reduction's status.  narrow: fewer workgroups for the reduction (n in
+// 1025..1280) at some speed, for calls that run beside other work.
 int64_t heev_native_work_bytes(int n);
 bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, double* w, void* work, int* info_dev,
-                 int* flags_host, bool own_tridiag = true, bool report = false);
+                 int* flags_host, bool own_tridiag = true, bool report = false, bool narrow = false);
 
 // ---------------------------------------------------------------- dc2_tridc.hip
 // Eigenpairs of the real symmetric tridiagonal (d in w, off-diagonal e[0 .. n-2]), 1 <= n <= 2048,

@@ -200,6 +200,8 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  determinant-space factorisation;
  *   "lanes":       1..4 concurrent streams (each with a host thread) for the subspace
  *                  solves (default 4: A, C, D and the rest; 3: A, C and the rest);
+ *   "lane_plan":   with 4 lanes, -1 (default) B and F on D's lane from 256 secondary spinors,
+ *                  0 never, 1 always;
  *   "graphs":      1 (default) block-Davidson iterations replayed from captured graphs
  *                  (dc2_get_graph_stats), 0 every launch issued directly;
  *   "schur":       1 (default) E2 of the determinant-space route through the compressed

@@ -124,6 +124,18 @@ def test_medium_caspt2(medium, metric, schur, lanes):
                 assert out.info[X][1] == sum(b.dimm for _, b in blocks), X   # (dimn, dimm, npert, ran)
 
 
+def test_medium_lane_plan(medium):
+    """B and F on D's lane (the plan chosen from 256 secondary spinors) at 192 spinors."""
+    from gpu_helpers import assert_energies, context_from_synth, say
+    sp, ref = medium
+    say("medium: gpu, lane_plan 1")
+    with context_from_synth(sp) as ctx:
+        ctx.set_option("lane_plan", 1)
+        ctx.casci(1)
+        out = ctx.caspt2(1, 0.0)
+        assert_energies(out, ref, 1e-9, "medium lane_plan 1")
+
+
 def test_small_davidson(small):
     """Block Davidson CASCI (forced at 70 determinants) against the oracle: the 5 lowest
     CASCI energies within 1e-10 Eh, and CASPT2 for roots 1 and 2 within 1e-9 Eh."""
