@@ -176,6 +176,7 @@ struct dc2_context {
   int eig_method = -1;
   int lane_plan = -1;        // 4 lanes: 0 D alone on lane 3, 1 D with B and F (their S/B solves
                              // run while the transforms do), -1 1 from 256 secondary spinors
+  bool no_thin = false;      // option "c_thin" 0: C's metric always as the m x m Gram matrix
   int ac_route = 0;          // A/C metric: 0 automatic, 1 RDM formulas (nact^3), 2 determinant space
   int nlanes = 4;            // concurrent subspace lanes (1 = sequential on the library stream)
   bool schur = true;         // E2 of the determinant-space route through the compressed resolvent
@@ -919,10 +920,22 @@ SolveOut solve_finish(dc2_context* c, Lane& L, char X, const BlockPrep& p, int64
 struct GramMetric {
   bool valid = false;
   bool btilde = false;    // B~ and its eigendecomposition already in X_bc1 / X_wbt
+  bool thin = false;      // C through its thin factor: T's eigenvectors in X_Tthin, t L^-1/2 in X_TSthin
   int64_t m = 0, dimm = 0;
   double margin = INFINITY;
   double seconds = 0.0;   // time spent on the lane before the subspace itself ran
 };
+
+// C's metric through its thin factor (build_thin_c) when that is smaller than the
+// (nelec-1)-electron space and the compressed resolvent cannot be used anyway
+bool gram_thin(const dc2_context* c, char X, int64_t m) {
+  const int na = c->na, ne = c->ne;
+  if (X != 'C' || ne < 2 || na < 2 || c->no_thin) return false;
+  const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+  // the blocks of M M^T have the eigenvalue 2 + kappa (na - 1) along sigma: invertible only
+  const double kappa = (double)(na - 2 * (ne - 1)) / ((double)(ne - 1) * (ne - 1));
+  return nred < m && m - nred > e2_schur_max_dropped() && 2.0 + kappa * (na - 1) > 1e-6;
+}
 
 // Phi from the E_uv|0> vectors Y of the CASCI state in one orbital basis, G = Phi Phi^H,
 // its eigendecomposition (lane buffers X_G, X_lam) and the retained count (w >= 1e-8,
@@ -946,11 +959,39 @@ GramMetric gram_eig(dc2_context* c, Lane& L, char X, int64_t n, const cplx* Y, c
   double* f = L.d(nm("_f").c_str(), m);
   double* lam = L.d(nm("_lam").c_str(), m);
   build_phi(s, na, c->ne, create, c->nfull, c->rank_full.get(), Y, eps_act, m, strings, Phi, f);
-  zherk_upper(s, m, n, Phi, m, G, m);   // Hermitian: the upper tiles only
-  check_finite(c, s, "Phi Phi^H", G, m * m);
   // a RAS-restricted CI vector leaves directions of the (nelec+-1) space unreached: G is then
   // rank-deficient, where plain divide and conquer can return NaN eigenvectors
   const bool ras = c->ras[0] + c->ras[1] + c->ras[2] != 0;
+  const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+  if (gram_thin(c, X, m)) {
+    g.thin = true;
+    // G = Psi' Psi'^H with the thin factor Psi' (m x nred): the nonzero eigenvalues of G are
+    // those of T = Psi'^H Psi' (nred x nred), its eigenvectors Psi' t / sqrt(l); only the
+    // retained ones are formed (the compressed resolvent, which needs all of G, is not used
+    // with m - nred > e2_schur_max_dropped() directions cut)
+    cplx* Av = L.z(nm("_Av").c_str(), m * na);
+    cplx* PsiH = L.z(nm("_PsiH").c_str(), nred * m);
+    cplx* T = L.z(nm("_Tthin").c_str(), nred * nred);
+    double* lt = L.d(nm("_lthin").c_str(), nred);
+    build_thin_c(s, na, c->ne, m, Phi, Av, PsiH);
+    zherk_upper(s, nred, m, PsiH, nred, T, nred);
+    check_finite(c, s, "Psi'^H Psi'", T, nred * nred);
+    zheevd(c, L, (int)nred, T, lt, "a determinant-space metric", ras ? EigKind::rank_deficient : EigKind::general);
+    g.margin = std::min(g.margin, thr);   // the m - nred eigenvalues of G that vanish identically
+    g.dimm = count_retained(L, nred, lt, g.margin);
+    const int64_t nd = m - g.dimm;
+    fill_zero(s, G, m * nd);
+    if (nd > 0) DC2_HIP(hipMemsetAsync(lam, 0, nd * sizeof(double), s));
+    if (g.dimm > 0) {
+      cplx* TS = L.z(nm("_TSthin").c_str(), nred * g.dimm);
+      scale_cols_tail(s, nred, g.dimm, T, lt, TS);
+      zgemm(s, 'C', 'N', m, g.dimm, nred, one, PsiH, nred, 0, TS, nred, 0, zero, G + m * nd, m, 0, 1);
+      DC2_HIP(hipMemcpyAsync(lam + nd, lt + (nred - g.dimm), g.dimm * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    return g;
+  }
+  zherk_upper(s, m, n, Phi, m, G, m);   // Hermitian: the upper tiles only
+  check_finite(c, s, "Phi Phi^H", G, m * m);
   zheevd(c, L, (int)m, G, lam, "a determinant-space metric", ras ? EigKind::rank_deficient : EigKind::general);
   g.dimm = count_retained(L, m, lam, g.margin);
   return g;
@@ -979,10 +1020,12 @@ void gram_rotate(dc2_context* c, Lane& L, char X, const GramMetric& g) {
   int off[4] = {c->ni, c->ni, 0, 0}, nn[4] = {na, na, 1, 1};
   extract4(s, c->U.get(), ld, off, nn, Ua);
   tracic_ds(s, na, X == 'A' ? c->ne + 1 : c->ne - 1, m, strings, Ua, M);
+  // the cut directions are needed only by the compressed resolvent
+  const int64_t nd = m - g.dimm, c0 = (c->schur && nd <= e2_schur_max_dropped()) ? 0 : nd;
   set_gemm_tag(4);
-  zgemm(s, 'C', 'N', m, m, m, one, M, m, 0, G, m, 0, zero, T, m, 0, 1);
+  zgemm(s, 'C', 'N', m, m - c0, m, one, M, m, 0, G + m * c0, m, 0, zero, T, m, 0, 1);
   set_gemm_tag(0);
-  DC2_HIP(hipMemcpyAsync(G, T, m * m * sizeof(cplx), hipMemcpyDeviceToDevice, s));
+  DC2_HIP(hipMemcpyAsync(G + m * c0, T, m * (m - c0) * sizeof(cplx), hipMemcpyDeviceToDevice, s));
 }
 
 // Phi and f of the rotated state (unless the metric was computed from it) and GL = G_k L^-1
@@ -1058,12 +1101,26 @@ SolveOut solve_block_gram(dc2_context* c, Lane& L, char X, const GramMetric& g, 
   const cplx* GL = L.z(nm("_GL").c_str(), m * dimm);
   const int64_t nd = m - dimm;
   const cplx* Gk = G + m * nd;
-  cplx* P = L.z(nm("_P").c_str(), m * npert);
   cplx* Y = L.z(nm("_Y").c_str(), dimm * npert);
-  set_gemm_tag(4);
-  zgemm(s, 'N', 'N', m, npert, n, one, Phi, m, 0, V, n, 0, zero, P, m, 0, 1);
-  zgemm(s, 'C', 'N', dimm, npert, m, one, GL, m, 0, P, m, 0, zero, Y, dimm, 0, 1);
-  set_gemm_tag(0);
+  if (g.thin) {
+    // (U L^-1/2)^H V = L^-1/2 t^H (M M^T)^(-1/2) M V in the thin factor's space (no division
+    // by the small eigenvalues beyond the reference's own L^-1/2)
+    const int na = c->na;
+    const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+    cplx* MV = L.z(nm("_MVthin").c_str(), nred * npert);
+    cplx* W = L.z(nm("_Wthin").c_str(), nred * npert);
+    const cplx* TS = L.z(nm("_TSthin").c_str(), nred * dimm);
+    thin_c_w(s, na, c->ne, npert, V, MV, W);
+    set_gemm_tag(4);
+    zgemm(s, 'C', 'N', dimm, npert, nred, one, TS, nred, 0, W, nred, 0, zero, Y, dimm, 0, 1);
+    set_gemm_tag(0);
+  } else {
+    cplx* P = L.z(nm("_P").c_str(), m * npert);
+    set_gemm_tag(4);
+    zgemm(s, 'N', 'N', m, npert, n, one, Phi, m, 0, V, n, 0, zero, P, m, 0, 1);
+    zgemm(s, 'C', 'N', dimm, npert, m, one, GL, m, 0, P, m, 0, zero, Y, dimm, 0, 1);
+    set_gemm_tag(0);
+  }
   if (c->schur && nd <= e2_schur_max_dropped()) {
     cplx* Z = L.z(nm("_Z").c_str(), m * npert);
     set_gemm_tag(4);
@@ -1482,10 +1539,12 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
         auto gram_k = [&](int k) { return (k == 0 || k == 2) && ac_route_of(c, kSub[k]).gram; };
         // the original-basis metric describes the rotated state only when the CASCI space is
         // closed under the active rotation (no RAS limits); with RAS limits tracic projects
-        // the rotated state back onto the restricted space, and the metric waits for it
+        // the rotated state back onto the restricted space, and the metric waits for it.  C's
+        // thin factor is built from the rotated state (its eigenvectors t live in the tuple
+        // basis, which the rotation mixes)
         if (!ras)
           for (int k : plan[w]) {
-            if (!gram_k(k)) continue;
+            if (!gram_k(k) || gram_thin(c, kSub[k], binom_host(na, c->ne - 1))) continue;
             Timer tm(Lw.stream);
             prep[k].g = gram_eig(c, Lw, kSub[k], n3, c->Y1o.get(), zeros_act);
             prep[k].seconds = tm.stop();
@@ -2329,6 +2388,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "ac_metric") {
       DC2_CHECK(value >= 0 && value <= 4, 1, "ac_metric must be 0..4");
       c->ac_route = (int)value;
+    } else if (k == "c_thin") {
+      DC2_CHECK(value == 0 || value == 1, 1, "c_thin must be 0 or 1");
+      c->no_thin = value == 0;
     } else if (k == "lane_plan") {
       DC2_CHECK(value >= -1 && value <= 1, 1, "lane_plan must be -1, 0 or 1");
       c->lane_plan = (int)value;

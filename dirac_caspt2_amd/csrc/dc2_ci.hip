@@ -632,6 +632,122 @@ __global__ void k_phi_adj(int na, int64_t m, const int64_t* __restrict__ strings
   }
 }
 
+// pair index p = w (w - 1) / 2 + s of s < w, and its inverse
+__device__ inline int thin_pair(int a, int b) {
+  const int s = a < b ? a : b, w = a < b ? b : a;
+  return w * (w - 1) / 2 + s;
+}
+__device__ inline void thin_unpair(int p, int& s, int& w) {
+  w = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+  while (w * (w - 1) / 2 > p) --w;
+  while ((w + 1) * w / 2 <= p) ++w;
+  s = p - w * (w - 1) / 2;
+}
+
+// A_v[r] = <r| a_v |0> from the C factor: sum_t a_t a+_t a_v |0> = (na - ne + 1) a_v |0> on the
+// (ne-1)-electron strings, Phi[r + m (t na na + u na + v)] = <r| a_t a+_u a_v |0>.
+__global__ void k_thin_avec(int na, int ne, int64_t m, const cplx* __restrict__ Phi, cplx* __restrict__ Av) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const int v = blockIdx.y;
+  const int64_t na2 = (int64_t)na * na;
+  cplx acc = mk(0.0, 0.0);
+  for (int t = 0; t < na; ++t) acc = cadd(acc, Phi[r + m * (t * na2 + (int64_t)t * na + v)]);
+  const double inv = 1.0 / (double)(na - ne + 1);
+  Av[r + m * v] = mk(acc.x * inv, acc.y * inv);
+}
+
+// Thin factor of the C metric G = Phi Phi^H, stored adjoint: with Psi_{u,(s,w)} = a+_u a_s a_w |0>
+// (s < w), Phi = Psi M for a fixed real M, and M M^T = 2 I + kappa sum_v sigma_v sigma_v^T,
+// kappa = (na - 2 (ne-1)) / (ne-1)^2, over disjoint blocks (the columns (u, (s, w)) with u in
+// {s, w}, v the other index, sigma = +1 for u < v, -1 for u > v).  Its square root gives
+// Psi' = Psi (M M^T)^(1/2) with Psi' Psi'^H = G:
+//   Psi'_{u,(s,w)} = sqrt2 (Phi(w,u,s) - [w = u] A_s) + [u = s] beta (ne-1) A_w - [u = w] beta (ne-1) A_s,
+//   beta = (sqrt(2 + kappa (na-1)) - sqrt2) / (na-1),
+// since Phi(w,u,s) = [w = u] a_s|0> + Psi_{u,(s,w)} and sum_x sigma_x Psi_{x,(x,v)} = (ne-1) a_v|0>.
+// Column j = u * npair + p, pair p = w (w - 1) / 2 + s (s < w); PsiH[j + nred r] = conj(Psi'[r, j]).
+__global__ void k_thin_c(int na, int ne, int64_t m, const cplx* __restrict__ Phi, const cplx* __restrict__ Av,
+                         int64_t nred, cplx* __restrict__ PsiH) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nred) return;
+  const int npair = na * (na - 1) / 2;
+  const int u = (int)(j / npair);
+  int sidx, w;
+  thin_unpair((int)(j % npair), sidx, w);
+  const int64_t na2 = (int64_t)na * na;
+  const double r2 = sqrt(2.0);
+  const double kappa = (double)(na - 2 * (ne - 1)) / ((double)(ne - 1) * (ne - 1));
+  const double beta = (sqrt(2.0 + kappa * (na - 1)) - r2) / (double)(na - 1) * (double)(ne - 1);
+  const int64_t col = (int64_t)w * na2 + (int64_t)u * na + sidx;
+  for (int64_t r = blockIdx.y; r < m; r += gridDim.y) {
+    cplx x = Phi[r + m * col];
+    if (w == u) x = csub(x, Av[r + m * sidx]);
+    x = mk(r2 * x.x, r2 * x.y);
+    if (u == sidx) cfma(x, mk(beta, 0.0), Av[r + m * w]);
+    if (u == w) cfma(x, mk(-beta, 0.0), Av[r + m * sidx]);
+    PsiH[j + nred * r] = cconj(x);
+  }
+}
+
+// MV = M V for the thin C factor (Phi = Psi M), V (na^3 x npert, row t na^2 + u na + v):
+//   (M V)_{u,(s,w)} = V(w,u,s) - V(s,u,w) + [u in {s,w}] sigma/(ne-1) sum_t V(t,t,v),
+// v the other pair index, sigma = +1 for u = s (u < v), -1 for u = w.
+__global__ void k_thin_mv(int na, int ne, int64_t npert, const cplx* __restrict__ V, int64_t nred,
+                          cplx* __restrict__ MV) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nred) return;
+  const int npair = na * (na - 1) / 2;
+  const int u = (int)(j / npair);
+  int sidx, w;
+  thin_unpair((int)(j % npair), sidx, w);
+  const int64_t na2 = (int64_t)na * na, na3 = na2 * na;
+  const int sg = u == sidx ? 1 : (u == w ? -1 : 0);
+  const int v = u == sidx ? w : sidx;
+  const double inv = 1.0 / (double)(ne - 1);
+  for (int64_t p = blockIdx.y; p < npert; p += gridDim.y) {
+    const cplx* Vp = V + na3 * p;
+    cplx x = csub(Vp[(int64_t)w * na2 + (int64_t)u * na + sidx], Vp[(int64_t)sidx * na2 + (int64_t)u * na + w]);
+    if (sg) {
+      cplx tr = mk(0.0, 0.0);
+      for (int t = 0; t < na; ++t) tr = cadd(tr, Vp[(int64_t)t * na2 + (int64_t)t * na + v]);
+      cfma(x, mk(sg * inv, 0.0), tr);
+    }
+    MV[j + nred * p] = x;
+  }
+}
+
+// W = (M M^T)^(-1/2) M V: 2^(-1/2) on the columns outside the blocks; on block v
+// (2 I + kappa sigma sigma^T)^(-1/2) = 2^(-1/2) I + gamma sigma sigma^T,
+// gamma = ((2 + kappa (na-1))^(-1/2) - 2^(-1/2)) / (na-1).
+__global__ void k_thin_w(int na, int ne, int64_t npert, const cplx* __restrict__ MV, int64_t nred,
+                         cplx* __restrict__ W) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nred) return;
+  const int npair = na * (na - 1) / 2;
+  const int u = (int)(j / npair);
+  int sidx, w;
+  thin_unpair((int)(j % npair), sidx, w);
+  const int sg = u == sidx ? 1 : (u == w ? -1 : 0);
+  const int v = u == sidx ? w : sidx;
+  const double kappa = (double)(na - 2 * (ne - 1)) / ((double)(ne - 1) * (ne - 1));
+  const double r2 = 1.0 / sqrt(2.0);
+  const double gam = (1.0 / sqrt(2.0 + kappa * (na - 1)) - r2) / (double)(na - 1);
+  for (int64_t p = blockIdx.y; p < npert; p += gridDim.y) {
+    const cplx* M = MV + nred * p;
+    cplx x = mk(r2 * M[j].x, r2 * M[j].y);
+    if (sg) {
+      cplx b = mk(0.0, 0.0);
+      for (int y = 0; y < na; ++y) {
+        if (y == v) continue;
+        const cplx e = M[(int64_t)y * npair + thin_pair(y, v)];
+        b = y < v ? cadd(b, e) : csub(b, e);
+      }
+      cfma(x, mk(gam * sg, 0.0), b);
+    }
+    W[j + nred * p] = x;
+  }
+}
+
 // f[r] = sum_w eps_w n_w(strings[r])
 __global__ void k_fdet(int na, int64_t m, const int64_t* strings, const double* eps_act, double* f) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -866,6 +982,25 @@ void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int
   hipLaunchKernelGGL(k_phi, dim3(cdiv(m, 256), na, na * na), dim3(256), 0, s, na, create ? 1 : 0, m, strings, nf,
                      rank_full, Y1, Phi);
   hipLaunchKernelGGL(k_fdet, dim3(cdiv(m, 256)), dim3(256), 0, s, na, m, strings, eps_act, fdiag);
+  DC2_HIP(hipGetLastError());
+}
+
+void build_thin_c(hipStream_t s, int na, int ne, int64_t m, const cplx* Phi, cplx* Av, cplx* PsiH) {
+  if (m == 0) return;
+  DC2_CHECK(na >= 2 && ne >= 2 && ne <= na, 1, "build_thin_c: needs 2 <= nelec <= nact");
+  const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+  hipLaunchKernelGGL(k_thin_avec, dim3(cdiv(m, 256), na), dim3(256), 0, s, na, ne, m, Phi, Av);
+  hipLaunchKernelGGL(k_thin_c, dim3(cdiv(nred, 256), (unsigned)std::min<int64_t>(m, 65535)), dim3(256), 0, s, na, ne,
+                     m, Phi, Av, nred, PsiH);
+  DC2_HIP(hipGetLastError());
+}
+
+void thin_c_w(hipStream_t s, int na, int ne, int64_t npert, const cplx* V, cplx* MV, cplx* W) {
+  const int64_t nred = (int64_t)na * na * (na - 1) / 2;
+  if (npert <= 0 || nred == 0) return;
+  const dim3 grid((unsigned)cdiv(nred, 256), (unsigned)std::min<int64_t>(npert, 65535));
+  hipLaunchKernelGGL(k_thin_mv, grid, dim3(256), 0, s, na, ne, npert, V, nred, MV);
+  hipLaunchKernelGGL(k_thin_w, grid, dim3(256), 0, s, na, ne, npert, MV, nred, W);
   DC2_HIP(hipGetLastError());
 }
 

@@ -70,6 +70,13 @@ int64_t binom_host(int n, int k);
 // S = Phi^H Phi and B = Phi^H diag(fdiag) Phi with fdiag[r] = sum_w eps_w n_w(r)
 void build_phi(hipStream_t s, int na, int ne, bool create, int64_t nf, const int* rank_full, const cplx* Y1,
                const double* eps_act, int64_t m, int64_t* strings, cplx* Phi, double* fdiag);
+// Thin factor of the C metric G = Phi Phi^H (Phi of build_phi, m x na^3, ne electrons in the
+// CASCI state): PsiH (nred x m, nred = na*na*(na-1)/2) with PsiH^H PsiH = G; Av: m x na scratch
+void build_thin_c(hipStream_t s, int na, int ne, int64_t m, const cplx* Phi, cplx* Av, cplx* PsiH);
+// W = (M M^T)^(-1/2) M V (nred x npert) for V (na^3 x npert) with Phi = Psi M of build_thin_c:
+// with T t = l t (T = PsiH PsiH^H), u = M^T (M M^T)^(-1/2) t are the orthonormal eigenvectors
+// of the tuple-space metric Phi^H Phi, so (U L^-1/2)^H V = L^-1/2 t^H W.  MV: nred x npert scratch
+void thin_c_w(hipStream_t s, int na, int ne, int64_t npert, const cplx* V, cplx* MV, cplx* W);
 // The antisymmetric reduction of the A factor in adjoint layout: the (ne+1)-electron strings,
 // Psi (nred x m) with Psi[k + nred*r] = conj(<r|a+_t E_uv|0>) for the tuple tup[k] = t*na*na +
 // u*na + v (t < u, nred = na*na*(na-1)/2 of them) and Psif = Psi diag(f), so that the reduced

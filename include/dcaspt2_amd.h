@@ -202,6 +202,9 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  solves (default 4: A, C, D and the rest; 3: A, C and the rest);
  *   "lane_plan":   with 4 lanes, -1 (default) B and F on D's lane from 256 secondary spinors,
  *                  0 never, 1 always;
+ *   "c_thin":      1 (default) the determinant-space C metric through its thin factor (an
+ *                  eigenproblem of nact^2 (nact-1)/2 instead of C(nact, nelec-1)) when that is
+ *                  smaller and more than 32 directions are cut anyway, 0 never;
  *   "graphs":      1 (default) block-Davidson iterations replayed from captured graphs
  *                  (dc2_get_graph_stats), 0 every launch issued directly;
  *   "schur":       1 (default) E2 of the determinant-space route through the compressed

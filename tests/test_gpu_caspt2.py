@@ -136,6 +136,29 @@ def test_medium_lane_plan(medium):
         assert_energies(out, ref, 1e-9, "medium lane_plan 1")
 
 
+def test_c_metric_thin_factor():
+    """14 active spinors with 6 electrons: C's metric through its thin factor (an eigenproblem
+    of 14*14*13/2 = 1274 instead of C(14, 5) = 2002) gives the energies of the full Gram
+    route (itself checked against the oracle) to 1e-10 Eh, with the same retained count."""
+    from dirac_caspt2_amd import synth
+    from gpu_helpers import context_from_synth, say
+    sp = synth.generate(synth.SynthSpec(ninact=4, nact=14, nsec=40, nelec=6, seed=7))
+    outs = {}
+    for thin in ("on", "off"):
+        say("14 active, 6 electrons: c_thin", thin)
+        with context_from_synth(sp) as ctx:
+            ctx.set_option("c_thin", thin)
+            ctx.casci(1)
+            outs[thin] = ctx.caspt2(1, 0.0)
+    a, b = outs["on"], outs["off"]
+    say("thin", a.e2, a.info["C"], "full", b.e2, b.info["C"])
+    assert a.info["C"][1] == b.info["C"][1]
+    for X in "ABCDEFGH":
+        assert abs(a.e2[X] - b.e2[X]) < 1e-10, X
+        assert abs(a.sumc2[X] - b.sumc2[X]) < 1e-10, X
+    assert abs(a.total - b.total) < 1e-10
+
+
 def test_small_davidson(small):
     """Block Davidson CASCI (forced at 70 determinants) against the oracle: the 5 lowest
     CASCI energies within 1e-10 Eh, and CASPT2 for roots 1 and 2 within 1e-9 Eh."""
