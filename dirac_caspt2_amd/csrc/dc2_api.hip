@@ -1400,11 +1400,14 @@ Lane& worker_lane(dc2_context* c, int i) {
     auto L = std::make_unique<Lane>();
     L->owned = true;
     DC2_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
-    DC2_ROCSOLVER(rocblas_create_handle(&L->blas));
-    DC2_ROCSOLVER(rocblas_set_stream(L->blas, L->stream));
     c->lanes.push_back(std::move(L));
   }
-  return *c->lanes[i - 1];
+  Lane& L = *c->lanes[i - 1];
+  if (!L.blas) {   // streams made at context creation get their handle here
+    DC2_ROCSOLVER(rocblas_create_handle(&L.blas));
+    DC2_ROCSOLVER(rocblas_set_stream(L.blas, L.stream));
+  }
+  return L;
 }
 
 // first: subspaces before it (0 = A .. 8 = none computed) are skipped, as with a caspt2_restart
@@ -1797,6 +1800,45 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
 }  // namespace
 
 namespace dc2 {
+// A context whose worker-lane streams (lanes - 1 of them) are made right after its own, before
+// anything uses the null stream (creating a rocBLAS handle does): HIP spreads streams over its
+// hardware queues in creation order, so with GPU_MAX_HW_QUEUES = 4 (HIP's default) the library
+// stream and three lanes get a queue each, and the null stream shares one.  Two lanes on one
+// queue wait for each other's stream waits and reductions (the C chain waited behind D's).
+int context_create_lanes(int32_t device, int lanes, dc2_context** out) {
+  return guarded([&] {
+    DC2_CHECK(out != nullptr, 1, "null output pointer");
+    DC2_CHECK(lanes >= 1 && lanes <= 4, 1, "lanes must be 1..4");
+    auto c = std::make_unique<dc2_context>();
+    c->device = device;
+    c->nlanes = lanes;
+    const char* dbg = std::getenv("DC2_DEBUG");
+    c->debug = dbg && dbg[0] && dbg[0] != '0';
+    if (const char* em = std::getenv("DC2_EIGEN")) {
+      const std::string m(em);
+      c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : m == "native" ? 4 : -1;
+    }
+    if (const char* r = std::getenv("DC2_AC_METRIC")) {
+      const std::string m(r);
+      c->ac_route = m == "formula" ? 1 : m == "determinant" ? 2 : 0;
+    }
+    DC2_HIP(hipSetDevice(device));
+    DC2_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int i = 1; i < lanes; ++i) {
+      auto L = std::make_unique<Lane>();
+      L->owned = true;
+      DC2_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+      c->lanes.push_back(std::move(L));
+    }
+    DC2_ROCSOLVER(rocblas_create_handle(&c->blas));
+    DC2_ROCSOLVER(rocblas_set_stream(c->blas, c->stream));
+    c->lane0 = std::make_unique<Lane>();
+    c->lane0->stream = c->stream;
+    c->lane0->blas = c->blas;
+    *out = c.release();
+  });
+}
+
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 void set_batcher(dc2_context* c, EigBatcher* b) { c->batcher = b; }
 int run_guarded(const std::function<void()>& f) { return guarded(f); }
@@ -1811,30 +1853,9 @@ const char* dc2_last_error(void) { return g_last_error.c_str(); }
 const char* dc2_version(void) { return "dcaspt2_amd 0.1 (gfx950)"; }
 
 int dc2_context_create(int32_t device, dc2_context** out) {
-  return guarded([&] {
-    DC2_CHECK(out != nullptr, 1, "null output pointer");
-    auto c = std::make_unique<dc2_context>();
-    c->device = device;
-    const char* dbg = std::getenv("DC2_DEBUG");
-    c->debug = dbg && dbg[0] && dbg[0] != '0';
-    if (const char* em = std::getenv("DC2_EIGEN")) {
-      const std::string m(em);
-      c->eig_method = m == "qr" ? 0 : m == "dc" ? 1 : m == "dcj" ? 2 : m == "jacobi" ? 3 : m == "native" ? 4 : -1;
-    }
-    if (const char* r = std::getenv("DC2_AC_METRIC")) {
-      const std::string m(r);
-      c->ac_route = m == "formula" ? 1 : m == "determinant" ? 2 : 0;
-    }
-    DC2_HIP(hipSetDevice(device));
-    DC2_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    DC2_ROCSOLVER(rocblas_create_handle(&c->blas));
-    DC2_ROCSOLVER(rocblas_set_stream(c->blas, c->stream));
-    c->lane0 = std::make_unique<Lane>();
-    c->lane0->stream = c->stream;
-    c->lane0->blas = c->blas;
-    if (const char* nl = std::getenv("DC2_LANES")) c->nlanes = std::max(1, std::min(4, std::atoi(nl)));
-    *out = c.release();
-  });
+  int lanes = 4;
+  if (const char* nl = std::getenv("DC2_LANES")) lanes = std::max(1, std::min(4, std::atoi(nl)));
+  return dc2::context_create_lanes(device, lanes, out);
 }
 
 int dc2_context_destroy(dc2_context* ctx) {

@@ -37,6 +37,7 @@
 namespace dc2 {
 void set_last_error(const std::string& msg);   // dc2_api.hip: the message dc2_last_error() returns
 void set_batcher(dc2_context* c, EigBatcher* b);   // dc2_api.hip: route the context's eigensolves
+int context_create_lanes(int32_t device, int lanes, dc2_context** out);   // dc2_api.hip: with lane streams
 
 // ---------------------------------------------------------------- EigBatcher
 EigBatcher::EigBatcher(int device, int max_batch, double window_s, int executors)
@@ -344,8 +345,7 @@ int renew_context(dc2_server* S, int k) {
   S->slot_ctx[k] = nullptr;
   S->slot_mol[k] = nullptr;
   dc2_context* c = nullptr;
-  int st = dc2_context_create(S->device, &c);
-  if (!st) st = dc2_set_option(c, "lanes", S->lanes);
+  int st = dc2::context_create_lanes(S->device, S->lanes, &c);
   if (st) {
     if (c) dc2_context_destroy(c);
     return st;
@@ -455,8 +455,7 @@ int dc2_server_create_batched(int32_t device, int32_t nslots, int64_t mem_budget
   }
   for (int k = 0; k < nslots; ++k) {
     dc2_context* c = nullptr;
-    int st = dc2_context_create(device, &c);
-    if (!st) st = dc2_set_option(c, "lanes", lanes_per_slot);
+    int st = dc2::context_create_lanes(device, lanes_per_slot, &c);
     if (st) {
       const std::string msg = dc2_last_error();
       if (c) dc2_context_destroy(c);
@@ -495,7 +494,7 @@ int dc2_server_add_molecule(dc2_server* S, int32_t ninact, int32_t nact, int32_t
   if (!S || !B || !mol_id || naux < 1) return fail(1, "dc2_server_add_molecule: invalid argument");
   auto m = std::make_unique<Molecule>();
   m->ni = ninact; m->na = nact; m->ns = nsec; m->ne = nelec;
-  int st = dc2_context_create(S->device, &m->ctx);
+  int st = dc2::context_create_lanes(S->device, 1, &m->ctx);   // holds the blocks only
   if (!st) st = dc2_set_spaces(m->ctx, ninact, nact, nsec, nelec);
   if (!st) st = dc2_set_integral_factors(m->ctx, B, naux, 0);
   if (st) {
