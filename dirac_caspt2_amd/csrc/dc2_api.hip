@@ -195,6 +195,7 @@ struct dc2_context {
   // when the eigensolves are batched across requests (0: never)
   int native_min = 0;
   int dav_maxsub = 0;   // block-Davidson subspace before a thick restart (0: max(6 b, 24))
+  int dav_block = 0;    // block-Davidson block size (0: max(nroot + 3, 4))
   int native_max = 2048;   // largest eigenproblem this library's eigensolver takes
   // the tridiagonal stage of this library's eigensolver: its own divide and conquer (true) or
   // rocSOLVER dstedc
@@ -516,7 +517,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   // iterations, against 19 with 6 / 48, whose larger host eigenproblems cost more); from 1500
   // determinants 12 b: on 3003 determinants 48 iterations and 13.8 ms against 107 and 22.6 ms
   // with 24 (96: 38 iterations, 22.6 ms)
-  const int b = (int)std::min<int64_t>(n, std::max(nroot + 3, 4));
+  const int b = (int)std::min<int64_t>(n, std::max(nroot + 3, c->dav_block > 0 ? c->dav_block : 4));
   const int auto_sub = n >= 1500 ? 12 * b : std::max(6 * b, 24);
   const int maxsub = (int)std::min<int64_t>(n, c->dav_maxsub > 0 ? std::max(c->dav_maxsub, 2 * b) : auto_sub);
   if (maxsub < 2 * b) return false;
@@ -2433,6 +2434,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "split_min_k") {
       DC2_CHECK(value >= 1 && value <= 4096, 1, "split_min_k must be in 1..4096");
       c->split_min_k = (int)value;
+    } else if (k == "davidson_block") {
+      DC2_CHECK(value >= 0 && value <= 64, 1, "davidson_block must be in 0..64");
+      c->dav_block = (int)value;
     } else if (k == "davidson_maxsub") {
       DC2_CHECK(value >= 0 && value <= 512, 1, "davidson_maxsub must be in 0..512");
       c->dav_maxsub = (int)value;

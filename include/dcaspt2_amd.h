@@ -216,6 +216,9 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *   "davidson_maxsub": vectors of the block Davidson's subspace before a thick restart
  *                  (0: automatic, max(6 b, 24) for a block of b vectors, 12 b from 1500
  *                  determinants);
+ *   "davidson_block": vectors per block (0: automatic, max(nroot + 3, 4); at 3003
+ *                  determinants 4 with 48 subspace vectors was fastest, 13.1 ms against 14.5
+ *                  to 26 ms with blocks of 6 to 12);
  *   "native_max":  largest n this library's eigensolver takes (default 2048, at most 2048);
  *                  rocSOLVER's divide and conquer above;
  *   "native_min":  n >= this (and n within this library's eigensolver, 3..2048) uses this
