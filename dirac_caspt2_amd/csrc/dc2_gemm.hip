@@ -320,6 +320,63 @@ __global__ __launch_bounds__(256) void k_zgemm_dot(int64_t M, int64_t N, int64_t
   }
 }
 
+// The same products with very few outputs (the block Davidson's Rayleigh columns and
+// Gram-Schmidt coefficients: at most a few hundred rows over K = ndet): one workgroup per row m
+// and up to 4 columns, its 256 threads striding over K; the partial sums meet in a butterfly
+// per wave and then in wave order through LDS, so the result does not depend on timing.
+template <char OPA, char OPB>
+__global__ __launch_bounds__(256) void k_zgemm_dotwg(int64_t M, int64_t N, int64_t K, cplx alpha,
+                                                     const cplx* __restrict__ A, int64_t lda,
+                                                     const cplx* __restrict__ B, int64_t ldb, cplx beta,
+                                                     cplx* __restrict__ C, int64_t ldc) {
+  __shared__ cplx part[4][DOT_NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t w = blockIdx.x;
+  const int64_t m = w % M, n0 = (w / M) * DOT_NB;
+  const int nn = (int)(N - n0 < DOT_NB ? N - n0 : DOT_NB);
+  cplx acc[DOT_NB];
+#pragma unroll
+  for (int j = 0; j < DOT_NB; ++j) acc[j] = mk(0.0, 0.0);
+  const cplx* a = A + lda * m;
+  const cplx* b = B + ldb * n0;
+  for (int64_t k0 = tid; k0 < K; k0 += 256 * DOT_U) {
+    cplx x[DOT_U], y[DOT_U][DOT_NB];
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u) {
+      const int64_t k = k0 + 256 * u;
+      x[u] = k < K ? a[k] : mk(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < DOT_NB; ++j) y[u][j] = (k < K && j < nn) ? b[k + ldb * j] : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u) {
+      if (k0 + 256 * u >= K) break;
+      const cplx xv = OPA == 'C' ? cconj(x[u]) : x[u];
+#pragma unroll
+      for (int j = 0; j < DOT_NB; ++j)
+        if (j < nn) cfma(acc[j], xv, OPB == 'R' ? cconj(y[u][j]) : y[u][j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < DOT_NB; ++j) {
+    double re = acc[j].x, im = acc[j].y;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      re += __shfl_xor(re, off);
+      im += __shfl_xor(im, off);
+    }
+    if (lane == 0) part[wave][j] = mk(re, im);
+  }
+  __syncthreads();
+  if (tid < nn) {
+    cplx s = part[0][tid];
+#pragma unroll
+    for (int q = 1; q < 4; ++q) s = cadd(s, part[q][tid]);
+    cplx* c = C + m + ldc * (n0 + tid);
+    *c = out_value(s, alpha, beta, c);
+  }
+}
+
 // op(A) = A, op(B) = B or conj(B) with a short K: one thread per output element
 template <char OPB>
 __global__ __launch_bounds__(256) void k_zgemm_smallk(int64_t M, int64_t N, int64_t K, cplx alpha,
@@ -494,6 +551,11 @@ bool launch_skinny(hipStream_t s, int64_t M, int64_t N, int64_t K, cplx alpha, c
     // reads its column of A once; on 64 x 64 MFMA tiles N = 1 would use 1/64 of each tile)
     if (M * N > 2048 && !(N <= DOT_NB && K >= 256)) return false;
     const int64_t waves = M * ((N + DOT_NB - 1) / DOT_NB);
+    if (waves < 512 && K >= 1024) {   // too few rows to keep the device busy: a workgroup per row
+      hipLaunchKernelGGL((k_zgemm_dotwg<OPA, OPB>), dim3((unsigned)waves), dim3(256), 0, s, M, N, K, alpha, A,
+                         lda, B, ldb, beta, C, ldc);
+      return true;
+    }
     hipLaunchKernelGGL((k_zgemm_dot<OPA, OPB>), dim3((unsigned)cdiv(waves, 4)), dim3(256), 0, s, M, N, K, alpha, A,
                        lda, B, ldb, beta, C, ldc);
     return true;

@@ -35,6 +35,8 @@ def _op(x, op):
                                    (33, 9, 700),
                                    # matrix-vector products over a long K (one wave per row)
                                    (3003, 1, 3003), (2600, 4, 256),
+                                   # few rows over a long K (a workgroup per row)
+                                   (48, 4, 3003), (1, 1, 1024), (7, 3, 1500),
                                    # K <= 16 with one output dimension <= 16 (index rotations
                                    # of the transforms: the MFMA kernel)
                                    (4100, 16, 16), (4100, 5, 12), (12, 4100, 12), (7, 3000, 16), (16, 300, 9)])
