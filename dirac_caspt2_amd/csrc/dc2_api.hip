@@ -538,7 +538,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   // (R^H R, T^H T); uploads up_y (Ritz coefficients, theta), up_sc, up_idx, whose contents
   // stay untouched until the next synchronising download
   const size_t nCh = (size_t)maxsub * b, nrg = 2 * (size_t)b * b, ny = (size_t)maxsub * b + b, nsc = (size_t)b * b;
-  zc* const Ch = reinterpret_cast<zc*>(L.host((nCh + nrg + ny + nsc) * sizeof(zc) + b * sizeof(int)));
+  zc* const Ch = reinterpret_cast<zc*>(L.host((nCh + nrg + ny + nsc) * sizeof(zc) + (b + 1) * sizeof(int)));
   zc* const rg = Ch + nCh;
   zc* const up_y = rg + nrg;
   zc* const up_sc = up_y + ny;
@@ -566,24 +566,31 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   int* sel = L.i("dv_sel", b);
   to_dev(sel, seq.data(), b * sizeof(int));
   int m = b, nk = b;   // subspace size; the last nk columns are new
+  int ldc = b;         // rows of the downloaded V^H W_new block (Ch)
   const double tol = 1e-11;
   std::vector<zc> Th((size_t)maxsub * maxsub), Tm, Yh, G, Q;
   std::vector<double> theta, gw;
   double* th = reinterpret_cast<double*>(Sy + (size_t)maxsub * b);
+  // the block orthonormalisation on the device for blocks of four (one root), so that an
+  // iteration needs one host round trip; the host's, in a round trip of its own, otherwise
+  const bool dev_orth = b == 4;
+  int* nk_dev = L.i("dv_nk", 1);
+  int* const nk_host = up_idx + b;
+  // the first block's Rayleigh columns
+  zgemm(s, 'C', 'N', m, nk, n, one, V, n, 0, W, n, 0, zero, Cn, ldc, 0, 1);
+  to_host(Ch, Cn, (size_t)ldc * nk * sizeof(cplx));
   for (int it = 1; it <= 200; ++it) {
     c->davidson_iters = it;
-    // Rayleigh matrix: the new columns V^H W[:, m-nk:m]
-    zgemm(s, 'C', 'N', m, nk, n, one, V, n, 0, W + n * (m - nk), n, 0, zero, Cn, m, 0, 1);
-    to_host(Ch, Cn, (size_t)m * nk * sizeof(cplx));
+    // Rayleigh matrix: the new columns V^H W[:, m-nk:m], downloaded with ldc rows
     for (int j = 0; j < nk; ++j)
       for (int i = 0; i < m; ++i) {
-        const zc x = Ch[i + (size_t)m * j];
+        const zc x = Ch[i + (size_t)ldc * j];
         const int jj = m - nk + j;
         if (i < m - nk) {
           Th[i + (size_t)maxsub * jj] = x;
           Th[jj + (size_t)maxsub * i] = std::conj(x);
         } else if (i <= jj) {   // the new diagonal block: its upper triangle, mirrored
-          const zc y = (i == jj) ? zc(x.real(), 0.0) : 0.5 * (x + std::conj(Ch[jj + (size_t)m * (i - (m - nk))]));
+          const zc y = (i == jj) ? zc(x.real(), 0.0) : 0.5 * (x + std::conj(Ch[jj + (size_t)ldc * (i - (m - nk))]));
           Th[i + (size_t)maxsub * jj] = y;
           Th[jj + (size_t)maxsub * i] = std::conj(y);
         }
@@ -601,7 +608,8 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     const bool restart = m + b > maxsub;
     const int m_in = m;
     // Ritz vectors and residuals, preconditioned corrections, (restart,) two classical
-    // Gram-Schmidt passes against V, then the Gram matrix of the block: launches only
+    // Gram-Schmidt passes against V, the Gram matrix of the block and (device_orth) its
+    // orthonormal combinations: launches only
     auto segment = [&](int mm, bool rs) {
       zgemm(s, 'N', 'N', n, b, mm, one, V, n, 0, Sy, mm, 0, zero, X, n, 0, 1);
       zgemm(s, 'N', 'N', n, b, mm, one, W, n, 0, Sy, mm, 0, zero, AX, n, 0, 1);
@@ -619,6 +627,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
         zgemm(s, 'N', 'N', n, b, mm, mone, V, n, 0, Sm, mm, 0, one, T, n, 0, 1);
       }
       zgemm(s, 'C', 'N', b, b, n, one, T, n, 0, T, n, 0, zero, RG + (size_t)b * b, b, 0, 1);
+      if (dev_orth) davidson_block_orth(s, RG + (size_t)b * b, Sc, nk_dev);
     };
     // Replayed from a captured graph when no restart copy is involved and the launches are
     // not being timed; the captured products run without split-K (their workspace can be
@@ -627,7 +636,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     if (graph_ok) {
       const std::vector<int64_t> key = {n, b, m_in, (int64_t)V, (int64_t)W, (int64_t)X, (int64_t)AX, (int64_t)R,
                                         (int64_t)T, (int64_t)Sy, (int64_t)th, (int64_t)RG, (int64_t)Sm, (int64_t)d,
-                                        (int64_t)sel, (int64_t)s};
+                                        (int64_t)sel, (int64_t)s, (int64_t)Sc, (int64_t)nk_dev, (int64_t)dev_orth};
       auto it = c->dav_graphs.find(key);
       if (it == c->dav_graphs.end()) {   // capture; on any failure the context stops using graphs
         hipGraph_t g = nullptr;
@@ -663,7 +672,31 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       segment(m_in, restart);
     }
     if (restart) m = b;
-    to_host(rg, RG, nrg * sizeof(cplx));
+    if (!dev_orth) {   // the host orthonormalises the block: a round trip of its own
+      to_host(rg, RG, nrg * sizeof(cplx));
+      G.assign(rg + (size_t)b * b, rg + nrg);
+      Q.resize(G.size());
+      gw.resize(b);
+      herm_eig(b, G.data(), gw.data(), Q.data());
+      int kept = 0;
+      for (int k = b - 1; k >= 0; --k) {
+        if (!(gw[k] > 1e-12)) continue;   // |projected unit vector|^2
+        for (int i = 0; i < b; ++i) up_sc[i + (size_t)b * kept] = Q[i + (size_t)b * k] / std::sqrt(gw[k]);
+        ++kept;
+      }
+      for (int k = kept; k < b; ++k)
+        for (int i = 0; i < b; ++i) up_sc[i + (size_t)b * k] = zc(0.0, 0.0);
+      *nk_host = kept;
+      to_dev(Sc, up_sc, (size_t)b * b * sizeof(cplx));
+    }
+    // the next block (b columns, the dropped ones zero) and its Rayleigh columns, then one
+    // download of the residual norms, the kept count and V^H W_new; m + b <= maxsub here
+    zgemm(s, 'N', 'N', n, b, b, one, T, n, 0, Sc, b, 0, zero, V + n * m, n, 0, 1);
+    zgemm(s, 'N', 'N', n, b, n, one, H, n, 0, V + n * m, n, 0, zero, W + n * m, n, 0, 1);
+    zgemm(s, 'C', 'N', m + b, b, n, one, V, n, 0, W + n * m, n, 0, zero, Cn, m + b, 0, 1);
+    DC2_HIP(hipMemcpyAsync(rg, RG, nrg * sizeof(cplx), hipMemcpyDeviceToHost, s));
+    if (dev_orth) DC2_HIP(hipMemcpyAsync(nk_host, nk_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    to_host(Ch, Cn, (size_t)(m + b) * b * sizeof(cplx));
     double worst = 0.0;
     for (int j = 0; j < nroot; ++j) worst = std::max(worst, std::sqrt(std::fabs(rg[j + (size_t)b * j].real())));
     if (worst < tol) {
@@ -678,20 +711,9 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       for (int j = 0; j < b; ++j)
         for (int i = 0; i < b; ++i) Th[i + (size_t)maxsub * j] = (i == j) ? zc(theta[j], 0.0) : zc(0.0, 0.0);
     }
-    G.assign(rg + (size_t)b * b, rg + nrg);
-    Q.resize(G.size());
-    gw.resize(b);
-    herm_eig(b, G.data(), gw.data(), Q.data());
-    nk = 0;
-    for (int k = b - 1; k >= 0; --k) {
-      if (!(gw[k] > 1e-12) || m + nk >= maxsub) continue;   // |projected unit vector|^2
-      for (int i = 0; i < b; ++i) up_sc[i + (size_t)b * nk] = Q[i + (size_t)b * k] / std::sqrt(gw[k]);
-      ++nk;
-    }
-    if (nk == 0) return false;
-    to_dev(Sc, up_sc, (size_t)b * nk * sizeof(cplx));
-    zgemm(s, 'N', 'N', n, nk, b, one, T, n, 0, Sc, b, 0, zero, V + n * m, n, 0, 1);
-    zgemm(s, 'N', 'N', n, nk, n, one, H, n, 0, V + n * m, n, 0, zero, W + n * m, n, 0, 1);
+    nk = *nk_host;
+    if (nk <= 0) return false;
+    ldc = m + b;
     m += nk;
   }
   return false;

@@ -1036,6 +1036,85 @@ void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const 
   DC2_HIP(hipGetLastError());
 }
 
+// Orthonormal combinations of a Davidson correction block of four columns from its Gram
+// matrix G = T^H T (4 x 4): a Cholesky factorisation G = R^H R that skips the columns whose
+// remaining squared norm is at most 1e-12 (their directions already lie in the block), then
+// Sc = R^-1 on the kept columns, written first, zero columns after them, and the kept count in
+// nk[0].  One thread, every index static.  This is the block orthonormalisation the host did
+// between two round trips, so that an iteration needs one host round trip instead of two.
+__global__ __launch_bounds__(64) void k_block_chol4(const cplx* __restrict__ G, cplx* __restrict__ Sc,
+                                                    int* __restrict__ nk) {
+  constexpr int B = 4;
+  if (threadIdx.x != 0) return;
+  cplx A[B][B], R[B][B];
+  bool keep[B];
+#pragma unroll
+  for (int j = 0; j < B; ++j)
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      A[i][j] = i <= j ? G[i + B * j] : cconj(G[j + B * i]);
+      R[i][j] = mk(0.0, 0.0);
+    }
+  // R[j][k] for kept rows j < k: (A[j][k] - sum_{i<j kept} conj(R[i][j]) R[i][k]) / R[j][j]
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    double d = A[k][k].x;
+#pragma unroll
+    for (int j = 0; j < k; ++j) {
+      if (!keep[j]) continue;
+      cplx x = A[j][k];
+#pragma unroll
+      for (int i = 0; i < j; ++i)
+        if (keep[i]) x = csub(x, cmul(cconj(R[i][j]), R[i][k]));
+      x = mk(x.x / R[j][j].x, x.y / R[j][j].x);
+      R[j][k] = x;
+      d -= cabs2(x);
+    }
+    keep[k] = d > 1e-12;
+    R[k][k] = mk(keep[k] ? sqrt(d) : 1.0, 0.0);
+  }
+  // X = R^-1 (upper triangular on the kept indices): X[k][k] = 1/R[k][k],
+  // X[i][k] = -(sum_{i<=j<k, kept} X[i][j] R[j][k]) / R[k][k]
+  cplx X[B][B];
+#pragma unroll
+  for (int k = 0; k < B; ++k)
+#pragma unroll
+    for (int i = 0; i < B; ++i) X[i][k] = mk(0.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    if (!keep[k]) continue;
+    const double inv = 1.0 / R[k][k].x;
+    X[k][k] = mk(inv, 0.0);
+#pragma unroll
+    for (int i = 0; i < k; ++i) {
+      if (!keep[i]) continue;
+      cplx sum = mk(0.0, 0.0);
+#pragma unroll
+      for (int j = i; j < k; ++j)
+        if (keep[j]) cfma(sum, X[i][j], R[j][k]);
+      X[i][k] = mk(-sum.x * inv, -sum.y * inv);
+    }
+  }
+  int kept = 0;
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    if (keep[k]) {
+#pragma unroll
+      for (int i = 0; i < B; ++i) Sc[i + B * kept] = X[i][k];
+      ++kept;
+    }
+  }
+  for (int k = kept; k < B; ++k)
+#pragma unroll
+    for (int i = 0; i < B; ++i) Sc[i + B * k] = mk(0.0, 0.0);
+  nk[0] = kept;
+}
+
+void davidson_block_orth(hipStream_t s, const cplx* G, cplx* Sc, int* nk) {
+  hipLaunchKernelGGL(k_block_chol4, dim3(1), dim3(64), 0, s, G, Sc, nk);
+  DC2_HIP(hipGetLastError());
+}
+
 void davidson_normalize(hipStream_t s, int64_t n, int ncol, cplx* T) {
   if (ncol <= 0) return;
   hipLaunchKernelGGL(k_col_normalize, dim3(ncol), dim3(256), 0, s, n, T);

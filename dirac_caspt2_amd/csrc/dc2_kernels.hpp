@@ -94,6 +94,9 @@ void davidson_residual(hipStream_t s, int64_t n, int ncol, const cplx* AX, const
 void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const cplx* R, const double* theta,
                       const double* d, cplx* T);
 void davidson_normalize(hipStream_t s, int64_t n, int ncol, cplx* T);
+// orthonormal combinations Sc (4 x 4, kept columns first, then zeros) of a correction block of
+// four columns from its Gram matrix G (4 x 4); the kept count in nk[0]
+void davidson_block_orth(hipStream_t s, const cplx* G, cplx* Sc, int* nk);
 
 // ---------------------------------------------------------------- dc2_ozaki.hip
 // Split-precision (Ozaki, INT8 MFMA) complex GEMM C = A^T-view * op(B) of the transform
