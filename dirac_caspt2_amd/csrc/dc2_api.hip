@@ -505,8 +505,9 @@ void upload_ints(hipStream_t s, DevBuf<int>& dst, const std::vector<int>& v) {
 // inside the block), subspace of at most maxsub vectors (thick restart on the Ritz vectors),
 // converged when every requested residual |H x - theta x| is below 1e-11.  The Rayleigh
 // matrix V^H H V is kept on the host and extended by the new block only (after a restart it
-// is diag(theta)); its eigenproblem and the block orthonormalisation run on the host
-// (herm_eig), with two host round trips per iteration.  Writes c->civec (n x nroot) and
+// is diag(theta)); its eigenproblem runs on the host (herm_eig), the orthonormalisation of a
+// block of four on the device (a second round trip to the host for larger blocks), so that
+// an iteration has one host round trip.  Writes c->civec (n x nroot) and
 // c->ecas; returns false (no result) when it does not converge, and the caller falls back to
 // the dense eigensolver.
 bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
