@@ -6,7 +6,7 @@ subspaces A-H) with the integrals already resident in HBM.  The default workload
 324-spinor one (30 inactive / 14 active with 6 electrons (3003 determinants) / 280
 secondary, complex128, seeded synthetic Kramers-paired integrals).
 
-    python bench.py [--steps K] [--warmup W] [--workload split|fp64|small|batch|serve|eig]
+    python bench.py [--steps K] [--warmup W] [--workload split|fp64|small|batch|serve|eig|mdcint]
                     [--served N] [--slots S] [--split SLICES] [--secondary fp64|''] [--stream 0|1]
                     [--requests N] [--rate R] [--no-autotune] [--no-cpu-baseline]
 
@@ -486,6 +486,49 @@ def eig_workload(args):
             "sizes": out}
 
 
+def mdcint_workload(args):
+    """The MDCINT unique set -> MDCINTNEW records conversion on the device (dc2_mdcint_records,
+    create_newmdcint.F90:86-177) on a seeded random set at the 324-spinor size: 4000 records of
+    2500 entries (10,000,000), Kramers labels in 1..162 with random signs, a random Kramers
+    table and CAS order; SCHEME 4 and 6, three calls each, records counted, not copied back.
+    Wall time per call includes the host-to-device upload of the entries."""
+    from dirac_caspt2_amd.api import Caspt2Context
+    rng = np.random.default_rng(11)
+    nkr, nrec, nzr = 162, 4000, 2500
+    nmo = 2 * nkr
+    lab = lambda n: rng.integers(1, nkr + 1, n) * rng.choice([-1, 1], n)
+    ikr, jkr = lab(nrec), lab(nrec)
+    nz = np.full(nrec, nzr, np.int64)
+    ne = int(nz.sum())
+    indk, indl = lab(ne), lab(ne)
+    rr = rng.standard_normal(ne)
+    perm = rng.permutation(nmo) + 1
+    kr = np.zeros(2 * nkr + 1, np.int64)
+    for m in range(1, nkr + 1):
+        kr[nkr + m] = perm[2 * (m - 1)]
+        kr[nkr - m] = perm[2 * (m - 1) + 1]
+    d2c = np.zeros(nmo + 1, np.int64)
+    d2c[1:] = rng.permutation(nmo) + 1
+    calls = {}
+    with Caspt2Context(0) as ctx:
+        for scheme in (4, 6):
+            ts = []
+            for _ in range(3):
+                t = time.perf_counter()
+                nout = ctx.mdcint_records(ikr, jkr, nz, indk, indl, rr, None, nkr, kr, d2c, scheme,
+                                          want_records=False)
+                ts.append(time.perf_counter() - t)
+            calls[f"scheme_{scheme}"] = {"records": int(nout), "ms_per_call": 1e3 * min(ts)}
+            log(f"scheme {scheme}: {ne} entries -> {nout} records, {1e3 * min(ts):.1f} ms per call")
+    best = min(v["ms_per_call"] for v in calls.values())
+    return {"metric": "MDCINT entries converted to MDCINTNEW records per second", "value": ne / (best * 1e-3),
+            "unit": "entries/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64 / int32",
+            "data": "synthetic (seeded random MDCINT set, 10,000,000 entries)",
+            "config": {"workload": "mdcint: 4000 records x 2500 entries, nkr 162, SCHEME 4 and 6, host upload "
+                                   "included"},
+            "calls": calls}
+
+
 AMPLITUDE_NOTE = ("two-electron factors at amplitude 0.25 x CN(0,1) (amplitude 1 puts a secondary Fock eigenvalue "
                   "below the inactive ones: an intruder state)")
 
@@ -567,10 +610,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="split", choices=["small", "fp64", "split", "batch", "serve", "eig"],
+    ap.add_argument("--workload", default="split", choices=["small", "fp64", "split", "batch", "serve", "eig", "mdcint"],
                     help="split: 324 spinors (30/14/280, 6 electrons, the default line); fp64: 192 spinors "
                          "(16/12/164); small: 52 spinors; batch: 64 independent instances; serve: Poisson stream; "
-                         "eig: the eigensolver alone on Gram matrices of the A/C metric shape")
+                         "eig: the eigensolver alone on Gram matrices of the A/C metric shape; mdcint: the "
+                         "MDCINT record conversion")
     ap.add_argument("--secondary", default="fp64", help="workload of the secondary object of the line ('' : none)")
     ap.add_argument("--secondary-steps", type=int, default=10)
     ap.add_argument("--instances", type=int, default=64, help="batch workload: independent instances")
@@ -609,6 +653,9 @@ def main():
     window = args.eig_window
     if args.workload == "eig":
         print(json.dumps(eig_workload(args)))
+        return
+    if args.workload == "mdcint":
+        print(json.dumps(mdcint_workload(args)))
         return
     if args.workload == "batch":
         args.eig_window = 0.1 if window is None else window

@@ -438,6 +438,7 @@ __global__ __launch_bounds__(512) void k_hetrd_back(int n, const double* __restr
 // D Z with the panels applied from the last to the first as X <- X - (V T) (V^H X), three
 // matrix products per panel on the MFMA kernel.
 constexpr int BT_NB = 64, BT_BLOCKED_MIN = 256;
+constexpr int BT_SB = 4 * BT_NB;   // reflectors per application: four panels merged
 
 // X = D Z (X complex, lda; Z real n x n)
 __global__ void k_dz(int n, const double* __restrict__ Z, const cplx* __restrict__ delta, cplx* __restrict__ X,
@@ -477,22 +478,56 @@ __global__ __launch_bounds__(BT_NB) void k_wy_t(int nrefl, const double* __restr
   for (int j = 0; j < BT_NB; ++j) t[i + BT_NB * j] = ts[j][i];
 }
 
-// X = Q D Z by panels; work: G and T (nb^2 per panel), VT (n x nb per panel), W (nb x n)
+// T of a merged panel: its four panels' T on the diagonal blocks (the rest zero)
+__global__ void k_ts_diag(const cplx* __restrict__ T64, cplx* __restrict__ TS) {
+  const int p = blockIdx.y, b = blockIdx.x >> 4;   // merged panel, its panel 0..3
+  const int e = (blockIdx.x & 15) * 256 + threadIdx.x, i = e % BT_NB, j = e / BT_NB;
+  cplx* ts = TS + (int64_t)BT_SB * BT_SB * p;
+  const cplx* t = T64 + (int64_t)BT_NB * BT_NB * (4 * p + b);
+  for (int bb = 0; bb < 4; ++bb)   // the whole column strip of block b: zeros off its diagonal block
+    ts[(BT_NB * bb + i) + (int64_t)BT_SB * (BT_NB * b + j)] = bb == b ? t[i + BT_NB * j] : mk(0.0, 0.0);
+}
+
+// X = Q D Z with Q = H_0 H_1 ... H_{n-3}, four panels of 64 reflectors at a time: each
+// panel's compact-WY T from its Gram matrix (k_wy_t), the merged T of four panels from
+// (I - V1 T1 V1^H)(I - V2 T2 V2^H) = I - [V1 V2] [[T1, -T1 (V1^H V2) T2], [0, T2]] [V1 V2]^H
+// (pairs, then the two pairs), then X <- X - (V T)(V^H X) per merged panel, last first: half
+// as many products as panels of 64, each four times taller (the products of the 64-reflector
+// panels ran at a few TF/s beside the transforms).  Work: G64, T64 (64^2 per panel), GS, TS,
+// TM (256^2 per merged panel), VT (n x 256 per merged panel), W (256 x n).
 void back_blocked(hipStream_t s, int n, const double* Z, const cplx* delta, const double* tau, const cplx* Vh,
-                  cplx* X, int64_t ldx, cplx* G, cplx* T, cplx* VT, cplx* W) {
-  const int nrefl = n - 2, np = (int)cdiv(nrefl, BT_NB);
+                  cplx* X, int64_t ldx, cplx* G, cplx* T, cplx* GS, cplx* TS, cplx* TM, cplx* VT, cplx* W) {
+  const int nrefl = n - 2, nps = (int)cdiv(nrefl, BT_SB), np = 4 * nps;
   const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0), mone = mk(-1.0, 0.0);
-  const int64_t nn = (int64_t)n * n;
+  const int64_t nn = (int64_t)n * n, sq = (int64_t)BT_SB * BT_SB, ld = BT_SB;
   hipLaunchKernelGGL(k_dz, dim3(cdiv(nn, 256)), dim3(256), 0, s, n, Z, delta, X, ldx);
-  // every panel's G = V^H V and V T in batched products (the columns of Vh past the last
-  // reflector are zero)
+  // every panel's G = V^H V and T; every merged panel's Gram (the columns of Vh past the last
+  // reflector are zero, so are their T columns)
   zgemm(s, 'C', 'N', BT_NB, BT_NB, n, one, Vh, n, (int64_t)n * BT_NB, Vh, n, (int64_t)n * BT_NB, zero, G, BT_NB,
         (int64_t)BT_NB * BT_NB, np);
   hipLaunchKernelGGL(k_wy_t, dim3(np), dim3(BT_NB), 0, s, nrefl, tau, G, T);
-  zgemm(s, 'N', 'N', n, BT_NB, BT_NB, one, Vh, n, (int64_t)n * BT_NB, T, BT_NB, (int64_t)BT_NB * BT_NB, zero, VT, n,
-        (int64_t)n * BT_NB, np);
-  for (int p = np - 1; p >= 0; --p) {
-    const int a = p * BT_NB, nb = std::min(BT_NB, nrefl - a), rows = n - a - 1;
+  zgemm(s, 'C', 'N', BT_SB, BT_SB, n, one, Vh, n, (int64_t)n * BT_SB, Vh, n, (int64_t)n * BT_SB, zero, GS, ld, sq,
+        nps);
+  hipLaunchKernelGGL(k_ts_diag, dim3(4 * 16, nps), dim3(256), 0, s, T, TS);
+  // pairs (0,1) and (2,3): TS_ab = -T_a (G_ab T_b); then the pairs: TS_[01][23]
+  for (int a = 0; a < 4; a += 2) {
+    const int b = a + 1;
+    const int64_t oab = BT_NB * a + ld * BT_NB * b, oaa = BT_NB * a + ld * BT_NB * a, obb = BT_NB * b + ld * BT_NB * b;
+    zgemm(s, 'N', 'N', BT_NB, BT_NB, BT_NB, one, GS + oab, ld, sq, TS + obb, ld, sq, zero, TM, BT_NB,
+          (int64_t)BT_NB * BT_NB, nps);
+    zgemm(s, 'N', 'N', BT_NB, BT_NB, BT_NB, mone, TS + oaa, ld, sq, TM, BT_NB, (int64_t)BT_NB * BT_NB, zero, TS + oab,
+          ld, sq, nps);
+  }
+  {
+    const int h = 2 * BT_NB;
+    const int64_t o12 = ld * h, o22 = h + ld * h;
+    zgemm(s, 'N', 'N', h, h, h, one, GS + o12, ld, sq, TS + o22, ld, sq, zero, TM, h, (int64_t)h * h, nps);
+    zgemm(s, 'N', 'N', h, h, h, mone, TS, ld, sq, TM, h, (int64_t)h * h, zero, TS + o12, ld, sq, nps);
+  }
+  zgemm(s, 'N', 'N', n, BT_SB, BT_SB, one, Vh, n, (int64_t)n * BT_SB, TS, ld, sq, zero, VT, n, (int64_t)n * BT_SB,
+        nps);
+  for (int p = nps - 1; p >= 0; --p) {
+    const int a = p * BT_SB, nb = std::min(BT_SB, nrefl - a), rows = n - a - 1;
     // rows a+1 .. n-1 are the only ones V touches
     zgemm(s, 'C', 'N', nb, n, rows, one, Vh + (a + 1) + (int64_t)n * a, n, 0, X + (a + 1), ldx, 0, zero, W, nb, 0,
           1);
@@ -562,14 +597,15 @@ void hetrd_stats_reset() {
   g_hetrd_time.clear();
 }
 
-// the blocked back-transformation's G, T, VT and W
+// the blocked back-transformation's G, T, GS, TS, TM, VT and W (each rounded to 256 bytes)
 int64_t back_blocked_bytes(int n) {
-  const int64_t np = cdiv(n - 2, BT_NB);
-  return 16 * (2 * np * BT_NB * BT_NB + np * BT_NB * (int64_t)n + (int64_t)BT_NB * n) + 4 * 256;
+  const int64_t nps = cdiv(n - 2, BT_SB), np = 4 * nps;
+  return 16 * (2 * np * BT_NB * BT_NB + 3 * nps * BT_SB * BT_SB + nps * BT_SB * (int64_t)n + (int64_t)BT_SB * n) +
+         7 * 256;
 }
 
-// columns of the reflector array: whole panels of the blocked back-transformation
-int64_t vh_cols(int n) { return n >= BT_BLOCKED_MIN ? std::max<int64_t>(n, cdiv(n - 2, BT_NB) * BT_NB) : n; }
+// columns of the reflector array: whole merged panels of the blocked back-transformation
+int64_t vh_cols(int n) { return n >= BT_BLOCKED_MIN ? std::max<int64_t>(n, cdiv(n - 2, BT_SB) * BT_SB) : n; }
 
 int64_t heev_native_work_bytes(int n) {
   const int64_t nn = (int64_t)n * n;
@@ -668,12 +704,15 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     if (rs != rocblas_status_success) return false;
   }
   if (n >= BT_BLOCKED_MIN) {
-    const int64_t np = cdiv(n - 2, BT_NB);
+    const int64_t nps = cdiv(n - 2, BT_SB), np = 4 * nps;
     cplx* G = reinterpret_cast<cplx*>(take(16 * np * BT_NB * BT_NB));
     cplx* T = reinterpret_cast<cplx*>(take(16 * np * BT_NB * BT_NB));
-    cplx* VT = reinterpret_cast<cplx*>(take(16 * np * BT_NB * (int64_t)n));
-    cplx* W = reinterpret_cast<cplx*>(take(16 * (int64_t)BT_NB * n));
-    back_blocked(s, n, Z, delta, tau, Vh, A, lda, G, T, VT, W);
+    cplx* GS = reinterpret_cast<cplx*>(take(16 * nps * BT_SB * BT_SB));
+    cplx* TS = reinterpret_cast<cplx*>(take(16 * nps * BT_SB * BT_SB));
+    cplx* TM = reinterpret_cast<cplx*>(take(16 * nps * BT_SB * BT_SB));
+    cplx* VT = reinterpret_cast<cplx*>(take(16 * nps * BT_SB * (int64_t)n));
+    cplx* W = reinterpret_cast<cplx*>(take(16 * (int64_t)BT_SB * n));
+    back_blocked(s, n, Z, delta, tau, Vh, A, lda, G, T, GS, TS, TM, VT, W);
     DC2_HIP(hipGetLastError());
     return true;
   }
