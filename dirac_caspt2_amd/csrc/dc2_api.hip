@@ -528,20 +528,25 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   cplx* AX = L.z("dv_AX", n * b);
   cplx* R = L.z("dv_R", n * b);
   cplx* T = L.z("dv_T", n * b);
-  cplx* Cn = L.z("dv_C", (int64_t)maxsub * b);           // V^H W_new
   cplx* Sm = L.z("dv_small", (int64_t)maxsub * b);       // Gram-Schmidt coefficients
   cplx* Sy = L.z("dv_y", (int64_t)maxsub * b + b);       // Ritz coefficients, then theta
-  cplx* RG = L.z("dv_rg", 2 * (int64_t)b * b);           // R^H R and T^H T, one download
   cplx* Sc = L.z("dv_sc", (int64_t)b * b);
   double* d = L.d("dv_d", n);
   int* idx = L.i("dv_idx", b);
-  // page-locked host side of the per-iteration transfers: downloads Ch (V^H W_new) and rg
-  // (R^H R, T^H T); uploads up_y (Ritz coefficients, theta), up_sc, up_idx, whose contents
-  // stay untouched until the next synchronising download
+  // what an iteration downloads, contiguous so that it is one copy: R^H R and T^H T (RG), the
+  // kept count of the next block (one element's room), V^H W_new (Cn, m + b rows)
   const size_t nCh = (size_t)maxsub * b, nrg = 2 * (size_t)b * b, ny = (size_t)maxsub * b + b, nsc = (size_t)b * b;
-  zc* const Ch = reinterpret_cast<zc*>(L.host((nCh + nrg + ny + nsc) * sizeof(zc) + (b + 1) * sizeof(int)));
-  zc* const rg = Ch + nCh;
-  zc* const up_y = rg + nrg;
+  cplx* const DL = L.z("dv_dl", (int64_t)(nrg + 1 + nCh));
+  cplx* const RG = DL;
+  int* const nk_dev = reinterpret_cast<int*>(DL + nrg);
+  cplx* const Cn = DL + nrg + 1;
+  // page-locked host side: the download's mirror (rg, nk_host, Ch); uploads up_y (Ritz
+  // coefficients, then theta at maxsub * b: one copy), up_sc, up_idx, whose contents stay
+  // untouched until the next synchronising download
+  zc* const rg = reinterpret_cast<zc*>(L.host((nrg + 1 + nCh + ny + nsc) * sizeof(zc) + b * sizeof(int)));
+  int* const nk_host = reinterpret_cast<int*>(rg + nrg);
+  zc* const Ch = rg + nrg + 1;
+  zc* const up_y = Ch + nCh;
   zc* const up_sc = up_y + ny;
   int* const up_idx = reinterpret_cast<int*>(up_sc + nsc);
   auto to_host = [&](void* dst, const void* src, size_t bytes) {
@@ -575,8 +580,6 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
   // the block orthonormalisation on the device for blocks of four (one root), so that an
   // iteration needs one host round trip; the host's, in a round trip of its own, otherwise
   const bool dev_orth = b == 4;
-  int* nk_dev = L.i("dv_nk", 1);
-  int* const nk_host = up_idx + b;
   // the first block's Rayleigh columns
   zgemm(s, 'C', 'N', m, nk, n, one, V, n, 0, W, n, 0, zero, Cn, ldc, 0, 1);
   to_host(Ch, Cn, (size_t)ldc * nk * sizeof(cplx));
@@ -604,19 +607,22 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     herm_eig(m, Tm.data(), theta.data(), Yh.data());
     std::copy(Yh.begin(), Yh.begin() + (size_t)m * b, up_y);     // first b Ritz vectors
     std::memcpy(up_y + (size_t)maxsub * b, theta.data(), b * sizeof(double));
-    to_dev(Sy, up_y, (size_t)m * b * sizeof(cplx));
-    to_dev(th, up_y + (size_t)maxsub * b, b * sizeof(double));
+    to_dev(Sy, up_y, (size_t)maxsub * b * sizeof(cplx) + b * sizeof(double));   // Sy, then theta
     const bool restart = m + b > maxsub;
     const int m_in = m;
-    // Ritz vectors and residuals, preconditioned corrections, (restart,) two classical
-    // Gram-Schmidt passes against V, the Gram matrix of the block and (device_orth) its
-    // orthonormal combinations: launches only
+    // Ritz vectors and residuals, preconditioned corrections (one launch up to 64 subspace
+    // vectors), (restart,) two classical Gram-Schmidt passes against V, the Gram matrix of the
+    // block and (device_orth) its orthonormal combinations: launches only
     auto segment = [&](int mm, bool rs) {
-      zgemm(s, 'N', 'N', n, b, mm, one, V, n, 0, Sy, mm, 0, zero, X, n, 0, 1);
-      zgemm(s, 'N', 'N', n, b, mm, one, W, n, 0, Sy, mm, 0, zero, AX, n, 0, 1);
-      davidson_residual(s, n, b, AX, X, th, R);
+      if (mm <= 64) {   // sel is the identity
+        davidson_ritz(s, n, b, mm, V, W, Sy, th, d, X, AX, R, T);
+      } else {
+        zgemm(s, 'N', 'N', n, b, mm, one, V, n, 0, Sy, mm, 0, zero, X, n, 0, 1);
+        zgemm(s, 'N', 'N', n, b, mm, one, W, n, 0, Sy, mm, 0, zero, AX, n, 0, 1);
+        davidson_residual(s, n, b, AX, X, th, R);
+        davidson_precond(s, n, b, sel, R, th, d, T);
+      }
       zgemm(s, 'C', 'N', b, b, n, one, R, n, 0, R, n, 0, zero, RG, b, 0, 1);   // residual norms
-      davidson_precond(s, n, b, sel, R, th, d, T);
       davidson_normalize(s, n, b, T);
       if (rs) {   // thick restart on the b Ritz vectors: V^H H V = diag(theta)
         DC2_HIP(hipMemcpyAsync(V, X, n * b * sizeof(cplx), hipMemcpyDeviceToDevice, s));
@@ -673,6 +679,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       segment(m_in, restart);
     }
     if (restart) m = b;
+    int kept_host = 0;
     if (!dev_orth) {   // the host orthonormalises the block: a round trip of its own
       to_host(rg, RG, nrg * sizeof(cplx));
       G.assign(rg + (size_t)b * b, rg + nrg);
@@ -687,7 +694,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       }
       for (int k = kept; k < b; ++k)
         for (int i = 0; i < b; ++i) up_sc[i + (size_t)b * k] = zc(0.0, 0.0);
-      *nk_host = kept;
+      kept_host = kept;
       to_dev(Sc, up_sc, (size_t)b * b * sizeof(cplx));
     }
     // the next block (b columns, the dropped ones zero) and its Rayleigh columns, then one
@@ -695,9 +702,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     zgemm(s, 'N', 'N', n, b, b, one, T, n, 0, Sc, b, 0, zero, V + n * m, n, 0, 1);
     zgemm(s, 'N', 'N', n, b, n, one, H, n, 0, V + n * m, n, 0, zero, W + n * m, n, 0, 1);
     zgemm(s, 'C', 'N', m + b, b, n, one, V, n, 0, W + n * m, n, 0, zero, Cn, m + b, 0, 1);
-    DC2_HIP(hipMemcpyAsync(rg, RG, nrg * sizeof(cplx), hipMemcpyDeviceToHost, s));
-    if (dev_orth) DC2_HIP(hipMemcpyAsync(nk_host, nk_dev, sizeof(int), hipMemcpyDeviceToHost, s));
-    to_host(Ch, Cn, (size_t)(m + b) * b * sizeof(cplx));
+    to_host(rg, DL, (nrg + 1 + (size_t)(m + b) * b) * sizeof(cplx));
     double worst = 0.0;
     for (int j = 0; j < nroot; ++j) worst = std::max(worst, std::sqrt(std::fabs(rg[j + (size_t)b * j].real())));
     if (worst < tol) {
@@ -712,7 +717,7 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
       for (int j = 0; j < b; ++j)
         for (int i = 0; i < b; ++i) Th[i + (size_t)maxsub * j] = (i == j) ? zc(theta[j], 0.0) : zc(0.0, 0.0);
     }
-    nk = *nk_host;
+    nk = dev_orth ? *nk_host : kept_host;
     if (nk <= 0) return false;
     ldc = m + b;
     m += nk;

@@ -788,6 +788,35 @@ __global__ void k_precond(int64_t n, const int* sel, const cplx* R, const double
   T[i + n * c] = cscale(R[i + n * j], 1.0 / den);
 }
 
+// Ritz vectors, residuals and preconditioned corrections of one Davidson iteration, one thread
+// per (row i, column j): x = sum_k V[i, k] Y[k, j] and ax = sum_k W[i, k] Y[k, j] in the order
+// k = 0..mm-1, r = ax - theta_j x, t = r / (theta_j - d_i) with the denominator kept at least
+// 1e-8 in magnitude (k_precond's rule); Y's column is uniform over the workgroup
+__global__ __launch_bounds__(256) void k_ritz_fused(int64_t n, int mm, const cplx* __restrict__ V,
+                                                    const cplx* __restrict__ W, const cplx* __restrict__ Y,
+                                                    const double* __restrict__ theta, const double* __restrict__ d,
+                                                    cplx* __restrict__ X, cplx* __restrict__ AX,
+                                                    cplx* __restrict__ R, cplx* __restrict__ T) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n) return;
+  const cplx* y = Y + (int64_t)mm * j;
+  cplx x = mk(0.0, 0.0), ax = mk(0.0, 0.0);
+  for (int k = 0; k < mm; ++k) {
+    const cplx yk = y[k];
+    cfma(x, V[i + n * k], yk);
+    cfma(ax, W[i + n * k], yk);
+  }
+  const double th = theta[j];
+  const cplx r = csub(ax, cscale(x, th));
+  double den = th - d[i];
+  if (fabs(den) < 1e-8) den = den < 0 ? -1e-8 : 1e-8;
+  X[i + n * j] = x;
+  AX[i + n * j] = ax;
+  R[i + n * j] = r;
+  T[i + n * j] = cscale(r, 1.0 / den);
+}
+
 // one workgroup per column: T[:, c] /= |T[:, c]| (deterministic tree reduction)
 __global__ __launch_bounds__(256) void k_col_normalize(int64_t n, cplx* T) {
   __shared__ double sh[256];
@@ -1033,6 +1062,14 @@ void davidson_residual(hipStream_t s, int64_t n, int ncol, const cplx* AX, const
 void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const cplx* R, const double* theta,
                       const double* d, cplx* T) {
   hipLaunchKernelGGL(k_precond, dim3(cdiv(n, 256), ncol), dim3(256), 0, s, n, sel, R, theta, d, T);
+  DC2_HIP(hipGetLastError());
+}
+
+void davidson_ritz(hipStream_t s, int64_t n, int ncol, int mm, const cplx* V, const cplx* W, const cplx* Sy,
+                   const double* theta, const double* d, cplx* X, cplx* AX, cplx* R, cplx* T) {
+  DC2_CHECK(mm >= 1 && mm <= 64 && ncol >= 1 && ncol < 65536, 1, "davidson_ritz: subspace out of range");
+  hipLaunchKernelGGL(k_ritz_fused, dim3(cdiv(n, 256), ncol), dim3(256), 0, s, n, mm, V, W, Sy, theta, d, X, AX, R,
+                     T);
   DC2_HIP(hipGetLastError());
 }
 

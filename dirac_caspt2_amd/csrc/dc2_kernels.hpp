@@ -94,6 +94,10 @@ void davidson_residual(hipStream_t s, int64_t n, int ncol, const cplx* AX, const
 void davidson_precond(hipStream_t s, int64_t n, int ncol, const int* sel, const cplx* R, const double* theta,
                       const double* d, cplx* T);
 void davidson_normalize(hipStream_t s, int64_t n, int ncol, cplx* T);
+// the same four steps in one launch for a subspace of mm <= 64 vectors: X = V Y, AX = W Y,
+// R = AX - X diag(theta), T[:, c] = R[:, c] / (theta_c - d) (Y = Sy, mm x ncol)
+void davidson_ritz(hipStream_t s, int64_t n, int ncol, int mm, const cplx* V, const cplx* W, const cplx* Sy,
+                   const double* theta, const double* d, cplx* X, cplx* AX, cplx* R, cplx* T);
 // orthonormal combinations Sc (4 x 4, kept columns first, then zeros) of a correction block of
 // four columns from its Gram matrix G (4 x 4); the kept count in nk[0]
 void davidson_block_orth(hipStream_t s, const cplx* G, cplx* Sc, int* nk);
