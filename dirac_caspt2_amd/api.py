@@ -574,12 +574,13 @@ def write_restart(path: str, totsym: int, selectroot: int, next_subspace: int, e
                                                             _ptr(b)))
 
 
-HOST_HEEV_METHODS = {"auto": 0, "jacobi": 1, "ql": 2, "zheev": 3}
+HOST_HEEV_METHODS = {"auto": 0, "jacobi": 1, "ql": 2, "zheev": 3, "low": 4}
 
 
 def host_heev(a: np.ndarray, method: str = "auto"):
     """Host Hermitian eigensolver of the library (the Davidson subspace solver): (w, V).
-    method: "auto" (QL, Jacobi if QL fails), "jacobi" or "ql"."""
+    method: "auto" (QL, Jacobi if QL fails), "jacobi", "ql", "zheev" or "low" (every eigenvalue,
+    the eigenvectors of the min(n, 4) lowest only, the other columns zero: the Davidson's)."""
     A = np.asfortranarray(np.asarray(a, dtype=np.complex128)).copy(order="F")
     n = A.shape[0]
     w = np.zeros(n)

@@ -603,8 +603,8 @@ bool davidson(dc2_context* c, int64_t n, const cplx* H, int nroot) {
     for (int j = 0; j < m; ++j)
       for (int i = 0; i < m; ++i) Tm[i + (size_t)m * j] = Th[i + (size_t)maxsub * j];
     theta.resize(m);
-    Yh.resize((size_t)m * m);
-    herm_eig(m, Tm.data(), theta.data(), Yh.data());
+    Yh.resize((size_t)m * b);
+    herm_eig_low(m, Tm.data(), theta.data(), Yh.data(), b);   // every theta, the lowest b vectors
     std::copy(Yh.begin(), Yh.begin() + (size_t)m * b, up_y);     // first b Ritz vectors
     std::memcpy(up_y + (size_t)maxsub * b, theta.data(), b * sizeof(double));
     to_dev(Sy, up_y, (size_t)maxsub * b * sizeof(cplx) + b * sizeof(double));   // Sy, then theta
@@ -2572,9 +2572,10 @@ int dc2_get_transform_flops(dc2_context* c, double* out) {
 
 int dc2_host_heev(int32_t n, double* a, double* w, int32_t method) {
   return guarded([&] {
-    DC2_CHECK(a && w && n >= 1 && n <= 4096 && method >= 0 && method <= 3, 1, "invalid argument");
+    DC2_CHECK(a && w && n >= 1 && n <= 4096 && method >= 0 && method <= 4, 1, "invalid argument");
     std::vector<zc> A(reinterpret_cast<zc*>(a), reinterpret_cast<zc*>(a) + (size_t)n * n), V((size_t)n * n);
-    if (method == 3) DC2_CHECK(herm_eig_zheev(n, A.data(), w, V.data()), 3, "QL/QR iteration did not converge");
+    if (method == 4) herm_eig_low(n, A.data(), w, V.data(), std::min(n, 4));   // the other columns stay zero
+    else if (method == 3) DC2_CHECK(herm_eig_zheev(n, A.data(), w, V.data()), 3, "QL/QR iteration did not converge");
     else if (method == 0) herm_eig(n, A.data(), w, V.data());
     else if (method == 1) herm_jacobi(n, A.data(), w, V.data());
     else DC2_CHECK(herm_tql(n, A.data(), w, V.data()), 3, "QL iteration did not converge");

@@ -167,24 +167,24 @@ inline bool tql_rotations(int n, std::vector<double>& d, std::vector<double>& e,
   return true;
 }
 
-// Hermitian eigendecomposition by Householder reduction to a real symmetric tridiagonal
-// matrix and the implicit QL iteration with Wilkinson shifts (the tql2 scheme), for the
-// small projected problems of the Davidson solver, where it needs about a fifth of the
-// flops of Jacobi.  Same contract as herm_jacobi: A (column-major, overwritten), w ascending,
-// V orthonormal eigenvectors.  Returns false if an eigenvalue needs more than 90 QL steps.
-//
-// Reduction: for column k the reflector H_k = I - tau v v^H (tau = 2 / v^H v, real) maps
-// x = A[k+1:, k] to alpha e_1 with alpha = -e^{i arg x_0} |x|; the trailing block becomes
-// B - v q^H - q v^H with p = tau B v, q = p - (tau/2)(v^H p) v.  The complex subdiagonal
-// e_k = alpha is made real by D = diag(delta), delta_{k+1} = delta_k e_k / |e_k|, so that
-// A = Q D T D^H Q^H with T real; eigenvectors are Q D Z for the QL rotations Z.
-inline bool herm_tql(int n, zc* A, double* w, zc* V) {
+// Householder reduction of a Hermitian matrix to a real symmetric tridiagonal one.  For
+// column k the reflector H_k = I - tau v v^H (tau = 2 / v^H v, real) maps x = A[k+1:, k] to
+// alpha e_1 with alpha = -e^{i arg x_0} |x|; the trailing block becomes B - v q^H - q v^H with
+// p = tau B v, q = p - (tau/2)(v^H p) v.  The complex subdiagonal e_k = alpha is made real by
+// D = diag(delta), delta_{k+1} = delta_k e_k / |e_k|, so that A = Q D T D^H Q^H with T real;
+// eigenvectors are Q D Z for the eigenvectors Z of T.  A (full column-major storage) is
+// overwritten, v of column k kept in A[k+1:, k] with tau[k]; d, e: T's diagonal and
+// subdiagonal (e[k] = |alpha_k|).
+inline void herm_tridiag(int n, zc* A, std::vector<double>& d, std::vector<double>& e, std::vector<double>& tau,
+                         std::vector<zc>& delta) {
   auto at = [&](int i, int j) -> zc& { return A[i + (size_t)n * j]; };
   auto mul = [](zc x, zc y) { return zc(x.real() * y.real() - x.imag() * y.imag(), x.real() * y.imag() + x.imag() * y.real()); };
   auto cmulc = [](zc x, zc y) {   // conj(x) * y
     return zc(x.real() * y.real() + x.imag() * y.imag(), x.real() * y.imag() - x.imag() * y.real());
   };
-  std::vector<double> d(n), e(n, 0.0), tau(n, 0.0);
+  d.assign(n, 0.0);
+  e.assign(n, 0.0);
+  tau.assign(n, 0.0);
   std::vector<zc> sub(n, zc(0.0, 0.0)), p(n), q(n);
   for (int k = 0; k + 1 < n; ++k) {
     const int m = n - k - 1;
@@ -225,12 +225,94 @@ inline bool herm_tql(int n, zc* A, double* w, zc* V) {
     }
   }
   for (int i = 0; i < n; ++i) d[i] = at(i, i).real();
-  std::vector<zc> delta(n, zc(1.0, 0.0));
+  delta.assign(n, zc(1.0, 0.0));
   for (int k = 0; k + 1 < n; ++k) {
     const double ae = std::abs(sub[k]);
     e[k] = ae;
     delta[k + 1] = ae > 0.0 ? mul(delta[k], sub[k] / ae) : delta[k];
   }
+}
+
+// herm_tridiag on the lower triangle alone (the upper one is neither read nor written): the
+// same reflectors, half the flops; the rounding differs from herm_tridiag's.
+inline void herm_tridiag_lower(int n, zc* A, std::vector<double>& d, std::vector<double>& e, std::vector<double>& tau,
+                         std::vector<zc>& delta) {
+  auto at = [&](int i, int j) -> zc& { return A[i + (size_t)n * j]; };
+  auto mul = [](zc x, zc y) { return zc(x.real() * y.real() - x.imag() * y.imag(), x.real() * y.imag() + x.imag() * y.real()); };
+  auto cmulc = [](zc x, zc y) {   // conj(x) * y
+    return zc(x.real() * y.real() + x.imag() * y.imag(), x.real() * y.imag() - x.imag() * y.real());
+  };
+  d.assign(n, 0.0);
+  e.assign(n, 0.0);
+  tau.assign(n, 0.0);
+  std::vector<zc> sub(n, zc(0.0, 0.0)), p(n), q(n);
+  for (int k = 0; k + 1 < n; ++k) {
+    const int m = n - k - 1;
+    zc* x = &at(k + 1, k);
+    double xn2 = 0.0, tail2 = 0.0;
+    for (int i = 0; i < m; ++i) xn2 += std::norm(x[i]);
+    tail2 = xn2 - std::norm(x[0]);
+    if (m == 1 || tail2 <= 0.0) {   // already tridiagonal in this column
+      sub[k] = x[0];
+      tau[k] = 0.0;
+      continue;
+    }
+    const double xn = std::sqrt(xn2), ax0 = std::abs(x[0]);
+    const zc ph = ax0 > 0.0 ? x[0] / ax0 : zc(1.0, 0.0);
+    const zc alpha = -xn * ph;
+    x[0] -= alpha;                    // v = x - alpha e_1, kept in place of x
+    const double t = 2.0 / (2.0 * xn2 + 2.0 * ax0 * xn);
+    tau[k] = t;
+    sub[k] = alpha;
+    // p = tau B v from the lower triangle of B = A[k+1:, k+1:]: column j adds B_ij v_j to
+    // p_i (i > j) and conj(B_ij) v_i = B_ji v_i to p_j
+    for (int i = 0; i < m; ++i) p[i] = zc(0.0, 0.0);
+    for (int j = 0; j < m; ++j) {
+      const zc vj = x[j];
+      const zc* col = &at(k + 1, k + 1 + j);
+      zc acc = col[j].real() * vj;
+      for (int i = j + 1; i < m; ++i) {
+        p[i] += mul(col[i], vj);
+        acc += cmulc(col[i], x[i]);
+      }
+      p[j] += acc;
+    }
+    zc vhp(0.0, 0.0);
+    for (int i = 0; i < m; ++i) {
+      p[i] *= t;
+      vhp += cmulc(x[i], p[i]);
+    }
+    const double K = 0.5 * t * vhp.real();
+    for (int i = 0; i < m; ++i) q[i] = p[i] - K * x[i];
+    for (int j = 0; j < m; ++j) {
+      zc* col = &at(k + 1, k + 1 + j);
+      const zc cqj = std::conj(q[j]), cvj = std::conj(x[j]);
+      for (int i = j; i < m; ++i) col[i] -= mul(x[i], cqj) + mul(q[i], cvj);
+    }
+  }
+  for (int i = 0; i < n; ++i) d[i] = at(i, i).real();
+  delta.assign(n, zc(1.0, 0.0));
+  for (int k = 0; k + 1 < n; ++k) {
+    const double ae = std::abs(sub[k]);
+    e[k] = ae;
+    delta[k + 1] = ae > 0.0 ? mul(delta[k], sub[k] / ae) : delta[k];
+  }
+}
+
+// Hermitian eigendecomposition by the reduction above and the implicit QL iteration with
+// Wilkinson shifts (the tql2 scheme), for the small projected problems of the Davidson
+// solver, where it needs about a fifth of the flops of Jacobi.  Same contract as
+// herm_jacobi: A (column-major, overwritten), w ascending, V orthonormal eigenvectors.
+// Returns false if an eigenvalue needs more than 90 QL steps.
+inline bool herm_tql(int n, zc* A, double* w, zc* V) {
+  auto at = [&](int i, int j) -> zc& { return A[i + (size_t)n * j]; };
+  auto mul = [](zc x, zc y) { return zc(x.real() * y.real() - x.imag() * y.imag(), x.real() * y.imag() + x.imag() * y.real()); };
+  auto cmulc = [](zc x, zc y) {   // conj(x) * y
+    return zc(x.real() * y.real() + x.imag() * y.imag(), x.real() * y.imag() - x.imag() * y.real());
+  };
+  std::vector<double> d, e, tau;
+  std::vector<zc> delta;
+  herm_tridiag(n, A, d, e, tau, delta);
   // implicit QL on (d, e), its rotations accumulated into the real Z
   std::vector<double> cs, sn;
   std::vector<int> sweeps;
@@ -276,6 +358,68 @@ inline bool herm_tql(int n, zc* A, double* w, zc* V) {
 inline void herm_eig(int n, zc* A, double* w, zc* V) {
   std::vector<zc> keep(A, A + (size_t)n * n);
   if (!herm_tql(n, A, w, V)) herm_jacobi(n, keep.data(), w, V);
+}
+
+// All n eigenvalues (w, ascending) and the eigenvectors of the nvec lowest only (V: n x nvec),
+// for the Rayleigh matrices of the Davidson solver, which uses the lowest block alone.  The
+// reduction of herm_tridiag_lower (A's lower triangle) and the QL of herm_tql; with Z = R_1 R_2 ... R_K the product of QL's plane rotations in
+// the order they were made, column j of Z is R_1 (R_2 (... (R_K e_j))), so the rotations are
+// applied to the nvec unit vectors from the last to the first (O(K nvec) instead of the O(K n)
+// of accumulating Z), then D and the reflectors (O(n^2 nvec) instead of O(n^3)).  On a QL
+// failure, herm_jacobi on the saved matrix and its first nvec eigenvectors.
+inline void herm_eig_low(int n, zc* A, double* w, zc* V, int nvec) {
+  nvec = std::max(0, std::min(nvec, n));
+  std::vector<zc> keep(A, A + (size_t)n * n);
+  std::vector<double> d, e, tau, cs, sn;
+  std::vector<zc> delta;
+  std::vector<int> sweeps;
+  herm_tridiag_lower(n, A, d, e, tau, delta);
+  cs.reserve(2 * (size_t)n * n);
+  sn.reserve(2 * (size_t)n * n);
+  if (!tql_rotations(n, d, e, cs, sn, sweeps)) {
+    std::vector<zc> Vall((size_t)n * n);
+    herm_jacobi(n, keep.data(), w, Vall.data());
+    std::copy(Vall.begin(), Vall.begin() + (size_t)n * nvec, V);
+    return;
+  }
+  auto at = [&](int i, int j) -> zc& { return A[i + (size_t)n * j]; };
+  auto mul = [](zc x, zc y) { return zc(x.real() * y.real() - x.imag() * y.imag(), x.real() * y.imag() + x.imag() * y.real()); };
+  auto cmulc = [](zc x, zc y) {   // conj(x) * y
+    return zc(x.real() * y.real() + x.imag() * y.imag(), x.real() * y.imag() - x.imag() * y.real());
+  };
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+  for (int c = 0; c < n; ++c) w[c] = d[idx[c]];
+  // z (n x nvec, row-major by row r: z[r * nvec + c]) = Z e_{idx[c]}
+  std::vector<double> z((size_t)n * nvec, 0.0);
+  for (int c = 0; c < nvec; ++c) z[(size_t)idx[c] * nvec + c] = 1.0;
+  for (size_t k = sweeps.size(); k >= 3; k -= 3) {
+    const int m = sweeps[k - 3], istop = sweeps[k - 2], first = sweeps[k - 1];
+    for (int i = istop; i <= m - 1; ++i) {   // the sweep's rotations in reverse (i ascending)
+      const double c = cs[first + m - 1 - i], s = sn[first + m - 1 - i];
+      double* z0 = &z[(size_t)nvec * i];
+      double* z1 = &z[(size_t)nvec * (i + 1)];
+      for (int col = 0; col < nvec; ++col) {
+        const double a = z0[col], b = z1[col];
+        z0[col] = c * a + s * b;
+        z1[col] = c * b - s * a;
+      }
+    }
+  }
+  for (int c2 = 0; c2 < nvec; ++c2) {
+    zc* col = V + (size_t)n * c2;
+    for (int i = 0; i < n; ++i) col[i] = delta[i] * z[(size_t)nvec * i + c2];
+    for (int k = n - 2; k >= 0; --k) {
+      if (tau[k] == 0.0) continue;
+      const int m = n - k - 1;
+      const zc* v = &at(k + 1, k);
+      zc s(0.0, 0.0);
+      for (int i = 0; i < m; ++i) s += cmulc(v[i], col[k + 1 + i]);
+      s *= tau[k];
+      for (int i = 0; i < m; ++i) col[k + 1 + i] -= mul(v[i], s);
+    }
+  }
 }
 
 

@@ -257,11 +257,13 @@ int dc2_get_hetrd_stats(int32_t n, double* out);
 int dc2_get_casci_info(dc2_context* ctx, int64_t* out);
 /* Hermitian eigendecomposition on the host (the Davidson subspace solver): a (n x n
  * complex, column-major) is overwritten by the eigenvectors, w by the ascending eigenvalues.
- * method 0: Householder tridiagonalisation + implicit QL, Jacobi if QL does not converge
- * (what the Davidson solver uses); 1: cyclic Jacobi; 2: QL only (status 3 if it does not
- * converge); 3: the sequence of LAPACK's ZHEEV('V','U') (zhetd2, zungtr, zsteqr), whose
- * eigenvector signs the reference's IVO coefficients carry (diag.F90:66-128).  Only the upper
- * triangle is read.  No GPU needed. */
+ * method 0: Householder tridiagonalisation + implicit QL, Jacobi if QL does not converge;
+ * 1: cyclic Jacobi; 2: QL only (status 3 if it does not converge); 3: the sequence of
+ * LAPACK's ZHEEV('V','U') (zhetd2, zungtr, zsteqr), whose eigenvector signs the reference's
+ * IVO coefficients carry (diag.F90:66-128), only the upper triangle read; 4: every eigenvalue
+ * and the eigenvectors of the min(n, 4) lowest only, the other columns of a set to zero (what
+ * the Davidson solver uses on its Rayleigh matrices: the lower triangle reduced, QL, the
+ * rotations applied to the selected unit vectors alone).  No GPU needed. */
 int dc2_host_heev(int32_t n, double* a, double* w, int32_t method);
 /* fockivo (fockivo.F90:40-230, driven by r4divo.F90:60-80): the IVO Fock matrix of the
  * secondary spinors, F_ab = delta_ab eps_a + sum_k [(ak|kb) - (ab|kk)] / numh over the numh
