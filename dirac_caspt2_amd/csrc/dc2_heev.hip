@@ -41,12 +41,17 @@ namespace {
 constexpr int HT_WAVES = 8;
 constexpr int HT_THREADS = 64 * HT_WAVES;
 constexpr int HT_NMAX = 2048;
-// columns per wave: 2 up to n = 1024 (4 spills registers at n > 768); 1 above, where the
-// 2048 / 8 = 256 workgroups of the largest matrix take every compute unit.  narrow: 2 up to
-// n = 1280 as well, half the workgroups (80 instead of 160 at n = 1274) at 19% more time
-// per reduction, for reductions that run beside other work (the subspace lanes of a CASPT2
-// energy: 88.0 against 90.4 ms per 324-spinor energy, whose transforms get the freed units)
-inline int hetrd_cpw(int n, bool narrow = false) { return (n <= 1024 || (narrow && n <= 1280)) ? 2 : 1; }
+// columns per wave: 1 up to n = 512 (twice the workgroups of 2, half the work of a wave per
+// column step: 3-9% less time at n = 100-512), 2 up to n = 1024 (4 spills registers at
+// n > 768), 1 above, where the 2048 / 8 = 256 workgroups of the largest matrix take every
+// compute unit.  narrow: 2 up to n = 1280, half the workgroups (80 instead of 160 at
+// n = 1274) at 19% more time per reduction, for reductions that run beside other work (the
+// subspace lanes of a CASPT2 energy: 88.0 against 90.4 ms per 324-spinor energy, whose
+// transforms get the freed units)
+inline int hetrd_cpw(int n, bool narrow = false) {
+  if (narrow) return n <= 1280 ? 2 : 1;
+  return (n > 512 && n <= 1024) ? 2 : 1;
+}
 inline int hetrd_nwg(int n, bool narrow = false) {
   return (n + HT_WAVES * hetrd_cpw(n, narrow) - 1) / (HT_WAVES * hetrd_cpw(n, narrow));
 }
@@ -666,15 +671,22 @@ bool heev_native(hipStream_t s, rocblas_handle h, int n, cplx* A, int64_t lda, d
     CoopSlot slot(nwg, cus);
     DC2_HIP(hipEventRecord(ev[0], s));
     // S = rows per lane (n <= 64 S)
-    if (n <= 128) st = launch_hetrd<2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
-    else if (n <= 256) st = launch_hetrd<4>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
-    else if (n <= 512) st = launch_hetrd<8>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    const bool two = hetrd_cpw(n, narrow) == 2;
+    if (n <= 128)
+      st = two ? launch_hetrd<2, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh)
+               : launch_hetrd<2, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 256)
+      st = two ? launch_hetrd<4, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh)
+               : launch_hetrd<4, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+    else if (n <= 512)
+      st = two ? launch_hetrd<8, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh)
+               : launch_hetrd<8, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 768) st = launch_hetrd<12>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 832) st = launch_hetrd<13>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1024) st = launch_hetrd<16>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1280)
-      st = hetrd_cpw(n, narrow) == 2 ? launch_hetrd<20, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh)
-                                     : launch_hetrd<20, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
+      st = two ? launch_hetrd<20, 2>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh)
+               : launch_hetrd<20, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1536) st = launch_hetrd<24, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else if (n <= 1792) st = launch_hetrd<28, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
     else st = launch_hetrd<32, 1>(s, nwg, n, A, lda, hw, w, sub, tau, delta, Vh);
