@@ -538,7 +538,7 @@ def workload_name(name, spec):
             f"seed {spec.seed}, selectroot {spec.selectroot}; {AMPLITUDE_NOTE}")
 
 
-def run_workload(name, args, steps, warmup, served, slots, autotune_on, accounting):
+def run_workload(name, args, steps, warmup, served, slots, autotune_on, accounting, lanes=1, eig_batch=64):
     """FP64 and split-precision steps of one workload, its served run and (optionally) the
     roofline of its dominant kernels.  Returns (result dict, generated problem, spec)."""
     from dirac_caspt2_amd import synth
@@ -595,7 +595,7 @@ def run_workload(name, args, steps, warmup, served, slots, autotune_on, accounti
         ctx.set_option("split_slices", 0)
         pts = plain_points(ctx, spec, sp, served, idx)
         ctx.close()
-        srv = served_run(spec, sp, served, slots, args.slot_lanes, pts, args.eig_batch, args.eig_window, args.split,
+        srv = served_run(spec, sp, served, slots, lanes, pts, eig_batch, args.eig_window, args.split,
                          args.native_min)
         srv["speedup_vs_plain_fp64"] = srv["energies_per_s"] / fp64["energies_per_s"]
         res["served"] = srv
@@ -633,11 +633,14 @@ def main():
                          "(default 64 at 324 spinors, 128 at 192; 0: skip)")
     ap.add_argument("--slots", type=int, default=None, help="concurrent slots of the served runs (default: 32 at "
                                                             "324 spinors, 64 otherwise)")
-    ap.add_argument("--slot-lanes", type=int, default=1, help="subspace lanes per slot in the served run")
+    ap.add_argument("--slot-lanes", type=int, default=None,
+                    help="subspace lanes per slot in the served runs (default: 2 at 324 spinors, 1 otherwise)")
     ap.add_argument("--split", type=int, default=6,
                     help="INT8 slices of the split-precision transforms; 6: |dE2| <= 3e-11 Eh against the "
                          "oracle at 324 spinors (5: 3e-9, too close to the 1e-8 bound for the totals; 7: 3e-13)")
-    ap.add_argument("--eig-batch", type=int, default=64, help="eigensolves batched across served requests (<=1: off)")
+    ap.add_argument("--eig-batch", type=int, default=None,
+                    help="eigensolves batched across served requests (<=1: off; default: off at 324 spinors, where "
+                         "each slot's own eigensolver is faster, 64 otherwise)")
     ap.add_argument("--native-min", type=int, default=0,
                     help="served runs: eigenproblems of at least this size use this library's eigensolver instead "
                          "of the cross-request batched rocSOLVER one (0: never)")
@@ -646,6 +649,9 @@ def main():
                          "every running request waits for one.  Default 0.1 for the runs that submit all requests "
                          "at once, 0.003 for the Poisson stream")
     args = ap.parse_args()
+    big_lanes, big_eig_batch = args.slot_lanes or 2, 0 if args.eig_batch is None else args.eig_batch
+    args.slot_lanes = args.slot_lanes or 1
+    args.eig_batch = 64 if args.eig_batch is None else args.eig_batch
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(the sharding design is documented in DESIGN.md)", file=sys.stderr)
@@ -670,9 +676,12 @@ def main():
     args.eig_window = 0.1 if window is None else window
     big = args.workload == "split"
     served = args.served if args.served is not None else (64 if big else 128)
-    slots = args.slots or (32 if big else 64)
+    # 324 spinors: 8 slots of 2 lanes with their own eigensolvers (19.1 energies/s served
+    # against 15.8 with 32 slots of 1 lane and cross-request batched rocSOLVER eigensolves)
+    slots = args.slots or (8 if big else 64)
     main_res, sp, spec = run_workload(args.workload, args, args.steps, args.warmup, served, slots,
-                                      not args.no_autotune, True)
+                                      not args.no_autotune, True, big_lanes if big else args.slot_lanes,
+                                      big_eig_batch if big else args.eig_batch)
     split, fp64 = main_res["split"], main_res["fp64"]
     result = {
         "metric": "CASPT2 energies/s + p99 latency, split-prec vs FP64 GPU at |\u0394E2|<1e-8 Eh",
@@ -708,7 +717,7 @@ def main():
             copy_calibration(cc, args.copy_calibration)
     if args.secondary and args.secondary != args.workload:
         sec, _, _ = run_workload(args.secondary, args, args.secondary_steps, max(1, min(args.warmup, 2)),
-                                 128, 64, False, False)
+                                 128, 64, False, False, args.slot_lanes, args.eig_batch)
         result["secondary"] = sec
     if args.stream and big:
         args.eig_window = 0.003 if window is None else window
