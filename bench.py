@@ -631,16 +631,16 @@ def main():
     ap.add_argument("--served", type=int, default=None, metavar="N",
                     help="N scan points of the same molecule through the request server, all submitted at once "
                          "(default 64 at 324 spinors, 128 at 192; 0: skip)")
-    ap.add_argument("--slots", type=int, default=None, help="concurrent slots of the served runs (default: 32 at "
-                                                            "324 spinors, 64 otherwise)")
+    ap.add_argument("--slots", type=int, default=None, help="concurrent slots of the served runs (default: 8 at "
+                                                            "324 spinors, 32 for the Poisson stream, 64 otherwise)")
     ap.add_argument("--slot-lanes", type=int, default=None,
                     help="subspace lanes per slot in the served runs (default: 2 at 324 spinors, 1 otherwise)")
     ap.add_argument("--split", type=int, default=6,
                     help="INT8 slices of the split-precision transforms; 6: |dE2| <= 3e-11 Eh against the "
                          "oracle at 324 spinors (5: 3e-9, too close to the 1e-8 bound for the totals; 7: 3e-13)")
     ap.add_argument("--eig-batch", type=int, default=None,
-                    help="eigensolves batched across served requests (<=1: off; default: off at 324 spinors, where "
-                         "each slot's own eigensolver is faster, 64 otherwise)")
+                    help="eigensolves batched across served requests (<=1: off; default: off at 324 spinors and "
+                         "for the Poisson stream, where each slot's own eigensolver is faster, 64 otherwise)")
     ap.add_argument("--native-min", type=int, default=0,
                     help="served runs: eigenproblems of at least this size use this library's eigensolver instead "
                          "of the cross-request batched rocSOLVER one (0: never)")
@@ -650,6 +650,9 @@ def main():
                          "at once, 0.003 for the Poisson stream")
     args = ap.parse_args()
     big_lanes, big_eig_batch = args.slot_lanes or 2, 0 if args.eig_batch is None else args.eig_batch
+    # the Poisson stream: 32 slots with their own eigensolvers (p50 48 ms, p99 263 ms at 25/s
+    # against 164 / 607 ms with 64 slots and cross-request batched rocSOLVER eigensolves)
+    stream_eig_batch = 0 if args.eig_batch is None else args.eig_batch
     args.slot_lanes = args.slot_lanes or 1
     args.eig_batch = 64 if args.eig_batch is None else args.eig_batch
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -670,7 +673,8 @@ def main():
         return
     if args.workload == "serve":
         args.eig_window = 0.003 if window is None else window
-        args.slots = args.slots or 64
+        args.slots = args.slots or 32
+        args.eig_batch = stream_eig_batch
         print(json.dumps(serve_workload(args)))
         return
     args.eig_window = 0.1 if window is None else window
@@ -721,7 +725,8 @@ def main():
         result["secondary"] = sec
     if args.stream and big:
         args.eig_window = 0.003 if window is None else window
-        args.slots = 64
+        args.slots = 32
+        args.eig_batch = stream_eig_batch
         result["serve_stream"] = serve_workload(args)
     if not args.no_cpu_baseline:
         ref, cb = cpu_baseline(sp)
