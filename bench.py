@@ -421,6 +421,8 @@ def hetrd_roofline(ctx, ht, steps, workload):
                       f"barrier per column)",
             "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(workload, "k_hetrd", n),
+            "limited_by": "latency: one grid barrier and one round of device-coherent exchanges per column, "
+                          "not FP64 arithmetic (the roofline's bound) or HBM",
             "launches_per_step": calls / steps,
             "avg_launch_us": 1e6 * sec / calls, "algorithmic_flops_per_launch": fl, "seconds_per_step": sec / steps,
             "n": n}
