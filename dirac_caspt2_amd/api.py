@@ -363,7 +363,7 @@ class Caspt2Context:
                "lanes": {}, "schur": {"off": 0, "on": 1}, "split_slices": {}, "split_min_k": {},
                "casci_solver": {"auto": 0, "dense": 1, "davidson": 2}, "graphs": {"off": 0, "on": 1},
                "tridiag_solver": {"own": 0, "rocsolver": 1}, "lane_plan": {"auto": -1}, "c_thin": {"off": 0, "on": 1},
-               "davidson_block": {}, "davidson_maxsub": {}}
+               "davidson_block": {}, "davidson_maxsub": {}, "early_v": {"off": 0, "on": 1}}
 
     def set_option(self, key: str, value):
         """Options: eigensolver auto|qr|dc|dcj|jacobi|native, ac_metric auto|formula|determinant|

@@ -177,6 +177,7 @@ struct dc2_context {
   int lane_plan = -1;        // 4 lanes: 0 D alone on lane 3, 1 D with B and F (their S/B solves
                              // run while the transforms do), -1 1 from 256 secondary spinors
   bool no_thin = false;      // option "c_thin" 0: C's metric always as the m x m Gram matrix
+  bool early_v = true;       // option "early_v": V of A and C on lane 3 once their blocks exist
   int ac_route = 0;          // A/C metric: 0 automatic, 1 RDM formulas (nact^3), 2 determinant space
   int nlanes = 4;            // concurrent subspace lanes (1 = sequential on the library stream)
   bool schur = true;         // E2 of the determinant-space route through the compressed resolvent
@@ -1333,14 +1334,63 @@ void v3_part(dc2_context* c, Lane& L, char X, const cplx* blk, int nx, cplx* V) 
 
 // one subspace k (0..7 = A..H) on lane L: V and alpha, then the solve with the prepared
 // S/B half (prepared here if prep is null or not valid)
-void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, double* sumc2,
-                  const SubPrep* prep = nullptr) {
+// V = <X|H|0> of subspace k (not H) and its energy shifts alpha, in buffers of lane L (named
+// after the subspace, so that a lane can hold those of several subspaces); k's transformed
+// blocks, the densities and h' must be ready on L's stream
+void subspace_v(dc2_context* c, Lane& L, int k, double eshift, cplx*& V_out, double*& alpha_out) {
   hipStream_t s = L.stream;
   const int ni = c->ni, na = c->na, ns = c->ns, nmo = c->nmo();
-  const int64_t na2 = (int64_t)na * na, na3 = na2 * na;
-  const int64_t nip = (int64_t)ni * (ni - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
-  const cplx one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   Dens d{na, c->G1.get(), c->G1F.get(), c->G2.get(), c->G2F.get(), c->G3.get(), c->G3F.get()};
+  auto blk = [&](const char* name) { return c->tblk.at(name).get(); };
+  const char X = kSub[k];
+  auto nm = [X](const char* b) { return std::string(1, X) + b; };
+  int64_t n = 0, npert = 0;
+  subspace_dims(c, X, n, npert);
+  cplx* V = L.z(nm("_V").c_str(), n * npert);
+  double* alpha = L.d(nm("_alpha").c_str(), npert);
+  if (n > 0 && npert > 0) {
+    switch (X) {
+      case 'A': {
+        cplx* eff = L.z("A_eff", na * ni);
+        const cplx* A1 = blk("A1");
+        effh_A(s, ni, na, c->hp.get(), nmo, blk("A2"), eff);
+        v3_part(c, L, 'A', A1, ni, V);
+        V_A_rest(s, na, ni, d, A1, eff, V);
+      } break;
+      case 'B': V_B(s, na, ni, d, blk("B"), c->tpairs.get(), c->ipairs.get(), V); break;
+      case 'C': {
+        cplx* eff = L.z("C_eff", (int64_t)ns * na);
+        const cplx* C1 = blk("C1");
+        if (ni) effh_C(s, ni, na, ns, c->hp.get(), nmo, C1, blk("C2"), blk("C3"), eff);
+        else effh_C(s, 0, na, ns, c->hp.get(), nmo, C1, C1, C1, eff);
+        v3_part(c, L, 'C', C1, ns, V);
+        V_C_rest(s, na, ns, d, eff, V);
+      } break;
+      case 'D': {
+        cplx* eff = L.z("D_eff", (int64_t)ns * ni);
+        effh_D(s, ni, na, ns, c->hp.get(), nmo, blk("D3"), eff);
+        V_D(s, na, ni, ns, d, blk("D1"), blk("D2"), eff, V);
+      } break;
+      case 'E': V_E(s, na, ni, ns, d, blk("E"), c->ipairs.get(), V); break;
+      case 'F':
+        V_F(s, na, ns, d, blk("F"), c->tpairs.get(), c->spairs.get(), V, L.z("F_vscr", V_F_scratch(na, ns)));
+        break;
+      case 'G': V_G(s, na, ni, ns, d, blk("G"), c->spairs.get(), V); break;
+    }
+    alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert, alpha);
+  }
+  check_finite(c, s, nm("V").c_str(), V, n * npert);
+  check_finite(c, s, nm("alpha").c_str(), alpha, npert);
+  V_out = V;
+  alpha_out = alpha;
+}
+
+void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, double* sumc2,
+                  const SubPrep* prep = nullptr, const cplx* vpre = nullptr, const double* apre = nullptr,
+                  hipEvent_t vpre_ready = nullptr) {
+  hipStream_t s = L.stream;
+  const int ni = c->ni, na = c->na, ns = c->ns;
+  const int64_t nip = (int64_t)ni * (ni - 1) / 2, nsp = (int64_t)ns * (ns - 1) / 2;
   auto blk = [&](const char* name) { return c->tblk.at(name).get(); };
   const char X = kSub[k];
   auto nm = [X](const char* b) { return std::string(1, X) + b; };
@@ -1366,41 +1416,15 @@ void run_subspace(dc2_context* c, Lane& L, int k, double eshift, double* e2, dou
     n = 0;
   } else {
     const AcRoute route = ac_route_of(c, X);
-    cplx* V = L.z(nm("_V").c_str(), n * npert);
-    double* alpha = L.d(nm("_alpha").c_str(), npert);
-    if (n > 0 && npert > 0) {
-      switch (X) {
-        case 'A': {
-          cplx* eff = L.z("A_eff", na * ni);
-          const cplx* A1 = blk("A1");
-          effh_A(s, ni, na, c->hp.get(), nmo, blk("A2"), eff);
-          v3_part(c, L, 'A', A1, ni, V);
-          V_A_rest(s, na, ni, d, A1, eff, V);
-        } break;
-        case 'B': V_B(s, na, ni, d, blk("B"), c->tpairs.get(), c->ipairs.get(), V); break;
-        case 'C': {
-          cplx* eff = L.z("C_eff", (int64_t)ns * na);
-          const cplx* C1 = blk("C1");
-          if (ni) effh_C(s, ni, na, ns, c->hp.get(), nmo, C1, blk("C2"), blk("C3"), eff);
-          else effh_C(s, 0, na, ns, c->hp.get(), nmo, C1, C1, C1, eff);
-          v3_part(c, L, 'C', C1, ns, V);
-          V_C_rest(s, na, ns, d, eff, V);
-        } break;
-        case 'D': {
-          cplx* eff = L.z("D_eff", (int64_t)ns * ni);
-          effh_D(s, ni, na, ns, c->hp.get(), nmo, blk("D3"), eff);
-          V_D(s, na, ni, ns, d, blk("D1"), blk("D2"), eff, V);
-        } break;
-        case 'E': V_E(s, na, ni, ns, d, blk("E"), c->ipairs.get(), V); break;
-        case 'F':
-          V_F(s, na, ns, d, blk("F"), c->tpairs.get(), c->spairs.get(), V, L.z("F_vscr", V_F_scratch(na, ns)));
-          break;
-        case 'G': V_G(s, na, ni, ns, d, blk("G"), c->spairs.get(), V); break;
-      }
-      alphas(s, X, ni, na, ns, c->eps.get(), c->e0.get(), eshift, c->ipairs.get(), c->spairs.get(), npert, alpha);
+    cplx* V = nullptr;
+    double* alpha = nullptr;
+    if (vpre) {   // computed on another lane
+      DC2_HIP(hipStreamWaitEvent(s, vpre_ready, 0));
+      V = const_cast<cplx*>(vpre);
+      alpha = const_cast<double*>(apre);
+    } else {
+      subspace_v(c, L, k, eshift, V, alpha);
     }
-    check_finite(c, s, nm("V").c_str(), V, n * npert);
-    check_finite(c, s, nm("alpha").c_str(), alpha, npert);
     if (route.gram) {
       r = solve_block_gram(c, L, X, prep->g, n, npert, V, alpha);
     } else if (route.reduced) {
@@ -1559,6 +1583,29 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
   std::promise<bool> go_rot, go_dens, go;
   std::shared_future<bool> go_rot_f = go_rot.get_future().share(), go_dens_f = go_dens.get_future().share(),
                            go_f = go.get_future().share();
+  // V of A and C on lane 3 (D's lane, idle while the transforms run) as soon as their blocks
+  // are transformed (ac_ready, go_ac), so that the A and C lanes, the longest, find it ready
+  // after their two eigensolves (vpre_ready, go_vpre: set once by lane 3 on every path)
+  const bool early_v = c->early_v && plan.size() >= 4 && !plan[3].empty() && !ras &&
+                       (plan.size() > 1 && (doA || doC));
+  const cplx* vpre[8] = {};
+  const double* apre[8] = {};
+  hipEvent_t ac_ready = nullptr, vpre_ready = nullptr;
+  DC2_HIP(hipEventCreateWithFlags(&ac_ready, hipEventDisableTiming));
+  DC2_HIP(hipEventCreateWithFlags(&vpre_ready, hipEventDisableTiming));
+  std::promise<bool> go_ac, go_vpre;
+  std::shared_future<bool> go_ac_f = go_ac.get_future().share(), go_vpre_f = go_vpre.get_future().share();
+  std::atomic<bool> ac_set{false}, vpre_set{false};
+  auto set_ac = [&](bool v) {
+    if (!ac_set.exchange(true)) go_ac.set_value(v);
+  };
+  auto set_vpre = [&](bool v) {
+    if (!vpre_set.exchange(true)) go_vpre.set_value(v);
+  };
+  if (!early_v) {
+    set_ac(false);
+    set_vpre(false);
+  }
   std::vector<std::thread> threads;
   for (size_t w = 1; w < plan.size(); ++w) {
     if (plan[w].empty()) continue;
@@ -1599,21 +1646,50 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
         // A's reduced metric from the determinant-space factor needs the rotated state only
         for (int k : plan[w])
           if (k == 0 && ac_route_of(c, 'A').phi) prep[k] = subspace_prep(c, Lw, k);
-        if (!go_dens_f.get()) return;
+        if (!go_dens_f.get()) {
+          if (w == 3) set_vpre(false);
+          return;
+        }
         DC2_HIP(hipStreamWaitEvent(Lw.stream, dens_ready, 0));
         for (int k : plan[w])
           if (!prep[k].valid) prep[k] = subspace_prep(c, Lw, k);
+        if (w == 3 && early_v) {
+          if (!go_ac_f.get()) {
+            set_vpre(false);
+            return;
+          }
+          DC2_HIP(hipStreamWaitEvent(Lw.stream, ac_ready, 0));
+          cplx* V = nullptr;
+          double* alpha = nullptr;
+          for (int k : {0, 2})
+            if (do_sub[k]) {
+              subspace_v(c, Lw, k, eshift, V, alpha);
+              vpre[k] = V;
+              apre[k] = alpha;
+            }
+          DC2_HIP(hipEventRecord(vpre_ready, Lw.stream));
+          set_vpre(true);
+        }
         if (!go_f.get()) return;
         DC2_HIP(hipStreamWaitEvent(Lw.stream, ready, 0));
-        for (int k : plan[w]) run_subspace(c, Lw, k, eshift, e2, sumc2, &prep[k]);
+        for (int k : plan[w]) {
+          if ((k == 0 || k == 2) && early_v && go_vpre_f.get())
+            run_subspace(c, Lw, k, eshift, e2, sumc2, &prep[k], vpre[k], apre[k], vpre_ready);
+          else
+            run_subspace(c, Lw, k, eshift, e2, sumc2, &prep[k]);
+        }
         DC2_HIP(hipStreamSynchronize(Lw.stream));
       } catch (...) {
         errs[w] = std::current_exception();
+        if (w == 3) set_vpre(false);
       }
     });
   }
   auto finish_lanes = [&]() {
+    set_ac(false);   // no effect once set; releases lane 3 when the transforms never ran
     for (auto& t : threads) t.join();
+    (void)hipEventDestroy(ac_ready);
+    (void)hipEventDestroy(vpre_ready);
     (void)hipEventDestroy(y1o_ready);
     (void)hipEventDestroy(rot_ready);
     (void)hipEventDestroy(dens_ready);
@@ -1771,8 +1847,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     Timer t3(s);
     std::vector<std::string> need;
     if (doA) { need.push_back("A1"); need.push_back("A2"); }
-    if (doB) need.push_back("B");
     if (doC) { need.push_back("C1"); if (ni) { need.push_back("C2"); need.push_back("C3"); } }
+    const size_t n_ac = need.size();   // A's and C's blocks first: lane 3 forms their V early
+    if (doB) need.push_back("B");
     if (doD) { need.push_back("D1"); need.push_back("D2"); need.push_back("D3"); }
     if (doE) need.push_back("E");
     if (doF) need.push_back("F");
@@ -1780,7 +1857,16 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     if (doH) need.push_back("H");
     set_gemm_tag(1);
     c->split_flops = c->fp64_flops = 0.0;
-    for (const auto& nm : need) transform(c, nm);
+    // every block's map entry exists before lane 3 may look up A's and C's (transform then
+    // only finds entries; inserting one while another thread reads the map would race)
+    for (const auto& nm : need) (void)c->tblk[nm];
+    for (size_t i = 0; i < need.size(); ++i) {
+      transform(c, need[i]);
+      if (i + 1 == n_ac && early_v) {
+        DC2_HIP(hipEventRecord(ac_ready, s));
+        set_ac(true);
+      }
+    }
     set_gemm_tag(0);
     c->timings[3] = t3.stop();
     for (const auto& nm : need) check_finite(c, ("transformed " + nm).c_str(), c->tblk[nm].get(), block_count(c, shape_of(nm)));
@@ -2441,6 +2527,9 @@ int dc2_set_option(dc2_context* c, const char* key, int64_t value) {
     } else if (k == "c_thin") {
       DC2_CHECK(value == 0 || value == 1, 1, "c_thin must be 0 or 1");
       c->no_thin = value == 0;
+    } else if (k == "early_v") {
+      DC2_CHECK(value == 0 || value == 1, 1, "early_v must be 0 or 1");
+      c->early_v = value != 0;
     } else if (k == "lane_plan") {
       DC2_CHECK(value >= -1 && value <= 1, 1, "lane_plan must be -1, 0 or 1");
       c->lane_plan = (int)value;

@@ -202,6 +202,9 @@ int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
  *                  solves (default 4: A, C, D and the rest; 3: A, C and the rest);
  *   "lane_plan":   with 4 lanes, -1 (default) B and F on D's lane from 256 secondary spinors,
  *                  0 never, 1 always;
+ *   "early_v":     1 (default) with 4 lanes, V of A and C formed on D's lane as soon as their
+ *                  transformed blocks exist (the A and C lanes use it after their eigensolves),
+ *                  0 on their own lanes;
  *   "c_thin":      1 (default) the determinant-space C metric through its thin factor (an
  *                  eigenproblem of nact^2 (nact-1)/2 instead of C(nact, nelec-1)) when that is
  *                  smaller and more than 32 directions are cut anyway, 0 never;

@@ -136,6 +136,25 @@ def test_medium_lane_plan(medium):
         assert_energies(out, ref, 1e-9, "medium lane_plan 1")
 
 
+def test_medium_early_v(medium):
+    """V of A and C formed on D's lane while the transforms run (option early_v, the default
+    with 4 lanes) gives the energies of V formed on their own lanes, and both the oracle's."""
+    from gpu_helpers import assert_energies, context_from_synth, say
+    sp, ref = medium
+    outs = {}
+    for ev in ("on", "off"):
+        say("medium: gpu, early_v", ev)
+        with context_from_synth(sp) as ctx:
+            ctx.set_option("early_v", ev)
+            for _ in range(2):   # the second energy reuses the lanes' buffers
+                ctx.casci(1)
+                outs[ev] = ctx.caspt2(1, 0.0)
+            assert_energies(outs[ev], ref, 1e-9, "medium early_v " + ev)
+    for X in "ABCDEFGH":
+        assert abs(outs["on"].e2[X] - outs["off"].e2[X]) < 1e-12, X
+        assert abs(outs["on"].sumc2[X] - outs["off"].sumc2[X]) < 1e-12, X
+
+
 def test_c_metric_thin_factor():
     """14 active spinors with 6 electrons: C's metric through its thin factor (an eigenproblem
     of 14*14*13/2 = 1274 instead of C(14, 5) = 2002) gives the energies of the full Gram
