@@ -792,17 +792,41 @@ int64_t gemm_choice_count() {
   return (int64_t)g_tune->size();
 }
 
+namespace {
+// p[i] = seeded values in (-0.5, 0.5) + i (-0.5, 0.5) from a 64-bit integer hash of (seed, i)
+// (splitmix64's finaliser), a grid-stride loop
+__global__ __launch_bounds__(256) void k_fill_hash(cplx* __restrict__ p, int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double re = (double)(z >> 40) * (1.0 / 16777216.0) - 0.5;
+    const double im = (double)((z >> 16) & 0xFFFFFF) * (1.0 / 16777216.0) - 0.5;
+    p[i] = mk(re, im);
+  }
+}
+
+void fill_hash(hipStream_t s, cplx* p, int64_t n, uint64_t seed) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>(cdiv(n, 256), 65535);
+  hipLaunchKernelGGL(k_fill_hash, dim3((unsigned)blocks), dim3(256), 0, s, p, n, seed);
+  DC2_HIP(hipGetLastError());
+}
+}  // namespace
+
 void gemm_tune(hipStream_t s, char opA, char opB, int64_t M, int64_t N, int64_t K, int64_t batch, int reps,
                double* out) {
   DC2_CHECK(M > 0 && N > 0 && K > 0 && batch > 0 && batch < 65536 && reps >= 1, 1, "gemm_tune: invalid shape");
   DC2_CHECK(std::string("NTC").find(opA) != std::string::npos && std::string("NRTC").find(opB) != std::string::npos, 1,
             "gemm_tune: bad op");
-  // scratch operands (zeros: the timing does not depend on the values)
+  // scratch operands with seeded nonzero values (an all-zero product draws less power than
+  // the real one, so its clocks and timings could differ)
   const int64_t lda = opA == 'N' ? M : K, ldb = (opB == 'N' || opB == 'R') ? K : N;
   const int64_t sA = M * K, sB = K * N, sC = M * N;
   DevBuf<cplx> A((size_t)(sA * batch)), B((size_t)(sB * batch)), C((size_t)(sC * batch));
-  DC2_HIP(hipMemsetAsync(A.get(), 0, sA * batch * sizeof(cplx), s));
-  DC2_HIP(hipMemsetAsync(B.get(), 0, sB * batch * sizeof(cplx), s));
+  fill_hash(s, A.get(), sA * batch, 0x5eed0001ull);
+  fill_hash(s, B.get(), sB * batch, 0x5eed0002ull);
   DC2_HIP(hipMemsetAsync(C.get(), 0, sC * batch * sizeof(cplx), s));
   hipEvent_t e0, e1;
   DC2_HIP(hipEventCreate(&e0));
