@@ -1695,9 +1695,9 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
     (void)hipEventDestroy(dens_ready);
     (void)hipEventDestroy(ready);
   };
-  // blocks of at most kHostFockEig spinors are diagonalised on the host (a rocSOLVER call
-  // costs about a millisecond at any size); the canonical gauge makes the eigenvectors
-  // independent of the solver
+  // with the automatic solver choice, blocks of at most kHostFockEig spinors are diagonalised
+  // on the host (a device solve is a chain of dependent launches); the canonical gauge makes
+  // the eigenvectors independent of the solver
   constexpr int kHostFockEig = 64;
   auto diag_space = [&](int sp) {
     const int lo = sp == 0 ? 0 : (sp == 1 ? ni : ni + na), hi = sp == 0 ? ni : (sp == 1 ? ni + na : nmo);
@@ -1713,7 +1713,7 @@ void run_caspt2(dc2_context* c, int selectroot, double eshift, double* e2, doubl
       cplx* scr = L0.z("fd_scr", (int64_t)n * n);
       double* w = L0.d("fd_w", n);
       gather_block(s, n, dind, c->fock.get(), nmo, blk);
-      if (n <= kHostFockEig) host_heev_upper(L0, n, blk, w);
+      if (n <= kHostFockEig && c->eig_method < 0) host_heev_upper(L0, n, blk, w);   // not when a solver is chosen
       else zheevd(c, n, blk, w, "a Fock block");
       canonical_gauge(s, n, w, blk, scr, L0.i("fd_starts", n + 2));
       scatter_block(s, n, dind, blk, w, c->U.get(), nmo, c->eps.get());

@@ -182,9 +182,10 @@ int dc2_gemm_clear_choices(int64_t* count_before);
  * (products routed to the kernels for skinny shapes are counted by dc2_get_gemm_stats only) */
 int dc2_get_gemm_op_stats(dc2_context* ctx, char opA, char opB, double* out);
 /* Numerical options (no counterpart in the reference):
- *   "eigensolver": -1 automatic (default: matrices up to 32 on the host, 4 for 33 <= n <=
- *                  native_max outside the batched server, rocSOLVER divide and conquer
- *                  otherwise, with Jacobi leaves for rank-deficient S matrices), 0 tridiagonal
+ *   "eigensolver": -1 automatic (default: matrices up to 32, and Fock blocks up to 64, on the
+ *                  host, 4 for 33 <= n <= native_max outside the batched server, rocSOLVER
+ *                  divide and conquer otherwise, with Jacobi leaves for rank-deficient S
+ *                  matrices); any other value puts every eigenproblem on the device: 0 tridiagonal
  *                  QR (the ZHEEV algorithm), 1 divide and conquer, 2 D&C with Jacobi leaves,
  *                  3 Jacobi, 4 this library's Householder tridiagonalisation (one persistent
  *                  kernel) + its tridiagonal divide and conquer (or rocSOLVER dstedc, option

@@ -136,6 +136,21 @@ def test_medium_lane_plan(medium):
         assert_energies(out, ref, 1e-9, "medium lane_plan 1")
 
 
+@pytest.mark.parametrize("solver", ["native", "qr"])
+def test_medium_all_eigensolves_on_device(medium, solver):
+    """Every eigenproblem of the path on the device (an explicit eigensolver: the S and B~
+    matrices up to 32 and the Fock blocks up to 64 included, which the automatic choice
+    solves on the host): the oracle's energies within 1e-9 Eh."""
+    from gpu_helpers import assert_energies, context_from_synth, say
+    sp, ref = medium
+    say("medium: gpu, eigensolver", solver)
+    with context_from_synth(sp) as ctx:
+        ctx.set_option("eigensolver", solver)
+        ctx.casci(1)
+        out = ctx.caspt2(1, 0.0)
+        assert_energies(out, ref, 1e-9, "medium eigensolver " + solver)
+
+
 def test_medium_early_v(medium):
     """V of A and C formed on D's lane while the transforms run (option early_v, the default
     with 4 lanes) gives the energies of V formed on their own lanes, and both the oracle's."""
